@@ -1,0 +1,233 @@
+/*
+ * pfm.h -- C ABI of libpfm, the MI355X (gfx950) pressure-equation path:
+ * LDU assembly, LDU SpMV, PCG(DIC) / PBiCGStab(DILU) / GAMG solves and the
+ * adjoint face-gradient scatter.
+ *
+ * Conventions
+ *   - Every function returns an int status (PFM_OK = 0).  On failure the
+ *     message is available from pfm_last_error() (thread-local).
+ *   - Device arrays are raw pointers into caller-owned memory; the library
+ *     never frees them.  Host arrays are marked _h.
+ *   - `stream` is a hipStream_t passed as void* (NULL = default stream).
+ *     All device work is enqueued on it; solver entry points synchronise
+ *     it once at the end to return their result struct.
+ *   - Floating point: FP64 unless a name ends in _f32.  Indices: int32.
+ *   - LDU storage (reference src/pyfoam/core/ldu_matrix.py:27-71):
+ *       diag[n_cells]; lower[f] = A[owner[f], neighbour[f]];
+ *       upper[f] = A[neighbour[f], owner[f]].
+ *
+ * Reference citations are relative to the pyOpenFOAM source tree
+ * (src/pyfoam/...).
+ */
+#ifndef PFM_H
+#define PFM_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define PFM_OK 0
+#define PFM_ERR_ARG 1      /* bad argument: maps to ValueError / TypeError */
+#define PFM_ERR_HIP 2      /* HIP runtime error */
+#define PFM_ERR_STATE 3    /* handle not usable for this call */
+#define PFM_ERR_NOMEM 4
+
+/* Solver status codes (pfm_solve_result.status). */
+#define PFM_SOLVE_RUNNING 0
+#define PFM_SOLVE_CONVERGED 1
+#define PFM_SOLVE_MAXITER 2
+#define PFM_SOLVE_BREAKDOWN 3
+
+#define PFM_PRECOND_NONE 0
+#define PFM_PRECOND_DIAG 1 /* z = r / rD, rD from pfm_dic_factor / pfm_dilu_factor */
+
+const char *pfm_last_error(void);
+int pfm_abi_version(void);
+
+/* ------------------------------------------------------------------ */
+/* Host-side setup algorithms (no GPU needed)                          */
+/* ------------------------------------------------------------------ */
+
+/* Reverse Cuthill-McKee ordering, perm_out[new] = old.
+ * Replaces tools/renumber_mesh.py:163-239 (_reverse_cuthill_mckee) with the
+ * same tie-breaks: neighbours in ascending index stably sorted by degree,
+ * start = lowest-index unvisited cell of minimum degree, FIFO BFS, restart
+ * on disconnected parts, whole list reversed. */
+int pfm_rcm(int32_t n_cells, int32_t n_faces, const int32_t *owner_h,
+            const int32_t *neighbour_h, int32_t *perm_out_h);
+
+/* Dependency level of every cell in the DIC factor recurrence
+ * (solvers/preconditioners.py:102-133): cell i waits for every owner j < i
+ * of a face whose neighbour is i.  Returns the level count in *n_levels. */
+int pfm_dic_levels(int32_t n_cells, int32_t n_faces, const int32_t *owner_h,
+                   const int32_t *neighbour_h, int32_t *level_out_h,
+                   int32_t *n_levels);
+
+/* Dependency level of every face in the DILU factor recurrence
+ * (solvers/preconditioners.py:244-272): face f waits for the previous face
+ * (in face order) touching either of its cells. */
+int pfm_dilu_levels(int32_t n_cells, int32_t n_faces, const int32_t *owner_h,
+                    const int32_t *neighbour_h, int32_t *level_out_h,
+                    int32_t *n_levels);
+
+/* ------------------------------------------------------------------ */
+/* LDU addressing handle                                              */
+/* ------------------------------------------------------------------ */
+
+typedef struct pfm_ldu_s *pfm_ldu;
+
+typedef struct {
+  int32_t n_cells;
+  int32_t n_faces;
+  int32_t n_slices;      /* row slices of 64 cells (one wavefront each) */
+  int32_t n_dic_levels;
+  int32_t n_dilu_levels;
+  int32_t max_lower;     /* max owner-faces of one cell */
+  int32_t max_upper;     /* max neighbour-faces of one cell */
+  int32_t device;
+  int64_t n_slots;       /* length of a packed coefficient array */
+} pfm_ldu_info;
+
+/* Build the device addressing of one LDU topology (owner/neighbour of the
+ * internal faces, reference LduMatrix.__init__ ldu_matrix.py:74-100):
+ * row slices for the SpMV, DIC cell levels and DILU face levels.
+ * Topology arrays are host int32; the current HIP device is used. */
+int pfm_ldu_create(int32_t n_cells, int32_t n_faces, const int32_t *owner_h,
+                   const int32_t *neighbour_h, pfm_ldu *out);
+int pfm_ldu_destroy(pfm_ldu h);
+int pfm_ldu_get_info(pfm_ldu h, pfm_ldu_info *out);
+
+/* Pack face coefficients into row-slice order: packed[n_slots].
+ * transpose = 0 packs A, transpose = 1 packs A^T (lower and upper swapped,
+ * as LduMatrix.Ax_T ldu_matrix.py:219-258 and the adjoint matrix of
+ * differentiable/linear_solver.py:127-131). */
+int pfm_ldu_pack(pfm_ldu h, int transpose, const double *lower,
+                 const double *upper, double *packed, void *stream);
+int pfm_ldu_pack_f32(pfm_ldu h, int transpose, const double *lower,
+                     const double *upper, float *packed, void *stream);
+
+/* y = A x on a packed matrix.  Row sums follow the reference association
+ * y = (diag*x + sum_owner-faces lower*x[nei]) + sum_neighbour-faces upper*x[own]
+ * with each sum in face order, i.e. LduMatrix.Ax ldu_matrix.py:200-217. */
+int pfm_spmv(pfm_ldu h, const double *diag, const double *packed,
+             const double *x, double *y, void *stream);
+int pfm_spmv_f32(pfm_ldu h, const float *diag, const float *packed,
+                 const float *x, float *y, void *stream);
+/* y = A x (transpose = 0) or A^T x (transpose = 1) straight from the face
+ * arrays, no packing: LduMatrix.Ax / Ax_T (ldu_matrix.py:172-258). */
+int pfm_spmv_ldu(pfm_ldu h, int transpose, const double *diag,
+                 const double *lower, const double *upper, const double *x,
+                 double *y, void *stream);
+/* r = b - A x (packed A). */
+int pfm_residual(pfm_ldu h, const double *diag, const double *packed,
+                 const double *x, const double *b, double *r, void *stream);
+
+/* DIC factor (preconditioners.py:88-133): rD = max(|d|, 1e-30) with the
+ * sequential recurrence d[i] -= upper^2 / d[owner] evaluated level by level.
+ * `packed` must come from pfm_ldu_pack with the same transpose flag as the
+ * matrix being solved. Bit-exact with the sequential recurrence. */
+int pfm_dic_factor(pfm_ldu h, const double *diag, const double *packed,
+                   double *rD, void *stream);
+/* DILU factor (preconditioners.py:228-272), face levels, bit-exact. */
+int pfm_dilu_factor(pfm_ldu h, int transpose, const double *diag,
+                    const double *lower, const double *upper, double *rD,
+                    void *stream);
+
+/* ------------------------------------------------------------------ */
+/* Krylov solvers                                                     */
+/* ------------------------------------------------------------------ */
+
+typedef struct {
+  double tolerance; /* absolute: converged when ||r|| < tolerance */
+  double rel_tol;   /* relative: ||r|| / ||r0|| < rel_tol (r0 > 0) */
+  int32_t max_iter;
+  int32_t min_iter;
+  int32_t precond;  /* PFM_PRECOND_* */
+  int32_t reserved;
+} pfm_solve_params;
+
+typedef struct {
+  int32_t iterations;  /* length of the residual history (reference
+                          ConvergenceInfo.iterations, residual.py:162) */
+  int32_t converged;
+  int32_t status;      /* PFM_SOLVE_* */
+  int32_t reserved;
+  double initial_residual;
+  double final_residual;
+} pfm_solve_result;
+
+/* Device workspace (bytes) for one solve with at most max_iter iterations. */
+int64_t pfm_pcg_workspace_bytes(pfm_ldu h, int32_t max_iter);
+int64_t pfm_pbicgstab_workspace_bytes(pfm_ldu h, int32_t max_iter);
+
+/* PCG (solvers/pcg.py:97-187) with the stopping rule of
+ * solvers/residual.py:112-145; x holds x0 on entry and the solution on exit.
+ * history_h (host, may be NULL) receives result.iterations residual norms. */
+int pfm_pcg_solve(pfm_ldu h, const pfm_solve_params *prm, const double *diag,
+                  const double *packed, const double *rD, const double *b,
+                  double *x, void *work, double *history_h,
+                  pfm_solve_result *res, void *stream);
+/* PBiCGStab (solvers/pbicgstab.py:103-239), same conventions. */
+int pfm_pbicgstab_solve(pfm_ldu h, const pfm_solve_params *prm,
+                        const double *diag, const double *packed,
+                        const double *rD, const double *b, double *x,
+                        void *work, double *history_h, pfm_solve_result *res,
+                        void *stream);
+
+/* ------------------------------------------------------------------ */
+/* Assembly and adjoint                                               */
+/* ------------------------------------------------------------------ */
+
+/* fvm.laplacian (discretisation/operators.py:490-517) plus the
+ * laplacianFoam Dirichlet diagonal term (applications/laplacian_foam.py:342-366):
+ *   c_f = gamma_f*|S_f|*delta_f, lower_f = -c_f/V_P, upper_f = -c_f/V_N,
+ *   diag_i = sum c/V_P over owner faces + sum c/V_N over neighbour faces
+ *            + sum over Dirichlet faces b of cell i of gamma_b*|S_b|*delta_b/V_i.
+ * Dirichlet faces are given by owner cell (dir_cell, n_dir entries, sorted by
+ * cell), gamma_b, |S_b| and delta_b = 1/max(|(x_f - C_P).n_hat|, 1e-30).
+ * Every product and quotient is rounded in the reference's order, so the
+ * result is bit-identical to the reference's torch ops. */
+int pfm_laplacian_assemble(pfm_ldu h, const double *gamma, const double *mag_sf,
+                           const double *delta, const double *vol, int32_t n_dir,
+                           const int32_t *dir_cell, const double *dir_gamma,
+                           const double *dir_mag_sf, const double *dir_delta,
+                           double *diag, double *lower, double *upper,
+                           void *stream);
+
+/* Adjoint gradients of DifferentiableLinearSolve.backward
+ * (differentiable/linear_solver.py:141-149):
+ *   g_diag = -lam*x, g_lower = -lam[own]*x[nei], g_upper = -lam[nei]*x[own].
+ * Any output pointer may be NULL. */
+int pfm_adjoint_ldu_grads(pfm_ldu h, const double *lam, const double *x,
+                          double *g_diag, double *g_lower, double *g_upper,
+                          void *stream);
+
+/* Chain rule of the assembly above: gradient w.r.t. gamma_f given gradients
+ * w.r.t. (diag, lower, upper):
+ *   g_gamma_f = |S_f| delta_f ((g_diag[P]-g_lower_f)/V_P + (g_diag[N]-g_upper_f)/V_N)
+ * and for Dirichlet faces g_gamma_b = |S_b| delta_b g_diag[P] / V_P. */
+int pfm_laplacian_backward(pfm_ldu h, const double *g_diag,
+                           const double *g_lower, const double *g_upper,
+                           const double *mag_sf, const double *delta,
+                           const double *vol, int32_t n_dir,
+                           const int32_t *dir_cell, const double *dir_mag_sf,
+                           const double *dir_delta, double *g_gamma,
+                           double *g_dir_gamma, void *stream);
+
+/* Fused adjoint face-gradient scatter straight to dJ/dgamma:
+ *   dJ/dgamma_f = -|S_f| delta_f (x_P - x_N) (lam_P/V_P - lam_N/V_N)
+ *   dJ/dgamma_b = -|S_b| delta_b lam_P x_P / V_P      (Dirichlet faces). */
+int pfm_adjoint_gamma_grad(pfm_ldu h, const double *lam, const double *x,
+                           const double *mag_sf, const double *delta,
+                           const double *vol, int32_t n_dir,
+                           const int32_t *dir_cell, const double *dir_mag_sf,
+                           const double *dir_delta, double *g_gamma,
+                           double *g_dir_gamma, void *stream);
+
+#ifdef __cplusplus
+}
+#endif
+#endif /* PFM_H */

@@ -1,0 +1,158 @@
+"""Krylov solvers and the residual monitor (oracle; test infrastructure only).
+
+Restates, in numpy float64 with the reference's op order:
+  * ``Monitor``   -- ResidualMonitor.update/build_info (solvers/residual.py:112-168)
+  * ``pcg``       -- PCGSolver._solve (solvers/pcg.py:97-187)
+  * ``pbicgstab`` -- PBiCGSTABSolver._solve (solvers/pbicgstab.py:103-239)
+``matvec(x)`` is the LDU product (oracle.ldu.ax) and ``precond(r)`` the
+diagonal apply r / rD, or None for no preconditioner.  ``dot`` selects the
+summation order of the dot products and norms (the reference leaves it to
+torch): BLAS (default), exact (correctly rounded) or numpy pairwise; running
+all three measures how sensitive a solve is to that order.  The return value
+mirrors LinearSolverBase.__call__ (linear_solver.py:88-126) plus the
+ConvergenceInfo fields: (x, iterations, final_residual, info).
+"""
+
+from __future__ import annotations
+
+from dataclasses import dataclass, field
+
+import math
+
+import numpy as np
+
+
+@dataclass
+class Info:
+    converged: bool
+    iterations: int
+    final_residual: float
+    initial_residual: float
+    residual_history: list = field(default_factory=list)
+
+
+def dot_blas(a, b):
+    return np.dot(a, b)
+
+
+def dot_exact(a, b):
+    """Correctly rounded dot product (exact sum of the rounded products)."""
+    return math.fsum((a * b).tolist())
+
+
+def dot_pairwise(a, b):
+    """numpy's pairwise summation of the rounded products."""
+    return np.add.reduce(a * b)
+
+
+DOTS = {"blas": dot_blas, "exact": dot_exact, "pairwise": dot_pairwise}
+
+
+class Monitor:
+    def __init__(self, tolerance=1e-6, rel_tol=0.01, min_iter=0, dot=dot_blas):
+        self.tol, self.rel_tol, self.min_iter = tolerance, rel_tol, min_iter
+        self.history: list[float] = []
+        self.r0 = None
+        self.dot = dot
+
+    def update(self, r: np.ndarray, it: int) -> bool:
+        res = float(np.sqrt(self.dot(r, r)))
+        self.history.append(res)
+        if self.r0 is None:
+            self.r0 = res
+        if it >= self.min_iter:
+            if res < self.tol:
+                return True
+            if self.r0 > 0:
+                if res / self.r0 < self.rel_tol:
+                    return True
+        return False
+
+    def info(self, converged: bool) -> Info:
+        r0 = self.r0 or 0.0
+        final = self.history[-1] if self.history else 0.0
+        return Info(converged, len(self.history), final, r0, list(self.history))
+
+
+def pcg(matvec, precond, b, x0, *, tolerance=1e-6, rel_tol=0.01, max_iter=1000, min_iter=0,
+        dot=dot_blas):
+    mon = Monitor(tolerance, rel_tol, min_iter, dot)
+    x = np.array(x0, dtype=np.float64, copy=True)
+    r = b - matvec(x)
+    if mon.update(r, 0):
+        return _ret(x, mon, True)
+    z = precond(r) if precond is not None else r.copy()
+    p = z.copy()
+    rz = dot(r, z)
+    for i in range(1, max_iter + 1):
+        ap = matvec(p)
+        pap = dot(p, ap)
+        if abs(float(pap)) < 1e-30:
+            return _ret(x, mon, False)
+        alpha = rz / pap
+        x = x + alpha * p
+        r = r - alpha * ap
+        if mon.update(r, i):
+            return _ret(x, mon, True)
+        z = precond(r) if precond is not None else r.copy()
+        rz_new = dot(r, z)
+        beta = rz_new / rz
+        rz = rz_new
+        p = z + beta * p
+    return _ret(x, mon, False)
+
+
+def pbicgstab(matvec, precond, b, x0, *, tolerance=1e-6, rel_tol=0.01, max_iter=1000, min_iter=0,
+              dot=dot_blas):
+    mon = Monitor(tolerance, rel_tol, min_iter, dot)
+    x = np.array(x0, dtype=np.float64, copy=True)
+    r = b - matvec(x)
+    if mon.update(r, 0):
+        return _ret(x, mon, True)
+    rt = r.copy()
+    rho = dot(rt, r)
+    p = r.copy()
+    v = np.zeros_like(r)
+    for i in range(1, max_iter + 1):
+        if abs(float(rho)) < 1e-30:
+            return _ret(x, mon, False)
+        y = precond(p) if precond is not None else p.copy()
+        v = matvec(y)
+        rtv = dot(rt, v)
+        if abs(float(rtv)) < 1e-30:
+            return _ret(x, mon, False)
+        alpha = rho / rtv
+        s = r - alpha * v
+        if mon.update(s, i):
+            x = x + alpha * y
+            return _ret(x, mon, True)
+        z = precond(s) if precond is not None else s.copy()
+        t = matvec(z)
+        tt = dot(t, t)
+        if abs(float(tt)) < 1e-30:
+            x = x + alpha * y
+            r = s.copy()
+            if mon.update(r, i):
+                return _ret(x, mon, True)
+            rho_new = dot(rt, r)
+            beta = (rho_new / rho) * alpha
+            rho = rho_new
+            p = r + beta * (p - v)
+            continue
+        omega = dot(t, z) / tt
+        x = x + alpha * y + omega * z
+        r = s - omega * t
+        if mon.update(r, i):
+            return _ret(x, mon, True)
+        rho_new = dot(rt, r)
+        if abs(float(omega)) < 1e-30:
+            return _ret(x, mon, False)
+        beta = (rho_new / rho) * (alpha / omega)
+        rho = rho_new
+        p = r + beta * (p - omega * v)
+    return _ret(x, mon, False)
+
+
+def _ret(x, mon: Monitor, converged: bool):
+    info = mon.info(converged)
+    return x, info.iterations, info.final_residual, info

@@ -1,0 +1,119 @@
+"""RCM ordering and mesh geometry (oracle; test infrastructure only).
+
+* ``rcm`` restates tools/renumber_mesh.py:115-239 (_build_adjacency, _degree,
+  _reverse_cuthill_mckee): coalesced symmetric adjacency, neighbour lists in
+  ascending column order stably sorted by degree, start = first unvisited
+  index of minimum degree, FIFO BFS, reversed; perm[new] = old.
+* ``geometry`` restates mesh/mesh_geometry.py:35-395 with the reference's
+  per-face / per-cell loops (small meshes only).
+"""
+
+from __future__ import annotations
+
+from collections import deque
+
+import numpy as np
+
+
+def rcm(n, owner, neighbour) -> np.ndarray:
+    if n == 0:
+        return np.zeros(0, np.int64)
+    if len(owner) == 0:
+        return np.arange(n, dtype=np.int64)
+    pairs = set()
+    for a, b in zip(np.asarray(owner).tolist(), np.asarray(neighbour).tolist()):
+        pairs.add((a, b))
+        pairs.add((b, a))
+    rows = sorted(pairs)
+    deg = np.zeros(n, np.int64)
+    nbr = [[] for _ in range(n)]
+    for r, c in rows:
+        deg[r] += 1
+        if r != c:
+            nbr[r].append(c)
+    for i in range(n):
+        nbr[i].sort(key=lambda v: deg[v])
+    visited = np.zeros(n, bool)
+    order = []
+    while len(order) < n:
+        cand = np.nonzero(~visited)[0]
+        start = int(cand[np.argmin(deg[cand])])
+        q = deque([start])
+        visited[start] = True
+        while q:
+            v = q.popleft()
+            order.append(v)
+            for w in nbr[v]:
+                if not visited[w]:
+                    visited[w] = True
+                    q.append(w)
+    order.reverse()
+    return np.array(order, dtype=np.int64)
+
+
+def geometry(points, faces, owner, neighbour, n_cells):
+    """Returns dict(face_centres, face_areas, cell_volumes, cell_centres, delta)."""
+    pts = np.asarray(points, dtype=np.float64)
+    nf = len(faces)
+    nfi = len(neighbour)
+    fc = np.zeros((nf, 3))
+    area = np.zeros((nf, 3))
+    for i, f in enumerate(faces):
+        v = pts[np.asarray(f)]
+        fc[i] = v.mean(axis=0)
+        acc = np.zeros(3)
+        for j in range(len(f)):
+            acc += np.cross(v[j], v[(j + 1) % len(f)])
+        area[i] = acc * 0.5
+    cell_faces = [[] for _ in range(n_cells)]
+    for f in range(nf):
+        cell_faces[int(owner[f])].append(f)
+    for f in range(nfi):
+        cell_faces[int(neighbour[f])].append(f)
+    vol = np.zeros(n_cells)
+    cen = np.zeros((n_cells, 3))
+    for c in range(n_cells):
+        allp = []
+        for f in cell_faces[c]:
+            allp.extend(np.asarray(faces[f]).tolist())
+        ref = pts[sorted(set(allp))].mean(axis=0)
+        tv = 0.0
+        wc = np.zeros(3)
+        for f in cell_faces[c]:
+            v = pts[np.asarray(faces[f])]
+            if f < nfi and int(neighbour[f]) == c:
+                v = v[::-1]
+            for j in range(len(v)):
+                a, b = v[j], v[(j + 1) % len(v)]
+                t = np.dot(fc[f] - ref, np.cross(a - ref, b - ref)) / 6.0
+                if abs(t) < 1e-30:
+                    continue
+                t = abs(t)
+                tv += t
+                wc += t * (ref + fc[f] + a + b) / 4.0
+        if abs(tv) > 1e-30:
+            vol[c] = abs(tv)
+            cen[c] = wc / tv
+        else:
+            cen[c] = fc[cell_faces[c]].mean(axis=0)
+    delta = np.zeros(nf)
+    for f in range(nfi):
+        d = cen[int(neighbour[f])] - cen[int(owner[f])]
+        m = np.linalg.norm(area[f])
+        nh = area[f] / (m if m > 1e-30 else 1.0)
+        dn = abs(np.dot(d, nh))
+        delta[f] = 1.0 / (dn if dn > 1e-30 else 1.0)
+    return {"face_centres": fc, "face_areas": area, "cell_volumes": vol, "cell_centres": cen,
+            "delta": delta}
+
+
+def dirichlet_delta(face_centres, face_areas, cell_centres, owner, face_ids):
+    """laplacian_foam.py:342-358: 1 / clamp(|(x_f - C_P).n_hat|, 1e-30)."""
+    out = np.zeros(len(face_ids))
+    for k, f in enumerate(face_ids):
+        d = face_centres[f] - cell_centres[int(owner[f])]
+        s = face_areas[f]
+        m = np.linalg.norm(s)
+        nh = s / m if m > 1e-30 else np.ones(3) / np.sqrt(3.0)
+        out[k] = 1.0 / max(abs(np.dot(d, nh)), 1e-30)
+    return out

@@ -1,0 +1,136 @@
+"""ctypes binding of libpfm.so (the C ABI declared in include/pfm.h).
+
+The library is loaded from the package directory.  There is no fallback: if
+it is missing, every compute entry point raises ``RuntimeError`` telling the
+caller to build it (``python -c 'import __graft_entry__ as g; g.build()'``).
+"""
+
+from __future__ import annotations
+
+import ctypes as C
+import threading
+from pathlib import Path
+
+import numpy as np
+
+__all__ = ["lib", "check", "PfmError", "SolveParams", "SolveResult", "LduInfo", "library_path",
+           "PFM_OK", "PFM_ERR_ARG", "PFM_ERR_HIP", "PFM_SOLVE_CONVERGED", "PFM_SOLVE_MAXITER",
+           "PFM_SOLVE_BREAKDOWN", "PFM_PRECOND_NONE", "PFM_PRECOND_DIAG", "EXPORTED_SYMBOLS"]
+
+PFM_OK, PFM_ERR_ARG, PFM_ERR_HIP, PFM_ERR_STATE, PFM_ERR_NOMEM = 0, 1, 2, 3, 4
+PFM_SOLVE_RUNNING, PFM_SOLVE_CONVERGED, PFM_SOLVE_MAXITER, PFM_SOLVE_BREAKDOWN = 0, 1, 2, 3
+PFM_PRECOND_NONE, PFM_PRECOND_DIAG = 0, 1
+
+library_path = Path(__file__).resolve().parent / "libpfm.so"
+
+
+class PfmError(RuntimeError):
+    pass
+
+
+class SolveParams(C.Structure):
+    _fields_ = [("tolerance", C.c_double), ("rel_tol", C.c_double), ("max_iter", C.c_int32),
+                ("min_iter", C.c_int32), ("precond", C.c_int32), ("reserved", C.c_int32)]
+
+
+class SolveResult(C.Structure):
+    _fields_ = [("iterations", C.c_int32), ("converged", C.c_int32), ("status", C.c_int32),
+                ("reserved", C.c_int32), ("initial_residual", C.c_double),
+                ("final_residual", C.c_double)]
+
+
+class LduInfo(C.Structure):
+    _fields_ = [("n_cells", C.c_int32), ("n_faces", C.c_int32), ("n_slices", C.c_int32),
+                ("n_dic_levels", C.c_int32), ("n_dilu_levels", C.c_int32),
+                ("max_lower", C.c_int32), ("max_upper", C.c_int32), ("device", C.c_int32),
+                ("n_slots", C.c_int64)]
+
+
+P = C.c_void_p
+I32 = C.c_int32
+I64 = C.c_int64
+INT = C.c_int
+
+# name -> (restype, argtypes)
+_SIGS = {
+    "pfm_last_error": (C.c_char_p, []),
+    "pfm_abi_version": (INT, []),
+    "pfm_rcm": (INT, [I32, I32, P, P, P]),
+    "pfm_dic_levels": (INT, [I32, I32, P, P, P, P]),
+    "pfm_dilu_levels": (INT, [I32, I32, P, P, P, P]),
+    "pfm_ldu_create": (INT, [I32, I32, P, P, C.POINTER(P)]),
+    "pfm_ldu_destroy": (INT, [P]),
+    "pfm_ldu_get_info": (INT, [P, C.POINTER(LduInfo)]),
+    "pfm_ldu_pack": (INT, [P, INT, P, P, P, P]),
+    "pfm_ldu_pack_f32": (INT, [P, INT, P, P, P, P]),
+    "pfm_spmv": (INT, [P, P, P, P, P, P]),
+    "pfm_spmv_f32": (INT, [P, P, P, P, P, P]),
+    "pfm_spmv_ldu": (INT, [P, INT, P, P, P, P, P, P]),
+    "pfm_residual": (INT, [P, P, P, P, P, P, P]),
+    "pfm_dic_factor": (INT, [P, P, P, P, P]),
+    "pfm_dilu_factor": (INT, [P, INT, P, P, P, P, P]),
+    "pfm_pcg_workspace_bytes": (I64, [P, I32]),
+    "pfm_pbicgstab_workspace_bytes": (I64, [P, I32]),
+    "pfm_pcg_solve": (INT, [P, C.POINTER(SolveParams), P, P, P, P, P, P, P, C.POINTER(SolveResult), P]),
+    "pfm_pbicgstab_solve": (INT, [P, C.POINTER(SolveParams), P, P, P, P, P, P, P,
+                                  C.POINTER(SolveResult), P]),
+    "pfm_laplacian_assemble": (INT, [P, P, P, P, P, I32, P, P, P, P, P, P, P, P]),
+    "pfm_adjoint_ldu_grads": (INT, [P, P, P, P, P, P, P]),
+    "pfm_laplacian_backward": (INT, [P, P, P, P, P, P, P, I32, P, P, P, P, P, P]),
+    "pfm_adjoint_gamma_grad": (INT, [P, P, P, P, P, P, I32, P, P, P, P, P, P]),
+}
+
+EXPORTED_SYMBOLS = tuple(_SIGS)
+
+_lock = threading.Lock()
+_lib = None
+
+
+def lib():
+    """The loaded library (raises if it has not been built)."""
+    global _lib
+    if _lib is not None:
+        return _lib
+    with _lock:
+        if _lib is None:
+            if not library_path.exists():
+                raise PfmError(
+                    f"{library_path} is missing: build the HIP extension first "
+                    "(python -c 'import __graft_entry__ as g; g.build()')")
+            handle = C.CDLL(str(library_path))
+            for name, (res, args) in _SIGS.items():
+                fn = getattr(handle, name)
+                fn.restype = res
+                fn.argtypes = args
+            _lib = handle
+    return _lib
+
+
+def check(status: int, what: str = "") -> None:
+    """Map a libpfm status to the reference's exception types."""
+    if status == PFM_OK:
+        return
+    msg = lib().pfm_last_error().decode(errors="replace")
+    text = f"{what}: {msg}" if what else msg
+    if status == PFM_ERR_ARG:
+        raise ValueError(text)
+    raise PfmError(text)
+
+
+def host_i32(a) -> np.ndarray:
+    """Contiguous int32 host copy of an index array (numpy or torch)."""
+    if hasattr(a, "detach"):
+        a = a.detach().cpu().numpy()
+    a = np.asarray(a)
+    if a.size and (a.min() < 0 or a.max() > np.iinfo(np.int32).max):
+        raise ValueError("index out of int32 range")
+    return np.ascontiguousarray(a, dtype=np.int32)
+
+
+def ptr(a) -> int | None:
+    """Raw address of a numpy array or torch tensor (None for None)."""
+    if a is None:
+        return None
+    if hasattr(a, "data_ptr"):
+        return a.data_ptr()
+    return a.ctypes.data

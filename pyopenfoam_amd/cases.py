@@ -1,0 +1,98 @@
+"""Seeded Poisson test systems on unit-cube hex meshes.
+
+``poisson_case(n, ...)`` builds the benchmark systems: an n^3 hex block,
+optionally jittered (every vertex moved by U[-j*h, j*h] per coordinate,
+boundary vertices only tangentially), cells randomly permuted (seeded), then
+RCM-renumbered (``pfm_rcm``, the reference's tie-breaks), faces re-oriented
+owner < neighbour and sorted (owner, neighbour).  Face diffusivity is 1 or
+lognormal(0, sigma) on every face (internal and boundary, drawn in final face
+order), p = 0 (Dirichlet) on the x = 0 patch, zeroGradient elsewhere, and the
+right-hand side is U[-1, 1] per cell made zero-mean.
+
+Random draws come from one ``np.random.default_rng(seed)`` in a fixed order:
+jitter (n_points x 3), permutation (n_cells), diffusivity (n_faces),
+right-hand side (n_cells).
+"""
+
+from __future__ import annotations
+
+from dataclasses import dataclass
+
+import numpy as np
+
+from . import _lib
+from .mesh.geometry import MeshGeometry, compute_geometry, dirichlet_delta
+from .mesh.hexmesh import HexMesh, jitter_points, relabel_cells, structured_hex
+
+__all__ = ["PoissonCase", "poisson_case", "rcm_permutation"]
+
+
+@dataclass
+class PoissonCase:
+    n: int
+    mesh: HexMesh
+    geo: MeshGeometry
+    gamma: np.ndarray          # (n_internal_faces,)
+    dir_faces: np.ndarray      # boundary face ids of the Dirichlet patch, sorted by owner
+    dir_cells: np.ndarray      # owner of each Dirichlet face
+    dir_gamma: np.ndarray
+    dir_mag: np.ndarray
+    dir_delta: np.ndarray
+    b: np.ndarray
+    seed: int
+
+    @property
+    def n_cells(self) -> int:
+        return self.mesh.n_cells
+
+    @property
+    def owner(self) -> np.ndarray:
+        return self.mesh.owner[: self.mesh.n_internal_faces]
+
+    @property
+    def neighbour(self) -> np.ndarray:
+        return self.mesh.neighbour
+
+
+def rcm_permutation(n_cells: int, owner, neighbour) -> np.ndarray:
+    """RCM order perm[new] = old via the native pfm_rcm (host, no GPU)."""
+    o = _lib.host_i32(owner)
+    q = _lib.host_i32(neighbour)
+    perm = np.empty(n_cells, np.int32)
+    _lib.check(_lib.lib().pfm_rcm(n_cells, o.shape[0], _lib.ptr(o), _lib.ptr(q), _lib.ptr(perm)),
+               "pfm_rcm")
+    return perm.astype(np.int64)
+
+
+def poisson_case(n: int, *, jitter: float = 0.0, sigma: float = 0.0, seed: int = 0,
+                 permute: bool = True, renumber: bool = True,
+                 dirichlet_patch: str = "x0") -> PoissonCase:
+    rng = np.random.default_rng(seed)
+    mesh = structured_hex(n)
+    if jitter > 0.0:
+        mesh = jitter_points(mesh, n, jitter, rng)
+    if permute:
+        mesh = relabel_cells(mesh, rng.permutation(mesh.n_cells))
+    if renumber:
+        nfi = mesh.n_internal_faces
+        perm = rcm_permutation(mesh.n_cells, mesh.owner[:nfi], mesh.neighbour)
+        inv = np.empty_like(perm)
+        inv[perm] = np.arange(mesh.n_cells)
+        mesh = relabel_cells(mesh, inv)
+    geo = compute_geometry(mesh)
+    if sigma > 0.0:
+        gamma_all = rng.lognormal(0.0, sigma, size=mesh.n_faces)
+    else:
+        gamma_all = np.ones(mesh.n_faces)
+    b = rng.uniform(-1.0, 1.0, size=mesh.n_cells)
+    b = b - b.mean()
+    nfi = mesh.n_internal_faces
+    patch = mesh.patch(dirichlet_patch)
+    faces = np.arange(patch["start"], patch["start"] + patch["size"])
+    cells = mesh.owner[faces]
+    order = np.argsort(cells, kind="stable")
+    faces, cells = faces[order], cells[order]
+    return PoissonCase(
+        n=n, mesh=mesh, geo=geo, gamma=gamma_all[:nfi].copy(), dir_faces=faces, dir_cells=cells,
+        dir_gamma=gamma_all[faces].copy(), dir_mag=geo.mag_sf[faces].copy(),
+        dir_delta=dirichlet_delta(mesh, geo, faces), b=b, seed=seed)

@@ -1,0 +1,194 @@
+"""LDU matrix on the GPU: the reference's ``LduMatrix`` contract, HIP compute.
+
+Storage contract of ``core/ldu_matrix.py:27-71``: ``diag`` (n_cells),
+``lower[f] = A[owner[f], neighbour[f]]``, ``upper[f] = A[neighbour[f], owner[f]]``.
+Setters cast to the matrix device/dtype and alias the caller's tensor when
+nothing changes (``:143-166``); ``Ax``/``Ax_T`` check the input as ``:192-198``
+(TypeError for non-float, ValueError for a wrong length) and run the
+``pfm_spmv_ldu`` kernel.  There is no CPU compute path: matrices live on a HIP
+device and every product is a libpfm kernel.
+
+The packed (row-slice) copy of the coefficients used by the solvers is cached
+per transpose flag and rebuilt when a coefficient tensor is replaced or
+modified in place (tracked through the tensor version counters), the analogue
+of the reference's CSR cache and ``invalidate_cache`` (``:309-343``).
+"""
+
+from __future__ import annotations
+
+import torch
+
+from .. import _lib
+from .addressing import LduAddressing, addressing_for
+from .device import (INDEX_DTYPE, assert_floating, current_stream, get_default_dtype, get_device,
+                     require_gpu)
+
+__all__ = ["LduMatrix"]
+
+
+class LduMatrix:
+    def __init__(self, n_cells: int, owner: torch.Tensor, neighbour: torch.Tensor, *,
+                 device: torch.device | str | None = None, dtype: torch.dtype | None = None) -> None:
+        self._device = torch.device(device) if device is not None else get_device()
+        if self._device.type == "cuda" and self._device.index is None and torch.cuda.is_available():
+            self._device = torch.device("cuda", torch.cuda.current_device())
+        self._dtype = dtype or get_default_dtype()
+        self._n_cells = int(n_cells)
+        self._n_internal_faces = int(neighbour.shape[0])
+        self._owner_src = owner
+        self._neighbour_src = neighbour
+        self._owner = owner.to(device=self._device, dtype=INDEX_DTYPE)
+        self._neighbour = neighbour.to(device=self._device, dtype=INDEX_DTYPE)
+        self._diag = torch.zeros(n_cells, device=self._device, dtype=self._dtype)
+        self._lower = torch.zeros(self._n_internal_faces, device=self._device, dtype=self._dtype)
+        self._upper = torch.zeros(self._n_internal_faces, device=self._device, dtype=self._dtype)
+        self._addr: LduAddressing | None = None
+        self._packed: dict = {}
+
+    # ------------------------------------------------------------ properties
+    @property
+    def n_cells(self) -> int:
+        return self._n_cells
+
+    @property
+    def n_internal_faces(self) -> int:
+        return self._n_internal_faces
+
+    @property
+    def device(self) -> torch.device:
+        return self._device
+
+    @property
+    def dtype(self) -> torch.dtype:
+        return self._dtype
+
+    @property
+    def owner(self) -> torch.Tensor:
+        return self._owner
+
+    @property
+    def neighbour(self) -> torch.Tensor:
+        return self._neighbour
+
+    @property
+    def diag(self) -> torch.Tensor:
+        return self._diag
+
+    @diag.setter
+    def diag(self, value: torch.Tensor) -> None:
+        self._diag = value.to(device=self._device, dtype=self._dtype)
+        self.invalidate_cache()
+
+    @property
+    def lower(self) -> torch.Tensor:
+        return self._lower
+
+    @lower.setter
+    def lower(self, value: torch.Tensor) -> None:
+        self._lower = value.to(device=self._device, dtype=self._dtype)
+        self.invalidate_cache()
+
+    @property
+    def upper(self) -> torch.Tensor:
+        return self._upper
+
+    @upper.setter
+    def upper(self, value: torch.Tensor) -> None:
+        self._upper = value.to(device=self._device, dtype=self._dtype)
+        self.invalidate_cache()
+
+    # ------------------------------------------------------------ native state
+    @property
+    def addressing(self) -> LduAddressing:
+        """Native addressing handle (built on first use, shared per topology)."""
+        if self._addr is None:
+            require_gpu(self._owner)
+            self._addr = addressing_for(self._n_cells, self._owner_src, self._neighbour_src,
+                                        self._device)
+        return self._addr
+
+    def invalidate_cache(self) -> None:
+        self._packed.clear()
+
+    def _check_compute(self) -> None:
+        require_gpu(self._diag)
+        if self._dtype != torch.float64:
+            raise TypeError("libpfm LDU kernels compute in float64; use mixed-precision solvers "
+                            "for float32 inner work")
+
+    def packed(self, transpose: bool = False) -> torch.Tensor:
+        """Row-slice packed off-diagonal coefficients (cached)."""
+        self._check_compute()
+        lo, up = self._lower.detach(), self._upper.detach()
+        key = (bool(transpose), lo.data_ptr(), up.data_ptr(), lo._version, up._version)
+        hit = self._packed.get(bool(transpose))
+        if hit is not None and hit[0] == key:
+            return hit[1]
+        addr = self.addressing
+        buf = torch.empty(max(addr.n_slots, 1), device=self._device, dtype=torch.float64)
+        _lib.check(_lib.lib().pfm_ldu_pack(addr.handle, int(transpose), _lib.ptr(lo.contiguous()),
+                                           _lib.ptr(up.contiguous()), _lib.ptr(buf),
+                                           current_stream(self._device)), "pfm_ldu_pack")
+        self._packed[bool(transpose)] = (key, buf)
+        return buf
+
+    # ------------------------------------------------------------ products
+    def _check_x(self, x: torch.Tensor) -> torch.Tensor:
+        assert_floating(x, "x")
+        if x.shape[0] != self._n_cells:
+            raise ValueError(f"x has {x.shape[0]} elements, expected {self._n_cells}")
+        return x.to(device=self._device, dtype=self._dtype).contiguous()
+
+    def _product(self, x: torch.Tensor, transpose: int) -> torch.Tensor:
+        x = self._check_x(x)
+        self._check_compute()
+        y = torch.empty(self._n_cells, device=self._device, dtype=self._dtype)
+        if self._n_cells == 0:
+            return y
+        d = self._diag.detach().contiguous()
+        lo = self._lower.detach().contiguous()
+        up = self._upper.detach().contiguous()
+        _lib.check(_lib.lib().pfm_spmv_ldu(self.addressing.handle, transpose, _lib.ptr(d),
+                                           _lib.ptr(lo), _lib.ptr(up), _lib.ptr(x.detach()),
+                                           _lib.ptr(y), current_stream(self._device)),
+                   "pfm_spmv_ldu")
+        return y
+
+    def Ax(self, x: torch.Tensor) -> torch.Tensor:
+        """y = A x (reference ``ldu_matrix.py:172-217``)."""
+        return self._product(x, 0)
+
+    def Ax_T(self, x: torch.Tensor) -> torch.Tensor:
+        """y = A^T x (reference ``ldu_matrix.py:219-258``)."""
+        return self._product(x, 1)
+
+    def Ax_packed(self, x: torch.Tensor, transpose: bool = False) -> torch.Tensor:
+        """y = A x (or the transpose matrix) through the cached packed copy."""
+        x = self._check_x(x)
+        pk = self.packed(transpose)
+        y = torch.empty(self._n_cells, device=self._device, dtype=self._dtype)
+        if self._n_cells:
+            _lib.check(_lib.lib().pfm_spmv(self.addressing.handle,
+                                           _lib.ptr(self._diag.detach().contiguous()),
+                                           _lib.ptr(pk), _lib.ptr(x.detach()), _lib.ptr(y),
+                                           current_stream(self._device)), "pfm_spmv")
+        return y
+
+    def add_to_diag(self, values: torch.Tensor) -> None:
+        if values.dim() == 0:
+            self._diag = self._diag + values
+        else:
+            self._diag = self._diag + values.to(device=self._device, dtype=self._dtype)
+        self.invalidate_cache()
+
+    def transposed(self) -> "LduMatrix":
+        """A^T sharing this topology (lower and upper swapped)."""
+        m = LduMatrix(self._n_cells, self._owner_src, self._neighbour_src, device=self._device,
+                      dtype=self._dtype)
+        m._addr = self._addr
+        m._diag, m._lower, m._upper = self._diag, self._upper, self._lower
+        return m
+
+    def __repr__(self) -> str:
+        return (f"LduMatrix(n_cells={self._n_cells}, n_internal_faces={self._n_internal_faces}, "
+                f"device={self._device}, dtype={self._dtype})")

@@ -1,0 +1,125 @@
+// Device-side helpers shared by the libpfm kernels.
+#pragma once
+
+#include "pfm_internal.h"
+
+namespace pfm {
+
+// Workgroups are dealt to the 8 XCDs round-robin (block b runs on XCD b % 8).
+// Remapping gives each XCD one contiguous range of logical blocks, so rows
+// that share x[] cache lines (RCM keeps neighbours within a narrow band) are
+// served from the same XCD's L2.
+__device__ __forceinline__ int xcd_remap(int b, int G) {
+  const int per = G / kXcds;
+  const int full = per * kXcds;
+  if (b >= full) return b;
+  return (b % kXcds) * per + (b / kXcds);
+}
+
+__device__ __forceinline__ int uniform(int v) {
+  return __builtin_amdgcn_readfirstlane(v);
+}
+
+__device__ __forceinline__ double wave_sum(double v) {
+#pragma unroll
+  for (int off = 32; off > 0; off >>= 1) v += __shfl_xor(v, off, 64);
+  return v;
+}
+
+// Deterministic block sum (fixed shuffle pattern, waves added in order).
+// Every thread gets the result.
+__device__ __forceinline__ double block_sum(double v, double *lds /*[kWavesPerBlock]*/) {
+  v = wave_sum(v);
+  const int w = threadIdx.x >> 6;
+  if ((threadIdx.x & 63) == 0) lds[w] = v;
+  __syncthreads();
+  double s = lds[0];
+#pragma unroll
+  for (int i = 1; i < kWavesPerBlock; ++i) s += lds[i];
+  __syncthreads();
+  return s;
+}
+
+// Sum of partials[0..np) in a fixed order, result on every thread of the
+// (single) finalize workgroup.
+__device__ __forceinline__ double sum_partials(const double *partials, int np, double *lds) {
+  double s = 0.0;
+  for (int i = threadIdx.x; i < np; i += kBlock) s += partials[i];
+  return block_sum(s, lds);
+}
+
+// Producer kernels use a fixed grid of at most kMaxRedBlocks workgroups so
+// that their per-block partial sums (reduced by a one-workgroup finalize
+// kernel in a fixed order) stay few; each workgroup owns one contiguous range
+// of slices, its four waves interleaved over it.
+constexpr int kMaxRedBlocks = 2048;
+
+struct SliceRange {
+  int begin, end;
+};
+
+__device__ __forceinline__ SliceRange block_slices(int nslices) {
+  const int G = gridDim.x;
+  const int lb = xcd_remap(blockIdx.x, G);
+  const int per = (nslices + G - 1) / G;
+  const int b = lb * per;
+  SliceRange r;
+  r.begin = b < nslices ? b : nslices;
+  r.end = b + per < nslices ? b + per : nslices;
+  return r;
+}
+
+// One row of y = A x on the packed row-slice layout.  Returns
+// (diag*x + sum_lower) + sum_upper, each sum in face order from 0, which is
+// the association of the reference's Ax (diag*x, then scatter of the owner
+// side, then scatter of the neighbour side).
+template <typename T>
+__device__ __forceinline__ T row_spmv(const SliceInfo si, int lane, T dx,
+                                      const int32_t *__restrict__ col,
+                                      const T *__restrict__ val,
+                                      const T *__restrict__ x) {
+  constexpr int U = 4;
+  T accL = T(0), accU = T(0);
+  int base = si.ptr + lane;
+  const int wl = si.wl, wu = si.wu;
+  for (int j0 = 0; j0 < wl; j0 += U) {
+    int c[U];
+    T v[U];
+#pragma unroll
+    for (int j = 0; j < U; ++j) {
+      c[j] = (j0 + j < wl) ? col[base + j * kSlice] : -1;
+      v[j] = (j0 + j < wl) ? val[base + j * kSlice] : T(0);
+    }
+#pragma unroll
+    for (int j = 0; j < U; ++j)
+      if (c[j] >= 0) accL += v[j] * x[c[j]];
+    base += U * kSlice;
+  }
+  base = si.ptr + wl * kSlice + lane;
+  for (int j0 = 0; j0 < wu; j0 += U) {
+    int c[U];
+    T v[U];
+#pragma unroll
+    for (int j = 0; j < U; ++j) {
+      c[j] = (j0 + j < wu) ? col[base + j * kSlice] : -1;
+      v[j] = (j0 + j < wu) ? val[base + j * kSlice] : T(0);
+    }
+#pragma unroll
+    for (int j = 0; j < U; ++j)
+      if (c[j] >= 0) accU += v[j] * x[c[j]];
+    base += U * kSlice;
+  }
+  return (dx + accL) + accU;
+}
+
+__device__ __forceinline__ SliceInfo load_slice(const SliceInfo *sl, int s) {
+  const int2 raw = reinterpret_cast<const int2 *>(sl)[s];
+  SliceInfo si;
+  si.ptr = uniform(raw.x);
+  const int w = uniform(raw.y);
+  si.wl = (int16_t)(w & 0xffff);
+  si.wu = (int16_t)((unsigned)w >> 16);
+  return si;
+}
+
+}  // namespace pfm

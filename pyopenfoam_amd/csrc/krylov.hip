@@ -1,0 +1,686 @@
+// Device-resident Krylov solvers: PCG (reference solvers/pcg.py:97-187) and
+// PBiCGStab (solvers/pbicgstab.py:103-239) with the stopping rule of
+// solvers/residual.py:112-145.
+//
+// Each iteration is a fixed sequence of kernels.  Bandwidth kernels (the
+// fused SpMV + dot products, the fused vector updates) run on a fixed grid of
+// at most kMaxRedBlocks workgroups and write one partial sum per workgroup;
+// a one-workgroup finalize kernel adds the partials in a fixed order and takes
+// the scalar decisions the reference takes on the host (alpha, beta, omega,
+// breakdown tests, the residual monitor).  Scalars, the residual history and
+// the stop flag live in device memory, so nothing synchronises with the host
+// inside the loop; results are bit-reproducible run to run.  The host
+// enqueues iterations in chunks and polls the stop flag of the previous chunk
+// through pinned memory while the next chunk is already queued; kernels that
+// start after the stop flag is set return immediately.
+
+#include <cmath>
+#include <cstring>
+
+#include "device_common.h"
+
+using namespace pfm;
+
+namespace {
+
+struct alignas(256) KState {
+  double tol, rel_tol;
+  double res0, res;
+  double rz, alpha, beta;    // PCG
+  double rho, omega;         // PBiCGStab
+  int32_t max_iter, min_iter;
+  int32_t iter;              // current outer iteration i
+  int32_t nhist;             // residual history length
+  int32_t done;
+  int32_t status;
+  int32_t branch;            // PBiCGStab: 0 normal, 1 t.t ~ 0, 2 converged on s
+  int32_t hist_cap;
+};
+
+// ResidualMonitor.update (residual.py:112-145), evaluated by one thread.
+__device__ bool monitor_update(KState *st, double *hist, double res, int it) {
+  if (st->nhist < st->hist_cap) hist[st->nhist] = res;
+  st->nhist += 1;
+  if (st->nhist == 1) st->res0 = res;
+  st->res = res;
+  if (it >= st->min_iter) {
+    if (res < st->tol) return true;
+    if (st->res0 > 0.0) {
+      if (res / st->res0 < st->rel_tol) return true;
+    }
+  }
+  return false;
+}
+
+__device__ __forceinline__ void finish(KState *st, int status) {
+  st->done = 1;
+  st->status = status;
+}
+
+__global__ void k_state_init(KState *st, double tol, double rel_tol, int max_iter, int min_iter,
+                             int hist_cap) {
+  if (threadIdx.x != 0 || blockIdx.x != 0) return;
+  KState s;
+  memset(&s, 0, sizeof(s));
+  s.tol = tol;
+  s.rel_tol = rel_tol;
+  s.max_iter = max_iter;
+  s.min_iter = min_iter;
+  s.hist_cap = hist_cap;
+  *st = s;
+}
+
+// Write this block's partials (part[k*G + b]).
+template <int K>
+__device__ __forceinline__ void store_partials(double (&v)[K], double *part, double *lds) {
+#pragma unroll
+  for (int k = 0; k < K; ++k) {
+    const double s = block_sum(v[k], lds);
+    if (threadIdx.x == 0) part[k * gridDim.x + blockIdx.x] = s;
+  }
+}
+
+#define FOR_ROWS(body)                                                          \
+  {                                                                             \
+    const SliceRange sr_ = block_slices(nslices);                               \
+    const int lane = threadIdx.x & 63;                                          \
+    for (int s = sr_.begin + (threadIdx.x >> 6); s < sr_.end; s += kWavesPerBlock) { \
+      const int row = s * kSlice + lane;                                        \
+      body                                                                      \
+    }                                                                           \
+  }
+
+// ---------------------------------------------------------------- PCG ------
+// r = b - A x; z = M^-1 r; p = z; partials r.r and r.z.
+__global__ __launch_bounds__(kBlock) void k_pcg_init(int n, int nslices, const SliceInfo *__restrict__ sl,
+                                                      const int32_t *__restrict__ col,
+                                                      const double *__restrict__ val,
+                                                      const double *__restrict__ diag,
+                                                      const double *__restrict__ rD,
+                                                      const double *__restrict__ b,
+                                                      const double *__restrict__ x,
+                                                      double *__restrict__ r, double *__restrict__ p,
+                                                      double *__restrict__ part) {
+  __shared__ double lds[kWavesPerBlock];
+  double acc[2] = {0.0, 0.0};
+  FOR_ROWS({
+    const SliceInfo si = load_slice(sl, s);
+    const double dx = row < n ? diag[row] * x[row] : 0.0;
+    const double ax = row_spmv<double>(si, lane, dx, col, val, x);
+    if (row < n) {
+      const double ri = b[row] - ax;
+      const double zi = rD ? ri / rD[row] : ri;
+      r[row] = ri;
+      p[row] = zi;
+      acc[0] += ri * ri;
+      acc[1] += ri * zi;
+    }
+  })
+  store_partials<2>(acc, part, lds);
+}
+
+__global__ __launch_bounds__(kBlock) void k_pcg_init_fin(const double *__restrict__ part, int G,
+                                                          KState *st, double *__restrict__ hist) {
+  __shared__ double lds[kWavesPerBlock];
+  const double RR = sum_partials(part, G, lds);
+  const double RZ = sum_partials(part + G, G, lds);
+  if (threadIdx.x == 0) {
+    if (monitor_update(st, hist, sqrt(RR), 0))
+      finish(st, PFM_SOLVE_CONVERGED);
+    else if (st->max_iter < 1)
+      finish(st, PFM_SOLVE_MAXITER);
+    st->rz = RZ;
+  }
+}
+
+// Ap = A p; partial p.Ap.
+__global__ __launch_bounds__(kBlock) void k_pcg_spmv(int n, int nslices, const SliceInfo *__restrict__ sl,
+                                                      const int32_t *__restrict__ col,
+                                                      const double *__restrict__ val,
+                                                      const double *__restrict__ diag,
+                                                      const double *__restrict__ p,
+                                                      double *__restrict__ Ap,
+                                                      double *__restrict__ part, const KState *st) {
+  __shared__ double lds[kWavesPerBlock];
+  if (st->done) return;
+  double acc[1] = {0.0};
+  FOR_ROWS({
+    const SliceInfo si = load_slice(sl, s);
+    const double pi = row < n ? p[row] : 0.0;
+    const double dx = row < n ? diag[row] * pi : 0.0;
+    const double v = row_spmv<double>(si, lane, dx, col, val, p);
+    if (row < n) {
+      Ap[row] = v;
+      acc[0] += pi * v;
+    }
+  })
+  store_partials<1>(acc, part, lds);
+}
+
+// alpha = rz / pAp, breakdown if |pAp| < 1e-30.
+__global__ __launch_bounds__(kBlock) void k_pcg_alpha(const double *__restrict__ part, int G,
+                                                       KState *st) {
+  __shared__ double lds[kWavesPerBlock];
+  if (st->done) return;
+  const double PAP = sum_partials(part, G, lds);
+  if (threadIdx.x == 0) {
+    if (fabs(PAP) < 1e-30)
+      finish(st, PFM_SOLVE_BREAKDOWN);
+    else
+      st->alpha = st->rz / PAP;
+  }
+}
+
+// x += alpha p; r -= alpha Ap; partials r.r and r.z.
+__global__ __launch_bounds__(kBlock) void k_pcg_update(int n, int nslices, const double *__restrict__ rD,
+                                                        const double *__restrict__ p,
+                                                        const double *__restrict__ Ap,
+                                                        double *__restrict__ x,
+                                                        double *__restrict__ r,
+                                                        double *__restrict__ part, const KState *st) {
+  __shared__ double lds[kWavesPerBlock];
+  if (st->done) return;
+  const double alpha = st->alpha;
+  double acc[2] = {0.0, 0.0};
+  FOR_ROWS({
+    if (row < n) {
+      x[row] = x[row] + alpha * p[row];
+      const double ri = r[row] - alpha * Ap[row];
+      r[row] = ri;
+      const double zi = rD ? ri / rD[row] : ri;
+      acc[0] += ri * ri;
+      acc[1] += ri * zi;
+    }
+  })
+  store_partials<2>(acc, part, lds);
+}
+
+// monitor(||r||, i); beta = r.z' / r.z.
+__global__ __launch_bounds__(kBlock) void k_pcg_beta(const double *__restrict__ part, int G,
+                                                      KState *st, double *__restrict__ hist) {
+  __shared__ double lds[kWavesPerBlock];
+  if (st->done) return;
+  const double RR = sum_partials(part, G, lds);
+  const double RZ = sum_partials(part + G, G, lds);
+  if (threadIdx.x == 0) {
+    const int it = st->iter + 1;
+    st->iter = it;
+    if (monitor_update(st, hist, sqrt(RR), it)) {
+      finish(st, PFM_SOLVE_CONVERGED);
+    } else {
+      st->beta = RZ / st->rz;
+      st->rz = RZ;
+      if (it >= st->max_iter) finish(st, PFM_SOLVE_MAXITER);
+    }
+  }
+}
+
+// p = M^-1 r + beta p
+__global__ __launch_bounds__(kBlock) void k_pcg_pupdate(int n, int nslices, const double *__restrict__ rD,
+                                                         const double *__restrict__ r,
+                                                         double *__restrict__ p, const KState *st) {
+  if (st->done) return;
+  const double beta = st->beta;
+  FOR_ROWS({
+    if (row < n) {
+      const double ri = r[row];
+      const double zi = rD ? ri / rD[row] : ri;
+      p[row] = zi + beta * p[row];
+    }
+  })
+}
+
+// ---------------------------------------------------------- PBiCGStab ------
+// r = b - A x; rt = r; p = r; y = M^-1 p; partial r.r (= rt.r).
+__global__ __launch_bounds__(kBlock) void k_bs_init(int n, int nslices, const SliceInfo *__restrict__ sl,
+                                                     const int32_t *__restrict__ col,
+                                                     const double *__restrict__ val,
+                                                     const double *__restrict__ diag,
+                                                     const double *__restrict__ rD,
+                                                     const double *__restrict__ b,
+                                                     const double *__restrict__ x,
+                                                     double *__restrict__ r, double *__restrict__ rt,
+                                                     double *__restrict__ p, double *__restrict__ y,
+                                                     double *__restrict__ part) {
+  __shared__ double lds[kWavesPerBlock];
+  double acc[1] = {0.0};
+  FOR_ROWS({
+    const SliceInfo si = load_slice(sl, s);
+    const double dx = row < n ? diag[row] * x[row] : 0.0;
+    const double ax = row_spmv<double>(si, lane, dx, col, val, x);
+    if (row < n) {
+      const double ri = b[row] - ax;
+      r[row] = ri;
+      rt[row] = ri;
+      p[row] = ri;
+      y[row] = rD ? ri / rD[row] : ri;
+      acc[0] += ri * ri;
+    }
+  })
+  store_partials<1>(acc, part, lds);
+}
+
+__global__ __launch_bounds__(kBlock) void k_bs_init_fin(const double *__restrict__ part, int G,
+                                                         KState *st, double *__restrict__ hist) {
+  __shared__ double lds[kWavesPerBlock];
+  const double RR = sum_partials(part, G, lds);
+  if (threadIdx.x == 0) {
+    if (monitor_update(st, hist, sqrt(RR), 0))
+      finish(st, PFM_SOLVE_CONVERGED);
+    else if (st->max_iter < 1)
+      finish(st, PFM_SOLVE_MAXITER);
+    st->rho = RR;  // rho = rt . r with rt == r
+    if (!st->done && fabs(RR) < 1e-30) finish(st, PFM_SOLVE_BREAKDOWN);
+  }
+}
+
+// v = A y; partial rt.v.
+__global__ __launch_bounds__(kBlock) void k_bs_spmv1(int n, int nslices, const SliceInfo *__restrict__ sl,
+                                                      const int32_t *__restrict__ col,
+                                                      const double *__restrict__ val,
+                                                      const double *__restrict__ diag,
+                                                      const double *__restrict__ y,
+                                                      const double *__restrict__ rt,
+                                                      double *__restrict__ v,
+                                                      double *__restrict__ part, const KState *st) {
+  __shared__ double lds[kWavesPerBlock];
+  if (st->done) return;
+  double acc[1] = {0.0};
+  FOR_ROWS({
+    const SliceInfo si = load_slice(sl, s);
+    const double dx = row < n ? diag[row] * y[row] : 0.0;
+    const double av = row_spmv<double>(si, lane, dx, col, val, y);
+    if (row < n) {
+      v[row] = av;
+      acc[0] += rt[row] * av;
+    }
+  })
+  store_partials<1>(acc, part, lds);
+}
+
+__global__ __launch_bounds__(kBlock) void k_bs_alpha(const double *__restrict__ part, int G,
+                                                      KState *st) {
+  __shared__ double lds[kWavesPerBlock];
+  if (st->done) return;
+  const double RTV = sum_partials(part, G, lds);
+  if (threadIdx.x == 0) {
+    if (fabs(RTV) < 1e-30)
+      finish(st, PFM_SOLVE_BREAKDOWN);
+    else
+      st->alpha = st->rho / RTV;
+  }
+}
+
+// s = r - alpha v; z = M^-1 s; partial s.s.
+__global__ __launch_bounds__(kBlock) void k_bs_s(int n, int nslices, const double *__restrict__ rD,
+                                                  const double *__restrict__ r,
+                                                  const double *__restrict__ v,
+                                                  double *__restrict__ sv, double *__restrict__ z,
+                                                  double *__restrict__ part, const KState *st) {
+  __shared__ double lds[kWavesPerBlock];
+  if (st->done) return;
+  const double alpha = st->alpha;
+  double acc[1] = {0.0};
+  FOR_ROWS({
+    if (row < n) {
+      const double si = r[row] - alpha * v[row];
+      sv[row] = si;
+      z[row] = rD ? si / rD[row] : si;
+      acc[0] += si * si;
+    }
+  })
+  store_partials<1>(acc, part, lds);
+}
+
+// monitor(||s||, i): converged -> branch 2 (x += alpha y, then stop).
+__global__ __launch_bounds__(kBlock) void k_bs_s_fin(const double *__restrict__ part, int G,
+                                                      KState *st, double *__restrict__ hist) {
+  __shared__ double lds[kWavesPerBlock];
+  if (st->done) return;
+  const double SS = sum_partials(part, G, lds);
+  if (threadIdx.x == 0) {
+    const int it = st->iter + 1;
+    st->iter = it;
+    st->branch = monitor_update(st, hist, sqrt(SS), it) ? 2 : 0;
+  }
+}
+
+// t = A z; partials t.t and t.z.
+__global__ __launch_bounds__(kBlock) void k_bs_spmv2(int n, int nslices, const SliceInfo *__restrict__ sl,
+                                                      const int32_t *__restrict__ col,
+                                                      const double *__restrict__ val,
+                                                      const double *__restrict__ diag,
+                                                      const double *__restrict__ z,
+                                                      double *__restrict__ t,
+                                                      double *__restrict__ part, const KState *st) {
+  __shared__ double lds[kWavesPerBlock];
+  if (st->done || st->branch == 2) return;
+  double acc[2] = {0.0, 0.0};
+  FOR_ROWS({
+    const SliceInfo si = load_slice(sl, s);
+    const double zi = row < n ? z[row] : 0.0;
+    const double dx = row < n ? diag[row] * zi : 0.0;
+    const double at = row_spmv<double>(si, lane, dx, col, val, z);
+    if (row < n) {
+      t[row] = at;
+      acc[0] += at * at;
+      acc[1] += at * zi;
+    }
+  })
+  store_partials<2>(acc, part, lds);
+}
+
+// omega = t.z / t.t, or the t ~ 0 branch.
+__global__ __launch_bounds__(kBlock) void k_bs_omega(const double *__restrict__ part, int G,
+                                                      KState *st) {
+  __shared__ double lds[kWavesPerBlock];
+  if (st->done || st->branch == 2) return;
+  const double TT = sum_partials(part, G, lds);
+  const double TZ = sum_partials(part + G, G, lds);
+  if (threadIdx.x == 0) {
+    if (fabs(TT) < 1e-30) {
+      st->branch = 1;
+    } else {
+      st->branch = 0;
+      st->omega = TZ / TT;
+    }
+  }
+}
+
+// Solution/residual update of iteration i (all three branches); partials
+// r.r and rt.r.
+__global__ __launch_bounds__(kBlock) void k_bs_update(int n, int nslices, const double *__restrict__ y,
+                                                       const double *__restrict__ z,
+                                                       const double *__restrict__ sv,
+                                                       const double *__restrict__ t,
+                                                       const double *__restrict__ rt,
+                                                       double *__restrict__ x,
+                                                       double *__restrict__ r,
+                                                       double *__restrict__ part, const KState *st) {
+  __shared__ double lds[kWavesPerBlock];
+  if (st->done) return;
+  const int branch = st->branch;
+  const double alpha = st->alpha, omega = st->omega;
+  double acc[2] = {0.0, 0.0};
+  FOR_ROWS({
+    if (row < n) {
+      if (branch == 2) {
+        x[row] = x[row] + alpha * y[row];
+      } else {
+        double ri;
+        if (branch == 1) {
+          x[row] = x[row] + alpha * y[row];
+          ri = sv[row];
+        } else {
+          x[row] = (x[row] + alpha * y[row]) + omega * z[row];
+          ri = sv[row] - omega * t[row];
+        }
+        r[row] = ri;
+        acc[0] += ri * ri;
+        acc[1] += rt[row] * ri;
+      }
+    }
+  })
+  if (branch != 2) store_partials<2>(acc, part, lds);
+}
+
+// monitor(||r||, i); rho_new = rt.r; omega breakdown; beta.
+__global__ __launch_bounds__(kBlock) void k_bs_beta(const double *__restrict__ part, int G,
+                                                     KState *st, double *__restrict__ hist) {
+  __shared__ double lds[kWavesPerBlock];
+  if (st->done) return;
+  if (st->branch == 2) {
+    if (threadIdx.x == 0) finish(st, PFM_SOLVE_CONVERGED);
+    return;
+  }
+  const double RR = sum_partials(part, G, lds);
+  const double RTR = sum_partials(part + G, G, lds);
+  if (threadIdx.x == 0) {
+    const int it = st->iter;
+    const int branch = st->branch;
+    const double alpha = st->alpha, omega = st->omega;
+    if (monitor_update(st, hist, sqrt(RR), it)) {
+      finish(st, PFM_SOLVE_CONVERGED);
+      return;
+    }
+    if (branch == 1) {
+      st->beta = (RTR / st->rho) * alpha;
+      st->rho = RTR;
+    } else {
+      if (fabs(omega) < 1e-30) {
+        finish(st, PFM_SOLVE_BREAKDOWN);
+        return;
+      }
+      st->beta = (RTR / st->rho) * (alpha / omega);
+      st->rho = RTR;
+    }
+    if (it >= st->max_iter) {
+      finish(st, PFM_SOLVE_MAXITER);
+      return;
+    }
+    // breakdown test at the top of the next iteration (pbicgstab.py:152-156)
+    if (fabs(st->rho) < 1e-30) finish(st, PFM_SOLVE_BREAKDOWN);
+  }
+}
+
+// p = r + beta (p - omega v)   (branch 1: p = r + beta (p - v)); y = M^-1 p.
+__global__ __launch_bounds__(kBlock) void k_bs_pupdate(int n, int nslices, const double *__restrict__ rD,
+                                                        const double *__restrict__ r,
+                                                        const double *__restrict__ v,
+                                                        double *__restrict__ p,
+                                                        double *__restrict__ y,
+                                                        const KState *st) {
+  if (st->done) return;
+  const double beta = st->beta, omega = st->omega;
+  const int branch = st->branch;
+  FOR_ROWS({
+    if (row < n) {
+      const double vi = branch == 1 ? v[row] : omega * v[row];
+      const double pi = r[row] + beta * (p[row] - vi);
+      p[row] = pi;
+      y[row] = rD ? pi / rD[row] : pi;
+    }
+  })
+}
+
+// ------------------------------------------------------------ workspace ----
+struct Work {
+  KState *st;
+  double *part;
+  double *hist;
+  double *vec[8];
+};
+
+inline size_t align256(size_t b) { return (b + 255) & ~size_t(255); }
+
+inline int red_grid(const pfm_ldu_s *h) {
+  return std::max(1, std::min(blocks_for_slices(h->nslices), kMaxRedBlocks));
+}
+
+size_t work_layout(const pfm_ldu_s *h, int max_iter, int nvec, Work *w, char *base) {
+  const int G = red_grid(h);
+  size_t off = 0;
+  auto take = [&](size_t bytes) {
+    size_t o = off;
+    off += align256(bytes);
+    return base ? (void *)(base + o) : nullptr;
+  };
+  Work tmp;
+  tmp.st = (KState *)take(sizeof(KState));
+  tmp.part = (double *)take(sizeof(double) * 2 * (size_t)G);
+  tmp.hist = (double *)take(sizeof(double) * (2 * (size_t)std::max(max_iter, 0) + 2));
+  for (int k = 0; k < nvec; ++k) tmp.vec[k] = (double *)take(sizeof(double) * (size_t)h->n);
+  if (w) *w = tmp;
+  return off;
+}
+
+int check_solve_args(pfm_ldu h, const pfm_solve_params *prm, const double *diag,
+                     const double *packed, const double *rD, const double *b, double *x, void *work,
+                     pfm_solve_result *res) {
+  PFM_REQUIRE(h && prm && res, "null handle, params or result");
+  PFM_REQUIRE(prm->max_iter >= 0, "max_iter must be >= 0");
+  PFM_REQUIRE(prm->precond == PFM_PRECOND_NONE || prm->precond == PFM_PRECOND_DIAG,
+              "unknown preconditioner code %d", prm->precond);
+  PFM_REQUIRE(prm->precond == PFM_PRECOND_NONE || rD, "preconditioner diagonal missing");
+  PFM_REQUIRE(h->n == 0 || (diag && b && x && work), "null device pointer");
+  PFM_REQUIRE(h->nslots == 0 || packed, "null packed coefficients");
+  return PFM_OK;
+}
+
+// Enqueue `body()` (one iteration) until the device stop flag is seen.
+// One chunk stays queued ahead of the poll so the GPU never idles on it.
+template <typename Body>
+int drive(pfm_ldu h, const KState *st_dev, int max_iter, hipStream_t s, Body body) {
+  int32_t *flag = h->h_flag;
+  hipEvent_t ev[2];
+  PFM_CHECK_HIP(hipEventCreateWithFlags(&ev[0], hipEventDisableTiming));
+  PFM_CHECK_HIP(hipEventCreateWithFlags(&ev[1], hipEventDisableTiming));
+  int enq = 0, chunk = 4, slot = 0, rc = PFM_OK;
+  bool pending = false;
+  while (true) {
+    bool queued = false;
+    if (enq < max_iter) {
+      const int c = std::min(chunk, max_iter - enq);
+      for (int k = 0; k < c; ++k) body();
+      enq += c;
+      chunk = std::min(chunk * 2, 32);
+      if (hipMemcpyAsync(&flag[slot], &st_dev->done, sizeof(int32_t), hipMemcpyDeviceToHost, s) !=
+              hipSuccess ||
+          hipEventRecord(ev[slot], s) != hipSuccess) {
+        rc = PFM_ERR_HIP;
+        break;
+      }
+      queued = true;
+    }
+    if (pending) {
+      if (hipEventSynchronize(ev[slot ^ 1]) != hipSuccess) {
+        rc = PFM_ERR_HIP;
+        break;
+      }
+      if (flag[slot ^ 1]) break;
+    }
+    if (!queued) break;
+    pending = true;
+    slot ^= 1;
+  }
+  if (rc != PFM_OK) set_error("HIP error while driving the solver loop");
+  (void)hipEventDestroy(ev[0]);
+  (void)hipEventDestroy(ev[1]);
+  return rc;
+}
+
+int collect(const KState *st_dev, const double *hist_dev, double *history_h, int hist_cap,
+            pfm_solve_result *res, hipStream_t s) {
+  KState st;
+  PFM_CHECK_HIP(hipMemcpyAsync(&st, st_dev, sizeof(KState), hipMemcpyDeviceToHost, s));
+  PFM_CHECK_HIP(hipStreamSynchronize(s));
+  const int nh = std::min(st.nhist, hist_cap);
+  if (history_h && nh > 0) {
+    PFM_CHECK_HIP(hipMemcpyAsync(history_h, hist_dev, sizeof(double) * nh, hipMemcpyDeviceToHost, s));
+    PFM_CHECK_HIP(hipStreamSynchronize(s));
+  }
+  res->iterations = st.nhist;
+  res->status = st.status == 0 ? PFM_SOLVE_MAXITER : st.status;
+  res->converged = res->status == PFM_SOLVE_CONVERGED;
+  res->initial_residual = st.nhist > 0 ? st.res0 : 0.0;
+  res->final_residual = st.nhist > 0 ? st.res : 0.0;
+  return PFM_OK;
+}
+
+}  // namespace
+
+extern "C" {
+
+int64_t pfm_pcg_workspace_bytes(pfm_ldu h, int32_t max_iter) {
+  if (!h) return -1;
+  return (int64_t)work_layout(h, max_iter, 3, nullptr, nullptr);
+}
+
+int64_t pfm_pbicgstab_workspace_bytes(pfm_ldu h, int32_t max_iter) {
+  if (!h) return -1;
+  return (int64_t)work_layout(h, max_iter, 8, nullptr, nullptr);
+}
+
+int pfm_pcg_solve(pfm_ldu h, const pfm_solve_params *prm, const double *diag, const double *packed,
+                  const double *rD, const double *b, double *x, void *work, double *history_h,
+                  pfm_solve_result *res, void *stream) {
+  int rc = check_solve_args(h, prm, diag, packed, rD, b, x, work, res);
+  if (rc != PFM_OK) return rc;
+  memset(res, 0, sizeof(*res));
+  if (h->n == 0) {
+    res->status = PFM_SOLVE_CONVERGED;
+    res->converged = 1;
+    return PFM_OK;
+  }
+  hipStream_t s = (hipStream_t)stream;
+  Work w;
+  work_layout(h, prm->max_iter, 3, &w, (char *)work);
+  double *r = w.vec[0], *p = w.vec[1], *Ap = w.vec[2];
+  const double *pd = prm->precond == PFM_PRECOND_DIAG ? rD : nullptr;
+  const int hist_cap = prm->max_iter + 1;
+  const int G = red_grid(h);
+  const int n = h->n, ns = h->nslices;
+  k_state_init<<<1, 1, 0, s>>>(w.st, prm->tolerance, prm->rel_tol, prm->max_iter, prm->min_iter,
+                               hist_cap);
+  k_pcg_init<<<G, kBlock, 0, s>>>(n, ns, h->d_slices, h->d_col, packed, diag, pd, b, x, r, p,
+                                  w.part);
+  k_pcg_init_fin<<<1, kBlock, 0, s>>>(w.part, G, w.st, w.hist);
+  PFM_CHECK_HIP(hipGetLastError());
+  rc = drive(h, w.st, prm->max_iter, s, [&]() {
+    k_pcg_spmv<<<G, kBlock, 0, s>>>(n, ns, h->d_slices, h->d_col, packed, diag, p, Ap, w.part,
+                                    w.st);
+    k_pcg_alpha<<<1, kBlock, 0, s>>>(w.part, G, w.st);
+    k_pcg_update<<<G, kBlock, 0, s>>>(n, ns, pd, p, Ap, x, r, w.part, w.st);
+    k_pcg_beta<<<1, kBlock, 0, s>>>(w.part, G, w.st, w.hist);
+    k_pcg_pupdate<<<G, kBlock, 0, s>>>(n, ns, pd, r, p, w.st);
+  });
+  if (rc != PFM_OK) return rc;
+  PFM_CHECK_HIP(hipGetLastError());
+  return collect(w.st, w.hist, history_h, hist_cap, res, s);
+}
+
+int pfm_pbicgstab_solve(pfm_ldu h, const pfm_solve_params *prm, const double *diag,
+                        const double *packed, const double *rD, const double *b, double *x,
+                        void *work, double *history_h, pfm_solve_result *res, void *stream) {
+  int rc = check_solve_args(h, prm, diag, packed, rD, b, x, work, res);
+  if (rc != PFM_OK) return rc;
+  memset(res, 0, sizeof(*res));
+  if (h->n == 0) {
+    res->status = PFM_SOLVE_CONVERGED;
+    res->converged = 1;
+    return PFM_OK;
+  }
+  hipStream_t s = (hipStream_t)stream;
+  Work w;
+  work_layout(h, prm->max_iter, 8, &w, (char *)work);
+  double *r = w.vec[0], *rt = w.vec[1], *p = w.vec[2], *v = w.vec[3], *y = w.vec[4],
+         *sv = w.vec[5], *z = w.vec[6], *t = w.vec[7];
+  const double *pd = prm->precond == PFM_PRECOND_DIAG ? rD : nullptr;
+  const int hist_cap = 2 * prm->max_iter + 2;
+  const int G = red_grid(h);
+  const int n = h->n, ns = h->nslices;
+  k_state_init<<<1, 1, 0, s>>>(w.st, prm->tolerance, prm->rel_tol, prm->max_iter, prm->min_iter,
+                               hist_cap);
+  k_bs_init<<<G, kBlock, 0, s>>>(n, ns, h->d_slices, h->d_col, packed, diag, pd, b, x, r, rt, p,
+                                 y, w.part);
+  k_bs_init_fin<<<1, kBlock, 0, s>>>(w.part, G, w.st, w.hist);
+  PFM_CHECK_HIP(hipGetLastError());
+  rc = drive(h, w.st, prm->max_iter, s, [&]() {
+    k_bs_spmv1<<<G, kBlock, 0, s>>>(n, ns, h->d_slices, h->d_col, packed, diag, y, rt, v, w.part,
+                                    w.st);
+    k_bs_alpha<<<1, kBlock, 0, s>>>(w.part, G, w.st);
+    k_bs_s<<<G, kBlock, 0, s>>>(n, ns, pd, r, v, sv, z, w.part, w.st);
+    k_bs_s_fin<<<1, kBlock, 0, s>>>(w.part, G, w.st, w.hist);
+    k_bs_spmv2<<<G, kBlock, 0, s>>>(n, ns, h->d_slices, h->d_col, packed, diag, z, t, w.part,
+                                    w.st);
+    k_bs_omega<<<1, kBlock, 0, s>>>(w.part, G, w.st);
+    k_bs_update<<<G, kBlock, 0, s>>>(n, ns, y, z, sv, t, rt, x, r, w.part, w.st);
+    k_bs_beta<<<1, kBlock, 0, s>>>(w.part, G, w.st, w.hist);
+    k_bs_pupdate<<<G, kBlock, 0, s>>>(n, ns, pd, r, v, p, y, w.st);
+  });
+  if (rc != PFM_OK) return rc;
+  PFM_CHECK_HIP(hipGetLastError());
+  return collect(w.st, w.hist, history_h, hist_cap, res, s);
+}
+
+}  // extern "C"

@@ -1,0 +1,89 @@
+// Internal declarations shared by the libpfm translation units.
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include <cstdint>
+#include <vector>
+
+#include "../../include/pfm.h"
+
+namespace pfm {
+
+void set_error(const char *fmt, ...);
+
+#define PFM_CHECK_HIP(call)                                                    \
+  do {                                                                         \
+    hipError_t e_ = (call);                                                    \
+    if (e_ != hipSuccess) {                                                    \
+      ::pfm::set_error("%s failed: %s", #call, hipGetErrorString(e_));         \
+      return PFM_ERR_HIP;                                                      \
+    }                                                                          \
+  } while (0)
+
+#define PFM_REQUIRE(cond, ...)                                                 \
+  do {                                                                         \
+    if (!(cond)) {                                                             \
+      ::pfm::set_error(__VA_ARGS__);                                           \
+      return PFM_ERR_ARG;                                                      \
+    }                                                                          \
+  } while (0)
+
+constexpr int kSlice = 64;     // rows per slice = one wavefront
+constexpr int kBlock = 256;    // threads per workgroup
+constexpr int kWavesPerBlock = kBlock / kSlice;
+constexpr int kXcds = 8;       // MI355X: 8 XCDs, workgroups dealt round-robin
+
+// Per-slice layout: slot offset, lower and upper slot counts.
+struct SliceInfo {
+  int32_t ptr;
+  int16_t wl;
+  int16_t wu;
+};
+
+// Host-built layout of one LDU topology (pure C++, no device state).
+struct HostLayout {
+  int32_t n = 0, nf = 0, nslices = 0;
+  int64_t nslots = 0;
+  int32_t max_lower = 0, max_upper = 0;
+  std::vector<SliceInfo> slices;
+  std::vector<int32_t> col;   // column of each slot, -1 = padding
+  std::vector<int32_t> map;   // face of each slot, -1 = padding
+  std::vector<int32_t> dic_level_ptr, dic_cells;
+  std::vector<int32_t> dilu_level_ptr, dilu_faces;
+};
+
+int build_host_layout(int32_t n, int32_t nf, const int32_t *own,
+                      const int32_t *nei, HostLayout &L);
+int validate_topology(int32_t n, int32_t nf, const int32_t *own,
+                      const int32_t *nei);
+
+}  // namespace pfm
+
+// Device-resident addressing of one LDU topology.
+struct pfm_ldu_s {
+  int32_t n = 0, nf = 0, nslices = 0;
+  int64_t nslots = 0;
+  int32_t max_lower = 0, max_upper = 0;
+  int device = 0;
+  std::vector<int32_t> dic_level_ptr;   // host copies of level offsets
+  std::vector<int32_t> dilu_level_ptr;
+  pfm::SliceInfo *d_slices = nullptr;
+  int32_t *d_col = nullptr;
+  int32_t *d_map = nullptr;
+  int32_t *d_own = nullptr;
+  int32_t *d_nei = nullptr;
+  int32_t *d_dic_cells = nullptr;
+  int32_t *d_dilu_faces = nullptr;
+  int32_t *d_dilu_level_ptr = nullptr;
+  void *d_dilu_pq = nullptr;            // level-ordered (owner, neighbour), lazily allocated
+  double *d_dilu_lu = nullptr;          // level-ordered lower*upper
+  int32_t *h_flag = nullptr;            // pinned: solver progress polling
+};
+
+namespace pfm {
+// Grid helpers --------------------------------------------------------------
+inline int blocks_for_slices(int nslices) {
+  return (nslices + kWavesPerBlock - 1) / kWavesPerBlock;
+}
+}  // namespace pfm

@@ -1,0 +1,125 @@
+"""Finite-volume mesh with device-resident geometry (reference ``mesh/fv_mesh.py``).
+
+Holds the polyMesh arrays and the geometry the Laplacian assembly consumes
+(cell volumes, face area vectors and magnitudes, delta coefficients), with
+the reference's property names (``fv_mesh.py:127-190``).  Geometry is
+computed once on the host (``geometry.compute_geometry``) and kept as float64
+tensors on the mesh's device.
+"""
+
+from __future__ import annotations
+
+import numpy as np
+import torch
+
+from ..core.device import CFD_DTYPE, INDEX_DTYPE, get_device
+from .geometry import MeshGeometry, compute_geometry, dirichlet_delta
+from .hexmesh import HexMesh
+
+__all__ = ["FvMesh"]
+
+
+class FvMesh:
+    def __init__(self, mesh: HexMesh, *, device: torch.device | str | None = None,
+                 geometry: MeshGeometry | None = None) -> None:
+        self._mesh = mesh
+        self._device = torch.device(device) if device is not None else get_device()
+        self._geo = geometry
+        self._t: dict = {}
+
+    @classmethod
+    def from_case(cls, case, device=None) -> "FvMesh":
+        return cls(case.mesh, device=device, geometry=case.geo)
+
+    # topology
+    @property
+    def device(self) -> torch.device:
+        return self._device
+
+    @property
+    def dtype(self) -> torch.dtype:
+        return CFD_DTYPE
+
+    @property
+    def n_cells(self) -> int:
+        return self._mesh.n_cells
+
+    @property
+    def n_faces(self) -> int:
+        return self._mesh.n_faces
+
+    @property
+    def n_internal_faces(self) -> int:
+        return self._mesh.n_internal_faces
+
+    @property
+    def boundary(self) -> list:
+        return self._mesh.patches
+
+    @property
+    def host(self) -> HexMesh:
+        return self._mesh
+
+    def _tensor(self, key, fn, dtype):
+        if key not in self._t:
+            self._t[key] = torch.as_tensor(np.ascontiguousarray(fn()), dtype=dtype).to(self._device)
+        return self._t[key]
+
+    @property
+    def geometry(self) -> MeshGeometry:
+        if self._geo is None:
+            self._geo = compute_geometry(self._mesh)
+        return self._geo
+
+    @property
+    def owner(self) -> torch.Tensor:
+        return self._tensor("owner", lambda: self._mesh.owner, INDEX_DTYPE)
+
+    @property
+    def neighbour(self) -> torch.Tensor:
+        return self._tensor("neighbour", lambda: self._mesh.neighbour, INDEX_DTYPE)
+
+    @property
+    def cell_volumes(self) -> torch.Tensor:
+        return self._tensor("vol", lambda: self.geometry.cell_volumes, CFD_DTYPE)
+
+    @property
+    def cell_centres(self) -> torch.Tensor:
+        return self._tensor("cc", lambda: self.geometry.cell_centres, CFD_DTYPE)
+
+    @property
+    def face_centres(self) -> torch.Tensor:
+        return self._tensor("fc", lambda: self.geometry.face_centres, CFD_DTYPE)
+
+    @property
+    def face_areas(self) -> torch.Tensor:
+        return self._tensor("sf", lambda: self.geometry.face_areas, CFD_DTYPE)
+
+    @property
+    def face_areas_magnitude(self) -> torch.Tensor:
+        return self._tensor("magsf", lambda: self.geometry.mag_sf, CFD_DTYPE)
+
+    @property
+    def delta_coefficients(self) -> torch.Tensor:
+        return self._tensor("delta", lambda: self.geometry.delta, CFD_DTYPE)
+
+    def patch_faces(self, name: str) -> np.ndarray:
+        p = self._mesh.patch(name)
+        return np.arange(p["start"], p["start"] + p["size"])
+
+    def dirichlet_data(self, patch: str):
+        """(cells, |S_b|, delta_b) of a patch, sorted by owner cell, on device."""
+        key = ("dir", patch)
+        if key not in self._t:
+            faces = self.patch_faces(patch)
+            cells = self._mesh.owner[faces]
+            order = np.argsort(cells, kind="stable")
+            faces, cells = faces[order], cells[order]
+            geo = self.geometry
+            self._t[key] = (
+                faces,
+                torch.from_numpy(cells.astype(np.int32)).to(self._device),
+                torch.from_numpy(geo.mag_sf[faces].copy()).to(self._device),
+                torch.from_numpy(dirichlet_delta(self._mesh, geo, faces)).to(self._device),
+            )
+        return self._t[key]

@@ -1,0 +1,120 @@
+"""Finite-volume geometry of all-quad polyMeshes (host, vectorised numpy).
+
+Formulas are those of the reference's ``src/pyfoam/mesh/mesh_geometry.py``:
+
+* face centre = arithmetic mean of the face vertices            (:35-63)
+* face area vector S_f = 1/2 sum_j v_j x v_{j+1}                  (:66-108)
+* cell volume = sum over faces and face edges of
+  |tet(ref, fc, v_j, v_{j+1})|, ref = mean of the cell's vertices,
+  faces seen from the neighbour side reversed; cell centre =
+  volume-weighted tet centroids                                   (:139-254)
+* delta_f = 1/|(C_N - C_P) . n_hat| on internal faces, 0 on
+  boundary faces                                                  (:330-395)
+* boundary delta for a Dirichlet patch (laplacianFoam convention)
+  1/max(|(x_f - C_P) . n_hat|, 1e-30)   (``src/pyfoam/applications/laplacian_foam.py:342-358``)
+
+The reference evaluates these with per-face and per-cell Python loops; here
+each formula is one vectorised pass, so a 16.8M-cell mesh takes seconds.
+"""
+
+from __future__ import annotations
+
+from dataclasses import dataclass
+
+import numpy as np
+
+from .hexmesh import HexMesh
+
+__all__ = ["MeshGeometry", "compute_geometry", "dirichlet_delta"]
+
+
+@dataclass
+class MeshGeometry:
+    face_centres: np.ndarray       # (n_faces, 3)
+    face_areas: np.ndarray         # (n_faces, 3) area vectors
+    mag_sf: np.ndarray             # (n_faces,)
+    cell_volumes: np.ndarray       # (n_cells,)
+    cell_centres: np.ndarray       # (n_cells, 3)
+    delta: np.ndarray              # (n_faces,) internal 1/|d.n|, boundary 0
+
+
+def _cross(a, b):
+    return np.stack([a[..., 1] * b[..., 2] - a[..., 2] * b[..., 1],
+                     a[..., 2] * b[..., 0] - a[..., 0] * b[..., 2],
+                     a[..., 0] * b[..., 1] - a[..., 1] * b[..., 0]], axis=-1)
+
+
+def _dot(a, b):
+    return a[..., 0] * b[..., 0] + a[..., 1] * b[..., 1] + a[..., 2] * b[..., 2]
+
+
+def compute_geometry(mesh: HexMesh) -> MeshGeometry:
+    pts = mesh.points
+    fv = pts[mesh.faces]                        # (nf, 4, 3)
+    nv = fv.shape[1]
+    fc = fv.sum(axis=1) / nv
+    area = np.zeros((fv.shape[0], 3))
+    for j in range(nv):
+        area += _cross(fv[:, j], fv[:, (j + 1) % nv])
+    area *= 0.5
+    mag = np.sqrt(_dot(area, area))
+
+    n_cells = mesh.n_cells
+    nfi = mesh.n_internal_faces
+    # cell -> (face, flipped) lists; all-hex meshes have 6 faces per cell
+    fidx = np.concatenate([np.arange(mesh.n_faces), np.arange(nfi)])
+    cell = np.concatenate([mesh.owner, mesh.neighbour])
+    flip = np.concatenate([np.zeros(mesh.n_faces, bool), np.ones(nfi, bool)])
+    order = np.argsort(cell, kind="stable")
+    counts = np.bincount(cell, minlength=n_cells)
+    if counts.min() != counts.max():
+        raise ValueError("compute_geometry expects cells with equal face counts")
+    nfc = int(counts[0])
+    fidx = fidx[order].reshape(n_cells, nfc)
+    flip = flip[order].reshape(n_cells, nfc)
+
+    # reference point: mean of the cell's distinct vertices
+    cverts = mesh.faces[fidx].reshape(n_cells, nfc * nv)
+    cverts.sort(axis=1)
+    first = np.ones_like(cverts, dtype=bool)
+    first[:, 1:] = cverts[:, 1:] != cverts[:, :-1]
+    n_unique = first.sum(axis=1)
+    ref = (pts[cverts] * first[..., None]).sum(axis=1) / n_unique[:, None]
+
+    vol = np.zeros(n_cells)
+    wc = np.zeros((n_cells, 3))
+    loops = mesh.faces[fidx]                    # (n_cells, nfc, nv)
+    loops = np.where(flip[..., None], loops[..., ::-1], loops)
+    fcc = fc[fidx]                              # (n_cells, nfc, 3)
+    refb = ref[:, None, :]
+    for j in range(nv):
+        va = pts[loops[:, :, j]]
+        vb = pts[loops[:, :, (j + 1) % nv]]
+        v = _dot(fcc - refb, _cross(va - refb, vb - refb)) / 6.0
+        v = np.abs(v)
+        v = np.where(v < 1e-30, 0.0, v)
+        cen = (refb + fcc + va + vb) / 4.0
+        vol += v.sum(axis=1)
+        wc += (v[..., None] * cen).sum(axis=1)
+    centres = wc / np.where(vol > 1e-30, vol, 1.0)[:, None]
+
+    delta = np.zeros(mesh.n_faces)
+    if nfi:
+        d = centres[mesh.neighbour] - centres[mesh.owner[:nfi]]
+        m = mag[:nfi]
+        nh = area[:nfi] / np.where(m > 1e-30, m, 1.0)[:, None]
+        dn = np.abs(_dot(d, nh))
+        delta[:nfi] = 1.0 / np.where(dn > 1e-30, dn, 1.0)
+    return MeshGeometry(face_centres=fc, face_areas=area, mag_sf=mag, cell_volumes=vol,
+                        cell_centres=centres, delta=delta)
+
+
+def dirichlet_delta(mesh: HexMesh, geo: MeshGeometry, face_ids: np.ndarray) -> np.ndarray:
+    """Boundary delta 1/max(|(x_f - C_P).n_hat|, 1e-30) on the given boundary faces."""
+    face_ids = np.asarray(face_ids, dtype=np.int64)
+    d = geo.face_centres[face_ids] - geo.cell_centres[mesh.owner[face_ids]]
+    s = geo.face_areas[face_ids]
+    m = geo.mag_sf[face_ids]
+    nh = s / np.where(m > 1e-30, m, 1.0)[:, None]
+    dn = np.abs(_dot(d, nh))
+    return 1.0 / np.maximum(dn, 1e-30)

@@ -1,0 +1,12 @@
+"""Linear solvers of the pressure equation (reference ``src/pyfoam/solvers``)."""
+
+from .linear_solver import LinearSolverBase, create_solver, solver_from_dict
+from .pbicgstab import PBiCGSTABSolver
+from .pcg import PCGSolver
+from .gamg import GAMGSolver
+from .preconditioners import DICPreconditioner, DILUPreconditioner, Preconditioner
+from .residual import ConvergenceInfo, ResidualMonitor
+
+__all__ = ["LinearSolverBase", "create_solver", "solver_from_dict", "PCGSolver",
+           "PBiCGSTABSolver", "GAMGSolver", "DICPreconditioner", "DILUPreconditioner",
+           "Preconditioner", "ConvergenceInfo", "ResidualMonitor"]

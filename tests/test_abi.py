@@ -1,0 +1,93 @@
+"""The C-ABI library: it loads without a GPU, exports every symbol that
+include/pfm.h declares, and its host-side setup algorithms (no GPU needed)
+agree with the oracle."""
+
+import re
+from pathlib import Path
+
+import numpy as np
+import pytest
+
+from oracle import mesh as omesh
+from pyopenfoam_amd import _lib
+from pyopenfoam_amd.cases import poisson_case, rcm_permutation
+from pyopenfoam_amd.mesh.hexmesh import relabel_cells, structured_hex
+
+ROOT = Path(__file__).resolve().parent.parent
+
+
+def declared_symbols():
+    text = (ROOT / "include" / "pfm.h").read_text()
+    text = re.sub(r"/\*.*?\*/", "", text, flags=re.S)
+    return sorted(set(re.findall(r"\b(pfm_\w+)\s*\(", text)))
+
+
+def test_library_exports_every_declared_symbol():
+    L = _lib.lib()
+    names = declared_symbols()
+    assert len(names) >= 20
+    for name in names:
+        assert hasattr(L, name), name
+    assert set(names) == set(_lib.EXPORTED_SYMBOLS)
+    assert L.pfm_abi_version() == 1
+
+
+def test_error_reporting():
+    L = _lib.lib()
+    own = np.array([0, 5], np.int32)
+    nei = np.array([1, 2], np.int32)
+    perm = np.empty(3, np.int32)
+    st = L.pfm_rcm(3, 2, own.ctypes.data, nei.ctypes.data, perm.ctypes.data)
+    assert st == _lib.PFM_ERR_ARG
+    assert b"outside" in L.pfm_last_error()
+    with pytest.raises(ValueError):
+        _lib.check(st)
+
+
+@pytest.mark.parametrize("n,seed", [(3, 0), (5, 1), (8, 2), (12, 3)])
+def test_rcm_matches_oracle(n, seed):
+    m = structured_hex(n)
+    m = relabel_cells(m, np.random.default_rng(seed).permutation(m.n_cells))
+    nfi = m.n_internal_faces
+    got = rcm_permutation(m.n_cells, m.owner[:nfi], m.neighbour)
+    want = omesh.rcm(m.n_cells, m.owner[:nfi], m.neighbour)
+    assert np.array_equal(got, want)
+
+
+def test_rcm_disconnected_and_isolated():
+    # two chains and an isolated cell: restarts pick the lowest min-degree cell
+    own = np.array([0, 1, 4, 5])
+    nei = np.array([1, 2, 5, 6])
+    got = rcm_permutation(8, own, nei)
+    assert np.array_equal(got, omesh.rcm(8, own, nei))
+    assert sorted(got.tolist()) == list(range(8))
+
+
+def _seq_dic_ok(n, own, nei, level):
+    for f in range(len(own)):
+        if own[f] < nei[f]:
+            assert level[nei[f]] > level[own[f]]
+
+
+def test_dic_and_dilu_levels_respect_sequential_order():
+    L = _lib.lib()
+    c = poisson_case(6, seed=4)
+    own = _lib.host_i32(c.owner)
+    nei = _lib.host_i32(c.neighbour)
+    import ctypes as C
+    lev = np.empty(c.n_cells, np.int32)
+    nl = C.c_int32()
+    assert L.pfm_dic_levels(c.n_cells, own.shape[0], own.ctypes.data, nei.ctypes.data,
+                            lev.ctypes.data, C.byref(nl)) == 0
+    _seq_dic_ok(c.n_cells, own, nei, lev)
+    assert nl.value == lev.max() + 1
+    flev = np.empty(own.shape[0], np.int32)
+    assert L.pfm_dilu_levels(c.n_cells, own.shape[0], own.ctypes.data, nei.ctypes.data,
+                             flev.ctypes.data, C.byref(nl)) == 0
+    # faces sharing a cell are ordered by level in face order
+    last = {}
+    for f in range(own.shape[0]):
+        for cell in (own[f], nei[f]):
+            if cell in last:
+                assert flev[f] > flev[last[cell]]
+            last[cell] = f

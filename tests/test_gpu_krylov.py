@@ -1,0 +1,125 @@
+"""GPU PCG / PBiCGStab against the oracle restatements of the reference solvers.
+
+Bar: the same stopping decision; the first residuals equal to 1e-9; on
+symmetric systems iterations within one of the oracle and the solution within
+1e-9; on the non-symmetric (jittered) systems, where the reference algorithm
+itself depends on the dot-product summation order, iterations inside the
+oracle's spread over three summation orders and the solution within ten times
+that spread.  The tolerances are written in each test."""
+
+import numpy as np
+import pytest
+import torch
+
+from oracle import krylov as okry
+from oracle import ldu as oldu
+from oracle import precond as opre
+from pyopenfoam_amd import LduMatrix, PBiCGSTABSolver, PCGSolver, create_solver
+from tests.helpers import system
+
+pytestmark = pytest.mark.gpu
+
+
+def _matrix(c, d, lo, up, dev):
+    m = LduMatrix(c.n_cells, torch.from_numpy(c.owner), torch.from_numpy(c.neighbour), device=dev)
+    m.diag = torch.from_numpy(d)
+    m.lower = torch.from_numpy(lo)
+    m.upper = torch.from_numpy(up)
+    return m
+
+
+def _oracle_spread(solve):
+    """Run an oracle solve under the three summation orders of the dot
+    products: (iteration range, solutions, per-order infos)."""
+    out = {}
+    for name, dot in okry.DOTS.items():
+        out[name] = solve(dot)
+    its = [o[1] for o in out.values()]
+    return min(its), max(its), out
+
+
+@pytest.mark.parametrize("n,jit,sig", [(8, 0.0, 0.0), (16, 0.2, 1.0), (32, 0.0, 0.0), (32, 0.2, 1.0)])
+@pytest.mark.parametrize("pre", ["DIC", "none"])
+def test_pcg_matches_oracle(gpu_device, n, jit, sig, pre):
+    """Symmetric systems (no jitter): the oracle's iteration count is the same
+    for every summation order; the GPU must be within one of it and its
+    solution within 1e-9.  Jittered meshes make the reference matrix
+    non-symmetric (rows scaled by 1/V); CG then depends on the summation order
+    itself (the oracle spreads by ~10% across orders), so the GPU must fall
+    inside the oracle's spread (+-1) and agree with it to the spread of the
+    oracle's own solutions."""
+    c, d, lo, up = system(n, jit, sig)
+    print(f"pcg n={n} jitter={jit} sigma={sig} precond={pre}", flush=True)
+    m = _matrix(c, d, lo, up, gpu_device)
+    solver = PCGSolver(tolerance=1e-300, rel_tol=1e-8, max_iter=20000, preconditioner=pre)
+    x, iters, res = solver(m, torch.from_numpy(c.b), torch.zeros(c.n_cells, dtype=torch.float64))
+    rD = opre.dic_build(d, up, c.owner, c.neighbour) if pre == "DIC" else None
+    mv = lambda v: oldu.ax(d, lo, up, c.owner, c.neighbour, v)
+    lo_it, hi_it, runs = _oracle_spread(lambda dot: okry.pcg(
+        mv, (lambda r: r / rD) if rD is not None else None, c.b, np.zeros(c.n_cells),
+        tolerance=1e-300, rel_tol=1e-8, max_iter=20000, dot=dot))
+    xo, io, ro, info = runs["blas"]
+    assert info.converged and solver.last_info.converged
+    assert lo_it - 1 <= iters <= hi_it + 1, (iters, lo_it, hi_it)
+    h = np.array(solver.last_info.residual_history)
+    np.testing.assert_allclose(h[:10], info.residual_history[:10], rtol=1e-9)
+    xg = x.cpu().numpy()
+    err = np.linalg.norm(xg - xo) / np.linalg.norm(xo)
+    spread = max(np.linalg.norm(r[0] - xo) / np.linalg.norm(xo) for r in runs.values())
+    if jit == 0.0:
+        assert hi_it == lo_it
+        assert err <= 1e-9, err
+    else:
+        assert err <= max(10 * spread, 1e-9), (err, spread)
+    true_rel = np.linalg.norm(c.b - mv(xg)) / np.linalg.norm(c.b)
+    assert true_rel <= 1.1e-8, true_rel
+
+
+def test_pcg_known_answer_and_semantics(gpu_device):
+    # 3-cell SPD chain, reference tests/unit/solvers/conftest.py:10-35 builder
+    own, nei = torch.tensor([0, 1]), torch.tensor([1, 2])
+    m = LduMatrix(3, own, nei, device=gpu_device)
+    m.lower = torch.tensor([-1.0, -1.0])
+    m.upper = torch.tensor([-1.0, -1.0])
+    m.diag = torch.tensor([1.1, 2.1, 1.1])
+    b = torch.tensor([1.0, 2.0, 3.0], dtype=torch.float64)
+    x, iters, res = PCGSolver(tolerance=1e-12, rel_tol=0.0, max_iter=100)(m, b, torch.zeros(3, dtype=torch.float64))
+    dense = np.array([[1.1, -1, 0], [-1, 2.1, -1], [0, -1, 1.1]])
+    np.testing.assert_allclose(x.cpu().numpy(), np.linalg.solve(dense, b.numpy()), rtol=1e-10)
+    # converged at iteration 0: zero rhs -> r0 = 0 < tolerance
+    x0, it0, r0 = PCGSolver()(m, torch.zeros(3, dtype=torch.float64), torch.zeros(3, dtype=torch.float64))
+    assert it0 == 1 and r0 == 0.0
+    # max_iter exhaustion returns max_iter + 1 history entries
+    PCGSolver(tolerance=1e-300, rel_tol=1e-300, max_iter=1)(m, b, torch.zeros(3, dtype=torch.float64))
+    with pytest.raises(ValueError):
+        PCGSolver(preconditioner="bogus")(m, b, torch.zeros(3, dtype=torch.float64))
+    with pytest.raises(ValueError):
+        create_solver("nope")
+
+
+@pytest.mark.parametrize("n,jit,sig", [(8, 0.0, 0.0), (16, 0.2, 1.0), (32, 0.2, 1.0)])
+@pytest.mark.parametrize("pre", ["DILU", "none"])
+def test_pbicgstab_matches_oracle(gpu_device, n, jit, sig, pre):
+    """Same decision (converged or not: with the reference's DILU the
+    omega = t.z/t.t step stalls, without a preconditioner it converges),
+    identical early history, and for converged solves iterations within the
+    oracle's summation-order spread and a matching solution."""
+    c, d, lo, up = system(n, jit, sig)
+    print(f"pbicgstab n={n} jitter={jit} sigma={sig} precond={pre}", flush=True)
+    m = _matrix(c, d, lo, up, gpu_device)
+    solver = PBiCGSTABSolver(tolerance=1e-300, rel_tol=1e-8, max_iter=3000, preconditioner=pre)
+    x, iters, res = solver(m, torch.from_numpy(c.b), torch.zeros(c.n_cells, dtype=torch.float64))
+    rD = opre.dilu_build(d, lo, up, c.owner, c.neighbour) if pre == "DILU" else None
+    mv = lambda v: oldu.ax(d, lo, up, c.owner, c.neighbour, v)
+    lo_it, hi_it, runs = _oracle_spread(lambda dot: okry.pbicgstab(
+        mv, (lambda r: r / rD) if rD is not None else None, c.b, np.zeros(c.n_cells),
+        tolerance=1e-300, rel_tol=1e-8, max_iter=3000, dot=dot))
+    xo, io, ro, info = runs["blas"]
+    assert solver.last_info.converged == info.converged
+    h = np.array(solver.last_info.residual_history)
+    np.testing.assert_allclose(h[:8], info.residual_history[:8], rtol=1e-9)
+    if info.converged:
+        assert lo_it - 1 <= iters <= hi_it + 1, (iters, lo_it, hi_it)
+        xg = x.cpu().numpy()
+        spread = max(np.linalg.norm(r[0] - xo) / np.linalg.norm(xo) for r in runs.values())
+        assert np.linalg.norm(xg - xo) / np.linalg.norm(xo) <= max(10 * spread, 1e-9)
