@@ -73,6 +73,19 @@ int pfm_dilu_levels(int32_t n_cells, int32_t n_faces, const int32_t *owner_h,
                     const int32_t *neighbour_h, int32_t *level_out_h,
                     int32_t *n_levels);
 
+/* Greedy aggregation of GAMGSolver._build_aggregates (solvers/gamg.py:164-239)
+ * on one level: f2c_out[n_cells], *n_coarse = aggregate count. */
+int pfm_gamg_aggregate(int32_t n_cells, int32_t n_faces, const int32_t *owner_h,
+                       const int32_t *neighbour_h, int32_t target, int32_t *f2c_out_h,
+                       int32_t *n_coarse);
+
+/* PairGamgAgglomeration.build_levels (fv_agglomeration/pair_agglomeration.py:47-131):
+ * cumulative finest->level maps, level k in maps_out[k*n_cells ...],
+ * coarse counts in n_coarse_out[k]; *n_levels_out levels were made. */
+int pfm_pair_agglomerate(int32_t n_cells, int32_t n_faces, const int32_t *owner_h,
+                         const int32_t *neighbour_h, int32_t n_levels, int32_t *maps_out_h,
+                         int32_t *n_coarse_out_h, int32_t *n_levels_out);
+
 /* ------------------------------------------------------------------ */
 /* LDU addressing handle                                              */
 /* ------------------------------------------------------------------ */
@@ -176,6 +189,61 @@ int pfm_pbicgstab_solve(pfm_ldu h, const pfm_solve_params *prm,
                         const double *rD, const double *b, double *x,
                         void *work, double *history_h, pfm_solve_result *res,
                         void *stream);
+
+/* ------------------------------------------------------------------ */
+/* GAMG                                                               */
+/* ------------------------------------------------------------------ */
+
+typedef struct pfm_gamg_s *pfm_gamg;
+
+#define PFM_AGGLOM_GREEDY 0   /* GAMGSolver._build_aggregates (solvers/gamg.py:164-239) */
+#define PFM_AGGLOM_PAIR 1     /* PairGamgAgglomeration (fv_agglomeration/pair_agglomeration.py:47-131) */
+#define PFM_COARSE_REFERENCE 0 /* _build_coarse_matrix (gamg.py:242-333): couplings
+                                  inside an aggregate are dropped */
+#define PFM_COARSE_GALERKIN 1  /* extension: they are added to the coarse diagonal,
+                                  the Galerkin product R A P of piecewise-constant P */
+
+typedef struct {
+  int32_t max_levels;       /* coarsening steps (reference default 10) */
+  int32_t min_cells_coarse; /* stop at or below this many cells (default 10) */
+  int32_t n_pre;            /* pre-smoothing Jacobi sweeps (default 2) */
+  int32_t n_post;           /* post-smoothing Jacobi sweeps (default 2) */
+  int32_t n_coarsest;       /* sweeps on the coarsest level (reference: 50) */
+  int32_t agglomeration;    /* PFM_AGGLOM_* */
+  int32_t coarse;           /* PFM_COARSE_* */
+  int32_t reserved;
+} pfm_gamg_params;
+
+/* Build the hierarchy of GAMGSolver._solve (gamg.py:541-570) from the fine
+ * topology: aggregation maps and coarse face maps per level (host, topology
+ * only) and an LDU handle per coarse level.  Coefficients are coarsened on the
+ * device by every solve. `fine` must outlive the hierarchy. */
+int pfm_gamg_create(pfm_ldu fine, const pfm_gamg_params *prm, pfm_gamg *out);
+int pfm_gamg_destroy(pfm_gamg g);
+int pfm_gamg_num_levels(pfm_gamg g);
+int pfm_gamg_level_info(pfm_gamg g, int32_t level, int32_t *n_cells, int32_t *n_faces);
+/* Maps of level -> level+1 (host arrays; any may be NULL): f2c[n_cells(level)],
+ * face_map[n_faces(level)] (coarse face or -1), coarse owner/neighbour. */
+int pfm_gamg_level_maps(pfm_gamg g, int32_t level, int32_t *f2c_h, int32_t *face_map_h,
+                        int32_t *c_owner_h, int32_t *c_neighbour_h);
+
+int64_t pfm_gamg_workspace_bytes(pfm_gamg g, int32_t max_iter);
+/* Coarsen the coefficients into the workspace (diag by aggregate sums in fine
+ * cell order, off-diagonals by coarse-face sums in fine face order), pack every
+ * level and compute the Jacobi inverse diagonals.  pfm_gamg_solve calls it. */
+int pfm_gamg_setup_values(pfm_gamg g, const double *diag, const double *lower,
+                          const double *upper, void *work, void *stream);
+/* Copy the coarsened coefficients of `level` (>= 1) out of the workspace. */
+int pfm_gamg_level_values(pfm_gamg g, void *work, int32_t level, double *diag,
+                          double *lower, double *upper, void *stream);
+/* GAMGSolver._solve (gamg.py:499-585): V-cycles with undamped Jacobi
+ * (1/|diag|) smoothing, sum restriction, injection prolongation and the
+ * coarsest level smoothed n_coarsest times; the residual b - A x after every
+ * cycle feeds the monitor (no initial residual is recorded). */
+int pfm_gamg_solve(pfm_gamg g, const pfm_solve_params *prm, const double *diag,
+                   const double *lower, const double *upper, const double *b,
+                   double *x, void *work, double *history_h, pfm_solve_result *res,
+                   void *stream);
 
 /* ------------------------------------------------------------------ */
 /* Assembly and adjoint                                               */

@@ -25,6 +25,7 @@ _SIGS = {
     "orc_gamg_aggregate": (I64, [I64, I64, P, P, I64, P]),
     "orc_gamg_coarse_faces": (I64, [I64, P, P, P, P, P, P, P, P, P, P]),
     "orc_pair_level": (I64, [I64, I64, I64, P, P, P, P]),
+    "orc_norm3": (C.c_int, [I64, P, P]),
 }
 
 

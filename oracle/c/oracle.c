@@ -297,3 +297,14 @@ int64_t orc_pair_level(int64_t n_fine, int64_t n_cur, int64_t nf, const int64_t 
   free(dense);
   return k;
 }
+
+/* torch's CPU vector norm of 3-component rows (Tensor.norm(dim=1), used by
+ * fvm.laplacian for |S_f|, discretisation/operators.py:490) accumulates the
+ * squares with fused multiply-adds: sqrt(fma(z, z, fma(y, y, x*x))). */
+int orc_norm3(int64_t n, const double *v, double *out) {
+  for (int64_t i = 0; i < n; ++i) {
+    const double x = v[3 * i], y = v[3 * i + 1], z = v[3 * i + 2];
+    out[i] = sqrt(fma(z, z, fma(y, y, x * x)));
+  }
+  return 0;
+}

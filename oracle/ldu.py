@@ -51,6 +51,15 @@ def dense(diag, lower, upper, owner, neighbour) -> np.ndarray:
     return a
 
 
+def face_mag(face_areas) -> np.ndarray:
+    """|S_f| as the reference computes it (torch norm over dim 1, FMA order)."""
+    from ._clib import clib, f64, p
+    s = f64(face_areas)
+    out = np.empty(s.shape[0])
+    clib().orc_norm3(s.shape[0], p(s), p(out))
+    return out
+
+
 def laplacian(gamma_int, mag_sf_int, delta_int, vol, owner, neighbour):
     """Internal-face part of fvm.laplacian: returns (diag, lower, upper).
 

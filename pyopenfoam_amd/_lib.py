@@ -13,7 +13,8 @@ from pathlib import Path
 
 import numpy as np
 
-__all__ = ["lib", "check", "PfmError", "SolveParams", "SolveResult", "LduInfo", "library_path",
+__all__ = ["lib", "check", "PfmError", "SolveParams", "SolveResult", "LduInfo", "GamgParams",
+           "library_path",
            "PFM_OK", "PFM_ERR_ARG", "PFM_ERR_HIP", "PFM_SOLVE_CONVERGED", "PFM_SOLVE_MAXITER",
            "PFM_SOLVE_BREAKDOWN", "PFM_PRECOND_NONE", "PFM_PRECOND_DIAG", "EXPORTED_SYMBOLS"]
 
@@ -39,6 +40,16 @@ class SolveResult(C.Structure):
                 ("final_residual", C.c_double)]
 
 
+class GamgParams(C.Structure):
+    _fields_ = [("max_levels", C.c_int32), ("min_cells_coarse", C.c_int32), ("n_pre", C.c_int32),
+                ("n_post", C.c_int32), ("n_coarsest", C.c_int32), ("agglomeration", C.c_int32),
+                ("coarse", C.c_int32), ("reserved", C.c_int32)]
+
+
+PFM_AGGLOM_GREEDY, PFM_AGGLOM_PAIR = 0, 1
+PFM_COARSE_REFERENCE, PFM_COARSE_GALERKIN = 0, 1
+
+
 class LduInfo(C.Structure):
     _fields_ = [("n_cells", C.c_int32), ("n_faces", C.c_int32), ("n_slices", C.c_int32),
                 ("n_dic_levels", C.c_int32), ("n_dilu_levels", C.c_int32),
@@ -58,6 +69,8 @@ _SIGS = {
     "pfm_rcm": (INT, [I32, I32, P, P, P]),
     "pfm_dic_levels": (INT, [I32, I32, P, P, P, P]),
     "pfm_dilu_levels": (INT, [I32, I32, P, P, P, P]),
+    "pfm_gamg_aggregate": (INT, [I32, I32, P, P, I32, P, P]),
+    "pfm_pair_agglomerate": (INT, [I32, I32, P, P, I32, P, P, P]),
     "pfm_ldu_create": (INT, [I32, I32, P, P, C.POINTER(P)]),
     "pfm_ldu_destroy": (INT, [P]),
     "pfm_ldu_get_info": (INT, [P, C.POINTER(LduInfo)]),
@@ -74,6 +87,15 @@ _SIGS = {
     "pfm_pcg_solve": (INT, [P, C.POINTER(SolveParams), P, P, P, P, P, P, P, C.POINTER(SolveResult), P]),
     "pfm_pbicgstab_solve": (INT, [P, C.POINTER(SolveParams), P, P, P, P, P, P, P,
                                   C.POINTER(SolveResult), P]),
+    "pfm_gamg_create": (INT, [P, C.POINTER(GamgParams), C.POINTER(P)]),
+    "pfm_gamg_destroy": (INT, [P]),
+    "pfm_gamg_num_levels": (INT, [P]),
+    "pfm_gamg_level_info": (INT, [P, I32, P, P]),
+    "pfm_gamg_level_maps": (INT, [P, I32, P, P, P, P]),
+    "pfm_gamg_workspace_bytes": (I64, [P, I32]),
+    "pfm_gamg_setup_values": (INT, [P, P, P, P, P, P]),
+    "pfm_gamg_level_values": (INT, [P, P, I32, P, P, P, P]),
+    "pfm_gamg_solve": (INT, [P, C.POINTER(SolveParams), P, P, P, P, P, P, P, C.POINTER(SolveResult), P]),
     "pfm_laplacian_assemble": (INT, [P, P, P, P, P, I32, P, P, P, P, P, P, P, P]),
     "pfm_adjoint_ldu_grads": (INT, [P, P, P, P, P, P, P]),
     "pfm_laplacian_backward": (INT, [P, P, P, P, P, P, P, I32, P, P, P, P, P, P]),

@@ -40,6 +40,8 @@ def _compile(src: Path, verbose: bool) -> Path:
     cmd = [HIPCC, *COMMON, "-c", str(src), "-o", str(obj)]
     if src.suffix == ".hip":
         cmd[1:1] = HIPFLAGS + ["-x", "hip"]
+    else:  # host-only C++ (setup algorithms); hipcc would otherwise treat it as HIP
+        cmd[1:1] = ["-x", "c++", "-D__HIP_PLATFORM_AMD__", "-I/opt/rocm/include"]
     if verbose:
         print(" ".join(cmd), flush=True)
     r = subprocess.run(cmd, capture_output=True, text=True)

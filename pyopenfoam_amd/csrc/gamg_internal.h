@@ -1,0 +1,56 @@
+// GAMG hierarchy: host setup routines and the device-side level descriptors.
+#pragma once
+
+#include "pfm_internal.h"
+
+namespace pfm {
+
+int gamg_aggregate(int32_t n, int32_t nf, const int32_t *own, const int32_t *nei, int32_t target,
+                   std::vector<int32_t> &f2c, int32_t &nagg);
+int gamg_coarse_faces(int32_t nf, const int32_t *own, const int32_t *nei,
+                      const std::vector<int32_t> &f2c, std::vector<int32_t> &c_own,
+                      std::vector<int32_t> &c_nei, std::vector<int32_t> &face_map);
+int pair_level(int32_t n_fine, int32_t n_cur, int32_t nf, const int32_t *own, const int32_t *nei,
+               const std::vector<int32_t> &cell_map, std::vector<int32_t> &out, int32_t &n_new);
+
+// Device pointers of one level, as the kernels see them.
+struct LevelDev {
+  int32_t n, nslices;
+  const SliceInfo *slices;
+  const int32_t *col;
+  const double *packed;
+  const double *diag;
+  const double *inv;      // 1 / max(|diag|, 1e-30)
+  double *x[2];           // Jacobi double buffer
+  double *b;
+  double *r;
+  // restriction to the next coarser level (unused on the coarsest)
+  const int32_t *f2c;
+  const int32_t *agg_ptr;
+  const int32_t *agg_cells;
+  int32_t nc;
+};
+
+}  // namespace pfm
+
+// One level of a hierarchy (host side).
+struct GamgLevelHost {
+  int32_t n = 0, nf = 0;
+  pfm_ldu ldu = nullptr;          // level 0 borrows the caller's handle
+  bool owns_ldu = false;
+  std::vector<int32_t> own, nei;  // level topology (coarse levels)
+  // maps to the next coarser level
+  int32_t nc = 0, nfc = 0;
+  std::vector<int32_t> f2c, face_map;
+  int32_t *d_f2c = nullptr, *d_agg_ptr = nullptr, *d_agg_cells = nullptr;
+  int32_t *d_cf_ptr = nullptr, *d_cf_faces = nullptr;
+  int32_t *d_in_ptr = nullptr, *d_in_faces = nullptr;  // faces inside each aggregate
+};
+
+struct pfm_gamg_s {
+  pfm_ldu fine = nullptr;
+  pfm_gamg_params prm{};
+  std::vector<GamgLevelHost> lv;
+  int device = 0;
+  int32_t *h_flag = nullptr;
+};

@@ -79,9 +79,10 @@ def test_pcg_known_answer_and_semantics(gpu_device):
     # 3-cell SPD chain, reference tests/unit/solvers/conftest.py:10-35 builder
     own, nei = torch.tensor([0, 1]), torch.tensor([1, 2])
     m = LduMatrix(3, own, nei, device=gpu_device)
-    m.lower = torch.tensor([-1.0, -1.0])
-    m.upper = torch.tensor([-1.0, -1.0])
-    m.diag = torch.tensor([1.1, 2.1, 1.1])
+    f64 = dict(dtype=torch.float64)
+    m.lower = torch.tensor([-1.0, -1.0], **f64)
+    m.upper = torch.tensor([-1.0, -1.0], **f64)
+    m.diag = torch.tensor([1.1, 2.1, 1.1], **f64)
     b = torch.tensor([1.0, 2.0, 3.0], dtype=torch.float64)
     x, iters, res = PCGSolver(tolerance=1e-12, rel_tol=0.0, max_iter=100)(m, b, torch.zeros(3, dtype=torch.float64))
     dense = np.array([[1.1, -1, 0], [-1, 2.1, -1], [0, -1, 1.1]])
@@ -117,7 +118,13 @@ def test_pbicgstab_matches_oracle(gpu_device, n, jit, sig, pre):
     xo, io, ro, info = runs["blas"]
     assert solver.last_info.converged == info.converged
     h = np.array(solver.last_info.residual_history)
-    np.testing.assert_allclose(h[:8], info.residual_history[:8], rtol=1e-9)
+    # per-entry envelope: 10x the spread of the oracle's own summation orders
+    # (the reference's DILU step amplifies rounding differences ~1e4x per step)
+    k = min(12, len(h), *(len(r[3].residual_history) for r in runs.values()))
+    hb = np.array(info.residual_history[:k])
+    spread = np.max([np.abs(np.array(r[3].residual_history[:k]) / hb - 1) for r in runs.values()],
+                    axis=0)
+    assert np.all(np.abs(h[:k] / hb - 1) <= np.maximum(10 * spread, 1e-9)), (h[:k], hb, spread)
     if info.converged:
         assert lo_it - 1 <= iters <= hi_it + 1, (iters, lo_it, hi_it)
         xg = x.cpu().numpy()

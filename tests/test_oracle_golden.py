@@ -1,0 +1,241 @@
+"""The oracle against golden vectors written by the reference itself
+(tests/golden/make_golden.py) and against the reference's own known answers.
+
+Bit-exact where the reference computation has a fixed operation order
+(scatter sums, recurrences, sweeps, graph algorithms); to a stated
+tolerance where the reference leaves the order to torch (dot products and
+norms) or to Python's set ordering (cell reference points of the geometry).
+"""
+
+from pathlib import Path
+
+import numpy as np
+import pytest
+
+from oracle import adjoint as oadj
+from oracle import gamg as ogamg
+from oracle import krylov as okry
+from oracle import ldu as oldu
+from oracle import mesh as omesh
+from oracle import precond as opre
+from pyopenfoam_amd.cases import rcm_permutation
+from pyopenfoam_amd.mesh.geometry import compute_geometry
+from pyopenfoam_amd.mesh.hexmesh import HexMesh
+from tests.helpers import bits_equal
+
+G = Path(__file__).resolve().parent / "golden"
+
+
+def load(name):
+    return dict(np.load(G / f"{name}.npz"))
+
+
+@pytest.mark.parametrize("name", ["rcm_4", "rcm_6"])
+def test_rcm_golden(name):
+    g = load(name)
+    n = int(g["n_cells"])
+    assert np.array_equal(omesh.rcm(n, g["owner"], g["neighbour"]), g["perm"])
+    assert np.array_equal(rcm_permutation(n, g["owner"], g["neighbour"]), g["perm"])
+
+
+@pytest.fixture(params=["sys_4", "sys_6"], scope="module")
+def sysg(request):
+    g = load(request.param)
+    g["nfi"] = g["neighbour"].shape[0]
+    g["own"] = g["owner"][: g["nfi"]]
+    return g
+
+
+def test_geometry_golden(sysg):
+    g = sysg
+    n = int(g["n_cells"])
+    if n <= 64:  # loop restatement (slow), small mesh only
+        geo = omesh.geometry(g["points"], list(g["faces"]), g["owner"], g["neighbour"], n)
+        for k in ("face_centres", "face_areas", "cell_volumes", "cell_centres", "delta"):
+            np.testing.assert_allclose(geo[k], g[k], rtol=1e-13, atol=1e-16)
+    m = HexMesh(points=g["points"], faces=g["faces"], owner=g["owner"], neighbour=g["neighbour"],
+                n_cells=n)
+    pg = compute_geometry(m)
+    np.testing.assert_allclose(pg.face_centres, g["face_centres"], rtol=1e-14, atol=1e-16)
+    # components that cancel to ~0 are compared against the face area scale
+    np.testing.assert_allclose(pg.face_areas, g["face_areas"], rtol=1e-13,
+                               atol=1e-13 * np.abs(g["face_areas"]).max())
+    np.testing.assert_allclose(pg.cell_volumes, g["cell_volumes"], rtol=1e-13)
+    np.testing.assert_allclose(pg.cell_centres, g["cell_centres"], rtol=1e-13)
+    np.testing.assert_allclose(pg.delta, g["delta"], rtol=1e-12)
+    assert np.all(g["delta"][g["nfi"]:] == 0.0)  # boundary delta is 0 in the reference
+
+
+def test_laplacian_golden_bit_exact(sysg):
+    g = sysg
+    nfi = g["nfi"]
+    mag = oldu.face_mag(g["face_areas"])
+    d, lo, up = oldu.laplacian(g["gamma"][:nfi], mag[:nfi], g["delta"][:nfi], g["cell_volumes"],
+                               g["own"], g["neighbour"])
+    assert bits_equal(lo, g["lap_lower"])
+    assert bits_equal(up, g["lap_upper"])
+    assert bits_equal(d, g["lap_diag"])
+    dd = oldu.dirichlet_diag(d, g["gamma"][g["dir_faces"]], g["dir_mag"], g["dir_delta"],
+                             g["cell_volumes"], g["dir_cells"])
+    assert bits_equal(dd, g["diag"])
+
+
+def test_dic_dilu_golden_bit_exact(sysg):
+    g = sysg
+    d, lo, up = g["diag"], g["lap_lower"], g["lap_upper"]
+    own, nei = g["own"], g["neighbour"]
+    assert bits_equal(opre.dic_build(d, up, own, nei), g["dic"])
+    assert bits_equal(opre.dic_build_py(d, up, own, nei), g["dic"])
+    assert bits_equal(opre.dilu_build(d, lo, up, own, nei), g["dilu"])
+    assert bits_equal(opre.dilu_build_py(d, lo, up, own, nei), g["dilu"])
+    assert bits_equal(opre.dic_apply_full(g["dic"], up, own, nei, g["r"]), g["dic_full"])
+    assert bits_equal(opre.dilu_apply_full(g["dilu"], lo, up, own, nei, g["r"]), g["dilu_full"])
+
+
+@pytest.mark.parametrize("kind", ["pcg_dic", "pcg_none", "bicg_dilu", "bicg_none"])
+def test_krylov_golden(sysg, kind):
+    g = sysg
+    d, lo, up = g["diag"], g["lap_lower"], g["lap_upper"]
+    own, nei = g["own"], g["neighbour"]
+    mv = lambda v: oldu.ax(d, lo, up, own, nei, v)
+    if kind.endswith("dic"):
+        pre = lambda r: r / opre.dic_build(d, up, own, nei)
+    elif kind.endswith("dilu"):
+        pre = lambda r: r / opre.dilu_build(d, lo, up, own, nei)
+    else:
+        pre = None
+    fn = okry.pcg if kind.startswith("pcg") else okry.pbicgstab
+    mx = 300 if kind == "bicg_dilu" else 2000
+    runs = {k: fn(mv, pre, g["b"], np.zeros(len(d)), tolerance=1e-300, rel_tol=1e-8, max_iter=mx,
+                  dot=dot) for k, dot in okry.DOTS.items()}
+    ref_h = g[f"{kind}_hist"]
+    x, it, res, info = runs["blas"]
+    assert bool(g[f"{kind}_converged"]) == info.converged
+    # per-entry envelope: 10x the spread of the oracle's own summation orders
+    # (PBiCGStab with the reference's DILU amplifies rounding ~1e4x per step)
+    k = min(20, len(ref_h), *(len(r[3].residual_history) for r in runs.values()))
+    hb = np.array(info.residual_history[:k])
+    spread = np.max([np.abs(np.array(r[3].residual_history[:k]) / hb - 1) for r in runs.values()],
+                    axis=0)
+    assert np.all(np.abs(ref_h[:k] / hb - 1) <= np.maximum(10 * spread, 1e-12))
+    its = [r[1] for r in runs.values()]
+    if info.converged:
+        # the reference's count lies inside the oracle's summation-order spread
+        assert min(its) - 1 <= len(ref_h) <= max(its) + 1, (len(ref_h), its)
+        spread = max(np.linalg.norm(r[0] - x) for r in runs.values()) / np.linalg.norm(x)
+        assert np.linalg.norm(g[f"{kind}_x"] - x) / np.linalg.norm(x) <= max(10 * spread, 1e-10)
+
+
+def test_gamg_golden(sysg):
+    g = sysg
+    d, lo, up = g["diag"], g["lap_lower"], g["lap_upper"]
+    own, nei = g["own"], g["neighbour"]
+    n = len(d)
+    nagg, f2c = ogamg.aggregate(n, own, nei, max(n // 2, 4))
+    assert nagg == int(g["gamg_ncoarse"])
+    assert np.array_equal(f2c, g["gamg_f2c"])
+    nagg2, f2c2 = ogamg.aggregate_py(n, own, nei, max(n // 2, 4))
+    assert nagg2 == nagg and np.array_equal(f2c2, f2c)
+    L0 = ogamg.Level(n, d, lo, up, own, nei)
+    c, _ = ogamg.coarse_matrix(L0, nagg, f2c)
+    assert bits_equal(c.diag, g["gamg_c_diag"])
+    assert bits_equal(c.lower, g["gamg_c_lower"])
+    assert bits_equal(c.upper, g["gamg_c_upper"])
+    assert np.array_equal(c.owner, g["gamg_c_owner"])
+    assert np.array_equal(c.neighbour, g["gamg_c_neighbour"])
+    x, it, res, info = ogamg.gamg(L0, g["b"], np.zeros(n), tolerance=1e-300, rel_tol=1e-8,
+                                  max_iter=25, min_cells_coarse=4)
+    # the V-cycle is fixed-order arithmetic: x bit-exact, history to the norm's rounding
+    assert bits_equal(x, g["gamg_x"])
+    np.testing.assert_allclose(info.residual_history, g["gamg_hist"], rtol=1e-13)
+
+
+def test_pair_agglomeration_golden(sysg):
+    g = sysg
+    lv = ogamg.pair_levels(int(g["n_cells"]), g["own"], g["neighbour"], n_levels=5)
+    lvp = ogamg.pair_levels_py(int(g["n_cells"]), g["own"], g["neighbour"], n_levels=5)
+    assert len(lv) == int(g["pair_nlevels"]) == len(lvp)
+    for k, l in enumerate(lv):
+        assert np.array_equal(l["cell_map"], g[f"pair_map{k}"])
+        assert np.array_equal(lvp[k]["cell_map"], g[f"pair_map{k}"])
+        assert [l["n_cells_fine"], l["n_cells_coarse"]] == g[f"pair_n{k}"].tolist()
+
+
+def test_gauss_seidel_and_adjoint_golden():
+    g = load("sys_4")
+    nfi = g["neighbour"].shape[0]
+    own, nei = g["owner"][:nfi], g["neighbour"]
+    d, lo, up = g["diag"], g["lap_lower"], g["lap_upper"]
+    x = ogamg.gauss_seidel(d, lo, up, own, nei, g["b"], g["gs_x0"], 2)
+    assert bits_equal(x, g["gs_x"])
+    # DifferentiableLinearSolve: J = sum(w*x), solves to rel 1e-14
+    mv = lambda v: oldu.ax(d, lo, up, own, nei, v)
+    rD = opre.dic_build(d, up, own, nei)
+    xs, *_ = okry.pcg(mv, lambda r: r / rD, g["b"], np.zeros(len(d)), tolerance=1e-300,
+                      rel_tol=1e-14, max_iter=5000)
+    mvt = lambda v: oldu.ax(d, up, lo, own, nei, v)
+    rDt = opre.dic_build(d, lo, own, nei)
+    lam, *_ = okry.pcg(mvt, lambda r: r / rDt, g["adj_w"], np.zeros(len(d)), tolerance=1e-300,
+                       rel_tol=1e-14, max_iter=5000)
+    gd, gl, gu = oadj.ldu_grads(lam, xs, own, nei)
+    np.testing.assert_allclose(xs, g["adj_x"], rtol=1e-10, atol=1e-14)
+    for a, b in ((gd, "adj_gdiag"), (gl, "adj_glower"), (gu, "adj_gupper"), (lam, "adj_gb")):
+        np.testing.assert_allclose(a, g[b], rtol=1e-8, atol=1e-14 * np.abs(g[b]).max())
+
+
+def test_ax_known_answers():
+    # reference tests/unit/core/test_ldu_matrix.py:151-202
+    o, q = np.array([0]), np.array([1])
+    assert oldu.ax(np.array([4.0, 4.0]), np.array([-1.0]), np.array([-2.0]), o, q,
+                   np.array([1.0, 3.0])).tolist() == [1.0, 10.0]
+    assert oldu.ax_t(np.array([4.0, 4.0]), np.array([-1.0]), np.array([-2.0]), o, q,
+                     np.array([1.0, 3.0])).tolist() == [-2.0, 11.0]
+    o3, q3 = np.array([0, 1]), np.array([1, 2])
+    assert oldu.ax(np.array([4.0, 6.0, 4.0]), -np.ones(2), -np.ones(2), o3, q3,
+                   np.array([1.0, 2.0, 3.0])).tolist() == [2.0, 8.0, 10.0]
+
+
+def test_ax_matches_dense():
+    rng = np.random.default_rng(0)
+    g = load("sys_6")
+    nfi = g["neighbour"].shape[0]
+    own, nei = g["owner"][:nfi], g["neighbour"]
+    d, lo, up = g["diag"], g["lap_lower"], g["lap_upper"]
+    A = oldu.dense(d, lo, up, own, nei)
+    x = rng.standard_normal(len(d))
+    np.testing.assert_allclose(oldu.ax(d, lo, up, own, nei, x), A @ x, rtol=1e-12, atol=1e-9)
+    np.testing.assert_allclose(oldu.ax_t(d, lo, up, own, nei, x), A.T @ x, rtol=1e-12, atol=1e-9)
+
+
+def test_dic_is_exact_cholesky_on_a_chain():
+    """On a chain (tridiagonal) incomplete Cholesky has no dropped fill, so the
+    DIC diagonal satisfies A = (D + L) D^-1 (D + L^T) exactly for a symmetric
+    matrix: a property test independent of the recurrence's code."""
+    n = 12
+    rng = np.random.default_rng(5)
+    own, nei = np.arange(n - 1), np.arange(1, n)
+    off = -rng.uniform(0.5, 1.5, n - 1)
+    d = np.bincount(own, -off, n) + np.bincount(nei, -off, n) + rng.uniform(0.1, 1.0, n)
+    D = opre.dic_build(d, off, own, nei)
+    A = oldu.dense(d, off, off, own, nei)
+    Lm = np.tril(A, -1)
+    M = (np.diag(D) + Lm) @ np.diag(1.0 / D) @ (np.diag(D) + Lm.T)
+    np.testing.assert_allclose(M, A, rtol=1e-12, atol=1e-12)
+
+
+def test_dilu_two_sided_update_known_answer():
+    """DILU updates both cells of a face with the values they hold before the
+    face (preconditioners.py:258-268): on 2 cells with d = (4, 4),
+    lower = -1, upper = -2, both become 4 - 2/4 = 3.5 (a DIC-style one-sided
+    recurrence would leave d0 = 4).  On a 3-cell chain the second face sees
+    the first face's update of cell 1."""
+    o, q = np.array([0]), np.array([1])
+    d = np.array([4.0, 4.0])
+    assert opre.dilu_build(d, np.array([-1.0]), np.array([-2.0]), o, q).tolist() == [3.5, 3.5]
+    assert opre.dic_build(d, np.array([-2.0]), o, q).tolist() == [4.0, 3.0]
+    o3, q3 = np.array([0, 1]), np.array([1, 2])
+    d3 = np.array([4.0, 6.0, 4.0])
+    one = -np.ones(2)
+    got = opre.dilu_build(d3, one, one, o3, q3)
+    d1 = 6.0 - 1.0 / 4.0
+    assert got.tolist() == [4.0 - 1.0 / 6.0, d1 - 1.0 / 4.0, 4.0 - 1.0 / d1]
