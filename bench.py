@@ -88,10 +88,12 @@ def main() -> int:
     ap.add_argument("--cpu-budget", type=float, default=15.0, help="seconds of oracle CPU work")
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-adjoint", action="store_true")
-    ap.add_argument("--max-iter", type=int, default=50000)
+    ap.add_argument("--max-iter", type=int, default=200000)
     ap.add_argument("--jitter", type=float, default=0.2)
     ap.add_argument("--sigma", type=float, default=1.0)
     ap.add_argument("--history", default=None, help="write the residual history (.npy) here")
+    ap.add_argument("--copy-probe-mib", type=int, default=0,
+                    help="also run a plain device copy of this size (HBM counter calibration)")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py measures one GPU only; --gpus {args.gpus} is not supported (the path "
@@ -172,6 +174,15 @@ def main() -> int:
     spmv_gbs = spmv_bytes / (spmv_us * 1e-6) / 1e9
     log(f"SpMV: {spmv_us:.2f} us/launch, {spmv_gbs:.0f} GB/s algorithmic "
         f"({spmv_gbs / HBM_PEAK_GBS * 100:.1f}% of 8 TB/s)")
+
+    if args.copy_probe_mib:
+        nb = args.copy_probe_mib * (1 << 20)
+        src = torch.empty(nb, dtype=torch.uint8, device=dev)
+        dst = torch.empty_like(src)
+        for _ in range(5):
+            dst.copy_(src)
+        torch.cuda.synchronize()
+        log(f"copy probe: 5 x {args.copy_probe_mib} MiB device copies")
 
     # --- DIC factor alone ---------------------------------------------------
     from pyopenfoam_amd.solvers import DICPreconditioner

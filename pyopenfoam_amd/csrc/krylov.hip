@@ -33,18 +33,30 @@ __device__ __forceinline__ void store_partials(double (&v)[K], double *part, dou
   }
 }
 
+// Rows of a producer kernel: a grid-stride walk over slices in which the
+// workgroups of one XCD (consecutive logical blocks after xcd_remap) cover one
+// contiguous chunk of rows per step, so neighbouring rows' x[] values are
+// served from that XCD's L2 while the chunk moves through the matrix.
 #define FOR_ROWS(body)                                                          \
   {                                                                             \
-    const SliceRange sr_ = block_slices(nslices);                               \
+    const int lb_ = xcd_remap(blockIdx.x, gridDim.x);                           \
     const int lane = threadIdx.x & 63;                                          \
-    for (int s = sr_.begin + (threadIdx.x >> 6); s < sr_.end; s += kWavesPerBlock) { \
+    for (int s = lb_ * kWavesPerBlock + (threadIdx.x >> 6); s < nslices;        \
+         s += gridDim.x * kWavesPerBlock) {                                     \
       const int row = s * kSlice + lane;                                        \
       body                                                                      \
     }                                                                           \
   }
 
 // ---------------------------------------------------------------- PCG ------
-// r = b - A x; z = M^-1 r; p = z; partials r.r and r.z.
+// Per iteration i: k_pcg_spmv (Ap, p.Ap partials) -> k_pcg_update (every
+// workgroup reduces p.Ap itself: breakdown test, alpha; x, r; r.r and r.z
+// partials) -> k_pcg_pupdate (every workgroup reduces r.r, r.z: monitor
+// decision, beta; p).  Workgroup 0 alone writes the state; scalars another
+// workgroup of the same kernel still reads (r.z) are kept per iteration
+// parity.  Partial regions: A = part[0, G) (p.Ap), B = part[G, 3G) (r.r, r.z).
+
+// r = b - A x; z = M^-1 r; p = z; partials r.r and r.z (region B).
 __global__ __launch_bounds__(kBlock) void k_pcg_init(int n, int nslices, const SliceInfo *__restrict__ sl,
                                                       const int32_t *__restrict__ col,
                                                       const double *__restrict__ val,
@@ -69,24 +81,24 @@ __global__ __launch_bounds__(kBlock) void k_pcg_init(int n, int nslices, const S
       acc[1] += ri * zi;
     }
   })
-  store_partials<2>(acc, part, lds);
+  store_partials<2>(acc, part + gridDim.x, lds);
 }
 
 __global__ __launch_bounds__(kBlock) void k_pcg_init_fin(const double *__restrict__ part, int G,
                                                           KState *st, double *__restrict__ hist) {
   __shared__ double lds[kWavesPerBlock];
-  const double RR = sum_partials(part, G, lds);
-  const double RZ = sum_partials(part + G, G, lds);
+  const double RR = reduce_partials(part + G, G, lds);
+  const double RZ = reduce_partials(part + 2 * G, G, lds);
   if (threadIdx.x == 0) {
     if (monitor_update(st, hist, sqrt(RR), 0))
       finish(st, PFM_SOLVE_CONVERGED);
     else if (st->max_iter < 1)
       finish(st, PFM_SOLVE_MAXITER);
-    st->rz = RZ;
+    st->rzs[0] = RZ;
   }
 }
 
-// Ap = A p; partial p.Ap.
+// Ap = A p; partial p.Ap (region A).
 __global__ __launch_bounds__(kBlock) void k_pcg_spmv(int n, int nslices, const SliceInfo *__restrict__ sl,
                                                       const int32_t *__restrict__ col,
                                                       const double *__restrict__ val,
@@ -110,30 +122,24 @@ __global__ __launch_bounds__(kBlock) void k_pcg_spmv(int n, int nslices, const S
   store_partials<1>(acc, part, lds);
 }
 
-// alpha = rz / pAp, breakdown if |pAp| < 1e-30.
-__global__ __launch_bounds__(kBlock) void k_pcg_alpha(const double *__restrict__ part, int G,
-                                                       KState *st) {
-  __shared__ double lds[kWavesPerBlock];
-  if (st->done) return;
-  const double PAP = sum_partials(part, G, lds);
-  if (threadIdx.x == 0) {
-    if (fabs(PAP) < 1e-30)
-      finish(st, PFM_SOLVE_BREAKDOWN);
-    else
-      st->alpha = st->rz / PAP;
-  }
-}
-
-// x += alpha p; r -= alpha Ap; partials r.r and r.z.
-__global__ __launch_bounds__(kBlock) void k_pcg_update(int n, int nslices, const double *__restrict__ rD,
+// alpha = rz / pAp (breakdown if |pAp| < 1e-30); x += alpha p; r -= alpha Ap;
+// partials r.r and r.z (region B).
+__global__ __launch_bounds__(kBlock) void k_pcg_update(int n, int nslices, int it,
+                                                        const double *__restrict__ rD,
                                                         const double *__restrict__ p,
                                                         const double *__restrict__ Ap,
                                                         double *__restrict__ x,
                                                         double *__restrict__ r,
-                                                        double *__restrict__ part, const KState *st) {
+                                                        double *__restrict__ part, KState *st) {
   __shared__ double lds[kWavesPerBlock];
   if (st->done) return;
-  const double alpha = st->alpha;
+  const int G = gridDim.x;
+  const double PAP = reduce_partials(part, G, lds);
+  if (fabs(PAP) < 1e-30) {
+    if (blockIdx.x == 0 && threadIdx.x == 0) finish(st, PFM_SOLVE_BREAKDOWN);
+    return;
+  }
+  const double alpha = st->rzs[(it - 1) & 1] / PAP;
   double acc[2] = {0.0, 0.0};
   FOR_ROWS({
     if (row < n) {
@@ -145,35 +151,35 @@ __global__ __launch_bounds__(kBlock) void k_pcg_update(int n, int nslices, const
       acc[1] += ri * zi;
     }
   })
-  store_partials<2>(acc, part, lds);
+  store_partials<2>(acc, part + G, lds);
 }
 
-// monitor(||r||, i); beta = r.z' / r.z.
-__global__ __launch_bounds__(kBlock) void k_pcg_beta(const double *__restrict__ part, int G,
-                                                      KState *st, double *__restrict__ hist) {
+// monitor(||r||, i); beta = r.z' / r.z; p = M^-1 r + beta p.
+__global__ __launch_bounds__(kBlock) void k_pcg_pupdate(int n, int nslices, int it,
+                                                         const double *__restrict__ rD,
+                                                         const double *__restrict__ r,
+                                                         double *__restrict__ p,
+                                                         const double *__restrict__ part, KState *st,
+                                                         double *__restrict__ hist) {
   __shared__ double lds[kWavesPerBlock];
   if (st->done) return;
-  const double RR = sum_partials(part, G, lds);
-  const double RZ = sum_partials(part + G, G, lds);
-  if (threadIdx.x == 0) {
-    const int it = st->iter + 1;
+  const int G = gridDim.x;
+  const double RR = reduce_partials(part + G, G, lds);
+  const double RZ = reduce_partials(part + 2 * G, G, lds);
+  const double res = sqrt(RR);
+  const bool conv = monitor_decision(st, res, it);
+  const double rz_old = st->rzs[(it - 1) & 1];
+  if (blockIdx.x == 0 && threadIdx.x == 0) {
+    monitor_update(st, hist, res, it);
     st->iter = it;
-    if (monitor_update(st, hist, sqrt(RR), it)) {
+    st->rzs[it & 1] = RZ;
+    if (conv)
       finish(st, PFM_SOLVE_CONVERGED);
-    } else {
-      st->beta = RZ / st->rz;
-      st->rz = RZ;
-      if (it >= st->max_iter) finish(st, PFM_SOLVE_MAXITER);
-    }
+    else if (it >= st->max_iter)
+      finish(st, PFM_SOLVE_MAXITER);
   }
-}
-
-// p = M^-1 r + beta p
-__global__ __launch_bounds__(kBlock) void k_pcg_pupdate(int n, int nslices, const double *__restrict__ rD,
-                                                         const double *__restrict__ r,
-                                                         double *__restrict__ p, const KState *st) {
-  if (st->done) return;
-  const double beta = st->beta;
+  if (conv || it >= st->max_iter) return;
+  const double beta = RZ / rz_old;
   FOR_ROWS({
     if (row < n) {
       const double ri = r[row];
@@ -460,7 +466,7 @@ size_t work_layout(const pfm_ldu_s *h, int max_iter, int nvec, Work *w, char *ba
   };
   Work tmp;
   tmp.st = (KState *)take(sizeof(KState));
-  tmp.part = (double *)take(sizeof(double) * 2 * (size_t)G);
+  tmp.part = (double *)take(sizeof(double) * 3 * (size_t)G);
   tmp.hist = (double *)take(sizeof(double) * (2 * (size_t)std::max(max_iter, 0) + 2));
   for (int k = 0; k < nvec; ++k) tmp.vec[k] = (double *)take(sizeof(double) * (size_t)h->n);
   if (w) *w = tmp;
@@ -494,7 +500,7 @@ int drive(pfm_ldu h, const KState *st_dev, int max_iter, hipStream_t s, Body bod
     bool queued = false;
     if (enq < max_iter) {
       const int c = std::min(chunk, max_iter - enq);
-      for (int k = 0; k < c; ++k) body();
+      for (int k = 0; k < c; ++k) body(enq + k + 1);
       enq += c;
       chunk = std::min(chunk * 2, 32);
       if (hipMemcpyAsync(&flag[slot], &st_dev->done, sizeof(int32_t), hipMemcpyDeviceToHost, s) !=
@@ -579,13 +585,11 @@ int pfm_pcg_solve(pfm_ldu h, const pfm_solve_params *prm, const double *diag, co
                                   w.part);
   k_pcg_init_fin<<<1, kBlock, 0, s>>>(w.part, G, w.st, w.hist);
   PFM_CHECK_HIP(hipGetLastError());
-  rc = drive(h, w.st, prm->max_iter, s, [&]() {
+  rc = drive(h, w.st, prm->max_iter, s, [&](int it) {
     k_pcg_spmv<<<G, kBlock, 0, s>>>(n, ns, h->d_slices, h->d_col, packed, diag, p, Ap, w.part,
                                     w.st);
-    k_pcg_alpha<<<1, kBlock, 0, s>>>(w.part, G, w.st);
-    k_pcg_update<<<G, kBlock, 0, s>>>(n, ns, pd, p, Ap, x, r, w.part, w.st);
-    k_pcg_beta<<<1, kBlock, 0, s>>>(w.part, G, w.st, w.hist);
-    k_pcg_pupdate<<<G, kBlock, 0, s>>>(n, ns, pd, r, p, w.st);
+    k_pcg_update<<<G, kBlock, 0, s>>>(n, ns, it, pd, p, Ap, x, r, w.part, w.st);
+    k_pcg_pupdate<<<G, kBlock, 0, s>>>(n, ns, it, pd, r, p, w.part, w.st, w.hist);
   });
   if (rc != PFM_OK) return rc;
   PFM_CHECK_HIP(hipGetLastError());
@@ -618,7 +622,7 @@ int pfm_pbicgstab_solve(pfm_ldu h, const pfm_solve_params *prm, const double *di
                                  y, w.part);
   k_bs_init_fin<<<1, kBlock, 0, s>>>(w.part, G, w.st, w.hist);
   PFM_CHECK_HIP(hipGetLastError());
-  rc = drive(h, w.st, prm->max_iter, s, [&]() {
+  rc = drive(h, w.st, prm->max_iter, s, [&](int) {
     k_bs_spmv1<<<G, kBlock, 0, s>>>(n, ns, h->d_slices, h->d_col, packed, diag, y, rt, v, w.part,
                                     w.st);
     k_bs_alpha<<<1, kBlock, 0, s>>>(w.part, G, w.st);

@@ -13,6 +13,7 @@ struct alignas(256) KState {
   double tol, rel_tol;
   double res0, res;
   double rz, alpha, beta;    // PCG
+  double rzs[2];             // PCG r.z by iteration parity (written while read)
   double rho, omega;         // PBiCGStab
   int32_t max_iter, min_iter;
   int32_t iter;              // current outer iteration i
@@ -36,6 +37,35 @@ __device__ inline bool monitor_update(KState *st, double *hist, double res, int 
     }
   }
   return false;
+}
+
+// The convergence decision of ResidualMonitor.update alone (no side effects),
+// so that every workgroup of a kernel can take it; st->res0 must be set.
+__device__ __forceinline__ bool monitor_decision(const KState *st, double res, int it) {
+  if (it >= st->min_iter) {
+    if (res < st->tol) return true;
+    if (st->res0 > 0.0) {
+      if (res / st->res0 < st->rel_tol) return true;
+    }
+  }
+  return false;
+}
+
+// Partials of at most kMaxRedBlocks workgroups summed in a fixed order by
+// every workgroup that needs the scalar: all loads issued before the adds.
+__device__ __forceinline__ double reduce_partials(const double *__restrict__ part, int np,
+                                                  double *lds) {
+  constexpr int K = kMaxRedBlocks / kBlock;
+  double v[K];
+#pragma unroll
+  for (int k = 0; k < K; ++k) {
+    const int i = threadIdx.x + k * kBlock;
+    v[k] = i < np ? part[i] : 0.0;
+  }
+  double s = 0.0;
+#pragma unroll
+  for (int k = 0; k < K; ++k) s += v[k];
+  return block_sum(s, lds);
 }
 
 __device__ __forceinline__ void finish(KState *st, int status) {

@@ -124,6 +124,7 @@ def test_pbicgstab_matches_oracle(gpu_device, n, jit, sig, pre):
     hb = np.array(info.residual_history[:k])
     spread = np.max([np.abs(np.array(r[3].residual_history[:k]) / hb - 1) for r in runs.values()],
                     axis=0)
+    spread = np.maximum.accumulate(spread)  # a divergence, once started, only grows
     assert np.all(np.abs(h[:k] / hb - 1) <= np.maximum(10 * spread, 1e-9)), (h[:k], hb, spread)
     if info.converged:
         assert lo_it - 1 <= iters <= hi_it + 1, (iters, lo_it, hi_it)

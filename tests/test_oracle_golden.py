@@ -117,6 +117,7 @@ def test_krylov_golden(sysg, kind):
     hb = np.array(info.residual_history[:k])
     spread = np.max([np.abs(np.array(r[3].residual_history[:k]) / hb - 1) for r in runs.values()],
                     axis=0)
+    spread = np.maximum.accumulate(spread)  # a divergence, once started, only grows
     assert np.all(np.abs(ref_h[:k] / hb - 1) <= np.maximum(10 * spread, 1e-12))
     its = [r[1] for r in runs.values()]
     if info.converged:
