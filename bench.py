@@ -93,6 +93,8 @@ def main() -> int:
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-adjoint", action="store_true")
     ap.add_argument("--max-iter", type=int, default=200000)
+    ap.add_argument("--no-mixed", action="store_true")
+    ap.add_argument("--inner-rel-tol", type=float, default=1e-3)
     ap.add_argument("--jitter", type=float, default=0.2)
     ap.add_argument("--sigma", type=float, default=1.0)
     ap.add_argument("--history", default=None, help="write the residual history (.npy) here")
@@ -230,6 +232,27 @@ def main() -> int:
         log(f"adjoint: forward {tf[-1] * 1e3:.1f} ms, backward {tb[-1] * 1e3:.1f} ms, "
             f"ratio {tb[-1] / tf[-1]:.3f}")
 
+    # --- mixed precision: FP32 inner PCG in FP64 iterative refinement -------
+    mixed = None
+    if not args.no_mixed:
+        msolver = PCGSolver(tolerance=1e-300, rel_tol=1e-8, max_iter=max_iter, precision="mixed",
+                            inner_rel_tol=args.inner_rel_tol)
+        mat.solve(msolver, x0, 1e-300, max_iter)  # warm (packs the FP32 copy)
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        xm, outer, _ = mat.solve(msolver, x0, 1e-300, max_iter)
+        torch.cuda.synchronize()
+        mms = (time.perf_counter() - t0) * 1e3
+        mi = msolver.last_info
+        err = ((xm - x).norm() / x.norm()).item()
+        rt = (mat.source - mat.Ax(xm)).norm().item() / mat.source.norm().item()
+        mixed = {"ms": round(mms, 2), "speedup_vs_fp64": round(ms / mms, 3),
+                 "inner_iterations": mi.inner_iterations, "refinement_steps": outer - 1,
+                 "converged": bool(mi.converged), "true_rel_residual": rt,
+                 "rel_err_vs_fp64_solution": err, "inner_rel_tol": args.inner_rel_tol}
+        log(f"mixed: {mms:.1f} ms ({ms / mms:.2f}x), inner iters {mi.inner_iterations}, "
+            f"steps {outer - 1}, true rel residual {rt:.3e}, err vs fp64 {err:.2e}")
+
     # --- CPU baseline ---------------------------------------------------------
     cpu = None
     if not args.no_cpu:
@@ -267,6 +290,7 @@ def main() -> int:
         "spmv_us": round(spmv_us, 2),
         "spmv_gbs": round(spmv_gbs, 1),
         "adjoint_over_forward": adjoint["ratio"] if adjoint else None,
+        "mixed_precision": mixed,
         "adjoint": adjoint,
         "roofline": {"bound": "hbm", "kernel": "k_spmv<double> (pfm_spmv)",
                      "achieved": round(spmv_gbs, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",

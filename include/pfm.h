@@ -190,6 +190,21 @@ int pfm_pbicgstab_solve(pfm_ldu h, const pfm_solve_params *prm,
                         void *work, double *history_h, pfm_solve_result *res,
                         void *stream);
 
+/* Mixed precision (extension of PCGSolver, solvers/pcg.py:97-187): FP32
+ * coefficients (pfm_ldu_pack_f32), FP32 SpMV and FP32 diagonal DIC apply in the
+ * inner PCG loop (every dot product and norm accumulated in FP64), inside FP64
+ * iterative refinement: r = b - A x in FP64, inner solve to
+ * ||r_in|| < inner_rel_tol*||r||, x += e.  The reference's stopping rule is
+ * applied to the FP64 true residual after each refinement step; prm->max_iter
+ * bounds the total inner iterations; history_h receives the true residual
+ * norms (result.iterations of them); *inner_iterations the inner total. */
+int64_t pfm_pcg_mixed_workspace_bytes(pfm_ldu h, int32_t max_outer);
+int pfm_pcg_mixed_solve(pfm_ldu h, const pfm_solve_params *prm, double inner_rel_tol,
+                        int32_t max_outer, const double *diag, const double *packed,
+                        const float *packed32, const double *rD, const double *b, double *x,
+                        void *work, double *history_h, pfm_solve_result *res,
+                        int32_t *inner_iterations, void *stream);
+
 /* ------------------------------------------------------------------ */
 /* GAMG                                                               */
 /* ------------------------------------------------------------------ */

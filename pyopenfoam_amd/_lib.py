@@ -87,6 +87,9 @@ _SIGS = {
     "pfm_pcg_solve": (INT, [P, C.POINTER(SolveParams), P, P, P, P, P, P, P, C.POINTER(SolveResult), P]),
     "pfm_pbicgstab_solve": (INT, [P, C.POINTER(SolveParams), P, P, P, P, P, P, P,
                                   C.POINTER(SolveResult), P]),
+    "pfm_pcg_mixed_workspace_bytes": (I64, [P, I32]),
+    "pfm_pcg_mixed_solve": (INT, [P, C.POINTER(SolveParams), C.c_double, I32, P, P, P, P, P, P, P,
+                                  P, C.POINTER(SolveResult), P, P]),
     "pfm_gamg_create": (INT, [P, C.POINTER(GamgParams), C.POINTER(P)]),
     "pfm_gamg_destroy": (INT, [P]),
     "pfm_gamg_num_levels": (INT, [P]),

@@ -132,6 +132,23 @@ class LduMatrix:
         self._packed[bool(transpose)] = (key, buf)
         return buf
 
+    def packed32(self, transpose: bool = False) -> torch.Tensor:
+        """FP32 copy of the packed coefficients (mixed-precision solves, cached)."""
+        self._check_compute()
+        lo, up = self._lower.detach(), self._upper.detach()
+        key = ("f32", bool(transpose), lo.data_ptr(), up.data_ptr(), lo._version, up._version)
+        hit = self._packed.get(("f32", bool(transpose)))
+        if hit is not None and hit[0] == key:
+            return hit[1]
+        addr = self.addressing
+        buf = torch.empty(max(addr.n_slots, 1), device=self._device, dtype=torch.float32)
+        _lib.check(_lib.lib().pfm_ldu_pack_f32(addr.handle, int(transpose),
+                                               _lib.ptr(lo.contiguous()), _lib.ptr(up.contiguous()),
+                                               _lib.ptr(buf), current_stream(self._device)),
+                   "pfm_ldu_pack_f32")
+        self._packed[("f32", bool(transpose))] = (key, buf)
+        return buf
+
     # ------------------------------------------------------------ products
     def _check_x(self, x: torch.Tensor) -> torch.Tensor:
         assert_floating(x, "x")

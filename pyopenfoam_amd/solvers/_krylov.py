@@ -65,3 +65,37 @@ def device_solve(kind: str, matrix: LduMatrix, source: torch.Tensor, x0: torch.T
     info = monitor.build_info(converged=bool(res.converged))
     info.status = int(res.status)
     return x, info
+
+
+def device_solve_mixed(matrix: LduMatrix, source: torch.Tensor, x0: torch.Tensor,
+                       monitor: ResidualMonitor, max_iter: int, rD: torch.Tensor | None,
+                       workspace: KrylovWorkspace, inner_rel_tol: float, max_outer: int,
+                       *, transpose: bool = False) -> tuple[torch.Tensor, ConvergenceInfo]:
+    """FP32-inner / FP64-outer PCG (pfm_pcg_mixed_solve)."""
+    dev, dt = matrix.device, matrix.dtype
+    require_gpu(matrix.diag)
+    b = source.to(device=dev, dtype=dt).contiguous()
+    if b.shape[0] != matrix.n_cells:
+        raise ValueError(f"source has {b.shape[0]} elements, expected {matrix.n_cells}")
+    x = x0.to(device=dev, dtype=dt).clone().contiguous()
+    L = _lib.lib()
+    addr = matrix.addressing
+    pk = matrix.packed(transpose)
+    pk32 = matrix.packed32(transpose)
+    work = workspace.get(int(L.pfm_pcg_mixed_workspace_bytes(addr.handle, int(max_outer))), dev)
+    prm = _lib.SolveParams(float(monitor.tolerance), float(monitor.rel_tol), int(max_iter),
+                           int(monitor.min_iter),
+                           _lib.PFM_PRECOND_DIAG if rD is not None else _lib.PFM_PRECOND_NONE, 0)
+    res = _lib.SolveResult()
+    hist = np.zeros(int(max_outer) + 2)
+    inner = C.c_int32()
+    _lib.check(L.pfm_pcg_mixed_solve(addr.handle, C.byref(prm), float(inner_rel_tol),
+                                     int(max_outer), _lib.ptr(matrix.diag.detach().contiguous()),
+                                     _lib.ptr(pk), _lib.ptr(pk32), _lib.ptr(rD), _lib.ptr(b),
+                                     _lib.ptr(x), _lib.ptr(work), hist.ctypes.data, C.byref(res),
+                                     C.byref(inner), current_stream(dev)), "pfm_pcg_mixed_solve")
+    monitor.load_history(hist[: min(res.iterations, len(hist))].tolist())
+    info = monitor.build_info(converged=bool(res.converged))
+    info.status = int(res.status)
+    info.inner_iterations = int(inner.value)
+    return x, info

@@ -131,3 +131,30 @@ def test_pbicgstab_matches_oracle(gpu_device, n, jit, sig, pre):
         xg = x.cpu().numpy()
         spread = max(np.linalg.norm(r[0] - xo) / np.linalg.norm(xo) for r in runs.values())
         assert np.linalg.norm(xg - xo) / np.linalg.norm(xo) <= max(10 * spread, 1e-9)
+
+
+@pytest.mark.parametrize("n,jit,sig", [(16, 0.0, 0.0), (32, 0.0, 1.0), (32, 0.2, 1.0)])
+def test_mixed_precision_pcg(gpu_device, n, jit, sig):
+    """FP32 inner PCG in FP64 iterative refinement reaches the FP64 stopping
+    rule on the true residual; its solution is within 1e-6 of the oracle's
+    FP64 PCG solution (both at relative residual 1e-8)."""
+    c, d, lo, up = system(n, jit, sig)
+    print(f"mixed n={n} jitter={jit} sigma={sig}", flush=True)
+    m = _matrix(c, d, lo, up, gpu_device)
+    solver = PCGSolver(tolerance=1e-300, rel_tol=1e-8, max_iter=50000, precision="mixed",
+                       inner_rel_tol=1e-3)
+    x, it, res = solver(m, torch.from_numpy(c.b), torch.zeros(c.n_cells, dtype=torch.float64))
+    info = solver.last_info
+    assert info.converged
+    mv = lambda v: oldu.ax(d, lo, up, c.owner, c.neighbour, v)
+    xg = x.cpu().numpy()
+    true_rel = np.linalg.norm(c.b - mv(xg)) / np.linalg.norm(c.b)
+    assert true_rel < 1e-8
+    np.testing.assert_allclose(info.residual_history[-1] / info.residual_history[0], true_rel,
+                               rtol=1e-6)
+    rD = opre.dic_build(d, up, c.owner, c.neighbour)
+    xo, io, _, _ = okry.pcg(mv, lambda r: r / rD, c.b, np.zeros(c.n_cells), tolerance=1e-300,
+                            rel_tol=1e-8, max_iter=50000)
+    err = np.linalg.norm(xg - xo) / np.linalg.norm(xo)
+    print(f"  inner iterations {info.inner_iterations} (fp64 {io}), outer {it}, err {err:.2e}")
+    assert err < 1e-6
