@@ -83,6 +83,130 @@ def cpu_baseline(case, d, lo, up, gpu_iters: int, budget_s: float):
                       f"iterations; BLAS limited to 1 thread"}
 
 
+def bench_batch(args, dev):
+    """512 (default) jittered systems, side n ~ U{8..48}: batched PCG-DIC to
+    rel 1e-8 (graph-replayed), the batched adjoint of J = sum(w*x), and an
+    estimate of the same systems solved one at a time with the single-system
+    device PCG (a size-stratified sample, scaled to the whole batch)."""
+    import numpy as np
+    import torch
+
+    from pyopenfoam_amd import LduMatrix, PCGSolver
+    from pyopenfoam_amd.cases import poisson_batch
+    from pyopenfoam_amd.discretisation.fvm import LaplacianAssemble
+    from pyopenfoam_amd.differentiable.linear_solver import _Ctx
+    from pyopenfoam_amd.solvers.batch import BatchedLdu, BatchedPCGSolver, batch_poisson_solve
+
+    t0 = time.perf_counter()
+    bc = poisson_batch(args.batch_systems, args.batch_nmin, args.batch_nmax, seed=0,
+                       jitter=args.jitter, sigma=args.sigma, workers=min(16, os.cpu_count() or 1))
+    t_gen = time.perf_counter() - t0
+    ns, N, F = bc.n_systems, bc.n_cells, int(bc.face_offsets[-1])
+    log(f"batch: {ns} systems, {N} cells, {F} faces, built in {t_gen:.1f} s")
+    t = lambda a: torch.from_numpy(np.ascontiguousarray(a)).to(dev)
+    t0 = time.perf_counter()
+    B = BatchedLdu(bc.cell_offsets, bc.face_offsets, bc.owner_local, bc.neighbour_local, dev,
+                   global_owner=bc.owner, global_neighbour=bc.neighbour)
+    torch.cuda.synchronize()
+    t_create = time.perf_counter() - t0
+    log(f"batch handle + autotune {t_create:.1f} s: tile {B.tile_slices} slices, {B.n_tiles} tiles, "
+        f"us/iter {B.autotune_us}")
+    geo = (t(bc.mag), t(bc.delta), t(bc.vol), t(bc.dir_cells.astype(np.int32)), t(bc.dir_mag),
+           t(bc.dir_delta))
+    gam, gamb, b = t(bc.gamma), t(bc.dir_gamma), t(bc.b)
+    d, lo, up = LaplacianAssemble.forward(_Ctx(), gam, gamb, B.global_addressing, *geo)
+    max_iter = args.batch_max_iter
+    solver = BatchedPCGSolver(tolerance=1e-300, rel_tol=1e-8, max_iter=max_iter)
+    x, res = solver.solve(B, d, lo, up, b)  # warm
+    torch.cuda.synchronize()
+    reps = 2
+    t0 = time.perf_counter()
+    for _ in range(reps):
+        x, res = solver.solve(B, d, lo, up, b)
+    torch.cuda.synchronize()
+    ms = (time.perf_counter() - t0) / reps * 1e3
+    # true relative residual per system
+    own, nei = B.global_addressing.owner32.long(), B.global_addressing.neighbour32.long()
+    ax = d * x
+    ax.index_add_(0, own, lo * x[nei])
+    ax.index_add_(0, nei, up * x[own])
+    seg = torch.repeat_interleave(torch.arange(ns, device=dev),
+                                  t(np.diff(bc.cell_offsets)).long())
+    r2 = torch.zeros(ns, dtype=torch.float64, device=dev).index_add_(0, seg, (b - ax) ** 2)
+    b2 = torch.zeros(ns, dtype=torch.float64, device=dev).index_add_(0, seg, b * b)
+    true_rel = (r2.sqrt() / b2.sqrt()).max().item()
+    it = res.iterations
+    rel_k = (r2.sqrt() / b2.sqrt()).cpu().numpy()
+    bad = np.flatnonzero(~res.converged)
+    log(f"batch solve: {ms:.1f} ms for {ns} systems, iterations max {it.max()} mean {it.mean():.0f}, "
+        f"converged {ns - len(bad)}/{ns}, max true rel residual {true_rel:.2e}")
+    for k in bad[:20]:
+        log(f"  not converged: system {k} n={bc.sizes[k]} iterations {it[k]} status {res.status[k]} "
+            f"final {res.final_residual[k]:.3e} initial {res.initial_residual[k]:.3e} "
+            f"true rel {rel_k[k]:.3e}")
+    conv = res.converged
+    p50, p90, p99 = (np.percentile(it[conv], q) for q in (50, 90, 99)) if conv.any() else (0, 0, 0)
+    log(f"  iterations of converged systems: p50 {p50:.0f} p90 {p90:.0f} p99 {p99:.0f} "
+        f"max {it[conv].max() if conv.any() else 0}")
+    # adjoint: J = sum over systems of sum(w*x)
+    g = gam.clone().requires_grad_(True)
+    gb = gamb.clone().requires_grad_(True)
+    w = t(np.random.default_rng(1).standard_normal(N))
+    asolver = BatchedPCGSolver(tolerance=1e-300, rel_tol=1e-8, max_iter=max_iter)
+    tf = tb = 0.0
+    for rep in range(2):
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        xs = batch_poisson_solve(g, gb, b, asolver, B, geo)
+        J = (w * xs).sum()
+        torch.cuda.synchronize()
+        t1 = time.perf_counter()
+        J.backward()
+        torch.cuda.synchronize()
+        tf, tb = t1 - t0, time.perf_counter() - t1
+        g.grad = None
+        gb.grad = None
+    log(f"batch adjoint: forward {tf * 1e3:.1f} ms, backward {tb * 1e3:.1f} ms, ratio {tb / tf:.3f}")
+    # the same systems one at a time (size-stratified sample)
+    order = np.argsort(bc.sizes, kind="stable")
+    k_s = max(1, min(args.batch_seq_sample, ns))
+    sample = order[np.linspace(0, ns - 1, k_s).round().astype(int)]
+    ssolver = PCGSolver(tolerance=1e-300, rel_tol=1e-8, max_iter=max_iter)
+    seq_s = 0.0
+    for k in sample:
+        c0, c1 = int(bc.cell_offsets[k]), int(bc.cell_offsets[k + 1])
+        f0, f1 = int(bc.face_offsets[k]), int(bc.face_offsets[k + 1])
+        m = LduMatrix(c1 - c0, torch.from_numpy(bc.owner_local[f0:f1]),
+                      torch.from_numpy(bc.neighbour_local[f0:f1]), device=dev)
+        m.diag, m.lower, m.upper = d[c0:c1], lo[f0:f1], up[f0:f1]
+        _ = m.addressing
+        xk = torch.zeros(c1 - c0, dtype=torch.float64, device=dev)
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        ssolver(m, b[c0:c1], xk)
+        torch.cuda.synchronize()
+        seq_s += time.perf_counter() - t0
+    seq_ms = seq_s * 1e3 * ns / k_s
+    log(f"one at a time: {seq_s * 1e3:.1f} ms for {k_s} sampled systems -> ~{seq_ms:.0f} ms "
+        f"for {ns} ({seq_ms / ms:.1f}x the batched solve)")
+    return {"n_systems": ns, "n_cells": N, "n_internal_faces": F,
+            "sizes": f"n ~ U{{{args.batch_nmin}..{args.batch_nmax}}} seed 0",
+            "ms": round(ms, 2), "ms_per_system": round(ms / ns, 4), "max_iter": max_iter,
+            "iterations_max": int(it.max()), "iterations_mean": round(float(it.mean()), 1),
+            "all_converged": bool(res.converged.all()), "n_converged": int(conv.sum()),
+            "not_converged": [{"system": int(k), "n": int(bc.sizes[k]), "iterations": int(it[k]),
+                               "status": int(res.status[k]),
+                               "true_rel_residual": float(rel_k[k])} for k in bad[:20]],
+            "max_true_rel_residual": true_rel,
+            "tile_slices": B.tile_slices, "n_tiles": B.n_tiles,
+            "autotune_us_per_iter": {str(k): round(v, 2) for k, v in B.autotune_us.items()},
+            "adjoint": {"forward_ms": round(tf * 1e3, 2), "backward_ms": round(tb * 1e3, 2),
+                        "ratio": round(tb / tf, 3)},
+            "one_at_a_time_ms_est": round(seq_ms, 1), "one_at_a_time_sample": int(k_s),
+            "speedup_vs_one_at_a_time": round(seq_ms / ms, 2),
+            "setup_s": {"cases_host": round(t_gen, 1), "handle_autotune": round(t_create, 1)}}
+
+
 def main() -> int:
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -98,6 +222,13 @@ def main() -> int:
     ap.add_argument("--jitter", type=float, default=0.2)
     ap.add_argument("--sigma", type=float, default=1.0)
     ap.add_argument("--history", default=None, help="write the residual history (.npy) here")
+    ap.add_argument("--no-batch", action="store_true")
+    ap.add_argument("--batch-systems", type=int, default=512)
+    ap.add_argument("--batch-nmin", type=int, default=8)
+    ap.add_argument("--batch-nmax", type=int, default=48)
+    ap.add_argument("--batch-max-iter", type=int, default=30000)
+    ap.add_argument("--batch-seq-sample", type=int, default=32,
+                    help="systems solved one at a time to estimate the unbatched loop")
     ap.add_argument("--copy-probe-mib", type=int, default=0,
                     help="also run a plain device copy of this size (HBM counter calibration)")
     args = ap.parse_args()
@@ -253,6 +384,11 @@ def main() -> int:
         log(f"mixed: {mms:.1f} ms ({ms / mms:.2f}x), inner iters {mi.inner_iterations}, "
             f"steps {outer - 1}, true rel residual {rt:.3e}, err vs fp64 {err:.2e}")
 
+    # --- batching: many uneven systems in one graph-replayed PCG-DIC ---------
+    batch = None
+    if not args.no_batch:
+        batch = bench_batch(args, dev)
+
     # --- CPU baseline ---------------------------------------------------------
     cpu = None
     if not args.no_cpu:
@@ -292,6 +428,7 @@ def main() -> int:
         "adjoint_over_forward": adjoint["ratio"] if adjoint else None,
         "mixed_precision": mixed,
         "adjoint": adjoint,
+        "batch": batch,
         "roofline": {"bound": "hbm", "kernel": "k_spmv<double> (pfm_spmv)",
                      "achieved": round(spmv_gbs, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": round(spmv_gbs / HBM_PEAK_GBS, 4),

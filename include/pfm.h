@@ -206,6 +206,40 @@ int pfm_pcg_mixed_solve(pfm_ldu h, const pfm_solve_params *prm, double inner_rel
                         int32_t *inner_iterations, void *stream);
 
 /* ------------------------------------------------------------------ */
+/* Batched PCG                                                        */
+/* ------------------------------------------------------------------ */
+
+typedef struct pfm_batch_s *pfm_batch;
+
+/* Many independent LDU systems (same solver, uneven sizes) as one batch.
+ * System k has cells [cell_offsets[k], cell_offsets[k+1]) and faces
+ * [face_offsets[k], face_offsets[k+1]) of the concatenated arrays; owner_h /
+ * neighbour_h hold each system's LOCAL cell indices (0-based per system).
+ * tile_slices = 0 autotunes the work tile (slices of 64 rows per workgroup) by
+ * timing whole batched PCG iterations on this batch; no reference counterpart
+ * (the reference solves systems one at a time through PCGSolver). */
+int pfm_batch_create(int32_t n_systems, const int32_t *cell_offsets_h,
+                     const int32_t *face_offsets_h, const int32_t *owner_h,
+                     const int32_t *neighbour_h, int32_t tile_slices, pfm_batch *out);
+int pfm_batch_destroy(pfm_batch b);
+/* n_padded rows, tile count, chosen tile size, autotuner timings (us per
+ * iteration for tile sizes 1, 2, 4, 8, 16, 32; 0 if not tuned). */
+int pfm_batch_info(pfm_batch b, int32_t *n_padded, int32_t *n_tiles, int32_t *tile_slices,
+                   float *tuned_us_h);
+int64_t pfm_batch_workspace_bytes(pfm_batch b);
+/* PCG (solvers/pcg.py:97-187, reference diagonal DIC) on every system of the
+ * batch at once, each with its own monitor (residual.py:112-145) and stop.
+ * Arrays are concatenated in the caller's (unpadded) layout; transpose = 1
+ * solves A^T (the adjoint).  graph_chunk > 0 captures that many iterations in
+ * a hipGraph and replays it; 0 launches eagerly.  Per-system iterations,
+ * status and residuals go to the host arrays (any may be NULL). */
+int pfm_batch_pcg_solve(pfm_batch b, const pfm_solve_params *prm, int transpose,
+                        const double *diag, const double *lower, const double *upper,
+                        const double *rhs, double *x, void *work, int32_t graph_chunk,
+                        int32_t *iterations_h, int32_t *status_h, double *initial_res_h,
+                        double *final_res_h, void *stream);
+
+/* ------------------------------------------------------------------ */
 /* GAMG                                                               */
 /* ------------------------------------------------------------------ */
 

@@ -90,6 +90,12 @@ _SIGS = {
     "pfm_pcg_mixed_workspace_bytes": (I64, [P, I32]),
     "pfm_pcg_mixed_solve": (INT, [P, C.POINTER(SolveParams), C.c_double, I32, P, P, P, P, P, P, P,
                                   P, C.POINTER(SolveResult), P, P]),
+    "pfm_batch_create": (INT, [I32, P, P, P, P, I32, C.POINTER(P)]),
+    "pfm_batch_destroy": (INT, [P]),
+    "pfm_batch_info": (INT, [P, P, P, P, P]),
+    "pfm_batch_workspace_bytes": (I64, [P]),
+    "pfm_batch_pcg_solve": (INT, [P, C.POINTER(SolveParams), INT, P, P, P, P, P, P, I32, P, P, P,
+                                  P, P]),
     "pfm_gamg_create": (INT, [P, C.POINTER(GamgParams), C.POINTER(P)]),
     "pfm_gamg_destroy": (INT, [P]),
     "pfm_gamg_num_levels": (INT, [P]),

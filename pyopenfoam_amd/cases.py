@@ -96,3 +96,73 @@ def poisson_case(n: int, *, jitter: float = 0.0, sigma: float = 0.0, seed: int =
         n=n, mesh=mesh, geo=geo, gamma=gamma_all[:nfi].copy(), dir_faces=faces, dir_cells=cells,
         dir_gamma=gamma_all[faces].copy(), dir_mag=geo.mag_sf[faces].copy(),
         dir_delta=dirichlet_delta(mesh, geo, faces), b=b, seed=seed)
+
+
+@dataclass
+class BatchCase:
+    """Many Poisson systems concatenated block-diagonally (global cell indices)."""
+    sizes: np.ndarray          # n per system (cube side)
+    cell_offsets: np.ndarray   # (ns+1,)
+    face_offsets: np.ndarray   # (ns+1,)
+    owner: np.ndarray          # global, (F,)
+    neighbour: np.ndarray
+    owner_local: np.ndarray    # per-system local indices
+    neighbour_local: np.ndarray
+    vol: np.ndarray
+    mag: np.ndarray            # (F,)
+    delta: np.ndarray
+    gamma: np.ndarray
+    dir_cells: np.ndarray      # global, sorted
+    dir_gamma: np.ndarray
+    dir_mag: np.ndarray
+    dir_delta: np.ndarray
+    b: np.ndarray
+    seed: int
+
+    @property
+    def n_systems(self) -> int:
+        return int(self.sizes.shape[0])
+
+    @property
+    def n_cells(self) -> int:
+        return int(self.cell_offsets[-1])
+
+
+def _batch_member(args):
+    n, seed, jitter, sigma = args
+    return poisson_case(n, jitter=jitter, sigma=sigma, seed=seed)
+
+
+def poisson_batch(n_systems: int = 512, n_min: int = 8, n_max: int = 48, *, seed: int = 0,
+                  jitter: float = 0.2, sigma: float = 1.0, workers: int = 8) -> BatchCase:
+    """Batch of independent cube systems: side n_k ~ U{n_min..n_max} from
+    ``default_rng(seed)``; system k is ``poisson_case(n_k, seed=seed*100003 + k)``
+    (jittered, permuted, RCM-renumbered, lognormal diffusivity, zero-mean RHS)."""
+    rng = np.random.default_rng(seed)
+    sizes = rng.integers(n_min, n_max + 1, size=n_systems)
+    jobs = [(int(n), seed * 100003 + k, jitter, sigma) for k, n in enumerate(sizes)]
+    if workers > 1 and n_systems > 1:
+        from concurrent.futures import ProcessPoolExecutor
+        with ProcessPoolExecutor(max_workers=workers) as ex:
+            cases = list(ex.map(_batch_member, jobs, chunksize=4))
+    else:
+        cases = [_batch_member(j) for j in jobs]
+    cell_off = np.zeros(n_systems + 1, np.int64)
+    face_off = np.zeros(n_systems + 1, np.int64)
+    for k, c in enumerate(cases):
+        cell_off[k + 1] = cell_off[k] + c.n_cells
+        face_off[k + 1] = face_off[k] + c.mesh.n_internal_faces
+    cat = np.concatenate
+    nfi = [c.mesh.n_internal_faces for c in cases]
+    return BatchCase(
+        sizes=sizes, cell_offsets=cell_off, face_offsets=face_off,
+        owner=cat([c.owner + cell_off[k] for k, c in enumerate(cases)]),
+        neighbour=cat([c.neighbour + cell_off[k] for k, c in enumerate(cases)]),
+        owner_local=cat([c.owner for c in cases]), neighbour_local=cat([c.neighbour for c in cases]),
+        vol=cat([c.geo.cell_volumes for c in cases]),
+        mag=cat([c.geo.mag_sf[:f] for c, f in zip(cases, nfi)]),
+        delta=cat([c.geo.delta[:f] for c, f in zip(cases, nfi)]),
+        gamma=cat([c.gamma for c in cases]),
+        dir_cells=cat([c.dir_cells + cell_off[k] for k, c in enumerate(cases)]),
+        dir_gamma=cat([c.dir_gamma for c in cases]), dir_mag=cat([c.dir_mag for c in cases]),
+        dir_delta=cat([c.dir_delta for c in cases]), b=cat([c.b for c in cases]), seed=seed)

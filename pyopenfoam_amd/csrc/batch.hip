@@ -1,0 +1,713 @@
+// Batched PCG-DIC: many independent LDU systems of uneven size solved by one
+// sequence of kernels, replayed as a hipGraph.
+//
+// The systems are concatenated block-diagonally; each one is padded to whole
+// row slices (dummy rows: diag 1, b 0, x 0, no faces, so they contribute 0 to
+// every dot product), so every slice belongs to one system.  Work is cut into
+// tiles of `tile_slices` slices of one system; one workgroup per tile.  Each
+// system keeps its own scalars and stopping state (the reference's monitor,
+// residual.py:112-145, per system), reduced in a fixed order over its tiles,
+// so every system's solve is identical to solving it alone except for the
+// summation order of its dot products.  Systems that have stopped skip their
+// tiles.  Iterations are captured into a hipGraph in chunks; the host replays
+// the graph and polls a device counter of finished systems.  The tile size is
+// picked by an autotuner that times whole iterations on the actual batch.
+
+#include <algorithm>
+#include <cmath>
+#include <cstring>
+
+#include "solver_state.h"
+
+using namespace pfm;
+
+struct BSys {
+  double res0, res;
+  double rz[2];
+  int32_t nhist, done, status, pad;
+};
+
+struct alignas(256) BGlobal {
+  double tol, rel_tol;
+  int32_t it;        // completed iterations
+  int32_t ndone;     // systems finished
+  int32_t max_iter, min_iter;
+  int32_t nact;      // entries of the active-tile list
+};
+
+struct pfm_batch_s {
+  int32_t ns = 0, n = 0, nf = 0, np = 0;
+  std::vector<int32_t> cell_off, face_off, pad_off;
+  pfm_ldu ldu = nullptr;  // padded concatenation
+  int32_t ntiles = 0, tile_slices = 0;
+  int32_t *d_tile_sys = nullptr, *d_tile_s0 = nullptr, *d_tile_s1 = nullptr, *d_sys_t0 = nullptr;
+  int32_t *d_cell_map = nullptr;  // unpadded cell -> padded row
+  int32_t *h_flag = nullptr;
+  float tuned_us[8] = {0};
+};
+
+namespace {
+
+inline int grid1d(int64_t n) { return (int)((n + kBlock - 1) / kBlock); }
+
+template <typename T>
+int upload(T **dst, const std::vector<T> &src) {
+  *dst = nullptr;
+  if (src.empty()) return PFM_OK;
+  PFM_CHECK_HIP(hipMalloc((void **)dst, src.size() * sizeof(T)));
+  PFM_CHECK_HIP(hipMemcpy(*dst, src.data(), src.size() * sizeof(T), hipMemcpyHostToDevice));
+  return PFM_OK;
+}
+
+#define TILE_ROWS(...)                                                         \
+  {                                                                             \
+    const int lane = threadIdx.x & 63;                                          \
+    for (int s = s0 + (threadIdx.x >> 6); s < s1; s += kWavesPerBlock) {        \
+      const int row = s * kSlice + lane;                                        \
+      __VA_ARGS__                                                               \
+    }                                                                           \
+  }
+
+// Sum of this system's tile partials (fixed tile order), on every thread.
+__device__ __forceinline__ double sys_sum(const double *__restrict__ part, int t0, int t1,
+                                          double *lds) {
+  double s = 0.0;
+  for (int t = t0 + (int)threadIdx.x; t < t1; t += kBlock) s += part[t];
+  return block_sum(s, lds);
+}
+
+__global__ __launch_bounds__(kBlock) void k_b_fill(int np, double *__restrict__ d,
+                                                    double *__restrict__ x, double *__restrict__ b) {
+  const int i = blockIdx.x * kBlock + threadIdx.x;
+  if (i >= np) return;
+  d[i] = 1.0;
+  x[i] = 0.0;
+  b[i] = 0.0;
+}
+
+__global__ __launch_bounds__(kBlock) void k_b_scatter(int n, const int32_t *__restrict__ map,
+                                                       const double *__restrict__ d,
+                                                       const double *__restrict__ x,
+                                                       const double *__restrict__ b,
+                                                       double *__restrict__ dp,
+                                                       double *__restrict__ xp,
+                                                       double *__restrict__ bp) {
+  const int i = blockIdx.x * kBlock + threadIdx.x;
+  if (i >= n) return;
+  const int j = map[i];
+  dp[j] = d[i];
+  xp[j] = x[i];
+  bp[j] = b[i];
+}
+
+__global__ __launch_bounds__(kBlock) void k_b_gather(int n, const int32_t *__restrict__ map,
+                                                      const double *__restrict__ xp,
+                                                      double *__restrict__ x) {
+  const int i = blockIdx.x * kBlock + threadIdx.x;
+  if (i < n) x[i] = xp[map[i]];
+}
+
+__global__ void k_b_state_init(BGlobal *g, BSys *sys, int ns, double tol, double rel_tol,
+                               int max_iter, int min_iter) {
+  const int k = blockIdx.x * blockDim.x + threadIdx.x;
+  if (k == 0) {
+    g->tol = tol;
+    g->rel_tol = rel_tol;
+    g->it = 0;
+    g->ndone = 0;
+    g->max_iter = max_iter;
+    g->min_iter = min_iter;
+  }
+  if (k < ns) {
+    BSys z;
+    memset(&z, 0, sizeof(z));
+    sys[k] = z;
+  }
+}
+
+__device__ __forceinline__ bool sys_decision(const BGlobal *g, const BSys &S, double res, int it) {
+  if (it >= g->min_iter) {
+    if (res < g->tol) return true;
+    if (S.res0 > 0.0 && res / S.res0 < g->rel_tol) return true;
+  }
+  return false;
+}
+
+// r = b - A x; z = r/rD; p = z; tile partials r.r, r.z.
+__global__ __launch_bounds__(kBlock) void k_b_init(const int32_t *__restrict__ tsys,
+                                                    const int32_t *__restrict__ ts0,
+                                                    const int32_t *__restrict__ ts1, int ntiles,
+                                                    const SliceInfo *__restrict__ sl,
+                                                    const int32_t *__restrict__ col,
+                                                    const double *__restrict__ val,
+                                                    const double *__restrict__ diag,
+                                                    const double *__restrict__ rD,
+                                                    const double *__restrict__ b,
+                                                    const double *__restrict__ x,
+                                                    double *__restrict__ r, double *__restrict__ p,
+                                                    double *__restrict__ part) {
+  __shared__ double lds[kWavesPerBlock];
+  const int t = blockIdx.x;
+  const int s0 = ts0[t], s1 = ts1[t];
+  double rr = 0.0, rz = 0.0;
+  TILE_ROWS({
+    const SliceInfo si = load_slice(sl, s);
+    const double ax = row_spmv<double>(si, lane, diag[row] * x[row], col, val, x);
+    const double ri = b[row] - ax;
+    const double zi = rD ? ri / rD[row] : ri;
+    r[row] = ri;
+    p[row] = zi;
+    rr += ri * ri;
+    rz += ri * zi;
+  })
+  rr = block_sum(rr, lds);
+  rz = block_sum(rz, lds);
+  if (threadIdx.x == 0) {
+    part[t] = rr;
+    part[ntiles + t] = rz;
+  }
+  (void)tsys;
+}
+
+// One thread per system: initial residual, monitor(0), rz.
+__global__ void k_b_init_fin(int ns, const int32_t *__restrict__ sys_t0, int ntiles,
+                             const double *__restrict__ part, BGlobal *g, BSys *sys) {
+  const int k = blockIdx.x * blockDim.x + threadIdx.x;
+  if (k >= ns) return;
+  double RR = 0.0, RZ = 0.0;
+  for (int t = sys_t0[k]; t < sys_t0[k + 1]; ++t) RR += part[t];
+  for (int t = sys_t0[k]; t < sys_t0[k + 1]; ++t) RZ += part[ntiles + t];
+  BSys S = sys[k];
+  const double res = sqrt(RR);
+  S.res0 = res;
+  S.res = res;
+  S.nhist = 1;
+  S.rz[0] = RZ;
+  if (sys_decision(g, S, res, 0)) {
+    S.done = 1;
+    S.status = PFM_SOLVE_CONVERGED;
+  } else if (g->max_iter < 1) {
+    S.done = 1;
+    S.status = PFM_SOLVE_MAXITER;
+  }
+  sys[k] = S;
+  if (S.done) atomicAdd(&g->ndone, 1);
+}
+
+// Active-tile list: the tiles of every system not yet stopped, in system
+// order.  Rebuilt by one workgroup at the start of each chunk of iterations;
+// tiles of systems that stop inside a chunk skip themselves.
+__global__ __launch_bounds__(1024) void k_b_compact(int ns, const int32_t *__restrict__ sys_t0,
+                                                     const BSys *__restrict__ sys,
+                                                     int32_t *__restrict__ act, BGlobal *g) {
+  __shared__ int32_t scan[1024];
+  const int tid = threadIdx.x;
+  int base = 0;
+  for (int k0 = 0; k0 < ns; k0 += 1024) {
+    const int k = k0 + tid;
+    const int cnt = (k < ns && !sys[k].done) ? sys_t0[k + 1] - sys_t0[k] : 0;
+    scan[tid] = cnt;
+    __syncthreads();
+    for (int off = 1; off < 1024; off <<= 1) {
+      const int v = tid >= off ? scan[tid - off] : 0;
+      __syncthreads();
+      scan[tid] += v;
+      __syncthreads();
+    }
+    const int start = base + scan[tid] - cnt;
+    for (int j = 0; j < cnt; ++j) act[start + j] = sys_t0[k] + j;
+    base += scan[1023];
+    __syncthreads();
+  }
+  if (tid == 0) g->nact = base;
+}
+
+// The iteration kernels run a fixed grid that walks the active-tile list, so
+// the cost of an iteration follows the work still left in the batch.  A
+// system's stop flag can be set by another workgroup while this one reads it,
+// so thread 0 reads it once per tile and the workgroup takes that value: every
+// wave then runs the same sequence of tiles and barriers.  (Either value is
+// correct: a tile that sees the flag just set would have taken the same stop
+// decision itself.)
+#define FOR_ACTIVE_TILES(...)                                                   \
+  {                                                                             \
+    __shared__ int s_done;                                                      \
+    const int nact = g->nact;                                                   \
+    for (int i = blockIdx.x; i < nact; i += gridDim.x) {                        \
+      const int t = act[i];                                                     \
+      const int k = tsys[t];                                                    \
+      if (threadIdx.x == 0) s_done = sys[k].done;                               \
+      __syncthreads();                                                          \
+      const int done_k = s_done;                                                \
+      __syncthreads();                                                          \
+      if (done_k) continue;                                                     \
+      __VA_ARGS__                                                               \
+    }                                                                           \
+  }
+
+// Ap = A p; tile partial p.Ap.
+__global__ __launch_bounds__(kBlock) void k_b_spmv(const int32_t *__restrict__ act,
+                                                    const int32_t *__restrict__ tsys,
+                                                    const int32_t *__restrict__ ts0,
+                                                    const int32_t *__restrict__ ts1,
+                                                    const SliceInfo *__restrict__ sl,
+                                                    const int32_t *__restrict__ col,
+                                                    const double *__restrict__ val,
+                                                    const double *__restrict__ diag,
+                                                    const double *__restrict__ p,
+                                                    double *__restrict__ Ap,
+                                                    double *__restrict__ part,
+                                                    const BGlobal *__restrict__ g,
+                                                    const BSys *sys) {
+  __shared__ double lds[kWavesPerBlock];
+  FOR_ACTIVE_TILES({
+    const int s0 = ts0[t], s1 = ts1[t];
+    double pap = 0.0;
+    TILE_ROWS({
+      const SliceInfo si = load_slice(sl, s);
+      const double pi = p[row];
+      const double v = row_spmv<double>(si, lane, diag[row] * pi, col, val, p);
+      Ap[row] = v;
+      pap += pi * v;
+    })
+    pap = block_sum(pap, lds);
+    if (threadIdx.x == 0) part[t] = pap;
+  })
+}
+
+// alpha per system; x += alpha p; r -= alpha Ap; tile partials r.r, r.z.
+__global__ __launch_bounds__(kBlock) void k_b_update(const int32_t *__restrict__ act,
+                                                      const int32_t *__restrict__ tsys,
+                                                      const int32_t *__restrict__ ts0,
+                                                      const int32_t *__restrict__ ts1, int ntiles,
+                                                      const int32_t *__restrict__ sys_t0,
+                                                      const double *__restrict__ rD,
+                                                      const double *__restrict__ p,
+                                                      const double *__restrict__ Ap,
+                                                      double *__restrict__ x,
+                                                      double *__restrict__ r,
+                                                      double *__restrict__ part,
+                                                      BGlobal *g, BSys *sys) {
+  __shared__ double lds[kWavesPerBlock];
+  const int it = g->it + 1;
+  FOR_ACTIVE_TILES({
+    const int t0 = sys_t0[k], t1 = sys_t0[k + 1];
+    const double PAP = sys_sum(part, t0, t1, lds);
+    if (fabs(PAP) < 1e-30) {
+      if (t == t0 && threadIdx.x == 0) {
+        sys[k].done = 1;
+        sys[k].status = PFM_SOLVE_BREAKDOWN;
+        atomicAdd((int *)&g->ndone, 1);
+      }
+      continue;
+    }
+    const double alpha = sys[k].rz[(it - 1) & 1] / PAP;
+    const int s0 = ts0[t], s1 = ts1[t];
+    double rr = 0.0, rz = 0.0;
+    TILE_ROWS({
+      x[row] = x[row] + alpha * p[row];
+      const double ri = r[row] - alpha * Ap[row];
+      r[row] = ri;
+      const double zi = rD ? ri / rD[row] : ri;
+      rr += ri * ri;
+      rz += ri * zi;
+    })
+    rr = block_sum(rr, lds);
+    rz = block_sum(rz, lds);
+    if (threadIdx.x == 0) {
+      part[ntiles + t] = rr;
+      part[2 * ntiles + t] = rz;
+    }
+  })
+}
+
+// monitor per system; beta; p = z + beta p.
+__global__ __launch_bounds__(kBlock) void k_b_pupdate(const int32_t *__restrict__ act,
+                                                       const int32_t *__restrict__ tsys,
+                                                       const int32_t *__restrict__ ts0,
+                                                       const int32_t *__restrict__ ts1, int ntiles,
+                                                       const int32_t *__restrict__ sys_t0,
+                                                       const double *__restrict__ rD,
+                                                       const double *__restrict__ r,
+                                                       double *__restrict__ p,
+                                                       const double *__restrict__ part,
+                                                       BGlobal *g, BSys *sys) {
+  __shared__ double lds[kWavesPerBlock];
+  const int it = g->it + 1;
+  FOR_ACTIVE_TILES({
+    const int t0 = sys_t0[k], t1 = sys_t0[k + 1];
+    const double RR = sys_sum(part + ntiles, t0, t1, lds);
+    const double RZ = sys_sum(part + 2 * ntiles, t0, t1, lds);
+    const BSys S = sys[k];
+    const double res = sqrt(RR);
+    const bool conv = sys_decision(g, S, res, it);
+    const bool stop = conv || it >= g->max_iter;
+    if (t == t0 && threadIdx.x == 0) {
+      BSys W = S;
+      W.nhist += 1;
+      W.res = res;
+      W.rz[it & 1] = RZ;
+      if (stop) {
+        W.done = 1;
+        W.status = conv ? PFM_SOLVE_CONVERGED : PFM_SOLVE_MAXITER;
+      }
+      sys[k] = W;
+      if (stop) atomicAdd((int *)&g->ndone, 1);
+    }
+    if (stop) continue;
+    const double beta = RZ / S.rz[(it - 1) & 1];
+    const int s0 = ts0[t], s1 = ts1[t];
+    TILE_ROWS({
+      const double ri = r[row];
+      const double zi = rD ? ri / rD[row] : ri;
+      p[row] = zi + beta * p[row];
+    })
+  })
+}
+
+__global__ void k_b_tick(BGlobal *g) {
+  if (threadIdx.x == 0 && blockIdx.x == 0) g->it += 1;
+}
+
+inline size_t align256(size_t b) { return (b + 255) & ~size_t(255); }
+
+struct BWork {
+  BGlobal *g;
+  BSys *sys;
+  int32_t *act;
+  double *part;
+  double *dp, *xp, *bp, *rD, *r, *p, *Ap, *packed;
+};
+
+size_t batch_layout(const pfm_batch_s *B, int ntiles, BWork *w, char *base) {
+  size_t off = 0;
+  auto take = [&](size_t bytes) -> void * {
+    size_t o = off;
+    off += align256(bytes);
+    return base ? (void *)(base + o) : nullptr;
+  };
+  BWork t;
+  t.g = (BGlobal *)take(sizeof(BGlobal));
+  t.sys = (BSys *)take(sizeof(BSys) * (size_t)B->ns);
+  t.act = (int32_t *)take(sizeof(int32_t) * (size_t)std::max(ntiles, 1));
+  t.part = (double *)take(sizeof(double) * 3 * (size_t)std::max(ntiles, 1));
+  const size_t np = (size_t)B->np;
+  t.dp = (double *)take(sizeof(double) * np);
+  t.xp = (double *)take(sizeof(double) * np);
+  t.bp = (double *)take(sizeof(double) * np);
+  t.rD = (double *)take(sizeof(double) * np);
+  t.r = (double *)take(sizeof(double) * np);
+  t.p = (double *)take(sizeof(double) * np);
+  t.Ap = (double *)take(sizeof(double) * np);
+  t.packed = (double *)take(sizeof(double) * (size_t)std::max<int64_t>(B->ldu->nslots, 1));
+  if (w) *w = t;
+  return off;
+}
+
+int build_tiles(pfm_batch_s *B, int tile_slices) {
+  std::vector<int32_t> tsys, ts0, ts1, sys_t0(B->ns + 1, 0);
+  for (int k = 0; k < B->ns; ++k) {
+    sys_t0[k] = (int32_t)tsys.size();
+    const int s_begin = B->pad_off[k] / kSlice, s_end = B->pad_off[k + 1] / kSlice;
+    for (int s = s_begin; s < s_end; s += tile_slices) {
+      tsys.push_back(k);
+      ts0.push_back(s);
+      ts1.push_back(std::min(s + tile_slices, s_end));
+    }
+  }
+  sys_t0[B->ns] = (int32_t)tsys.size();
+  (void)hipFree(B->d_tile_sys);
+  (void)hipFree(B->d_tile_s0);
+  (void)hipFree(B->d_tile_s1);
+  (void)hipFree(B->d_sys_t0);
+  B->ntiles = (int32_t)tsys.size();
+  B->tile_slices = tile_slices;
+  int st;
+  if ((st = upload(&B->d_tile_sys, tsys)) || (st = upload(&B->d_tile_s0, ts0)) ||
+      (st = upload(&B->d_tile_s1, ts1)) || (st = upload(&B->d_sys_t0, sys_t0)))
+    return st;
+  return PFM_OK;
+}
+
+
+// Workgroups of the iteration kernels (they walk the active-tile list).
+constexpr int kBatchGrid = 2048;
+
+void launch_compact(const pfm_batch_s *B, const BWork &w, hipStream_t q) {
+  k_b_compact<<<1, 1024, 0, q>>>(B->ns, B->d_sys_t0, w.sys, w.act, w.g);
+}
+
+void launch_iteration(const pfm_batch_s *B, const BWork &w, const double *pd, hipStream_t q) {
+  const pfm_ldu L = B->ldu;
+  const int T = B->ntiles;
+  const int G = std::max(1, std::min(T, kBatchGrid));
+  k_b_spmv<<<G, kBlock, 0, q>>>(w.act, B->d_tile_sys, B->d_tile_s0, B->d_tile_s1, L->d_slices,
+                                L->d_col, w.packed, w.dp, w.p, w.Ap, w.part, w.g, w.sys);
+  k_b_update<<<G, kBlock, 0, q>>>(w.act, B->d_tile_sys, B->d_tile_s0, B->d_tile_s1, T, B->d_sys_t0,
+                                  pd, w.p, w.Ap, w.xp, w.r, w.part, w.g, w.sys);
+  k_b_pupdate<<<G, kBlock, 0, q>>>(w.act, B->d_tile_sys, B->d_tile_s0, B->d_tile_s1, T,
+                                   B->d_sys_t0, pd, w.r, w.p, w.part, w.g, w.sys);
+  k_b_tick<<<1, 1, 0, q>>>(w.g);
+}
+
+constexpr int kTileCand[6] = {1, 2, 4, 8, 16, 32};
+
+// Times `kTuneIters` full iterations per candidate tile size on synthetic
+// values (unit diagonal, no off-diagonal coefficients, p = 1, tolerances that
+// never stop a system) and keeps the fastest.  The values go to NaN after the
+// first iteration; the kernels' memory traffic and launch shapes do not
+// depend on them.
+int autotune(pfm_batch_s *B) {
+  constexpr int kTuneIters = 10;
+  const int max_tiles = std::max(B->np / kSlice, 1);
+  const size_t bytes = batch_layout(B, max_tiles, nullptr, nullptr);
+  char *base = nullptr;
+  PFM_CHECK_HIP(hipMalloc((void **)&base, bytes));
+  BWork w;
+  batch_layout(B, max_tiles, &w, base);
+  hipEvent_t e0 = nullptr, e1 = nullptr;
+  bool ok = hipMemset(base, 0, bytes) == hipSuccess && hipEventCreate(&e0) == hipSuccess &&
+            hipEventCreate(&e1) == hipSuccess;
+  const int np = B->np;
+  if (ok) {
+    k_b_fill<<<grid1d(np), kBlock>>>(np, w.dp, w.r, w.bp);  // diag 1
+    k_b_fill<<<grid1d(np), kBlock>>>(np, w.p, w.xp, w.Ap);  // p 1
+    k_b_fill<<<grid1d(np), kBlock>>>(np, w.rD, w.r, w.bp);  // rD 1
+    ok = hipGetLastError() == hipSuccess;
+  }
+  int best = 8;
+  float best_t = 1e30f;
+  for (int ci = 0; ok && ci < 6; ++ci) {
+    if (build_tiles(B, kTileCand[ci]) != PFM_OK) {
+      ok = false;
+      break;
+    }
+    k_b_state_init<<<(B->ns + 255) / 256, 256>>>(w.g, w.sys, B->ns, -1.0, -1.0, 0x7fffffff, 0);
+    launch_compact(B, w, nullptr);
+    for (int rep = 0; rep < 2; ++rep) launch_iteration(B, w, w.rD, nullptr);
+    ok = hipEventRecord(e0, nullptr) == hipSuccess;
+    for (int rep = 0; ok && rep < kTuneIters; ++rep) launch_iteration(B, w, w.rD, nullptr);
+    float ms = 0.f;
+    ok = ok && hipEventRecord(e1, nullptr) == hipSuccess && hipEventSynchronize(e1) == hipSuccess &&
+         hipEventElapsedTime(&ms, e0, e1) == hipSuccess;
+    B->tuned_us[ci] = ms * 1000.f / kTuneIters;  // us per iteration
+    if (ok && ms < best_t) {
+      best_t = ms;
+      best = kTileCand[ci];
+    }
+  }
+  if (e0) (void)hipEventDestroy(e0);
+  if (e1) (void)hipEventDestroy(e1);
+  (void)hipFree(base);
+  if (!ok) {
+    set_error("batch autotuning failed");
+    return PFM_ERR_HIP;
+  }
+  return build_tiles(B, best);
+}
+
+}  // namespace
+
+extern "C" {
+
+int pfm_batch_destroy(pfm_batch B) {
+  if (!B) return PFM_OK;
+  (void)hipFree(B->d_tile_sys);
+  (void)hipFree(B->d_tile_s0);
+  (void)hipFree(B->d_tile_s1);
+  (void)hipFree(B->d_sys_t0);
+  (void)hipFree(B->d_cell_map);
+  if (B->h_flag) (void)hipHostFree(B->h_flag);
+  if (B->ldu) pfm_ldu_destroy(B->ldu);
+  delete B;
+  return PFM_OK;
+}
+
+int pfm_batch_create(int32_t ns, const int32_t *cell_offsets, const int32_t *face_offsets,
+                     const int32_t *owner, const int32_t *neighbour, int32_t tile_slices,
+                     pfm_batch *out) {
+  PFM_REQUIRE(ns > 0 && cell_offsets && face_offsets && out, "bad batch description");
+  PFM_REQUIRE(tile_slices >= 0 && tile_slices <= 4096, "bad tile size");
+  *out = nullptr;
+  pfm_batch B = new pfm_batch_s();
+  B->ns = ns;
+  B->cell_off.assign(cell_offsets, cell_offsets + ns + 1);
+  B->face_off.assign(face_offsets, face_offsets + ns + 1);
+  B->n = B->cell_off[ns];
+  B->nf = B->face_off[ns];
+  B->pad_off.assign(ns + 1, 0);
+  for (int k = 0; k < ns; ++k) {
+    const int nk = B->cell_off[k + 1] - B->cell_off[k];
+    if (nk < 0 || B->face_off[k + 1] < B->face_off[k]) {
+      delete B;
+      set_error("offsets must be non-decreasing");
+      return PFM_ERR_ARG;
+    }
+    B->pad_off[k + 1] = B->pad_off[k] + (nk + kSlice - 1) / kSlice * kSlice;
+  }
+  B->np = B->pad_off[ns];
+  // padded concatenated topology (local indices of system k shifted to its block)
+  std::vector<int32_t> own(B->nf), nei(B->nf), cmap(B->n);
+  for (int k = 0; k < ns; ++k) {
+    const int nk = B->cell_off[k + 1] - B->cell_off[k];
+    for (int f = B->face_off[k]; f < B->face_off[k + 1]; ++f) {
+      if (owner[f] < 0 || owner[f] >= nk || neighbour[f] < 0 || neighbour[f] >= nk) {
+        delete B;
+        set_error("system %d face %d addresses a cell outside the system", k, f);
+        return PFM_ERR_ARG;
+      }
+      own[f] = owner[f] + B->pad_off[k];
+      nei[f] = neighbour[f] + B->pad_off[k];
+    }
+    for (int i = 0; i < nk; ++i) cmap[B->cell_off[k] + i] = B->pad_off[k] + i;
+  }
+  int st = pfm_ldu_create(B->np, B->nf, own.data(), nei.data(), &B->ldu);
+  if (st == PFM_OK) st = upload(&B->d_cell_map, cmap);
+  if (st == PFM_OK &&
+      hipHostMalloc((void **)&B->h_flag, 4 * sizeof(int32_t), hipHostMallocDefault) != hipSuccess) {
+    set_error("hipHostMalloc failed");
+    st = PFM_ERR_HIP;
+  }
+  if (st != PFM_OK) {
+    pfm_batch_destroy(B);
+    return st;
+  }
+  if (tile_slices > 0) {
+    st = build_tiles(B, tile_slices);
+  } else {
+    // autotune: time whole iterations for each candidate tile size
+    st = autotune(B);
+  }
+  if (st != PFM_OK) {
+    pfm_batch_destroy(B);
+    return st;
+  }
+  *out = B;
+  return PFM_OK;
+}
+
+int pfm_batch_info(pfm_batch B, int32_t *n_padded, int32_t *n_tiles, int32_t *tile_slices,
+                   float *tuned_us) {
+  PFM_REQUIRE(B, "null batch");
+  if (n_padded) *n_padded = B->np;
+  if (n_tiles) *n_tiles = B->ntiles;
+  if (tile_slices) *tile_slices = B->tile_slices;
+  if (tuned_us) memcpy(tuned_us, B->tuned_us, sizeof(B->tuned_us[0]) * 6);
+  return PFM_OK;
+}
+
+int64_t pfm_batch_workspace_bytes(pfm_batch B) {
+  if (!B) return -1;
+  return (int64_t)batch_layout(B, B->ntiles, nullptr, nullptr);
+}
+
+int pfm_batch_pcg_solve(pfm_batch B, const pfm_solve_params *prm, int transpose,
+                        const double *diag, const double *lower, const double *upper,
+                        const double *b, double *x, void *work, int32_t graph_chunk,
+                        int32_t *iterations_h, int32_t *status_h, double *initial_res_h,
+                        double *final_res_h, void *stream) {
+  PFM_REQUIRE(B && prm && work && diag && b && x, "null argument");
+  PFM_REQUIRE(B->nf == 0 || (lower && upper), "null lower/upper");
+  PFM_REQUIRE(prm->max_iter >= 0, "max_iter must be >= 0");
+  PFM_REQUIRE(prm->precond == PFM_PRECOND_NONE || prm->precond == PFM_PRECOND_DIAG,
+              "unknown preconditioner");
+  hipStream_t s = (hipStream_t)stream;
+  BWork w;
+  batch_layout(B, B->ntiles, &w, (char *)work);
+  pfm_ldu L = B->ldu;
+  const int np = B->np, ns = B->ns, T = B->ntiles;
+  k_b_fill<<<grid1d(np), kBlock, 0, s>>>(np, w.dp, w.xp, w.bp);
+  if (B->n) k_b_scatter<<<grid1d(B->n), kBlock, 0, s>>>(B->n, B->d_cell_map, diag, x, b, w.dp, w.xp, w.bp);
+  int st = pfm_ldu_pack(L, transpose, lower, upper, w.packed, stream);
+  if (st != PFM_OK) return st;
+  const bool pre = prm->precond == PFM_PRECOND_DIAG;
+  if (pre && (st = pfm_dic_factor(L, w.dp, w.packed, w.rD, stream)) != PFM_OK) return st;
+  const double *pd = pre ? w.rD : nullptr;
+  k_b_state_init<<<(ns + 255) / 256, 256, 0, s>>>(w.g, w.sys, ns, prm->tolerance, prm->rel_tol,
+                                                   prm->max_iter, prm->min_iter);
+  k_b_init<<<T, kBlock, 0, s>>>(B->d_tile_sys, B->d_tile_s0, B->d_tile_s1, T, L->d_slices, L->d_col,
+                                w.packed, w.dp, pd, w.bp, w.xp, w.r, w.p, w.part);
+  k_b_init_fin<<<(ns + 255) / 256, 256, 0, s>>>(ns, B->d_sys_t0, T, w.part, w.g, w.sys);
+  PFM_CHECK_HIP(hipGetLastError());
+  auto iteration = [&](hipStream_t q) { launch_iteration(B, w, pd, q); };
+  const int chunk = std::max(1, graph_chunk);
+  hipGraphExec_t exec = nullptr;
+  hipStream_t cap = nullptr;
+  if (graph_chunk > 0) {
+    PFM_CHECK_HIP(hipStreamCreateWithFlags(&cap, hipStreamNonBlocking));
+    hipGraph_t graph = nullptr;
+    PFM_CHECK_HIP(hipStreamBeginCapture(cap, hipStreamCaptureModeThreadLocal));
+    launch_compact(B, w, cap);
+    for (int k = 0; k < chunk; ++k) iteration(cap);
+    hipError_t e = hipStreamEndCapture(cap, &graph);
+    if (e == hipSuccess) e = hipGraphInstantiate(&exec, graph, nullptr, nullptr, 0);
+    if (graph) (void)hipGraphDestroy(graph);
+    if (e != hipSuccess) {
+      (void)hipStreamDestroy(cap);
+      set_error("graph capture failed: %s", hipGetErrorString(e));
+      return PFM_ERR_HIP;
+    }
+  }
+  // replay chunks until every system has stopped (one chunk queued ahead)
+  int32_t *flag = B->h_flag;
+  hipEvent_t ev[2];
+  PFM_CHECK_HIP(hipEventCreateWithFlags(&ev[0], hipEventDisableTiming));
+  PFM_CHECK_HIP(hipEventCreateWithFlags(&ev[1], hipEventDisableTiming));
+  int enq = 0, slot = 0, rc = PFM_OK;
+  bool pending = false;
+  while (true) {
+    bool queued = false;
+    if (enq < prm->max_iter) {
+      if (exec) {
+        if (hipGraphLaunch(exec, s) != hipSuccess) {
+          rc = PFM_ERR_HIP;
+          break;
+        }
+        enq += chunk;
+      } else {
+        const int c = std::min(8, prm->max_iter - enq);
+        launch_compact(B, w, s);
+        for (int k = 0; k < c; ++k) iteration(s);
+        enq += c;
+      }
+      if (hipMemcpyAsync(&flag[slot], &w.g->ndone, sizeof(int32_t), hipMemcpyDeviceToHost, s) !=
+              hipSuccess ||
+          hipEventRecord(ev[slot], s) != hipSuccess) {
+        rc = PFM_ERR_HIP;
+        break;
+      }
+      queued = true;
+    }
+    if (pending) {
+      if (hipEventSynchronize(ev[slot ^ 1]) != hipSuccess) {
+        rc = PFM_ERR_HIP;
+        break;
+      }
+      if (flag[slot ^ 1] >= ns) break;
+    }
+    if (!queued) break;
+    pending = true;
+    slot ^= 1;
+  }
+  (void)hipEventDestroy(ev[0]);
+  (void)hipEventDestroy(ev[1]);
+  if (exec) (void)hipGraphExecDestroy(exec);
+  if (cap) (void)hipStreamDestroy(cap);
+  if (rc != PFM_OK) {
+    set_error("HIP error while driving the batched solve");
+    return rc;
+  }
+  if (B->n) k_b_gather<<<grid1d(B->n), kBlock, 0, s>>>(B->n, B->d_cell_map, w.xp, x);
+  std::vector<BSys> hs(ns);
+  PFM_CHECK_HIP(hipMemcpyAsync(hs.data(), w.sys, sizeof(BSys) * ns, hipMemcpyDeviceToHost, s));
+  PFM_CHECK_HIP(hipStreamSynchronize(s));
+  for (int k = 0; k < ns; ++k) {
+    if (iterations_h) iterations_h[k] = hs[k].nhist;
+    if (status_h) status_h[k] = hs[k].status == 0 ? PFM_SOLVE_MAXITER : hs[k].status;
+    if (initial_res_h) initial_res_h[k] = hs[k].res0;
+    if (final_res_h) final_res_h[k] = hs[k].res;
+  }
+  return PFM_OK;
+}
+
+}  // extern "C"

@@ -60,6 +60,8 @@ int validate_topology(int32_t n, int32_t nf, const int32_t *own,
 
 }  // namespace pfm
 
+struct pfm_batch_s;
+
 // Device-resident addressing of one LDU topology.
 struct pfm_ldu_s {
   int32_t n = 0, nf = 0, nslices = 0;

@@ -2,6 +2,7 @@
 include/pfm.h declares, and its host-side setup algorithms (no GPU needed)
 agree with the oracle."""
 
+import ctypes as C
 import re
 from pathlib import Path
 
@@ -91,3 +92,23 @@ def test_dic_and_dilu_levels_respect_sequential_order():
             if cell in last:
                 assert flev[f] > flev[last[cell]]
             last[cell] = f
+
+
+def test_batch_create_rejects_bad_descriptions():
+    """Argument checks of pfm_batch_create run on the host before any device work."""
+    L = _lib.lib()
+    h = C.c_void_p()
+    co = np.array([0, 2, 4], np.int32)
+    fo = np.array([0, 1, 2], np.int32)
+    own = np.array([0, 0], np.int32)
+    nei = np.array([1, 2], np.int32)  # system 1 has 2 cells: local index 2 is out of range
+    st = L.pfm_batch_create(2, co.ctypes.data, fo.ctypes.data, own.ctypes.data, nei.ctypes.data,
+                            0, C.byref(h))
+    assert st == _lib.PFM_ERR_ARG and b"outside" in L.pfm_last_error() and not h.value
+    bad = np.array([0, 3, 2], np.int32)
+    st = L.pfm_batch_create(2, bad.ctypes.data, fo.ctypes.data, own.ctypes.data, nei.ctypes.data,
+                            0, C.byref(h))
+    assert st == _lib.PFM_ERR_ARG and b"non-decreasing" in L.pfm_last_error()
+    st = L.pfm_batch_create(0, co.ctypes.data, fo.ctypes.data, own.ctypes.data, nei.ctypes.data,
+                            0, C.byref(h))
+    assert st == _lib.PFM_ERR_ARG
