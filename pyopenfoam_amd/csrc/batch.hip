@@ -29,7 +29,8 @@ struct BSys {
 
 struct alignas(256) BGlobal {
   double tol, rel_tol;
-  int32_t it;        // completed iterations
+  int32_t it;        // iterations scheduled so far (chunks started)
+  int32_t base;      // iterations before the current chunk
   int32_t ndone;     // systems finished
   int32_t max_iter, min_iter;
   int32_t nact;      // entries of the active-tile list
@@ -59,21 +60,38 @@ int upload(T **dst, const std::vector<T> &src) {
   return PFM_OK;
 }
 
-#define TILE_ROWS(...)                                                         \
+// Tile kernels run 256-thread workgroups; a tile's slices are shared among
+// the waves.
+constexpr int kMaxWaves = kBlock / kSlice;
+
+#define TILE_ROWS(...)                                                          \
   {                                                                             \
     const int lane = threadIdx.x & 63;                                          \
-    for (int s = s0 + (threadIdx.x >> 6); s < s1; s += kWavesPerBlock) {        \
+    const int nw = blockDim.x >> 6;                                             \
+    for (int s = s0 + (threadIdx.x >> 6); s < s1; s += nw) {                    \
       const int row = s * kSlice + lane;                                        \
       __VA_ARGS__                                                               \
     }                                                                           \
   }
 
+// Deterministic workgroup sum for any wave count (waves added in order).
+__device__ __forceinline__ double wg_sum(double v, double *lds /*[kMaxWaves]*/) {
+  v = wave_sum(v);
+  const int nw = blockDim.x >> 6;
+  if ((threadIdx.x & 63) == 0) lds[threadIdx.x >> 6] = v;
+  __syncthreads();
+  double s = lds[0];
+  for (int i = 1; i < nw; ++i) s += lds[i];
+  __syncthreads();
+  return s;
+}
+
 // Sum of this system's tile partials (fixed tile order), on every thread.
 __device__ __forceinline__ double sys_sum(const double *__restrict__ part, int t0, int t1,
                                           double *lds) {
   double s = 0.0;
-  for (int t = t0 + (int)threadIdx.x; t < t1; t += kBlock) s += part[t];
-  return block_sum(s, lds);
+  for (int t = t0 + (int)threadIdx.x; t < t1; t += (int)blockDim.x) s += part[t];
+  return wg_sum(s, lds);
 }
 
 __global__ __launch_bounds__(kBlock) void k_b_fill(int np, double *__restrict__ d,
@@ -114,6 +132,8 @@ __global__ void k_b_state_init(BGlobal *g, BSys *sys, int ns, double tol, double
     g->tol = tol;
     g->rel_tol = rel_tol;
     g->it = 0;
+    g->base = 0;
+    g->nact = 0;
     g->ndone = 0;
     g->max_iter = max_iter;
     g->min_iter = min_iter;
@@ -146,7 +166,7 @@ __global__ __launch_bounds__(kBlock) void k_b_init(const int32_t *__restrict__ t
                                                     const double *__restrict__ x,
                                                     double *__restrict__ r, double *__restrict__ p,
                                                     double *__restrict__ part) {
-  __shared__ double lds[kWavesPerBlock];
+  __shared__ double lds[kMaxWaves];
   const int t = blockIdx.x;
   const int s0 = ts0[t], s1 = ts1[t];
   double rr = 0.0, rz = 0.0;
@@ -160,8 +180,8 @@ __global__ __launch_bounds__(kBlock) void k_b_init(const int32_t *__restrict__ t
     rr += ri * ri;
     rz += ri * zi;
   })
-  rr = block_sum(rr, lds);
-  rz = block_sum(rz, lds);
+  rr = wg_sum(rr, lds);
+  rz = wg_sum(rz, lds);
   if (threadIdx.x == 0) {
     part[t] = rr;
     part[ntiles + t] = rz;
@@ -194,12 +214,18 @@ __global__ void k_b_init_fin(int ns, const int32_t *__restrict__ sys_t0, int nti
   if (S.done) atomicAdd(&g->ndone, 1);
 }
 
-// Active-tile list: the tiles of every system not yet stopped, in system
-// order.  Rebuilt by one workgroup at the start of each chunk of iterations;
-// tiles of systems that stop inside a chunk skip themselves.
+// Active-tile list: descriptors {tile, system, first slice, end slice} of
+// the tiles of every system not yet stopped, in system order.  Rebuilt by one
+// workgroup at the start of each chunk of iterations; tiles of systems that
+// stop inside a chunk skip themselves.  It also fixes the chunk's first
+// iteration index: the kernels of the chunk's j-th iteration (a launch
+// argument, so a captured chunk replays unchanged) run iteration base + j + 1.
 __global__ __launch_bounds__(1024) void k_b_compact(int ns, const int32_t *__restrict__ sys_t0,
+                                                     const int32_t *__restrict__ ts0,
+                                                     const int32_t *__restrict__ ts1,
                                                      const BSys *__restrict__ sys,
-                                                     int32_t *__restrict__ act, BGlobal *g) {
+                                                     int4 *__restrict__ act, BGlobal *g,
+                                                     int chunk) {
   __shared__ int32_t scan[1024];
   const int tid = threadIdx.x;
   int base = 0;
@@ -215,11 +241,41 @@ __global__ __launch_bounds__(1024) void k_b_compact(int ns, const int32_t *__res
       __syncthreads();
     }
     const int start = base + scan[tid] - cnt;
-    for (int j = 0; j < cnt; ++j) act[start + j] = sys_t0[k] + j;
+    for (int j = 0; j < cnt; ++j) {
+      const int t = sys_t0[k] + j;
+      act[start + j] = make_int4(t, k, ts0[t], ts1[t]);
+    }
     base += scan[1023];
     __syncthreads();
   }
-  if (tid == 0) g->nact = base;
+  if (tid == 0) {
+    g->nact = base;
+    g->base = g->it;  // iterations of this chunk are base+1 .. base+chunk
+    g->it += chunk;
+  }
+}
+
+// Workgroup b runs on XCD b mod 8.  With a grid that is a multiple of 8, each
+// XCD walks its own contiguous eighth of the active list, so the tiles of a
+// system (whose rows read each other's vector entries) share one XCD's L2.
+struct XcdRange {
+  int begin, end, step;
+};
+
+__device__ __forceinline__ XcdRange xcd_range(int nact) {
+  const int G = gridDim.x, b = blockIdx.x;
+  XcdRange r;
+  if (G % kXcds) {
+    r.begin = b;
+    r.end = nact;
+    r.step = G;
+    return r;
+  }
+  const int x = b % kXcds;
+  r.begin = (int)((int64_t)nact * x / kXcds) + b / kXcds;
+  r.end = (int)((int64_t)nact * (x + 1) / kXcds);
+  r.step = G / kXcds;
+  return r;
 }
 
 // The iteration kernels run a fixed grid that walks the active-tile list, so
@@ -229,27 +285,27 @@ __global__ __launch_bounds__(1024) void k_b_compact(int ns, const int32_t *__res
 // wave then runs the same sequence of tiles and barriers.  (Either value is
 // correct: a tile that sees the flag just set would have taken the same stop
 // decision itself.)
-#define FOR_ACTIVE_TILES(...)                                                   \
+#define FOR_ACTIVE_TILES(CHECK_DONE, ...)                                       \
   {                                                                             \
     __shared__ int s_done;                                                      \
-    const int nact = g->nact;                                                   \
-    for (int i = blockIdx.x; i < nact; i += gridDim.x) {                        \
-      const int t = act[i];                                                     \
-      const int k = tsys[t];                                                    \
-      if (threadIdx.x == 0) s_done = sys[k].done;                               \
-      __syncthreads();                                                          \
-      const int done_k = s_done;                                                \
-      __syncthreads();                                                          \
-      if (done_k) continue;                                                     \
+    const XcdRange rg = xcd_range(g->nact);                                     \
+    for (int i = rg.begin; i < rg.end; i += rg.step) {                          \
+      const int4 dsc = act[i];                                                  \
+      const int t = dsc.x, k = dsc.y, s0 = dsc.z, s1 = dsc.w;                   \
+      if (CHECK_DONE) {                                                         \
+        if (threadIdx.x == 0) s_done = sys[k].done;                             \
+        __syncthreads();                                                        \
+        const int done_k = s_done;                                              \
+        __syncthreads();                                                        \
+        if (done_k) continue;                                                   \
+      }                                                                         \
       __VA_ARGS__                                                               \
     }                                                                           \
   }
 
-// Ap = A p; tile partial p.Ap.
-__global__ __launch_bounds__(kBlock) void k_b_spmv(const int32_t *__restrict__ act,
-                                                    const int32_t *__restrict__ tsys,
-                                                    const int32_t *__restrict__ ts0,
-                                                    const int32_t *__restrict__ ts1,
+// Ap = A p; tile partial p.Ap.  Tiles of a system that stopped earlier in the
+// chunk compute values nobody reads, so this kernel skips the stop check.
+__global__ __launch_bounds__(kBlock) void k_b_spmv(const int4 *__restrict__ act,
                                                     const SliceInfo *__restrict__ sl,
                                                     const int32_t *__restrict__ col,
                                                     const double *__restrict__ val,
@@ -259,9 +315,9 @@ __global__ __launch_bounds__(kBlock) void k_b_spmv(const int32_t *__restrict__ a
                                                     double *__restrict__ part,
                                                     const BGlobal *__restrict__ g,
                                                     const BSys *sys) {
-  __shared__ double lds[kWavesPerBlock];
-  FOR_ACTIVE_TILES({
-    const int s0 = ts0[t], s1 = ts1[t];
+  __shared__ double lds[kMaxWaves];
+  FOR_ACTIVE_TILES(false, {
+    (void)k;
     double pap = 0.0;
     TILE_ROWS({
       const SliceInfo si = load_slice(sl, s);
@@ -270,16 +326,13 @@ __global__ __launch_bounds__(kBlock) void k_b_spmv(const int32_t *__restrict__ a
       Ap[row] = v;
       pap += pi * v;
     })
-    pap = block_sum(pap, lds);
+    pap = wg_sum(pap, lds);
     if (threadIdx.x == 0) part[t] = pap;
   })
 }
 
 // alpha per system; x += alpha p; r -= alpha Ap; tile partials r.r, r.z.
-__global__ __launch_bounds__(kBlock) void k_b_update(const int32_t *__restrict__ act,
-                                                      const int32_t *__restrict__ tsys,
-                                                      const int32_t *__restrict__ ts0,
-                                                      const int32_t *__restrict__ ts1, int ntiles,
+__global__ __launch_bounds__(kBlock) void k_b_update(const int4 *__restrict__ act, int ntiles,
                                                       const int32_t *__restrict__ sys_t0,
                                                       const double *__restrict__ rD,
                                                       const double *__restrict__ p,
@@ -287,10 +340,10 @@ __global__ __launch_bounds__(kBlock) void k_b_update(const int32_t *__restrict__
                                                       double *__restrict__ x,
                                                       double *__restrict__ r,
                                                       double *__restrict__ part,
-                                                      BGlobal *g, BSys *sys) {
-  __shared__ double lds[kWavesPerBlock];
-  const int it = g->it + 1;
-  FOR_ACTIVE_TILES({
+                                                      BGlobal *g, BSys *sys, int j) {
+  __shared__ double lds[kMaxWaves];
+  const int it = g->base + j + 1;
+  FOR_ACTIVE_TILES(true, {
     const int t0 = sys_t0[k], t1 = sys_t0[k + 1];
     const double PAP = sys_sum(part, t0, t1, lds);
     if (fabs(PAP) < 1e-30) {
@@ -302,7 +355,6 @@ __global__ __launch_bounds__(kBlock) void k_b_update(const int32_t *__restrict__
       continue;
     }
     const double alpha = sys[k].rz[(it - 1) & 1] / PAP;
-    const int s0 = ts0[t], s1 = ts1[t];
     double rr = 0.0, rz = 0.0;
     TILE_ROWS({
       x[row] = x[row] + alpha * p[row];
@@ -312,8 +364,8 @@ __global__ __launch_bounds__(kBlock) void k_b_update(const int32_t *__restrict__
       rr += ri * ri;
       rz += ri * zi;
     })
-    rr = block_sum(rr, lds);
-    rz = block_sum(rz, lds);
+    rr = wg_sum(rr, lds);
+    rz = wg_sum(rz, lds);
     if (threadIdx.x == 0) {
       part[ntiles + t] = rr;
       part[2 * ntiles + t] = rz;
@@ -322,19 +374,16 @@ __global__ __launch_bounds__(kBlock) void k_b_update(const int32_t *__restrict__
 }
 
 // monitor per system; beta; p = z + beta p.
-__global__ __launch_bounds__(kBlock) void k_b_pupdate(const int32_t *__restrict__ act,
-                                                       const int32_t *__restrict__ tsys,
-                                                       const int32_t *__restrict__ ts0,
-                                                       const int32_t *__restrict__ ts1, int ntiles,
+__global__ __launch_bounds__(kBlock) void k_b_pupdate(const int4 *__restrict__ act, int ntiles,
                                                        const int32_t *__restrict__ sys_t0,
                                                        const double *__restrict__ rD,
                                                        const double *__restrict__ r,
                                                        double *__restrict__ p,
                                                        const double *__restrict__ part,
-                                                       BGlobal *g, BSys *sys) {
-  __shared__ double lds[kWavesPerBlock];
-  const int it = g->it + 1;
-  FOR_ACTIVE_TILES({
+                                                       BGlobal *g, BSys *sys, int j) {
+  __shared__ double lds[kMaxWaves];
+  const int it = g->base + j + 1;
+  FOR_ACTIVE_TILES(true, {
     const int t0 = sys_t0[k], t1 = sys_t0[k + 1];
     const double RR = sys_sum(part + ntiles, t0, t1, lds);
     const double RZ = sys_sum(part + 2 * ntiles, t0, t1, lds);
@@ -356,7 +405,6 @@ __global__ __launch_bounds__(kBlock) void k_b_pupdate(const int32_t *__restrict_
     }
     if (stop) continue;
     const double beta = RZ / S.rz[(it - 1) & 1];
-    const int s0 = ts0[t], s1 = ts1[t];
     TILE_ROWS({
       const double ri = r[row];
       const double zi = rD ? ri / rD[row] : ri;
@@ -365,16 +413,12 @@ __global__ __launch_bounds__(kBlock) void k_b_pupdate(const int32_t *__restrict_
   })
 }
 
-__global__ void k_b_tick(BGlobal *g) {
-  if (threadIdx.x == 0 && blockIdx.x == 0) g->it += 1;
-}
-
 inline size_t align256(size_t b) { return (b + 255) & ~size_t(255); }
 
 struct BWork {
   BGlobal *g;
   BSys *sys;
-  int32_t *act;
+  int4 *act;
   double *part;
   double *dp, *xp, *bp, *rD, *r, *p, *Ap, *packed;
 };
@@ -389,7 +433,7 @@ size_t batch_layout(const pfm_batch_s *B, int ntiles, BWork *w, char *base) {
   BWork t;
   t.g = (BGlobal *)take(sizeof(BGlobal));
   t.sys = (BSys *)take(sizeof(BSys) * (size_t)B->ns);
-  t.act = (int32_t *)take(sizeof(int32_t) * (size_t)std::max(ntiles, 1));
+  t.act = (int4 *)take(sizeof(int4) * (size_t)std::max(ntiles, 1));
   t.part = (double *)take(sizeof(double) * 3 * (size_t)std::max(ntiles, 1));
   const size_t np = (size_t)B->np;
   t.dp = (double *)take(sizeof(double) * np);
@@ -433,21 +477,26 @@ int build_tiles(pfm_batch_s *B, int tile_slices) {
 // Workgroups of the iteration kernels (they walk the active-tile list).
 constexpr int kBatchGrid = 2048;
 
-void launch_compact(const pfm_batch_s *B, const BWork &w, hipStream_t q) {
-  k_b_compact<<<1, 1024, 0, q>>>(B->ns, B->d_sys_t0, w.sys, w.act, w.g);
+inline int iter_grid(const pfm_batch_s *B) {
+  const int G = std::min(B->ntiles, kBatchGrid);
+  return G >= kXcds ? G / kXcds * kXcds : std::max(G, 1);
 }
 
-void launch_iteration(const pfm_batch_s *B, const BWork &w, const double *pd, hipStream_t q) {
+void launch_compact(const pfm_batch_s *B, const BWork &w, int chunk, hipStream_t q) {
+  k_b_compact<<<1, 1024, 0, q>>>(B->ns, B->d_sys_t0, B->d_tile_s0, B->d_tile_s1, w.sys, w.act,
+                                 w.g, chunk);
+}
+
+// Iteration j of a chunk: three kernels.
+void launch_iteration(const pfm_batch_s *B, const BWork &w, const double *pd, int j,
+                      hipStream_t q) {
   const pfm_ldu L = B->ldu;
-  const int T = B->ntiles;
-  const int G = std::max(1, std::min(T, kBatchGrid));
-  k_b_spmv<<<G, kBlock, 0, q>>>(w.act, B->d_tile_sys, B->d_tile_s0, B->d_tile_s1, L->d_slices,
-                                L->d_col, w.packed, w.dp, w.p, w.Ap, w.part, w.g, w.sys);
-  k_b_update<<<G, kBlock, 0, q>>>(w.act, B->d_tile_sys, B->d_tile_s0, B->d_tile_s1, T, B->d_sys_t0,
-                                  pd, w.p, w.Ap, w.xp, w.r, w.part, w.g, w.sys);
-  k_b_pupdate<<<G, kBlock, 0, q>>>(w.act, B->d_tile_sys, B->d_tile_s0, B->d_tile_s1, T,
-                                   B->d_sys_t0, pd, w.r, w.p, w.part, w.g, w.sys);
-  k_b_tick<<<1, 1, 0, q>>>(w.g);
+  const int T = B->ntiles, G = iter_grid(B);
+  k_b_spmv<<<G, kBlock, 0, q>>>(w.act, L->d_slices, L->d_col, w.packed, w.dp, w.p, w.Ap, w.part,
+                                w.g, w.sys);
+  k_b_update<<<G, kBlock, 0, q>>>(w.act, T, B->d_sys_t0, pd, w.p, w.Ap, w.xp, w.r, w.part, w.g,
+                                  w.sys, j);
+  k_b_pupdate<<<G, kBlock, 0, q>>>(w.act, T, B->d_sys_t0, pd, w.r, w.p, w.part, w.g, w.sys, j);
 }
 
 constexpr int kTileCand[6] = {1, 2, 4, 8, 16, 32};
@@ -483,10 +532,10 @@ int autotune(pfm_batch_s *B) {
       break;
     }
     k_b_state_init<<<(B->ns + 255) / 256, 256>>>(w.g, w.sys, B->ns, -1.0, -1.0, 0x7fffffff, 0);
-    launch_compact(B, w, nullptr);
-    for (int rep = 0; rep < 2; ++rep) launch_iteration(B, w, w.rD, nullptr);
+    launch_compact(B, w, kTuneIters + 2, nullptr);
+    for (int rep = 0; rep < 2; ++rep) launch_iteration(B, w, w.rD, rep, nullptr);
     ok = hipEventRecord(e0, nullptr) == hipSuccess;
-    for (int rep = 0; ok && rep < kTuneIters; ++rep) launch_iteration(B, w, w.rD, nullptr);
+    for (int rep = 0; ok && rep < kTuneIters; ++rep) launch_iteration(B, w, w.rD, rep + 2, nullptr);
     float ms = 0.f;
     ok = ok && hipEventRecord(e1, nullptr) == hipSuccess && hipEventSynchronize(e1) == hipSuccess &&
          hipEventElapsedTime(&ms, e0, e1) == hipSuccess;
@@ -629,7 +678,7 @@ int pfm_batch_pcg_solve(pfm_batch B, const pfm_solve_params *prm, int transpose,
                                 w.packed, w.dp, pd, w.bp, w.xp, w.r, w.p, w.part);
   k_b_init_fin<<<(ns + 255) / 256, 256, 0, s>>>(ns, B->d_sys_t0, T, w.part, w.g, w.sys);
   PFM_CHECK_HIP(hipGetLastError());
-  auto iteration = [&](hipStream_t q) { launch_iteration(B, w, pd, q); };
+  auto iteration = [&](int j, hipStream_t q) { launch_iteration(B, w, pd, j, q); };
   const int chunk = std::max(1, graph_chunk);
   hipGraphExec_t exec = nullptr;
   hipStream_t cap = nullptr;
@@ -637,8 +686,8 @@ int pfm_batch_pcg_solve(pfm_batch B, const pfm_solve_params *prm, int transpose,
     PFM_CHECK_HIP(hipStreamCreateWithFlags(&cap, hipStreamNonBlocking));
     hipGraph_t graph = nullptr;
     PFM_CHECK_HIP(hipStreamBeginCapture(cap, hipStreamCaptureModeThreadLocal));
-    launch_compact(B, w, cap);
-    for (int k = 0; k < chunk; ++k) iteration(cap);
+    launch_compact(B, w, chunk, cap);
+    for (int k = 0; k < chunk; ++k) iteration(k, cap);
     hipError_t e = hipStreamEndCapture(cap, &graph);
     if (e == hipSuccess) e = hipGraphInstantiate(&exec, graph, nullptr, nullptr, 0);
     if (graph) (void)hipGraphDestroy(graph);
@@ -666,8 +715,8 @@ int pfm_batch_pcg_solve(pfm_batch B, const pfm_solve_params *prm, int transpose,
         enq += chunk;
       } else {
         const int c = std::min(8, prm->max_iter - enq);
-        launch_compact(B, w, s);
-        for (int k = 0; k < c; ++k) iteration(s);
+        launch_compact(B, w, c, s);
+        for (int k = 0; k < c; ++k) iteration(k, s);
         enq += c;
       }
       if (hipMemcpyAsync(&flag[slot], &w.g->ndone, sizeof(int32_t), hipMemcpyDeviceToHost, s) !=

@@ -80,8 +80,33 @@ __device__ __forceinline__ T row_spmv(const SliceInfo si, int lane, T dx,
                                       const T *__restrict__ x) {
   constexpr int U = 4;
   T accL = T(0), accU = T(0);
-  int base = si.ptr + lane;
   const int wl = si.wl, wu = si.wu;
+  const int bl = si.ptr + lane, bu = si.ptr + wl * kSlice + lane;
+  if (wl <= U && wu <= U) {
+    // Common case (hexahedral and most polyhedral rows): all column and
+    // coefficient loads of both groups in flight at once, then the gathers,
+    // then the two sums in face order.
+    int c[2 * U];
+    T v[2 * U];
+#pragma unroll
+    for (int j = 0; j < U; ++j) {
+      c[j] = j < wl ? col[bl + j * kSlice] : -1;
+      v[j] = j < wl ? val[bl + j * kSlice] : T(0);
+      c[U + j] = j < wu ? col[bu + j * kSlice] : -1;
+      v[U + j] = j < wu ? val[bu + j * kSlice] : T(0);
+    }
+    T xv[2 * U];
+#pragma unroll
+    for (int j = 0; j < 2 * U; ++j) xv[j] = c[j] >= 0 ? x[c[j]] : T(0);
+#pragma unroll
+    for (int j = 0; j < U; ++j)
+      if (c[j] >= 0) accL += v[j] * xv[j];
+#pragma unroll
+    for (int j = 0; j < U; ++j)
+      if (c[U + j] >= 0) accU += v[U + j] * xv[U + j];
+    return (dx + accL) + accU;
+  }
+  int base = bl;
   for (int j0 = 0; j0 < wl; j0 += U) {
     int c[U];
     T v[U];
@@ -95,7 +120,7 @@ __device__ __forceinline__ T row_spmv(const SliceInfo si, int lane, T dx,
       if (c[j] >= 0) accL += v[j] * x[c[j]];
     base += U * kSlice;
   }
-  base = si.ptr + wl * kSlice + lane;
+  base = bu;
   for (int j0 = 0; j0 < wu; j0 += U) {
     int c[U];
     T v[U];
