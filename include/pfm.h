@@ -73,6 +73,15 @@ int pfm_dilu_levels(int32_t n_cells, int32_t n_faces, const int32_t *owner_h,
                     const int32_t *neighbour_h, int32_t *level_out_h,
                     int32_t *n_levels);
 
+/* Sweep levels of the symmetric Gauss-Seidel smoother
+ * (solvers/smoothers.py:92-149): in the forward sweep cell i reads the
+ * updated x of every neighbour j < i, so fwd_level[i] = 1 + max over those;
+ * in the backward sweep likewise for neighbours j > i.  Cells of one level
+ * are independent, so level-by-level updates equal the sequential sweeps. */
+int pfm_gs_levels(int32_t n_cells, int32_t n_faces, const int32_t *owner_h,
+                  const int32_t *neighbour_h, int32_t *fwd_level_out_h,
+                  int32_t *bwd_level_out_h, int32_t *n_fwd_levels, int32_t *n_bwd_levels);
+
 /* Greedy aggregation of GAMGSolver._build_aggregates (solvers/gamg.py:164-239)
  * on one level: f2c_out[n_cells], *n_coarse = aggregate count. */
 int pfm_gamg_aggregate(int32_t n_cells, int32_t n_faces, const int32_t *owner_h,
@@ -238,6 +247,26 @@ int pfm_batch_pcg_solve(pfm_batch b, const pfm_solve_params *prm, int transpose,
                         const double *rhs, double *x, void *work, int32_t graph_chunk,
                         int32_t *iterations_h, int32_t *status_h, double *initial_res_h,
                         double *final_res_h, void *stream);
+
+/* ------------------------------------------------------------------ */
+/* Smoothers                                                          */
+/* ------------------------------------------------------------------ */
+
+/* GaussSeidelSmoother.smooth (solvers/smoothers.py:92-149): n_sweeps times a
+ * forward sweep over cells 0..n-1 then a backward sweep n-1..0; each cell's
+ * row sum runs over its faces in face order with the latest x (diag*x first),
+ * off = row - diag*x, and x += omega*((b - off)/diag - x) where |diag| >
+ * 1e-30.  x is updated in place (the reference returns a smoothed copy).
+ * packed = pfm_ldu_pack(h, 0, lower, upper).  Level-scheduled: one launch
+ * per sweep level, or one workgroup walking every level for small meshes. */
+int pfm_gs_smooth(pfm_ldu h, const double *diag, const double *packed, const double *b,
+                  double *x, int32_t n_sweeps, double omega, void *stream);
+
+/* JacobiSmoother.smooth (solvers/smoothers.py:180-200): n_iter times
+ * x = x + (omega * 1/max(|diag|, 1e-30)) * (b - A x), in place on x; work
+ * holds n_cells doubles. */
+int pfm_jacobi_smooth(pfm_ldu h, const double *diag, const double *packed, const double *b,
+                      double *x, int32_t n_iter, double omega, double *work, void *stream);
 
 /* ------------------------------------------------------------------ */
 /* GAMG                                                               */

@@ -8,6 +8,7 @@ Restates:
   * ``pair_levels``    -- PairGamgAgglomeration.build_levels/_pair_cells
                           (fv_agglomeration/pair_agglomeration.py:47-131)
   * ``gauss_seidel``   -- GaussSeidelSmoother.smooth (solvers/smoothers.py:92-149)
+  * ``jacobi``         -- JacobiSmoother.smooth (solvers/smoothers.py:173-200)
 The sequential graph passes and sweeps run in ``c/oracle.c``; the V-cycle
 arithmetic is numpy with the reference's op order.
 """
@@ -210,4 +211,18 @@ def gauss_seidel(diag, lower, upper, owner, neighbour, b, x, n_iterations, omega
     xx = np.array(x, dtype=np.float64, copy=True)
     clib().orc_gauss_seidel(d.shape[0], o.shape[0], p(o), p(q), p(d), p(lo), p(up), p(bb),
                             float(omega), int(n_iterations), p(xx))
+    return xx
+
+
+def jacobi(diag, lower, upper, owner, neighbour, b, x, n_iterations, omega=2.0 / 3.0):
+    """JacobiSmoother: inv = 1/max(|diag|, 1e-30) once (:178); each iteration
+    r = b - A x, x = x + (omega * inv) * r (:196-198, Python's left-to-right
+    product)."""
+    d = f64(diag)
+    inv = 1.0 / np.maximum(np.abs(d), 1e-30)
+    xx = np.array(x, dtype=np.float64, copy=True)
+    bb = f64(b)
+    for _ in range(int(n_iterations)):
+        r = bb - ax(d, f64(lower), f64(upper), owner, neighbour, xx)
+        xx = xx + (omega * inv) * r
     return xx

@@ -69,6 +69,7 @@ _SIGS = {
     "pfm_rcm": (INT, [I32, I32, P, P, P]),
     "pfm_dic_levels": (INT, [I32, I32, P, P, P, P]),
     "pfm_dilu_levels": (INT, [I32, I32, P, P, P, P]),
+    "pfm_gs_levels": (INT, [I32, I32, P, P, P, P, P, P]),
     "pfm_gamg_aggregate": (INT, [I32, I32, P, P, I32, P, P]),
     "pfm_pair_agglomerate": (INT, [I32, I32, P, P, I32, P, P, P]),
     "pfm_ldu_create": (INT, [I32, I32, P, P, C.POINTER(P)]),
@@ -80,6 +81,8 @@ _SIGS = {
     "pfm_spmv_f32": (INT, [P, P, P, P, P, P]),
     "pfm_spmv_ldu": (INT, [P, INT, P, P, P, P, P, P]),
     "pfm_residual": (INT, [P, P, P, P, P, P, P]),
+    "pfm_gs_smooth": (INT, [P, P, P, P, P, I32, C.c_double, P]),
+    "pfm_jacobi_smooth": (INT, [P, P, P, P, P, I32, C.c_double, P, P]),
     "pfm_dic_factor": (INT, [P, P, P, P, P]),
     "pfm_dilu_factor": (INT, [P, INT, P, P, P, P, P]),
     "pfm_pcg_workspace_bytes": (I64, [P, I32]),

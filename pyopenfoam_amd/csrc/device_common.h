@@ -137,6 +137,9 @@ __device__ __forceinline__ T row_spmv(const SliceInfo si, int lane, T dx,
   return (dx + accL) + accU;
 }
 
+// Slice descriptor for a slice index that is the same on every lane of the
+// wave (the row-slice kernels: one wave per slice); the value is moved to
+// scalar registers.
 __device__ __forceinline__ SliceInfo load_slice(const SliceInfo *sl, int s) {
   const int2 raw = reinterpret_cast<const int2 *>(sl)[s];
   SliceInfo si;
@@ -144,6 +147,16 @@ __device__ __forceinline__ SliceInfo load_slice(const SliceInfo *sl, int s) {
   const int w = uniform(raw.y);
   si.wl = (int16_t)(w & 0xffff);
   si.wu = (int16_t)((unsigned)w >> 16);
+  return si;
+}
+
+// Slice descriptor when lanes of a wave may address different slices.
+__device__ __forceinline__ SliceInfo load_slice_lane(const SliceInfo *sl, int s) {
+  const int2 raw = reinterpret_cast<const int2 *>(sl)[s];
+  SliceInfo si;
+  si.ptr = raw.x;
+  si.wl = (int16_t)(raw.y & 0xffff);
+  si.wu = (int16_t)((unsigned)raw.y >> 16);
   return si;
 }
 

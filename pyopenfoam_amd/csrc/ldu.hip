@@ -344,29 +344,6 @@ constexpr int kDiluLaunchLevels = 64;
 inline int grid1d(int64_t n) { return (int)((n + kBlock - 1) / kBlock); }
 
 template <typename T>
-int check_dev(const T *p, const char *name) {
-  if (!p) {
-    set_error("null device pointer: %s", name);
-    return PFM_ERR_ARG;
-  }
-  return PFM_OK;
-}
-
-#define PFM_NONNULL(p)                                                         \
-  do {                                                                         \
-    if (check_dev(p, #p) != PFM_OK) return PFM_ERR_ARG;                        \
-  } while (0)
-
-#define PFM_LAUNCH_CHECK()                                                     \
-  do {                                                                         \
-    hipError_t e_ = hipGetLastError();                                         \
-    if (e_ != hipSuccess) {                                                    \
-      set_error("kernel launch failed: %s", hipGetErrorString(e_));            \
-      return PFM_ERR_HIP;                                                      \
-    }                                                                          \
-  } while (0)
-
-template <typename T>
 int upload(T **dst, const std::vector<T> &src) {
   *dst = nullptr;
   if (src.empty()) return PFM_OK;
@@ -394,6 +371,10 @@ int pfm_ldu_destroy(pfm_ldu h) {
   (void)hipFree(h->d_dilu_level_ptr);
   (void)hipFree(h->d_dilu_pq);
   (void)hipFree(h->d_dilu_lu);
+  (void)hipFree(h->d_gs_fwd_ptr);
+  (void)hipFree(h->d_gs_fwd_cells);
+  (void)hipFree(h->d_gs_bwd_ptr);
+  (void)hipFree(h->d_gs_bwd_cells);
   if (h->h_flag) (void)hipHostFree(h->h_flag);
   (void)hipSetDevice(dev_prev);
   delete h;
@@ -415,6 +396,8 @@ int pfm_ldu_create(int32_t n, int32_t nf, const int32_t *own, const int32_t *nei
   h->max_upper = L.max_upper;
   h->dic_level_ptr = L.dic_level_ptr;
   h->dilu_level_ptr = L.dilu_level_ptr;
+  h->gs_fwd_ptr = L.gs_fwd_ptr;
+  h->gs_bwd_ptr = L.gs_bwd_ptr;
   hipError_t e = hipGetDevice(&h->device);
   if (e != hipSuccess) {
     delete h;
@@ -426,7 +409,11 @@ int pfm_ldu_create(int32_t n, int32_t nf, const int32_t *own, const int32_t *nei
       (st = upload(&h->d_map, L.map)) || (st = upload(&h->d_own, o)) ||
       (st = upload(&h->d_nei, q)) || (st = upload(&h->d_dic_cells, L.dic_cells)) ||
       (st = upload(&h->d_dilu_faces, L.dilu_faces)) ||
-      (st = upload(&h->d_dilu_level_ptr, L.dilu_level_ptr))) {
+      (st = upload(&h->d_dilu_level_ptr, L.dilu_level_ptr)) ||
+      (st = upload(&h->d_gs_fwd_ptr, L.gs_fwd_ptr)) ||
+      (st = upload(&h->d_gs_fwd_cells, L.gs_fwd_cells)) ||
+      (st = upload(&h->d_gs_bwd_ptr, L.gs_bwd_ptr)) ||
+      (st = upload(&h->d_gs_bwd_cells, L.gs_bwd_cells))) {
     pfm_ldu_destroy(h);
     return st;
   }

@@ -29,6 +29,24 @@ void set_error(const char *fmt, ...);
     }                                                                          \
   } while (0)
 
+// Entry points reject null device pointers with a named error.
+#define PFM_NONNULL(p)                                                         \
+  do {                                                                         \
+    if (!(p)) {                                                                \
+      ::pfm::set_error("null device pointer: %s", #p);                         \
+      return PFM_ERR_ARG;                                                      \
+    }                                                                          \
+  } while (0)
+
+#define PFM_LAUNCH_CHECK()                                                     \
+  do {                                                                         \
+    hipError_t e_ = hipGetLastError();                                         \
+    if (e_ != hipSuccess) {                                                    \
+      ::pfm::set_error("kernel launch failed: %s", hipGetErrorString(e_));     \
+      return PFM_ERR_HIP;                                                      \
+    }                                                                          \
+  } while (0)
+
 constexpr int kSlice = 64;     // rows per slice = one wavefront
 constexpr int kBlock = 256;    // threads per workgroup
 constexpr int kWavesPerBlock = kBlock / kSlice;
@@ -51,6 +69,8 @@ struct HostLayout {
   std::vector<int32_t> map;   // face of each slot, -1 = padding
   std::vector<int32_t> dic_level_ptr, dic_cells;
   std::vector<int32_t> dilu_level_ptr, dilu_faces;
+  std::vector<int32_t> gs_fwd_ptr, gs_fwd_cells;  // Gauss-Seidel sweep levels
+  std::vector<int32_t> gs_bwd_ptr, gs_bwd_cells;
 };
 
 int build_host_layout(int32_t n, int32_t nf, const int32_t *own,
@@ -80,6 +100,9 @@ struct pfm_ldu_s {
   int32_t *d_dilu_level_ptr = nullptr;
   void *d_dilu_pq = nullptr;            // level-ordered (owner, neighbour), lazily allocated
   double *d_dilu_lu = nullptr;          // level-ordered lower*upper
+  std::vector<int32_t> gs_fwd_ptr, gs_bwd_ptr;  // host copies of GS level offsets
+  int32_t *d_gs_fwd_ptr = nullptr, *d_gs_fwd_cells = nullptr;
+  int32_t *d_gs_bwd_ptr = nullptr, *d_gs_bwd_cells = nullptr;
   int32_t *h_flag = nullptr;            // pinned: solver progress polling
 };
 

@@ -127,6 +127,22 @@ int build_host_layout(int32_t n, int32_t nf, const int32_t *own,
     std::vector<int32_t> c(L.dilu_level_ptr.begin(), L.dilu_level_ptr.end() - 1);
     for (int32_t f = 0; f < nf; ++f) L.dilu_faces[c[flev[f]]++] = f;
   }
+  // Gauss-Seidel forward / backward sweep levels, grouped.
+  std::vector<int32_t> gf(n), gb(n);
+  int32_t ngf = 0, ngb = 0;
+  if (pfm_gs_levels(n, nf, own, nei, gf.data(), gb.data(), &ngf, &ngb) != PFM_OK)
+    return PFM_ERR_ARG;
+  auto group = [n](const std::vector<int32_t> &lev, int32_t nl, std::vector<int32_t> &ptr,
+                   std::vector<int32_t> &cells) {
+    ptr.assign(nl + 1, 0);
+    for (int32_t i = 0; i < n; ++i) ptr[lev[i] + 1]++;
+    for (int32_t l = 0; l < nl; ++l) ptr[l + 1] += ptr[l];
+    cells.resize(n);
+    std::vector<int32_t> c(ptr.begin(), ptr.end() - 1);
+    for (int32_t i = 0; i < n; ++i) cells[c[lev[i]]++] = i;
+  };
+  group(gf, ngf, L.gs_fwd_ptr, L.gs_fwd_cells);
+  group(gb, ngb, L.gs_bwd_ptr, L.gs_bwd_cells);
   return PFM_OK;
 }
 
@@ -231,6 +247,45 @@ int pfm_dic_levels(int32_t n, int32_t nf, const int32_t *own, const int32_t *nei
     nl = std::max(nl, l + 1);
   }
   if (n_levels) *n_levels = nl;
+  return PFM_OK;
+}
+
+int pfm_gs_levels(int32_t n, int32_t nf, const int32_t *own, const int32_t *nei,
+                  int32_t *fwd_level, int32_t *bwd_level, int32_t *n_fwd, int32_t *n_bwd) {
+  int st = validate_topology(n, nf, own, nei);
+  if (st != PFM_OK) return st;
+  PFM_REQUIRE(n == 0 || (fwd_level && bwd_level), "null output");
+  // cell adjacency from both sides of every face
+  std::vector<int32_t> ptr(n + 1, 0), adj(2 * (size_t)nf);
+  for (int32_t f = 0; f < nf; ++f) {
+    ptr[own[f] + 1]++;
+    ptr[nei[f] + 1]++;
+  }
+  for (int32_t i = 0; i < n; ++i) ptr[i + 1] += ptr[i];
+  {
+    std::vector<int32_t> c(ptr.begin(), ptr.end() - 1);
+    for (int32_t f = 0; f < nf; ++f) {
+      adj[c[own[f]]++] = nei[f];
+      adj[c[nei[f]]++] = own[f];
+    }
+  }
+  int32_t nlf = 0, nlb = 0;
+  for (int32_t i = 0; i < n; ++i) {  // forward sweep: i reads the new x of every j < i
+    int32_t l = 0;
+    for (int32_t k = ptr[i]; k < ptr[i + 1]; ++k)
+      if (adj[k] < i) l = std::max(l, fwd_level[adj[k]] + 1);
+    fwd_level[i] = l;
+    nlf = std::max(nlf, l + 1);
+  }
+  for (int32_t i = n - 1; i >= 0; --i) {  // backward sweep: i reads the new x of every j > i
+    int32_t l = 0;
+    for (int32_t k = ptr[i]; k < ptr[i + 1]; ++k)
+      if (adj[k] > i) l = std::max(l, bwd_level[adj[k]] + 1);
+    bwd_level[i] = l;
+    nlb = std::max(nlb, l + 1);
+  }
+  if (n_fwd) *n_fwd = nlf;
+  if (n_bwd) *n_bwd = nlb;
   return PFM_OK;
 }
 

@@ -6,7 +6,9 @@ from .pcg import PCGSolver
 from .gamg import GAMGSolver
 from .preconditioners import DICPreconditioner, DILUPreconditioner, Preconditioner
 from .residual import ConvergenceInfo, ResidualMonitor
+from .smoothers import GaussSeidelSmoother, JacobiSmoother, Smoother
 
 __all__ = ["LinearSolverBase", "create_solver", "solver_from_dict", "PCGSolver",
            "PBiCGSTABSolver", "GAMGSolver", "DICPreconditioner", "DILUPreconditioner",
-           "Preconditioner", "ConvergenceInfo", "ResidualMonitor"]
+           "Preconditioner", "ConvergenceInfo", "ResidualMonitor", "GaussSeidelSmoother",
+           "JacobiSmoother", "Smoother"]

@@ -112,3 +112,34 @@ def test_batch_create_rejects_bad_descriptions():
     st = L.pfm_batch_create(0, co.ctypes.data, fo.ctypes.data, own.ctypes.data, nei.ctypes.data,
                             0, C.byref(h))
     assert st == _lib.PFM_ERR_ARG
+
+
+@pytest.mark.parametrize("n,renumber,swap", [(6, True, False), (8, False, False), (7, True, True)])
+def test_gs_levels_are_the_sequential_dependency_depth(n, renumber, swap):
+    """Forward level of i = 1 + max forward level of its neighbours j < i
+    (0 if none); backward level likewise over neighbours j > i; for faces of
+    either orientation."""
+    c = poisson_case(n, jitter=0.1, seed=n, renumber=renumber)
+    own, nei = c.owner.astype(np.int32), c.neighbour.astype(np.int32)
+    if swap:  # mixed orientation, as on agglomerated levels
+        m = np.random.default_rng(0).random(len(own)) < 0.5
+        own, nei = np.where(m, nei, own).astype(np.int32), np.where(m, own, nei).astype(np.int32)
+    N = c.n_cells
+    fl = np.empty(N, np.int32)
+    bl = np.empty(N, np.int32)
+    nf, nb = C.c_int32(), C.c_int32()
+    L = _lib.lib()
+    assert L.pfm_gs_levels(N, len(own), own.ctypes.data, nei.ctypes.data, fl.ctypes.data,
+                           bl.ctypes.data, C.byref(nf), C.byref(nb)) == 0
+    adj = [[] for _ in range(N)]
+    for a, b in zip(own, nei):
+        adj[a].append(b)
+        adj[b].append(a)
+    ef = np.zeros(N, np.int64)
+    for i in range(N):
+        ef[i] = max([ef[j] + 1 for j in adj[i] if j < i], default=0)
+    eb = np.zeros(N, np.int64)
+    for i in range(N - 1, -1, -1):
+        eb[i] = max([eb[j] + 1 for j in adj[i] if j > i], default=0)
+    assert np.array_equal(fl, ef) and np.array_equal(bl, eb)
+    assert nf.value == ef.max() + 1 and nb.value == eb.max() + 1

@@ -240,3 +240,29 @@ def test_dilu_two_sided_update_known_answer():
     got = opre.dilu_build(d3, one, one, o3, q3)
     d1 = 6.0 - 1.0 / 4.0
     assert got.tolist() == [4.0 - 1.0 / 6.0, d1 - 1.0 / 4.0, 4.0 - 1.0 / d1]
+
+
+def test_jacobi_oracle_known_answers():
+    """JacobiSmoother known answers of the reference's tests
+    (tests/unit/solvers/test_smoothers.py:180-206): identity -> x = b, diag(2)
+    -> x = b/2 after one step with omega = 1; plus a 2-cell hand computation
+    that fails if the off-diagonal term, omega or the 1/|d| scaling is
+    dropped: d = (4, 5), lower = upper = -1, x0 = (1, 2), b = (3, 1),
+    omega = 0.5: A x0 = (4*1 - 2, 5*2 - 1) = (2, 9), r = (1, -8),
+    x1 = (1 + 0.5/4 * 1, 2 + 0.5/5 * -8) = (1.125, 1.2)."""
+    e = np.zeros(0, np.int64)
+    x = ogamg.jacobi(np.ones(3), np.zeros(0), np.zeros(0), e, e, np.array([2.0, 3.0, 4.0]),
+                     np.zeros(3), 1, omega=1.0)
+    assert np.array_equal(x, [2.0, 3.0, 4.0])
+    x = ogamg.jacobi(np.full(2, 2.0), np.zeros(0), np.zeros(0), e, e, np.array([4.0, 6.0]),
+                     np.zeros(2), 1, omega=1.0)
+    assert np.array_equal(x, [2.0, 3.0])
+    x = ogamg.jacobi(np.array([4.0, 5.0]), np.array([-1.0]), np.array([-1.0]), np.array([0]),
+                     np.array([1]), np.array([3.0, 1.0]), np.array([1.0, 2.0]), 1, omega=0.5)
+    np.testing.assert_allclose(x, [1.125, 1.2], rtol=0, atol=1e-15)
+    # the exact solution is a fixed point
+    A = np.array([[4.0, -1.0], [-1.0, 5.0]])
+    xs = np.linalg.solve(A, [3.0, 1.0])
+    x = ogamg.jacobi(np.array([4.0, 5.0]), np.array([-1.0]), np.array([-1.0]), np.array([0]),
+                     np.array([1]), np.array([3.0, 1.0]), xs, 3, omega=0.7)
+    np.testing.assert_allclose(x, xs, rtol=1e-14)
