@@ -280,16 +280,20 @@ typedef struct pfm_gamg_s *pfm_gamg;
                                   inside an aggregate are dropped */
 #define PFM_COARSE_GALERKIN 1  /* extension: they are added to the coarse diagonal,
                                   the Galerkin product R A P of piecewise-constant P */
+#define PFM_SMOOTHER_JACOBI 0       /* GAMGSolver._smooth (gamg.py:385-429) */
+#define PFM_SMOOTHER_GAUSS_SEIDEL 1 /* extension: every smoothing step on every level is a
+                                       symmetric GaussSeidelSmoother sweep
+                                       (smoothers.py:92-149), in place */
 
 typedef struct {
   int32_t max_levels;       /* coarsening steps (reference default 10) */
   int32_t min_cells_coarse; /* stop at or below this many cells (default 10) */
-  int32_t n_pre;            /* pre-smoothing Jacobi sweeps (default 2) */
-  int32_t n_post;           /* post-smoothing Jacobi sweeps (default 2) */
+  int32_t n_pre;            /* pre-smoothing sweeps (default 2) */
+  int32_t n_post;           /* post-smoothing sweeps (default 2) */
   int32_t n_coarsest;       /* sweeps on the coarsest level (reference: 50) */
   int32_t agglomeration;    /* PFM_AGGLOM_* */
   int32_t coarse;           /* PFM_COARSE_* */
-  int32_t reserved;
+  int32_t smoother;         /* PFM_SMOOTHER_* */
 } pfm_gamg_params;
 
 /* Build the hierarchy of GAMGSolver._solve (gamg.py:541-570) from the fine

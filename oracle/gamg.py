@@ -120,29 +120,40 @@ def smooth(lv: Level, x, b, n_it):
     return x
 
 
-def v_cycle(levels, aggs, k, x, b, n_pre=2, n_post=2):
+def smooth_gs(lv: Level, x, b, n_it):
+    """Level smoothing by symmetric Gauss-Seidel sweeps (the build's
+    ``level_smoother="GaussSeidel"`` extension; the reference GAMG uses
+    ``smooth``)."""
+    if n_it <= 0:
+        return x
+    return gauss_seidel(lv.diag, lv.lower, lv.upper, lv.owner, lv.neighbour, b, x, n_it)
+
+
+def v_cycle(levels, aggs, k, x, b, n_pre=2, n_post=2, smoother=smooth):
     lv = levels[k]
     if k == len(levels) - 1:
-        return smooth(lv, x, b, 50)
-    x = smooth(lv, x, b, n_pre)
+        return smoother(lv, x, b, 50)
+    x = smoother(lv, x, b, n_pre)
     r = b - _lax(lv, x)
     nagg, f2c = aggs[k]
     rc = _scatter(r, f2c, nagg)
     ec = np.zeros_like(rc)
-    ec = v_cycle(levels, aggs, k + 1, ec, rc, n_pre, n_post)
+    ec = v_cycle(levels, aggs, k + 1, ec, rc, n_pre, n_post, smoother)
     x = x + ec[f2c]
-    x = smooth(lv, x, b, n_post)
+    x = smoother(lv, x, b, n_post)
     return x
 
 
 def gamg(level0: Level, b, x0, *, tolerance=1e-6, rel_tol=0.01, max_iter=100, min_iter=0,
-         n_pre_smooth=2, n_post_smooth=2, max_levels=10, min_cells_coarse=10):
+         n_pre_smooth=2, n_post_smooth=2, max_levels=10, min_cells_coarse=10,
+         level_smoother="jacobi"):
     levels, aggs = build_hierarchy(level0, max_levels, min_cells_coarse)
     mon = Monitor(tolerance, rel_tol, min_iter)
     x = np.array(x0, dtype=np.float64, copy=True)
     converged = False
+    sm = smooth_gs if level_smoother == "gauss_seidel" else smooth
     for i in range(1, max_iter + 1):
-        x = v_cycle(levels, aggs, 0, x, b, n_pre_smooth, n_post_smooth)
+        x = v_cycle(levels, aggs, 0, x, b, n_pre_smooth, n_post_smooth, sm)
         r = b - _lax(level0, x)
         if mon.update(r, i):
             converged = True

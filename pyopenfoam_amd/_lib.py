@@ -43,10 +43,11 @@ class SolveResult(C.Structure):
 class GamgParams(C.Structure):
     _fields_ = [("max_levels", C.c_int32), ("min_cells_coarse", C.c_int32), ("n_pre", C.c_int32),
                 ("n_post", C.c_int32), ("n_coarsest", C.c_int32), ("agglomeration", C.c_int32),
-                ("coarse", C.c_int32), ("reserved", C.c_int32)]
+                ("coarse", C.c_int32), ("smoother", C.c_int32)]
 
 
 PFM_AGGLOM_GREEDY, PFM_AGGLOM_PAIR = 0, 1
+PFM_SMOOTHER_JACOBI, PFM_SMOOTHER_GAUSS_SEIDEL = 0, 1
 PFM_COARSE_REFERENCE, PFM_COARSE_GALERKIN = 0, 1
 
 

@@ -8,7 +8,11 @@
 //   * restriction   b_c[c] = sum of r over the aggregate, fine-cell order
 //     (_restrict :340-360, scatter_add order);
 //   * prolongation  x[i] += e_c[f2c[i]] (_prolongate :363-379);
-//   * the coarsest level smoothed n_coarsest (50) times (:458-461).
+//   * the coarsest level smoothed n_coarsest (50) times (:458-461);
+//   * optionally (PFM_SMOOTHER_GAUSS_SEIDEL, not in the reference GAMG) every
+//     smoothing step is instead a symmetric Gauss-Seidel sweep in place,
+//     level-scheduled like pfm_gs_smooth (one launch per sweep level on large
+//     levels, the single-workgroup tail walks the levels of small ones).
 // Levels at or below kTailCells run, together with everything coarser, in a
 // single-workgroup kernel (one launch instead of dozens of tiny ones).  After
 // each cycle r = b - A x is reduced and the residual monitor runs on the
@@ -98,6 +102,15 @@ __global__ __launch_bounds__(kBlock) void k_jacobi(LevelDev L, int cur, const KS
   if (row < L.n) L.x[cur ^ 1][row] = x[row] + L.inv[row] * (L.b[row] - ax);
 }
 
+// One Gauss-Seidel sweep level of a level's x[cur], in place.
+__global__ __launch_bounds__(kBlock) void k_gs_level(LevelDev L, int cur,
+                                                      const int32_t *__restrict__ cells, int count,
+                                                      const KState *st) {
+  if (st->done) return;
+  const int t = blockIdx.x * kBlock + threadIdx.x;
+  if (t < count) gs_cell(cells[t], L.slices, L.col, L.map, L.packed, L.diag, L.b, L.x[cur], 1.0);
+}
+
 __global__ __launch_bounds__(kBlock) void k_resid(LevelDev L, int cur, const KState *st) {
   if (st->done) return;
   const int s = xcd_remap(blockIdx.x, gridDim.x) * kWavesPerBlock + (threadIdx.x >> 6);
@@ -138,18 +151,39 @@ __device__ void tail_jacobi(const LevelDev &L, int cur) {
   __syncthreads();
 }
 
+// One symmetric Gauss-Seidel sweep of a small level by the whole workgroup.
+__device__ void tail_gs(const LevelDev &L, int cur) {
+  for (int pass = 0; pass < 2; ++pass) {
+    const int32_t *ptr = pass ? L.gs_bptr : L.gs_fptr;
+    const int32_t *cells = pass ? L.gs_bcells : L.gs_fcells;
+    const int nl = pass ? L.gs_nbl : L.gs_nfl;
+    for (int l = 0; l < nl; ++l) {
+      for (int k = ptr[l] + (int)threadIdx.x; k < ptr[l + 1]; k += kTailThreads)
+        gs_cell(cells[k], L.slices, L.col, L.map, L.packed, L.diag, L.b, L.x[cur], 1.0);
+      __syncthreads();
+    }
+  }
+}
+
+// One smoothing step: Jacobi flips the level's buffer, Gauss-Seidel does not.
+__device__ __forceinline__ void tail_smooth(const LevelDev &L, int &cur, bool gs) {
+  if (gs) {
+    tail_gs(L, cur);
+  } else {
+    tail_jacobi(L, cur);
+    cur ^= 1;
+  }
+}
+
 __global__ __launch_bounds__(kTailThreads) void k_tail(const LevelDev *__restrict__ Ls, int k0,
                                                         int nlev, int cur0, int n_pre, int n_post,
-                                                        int n_coarsest, const KState *st) {
+                                                        int n_coarsest, int gs, const KState *st) {
   if (st->done) return;
   int cur[kMaxLevels];
   cur[k0] = cur0;
   for (int k = k0; k < nlev - 1; ++k) {
     const LevelDev L = Ls[k];
-    for (int it = 0; it < n_pre; ++it) {
-      tail_jacobi(L, cur[k]);
-      cur[k] ^= 1;
-    }
+    for (int it = 0; it < n_pre; ++it) tail_smooth(L, cur[k], gs);
     for (int row0 = threadIdx.x; row0 < L.nslices * kSlice; row0 += kTailThreads) {
       const double ax = level_ax(L, row0 >> 6, row0 & 63, row0, L.x[cur[k]]);
       if (row0 < L.n) L.r[row0] = L.b[row0] - ax;
@@ -167,10 +201,7 @@ __global__ __launch_bounds__(kTailThreads) void k_tail(const LevelDev *__restric
   }
   {
     const LevelDev L = Ls[nlev - 1];
-    for (int it = 0; it < n_coarsest; ++it) {
-      tail_jacobi(L, cur[nlev - 1]);
-      cur[nlev - 1] ^= 1;
-    }
+    for (int it = 0; it < n_coarsest; ++it) tail_smooth(L, cur[nlev - 1], gs);
   }
   for (int k = nlev - 2; k >= k0; --k) {
     const LevelDev L = Ls[k];
@@ -178,10 +209,7 @@ __global__ __launch_bounds__(kTailThreads) void k_tail(const LevelDev *__restric
     for (int i = threadIdx.x; i < L.n; i += kTailThreads)
       L.x[cur[k]][i] = L.x[cur[k]][i] + xc[L.f2c[i]];
     __syncthreads();
-    for (int it = 0; it < n_post; ++it) {
-      tail_jacobi(L, cur[k]);
-      cur[k] ^= 1;
-    }
+    for (int it = 0; it < n_post; ++it) tail_smooth(L, cur[k], gs);
   }
 }
 
@@ -259,6 +287,13 @@ size_t gamg_layout(const pfm_gamg_s *g, int max_iter, GamgWork *w, char *base) {
     D.nslices = H.ldu->nslices;
     D.slices = H.ldu->d_slices;
     D.col = H.ldu->d_col;
+    D.map = H.ldu->d_map;
+    D.gs_fptr = H.ldu->d_gs_fwd_ptr;
+    D.gs_fcells = H.ldu->d_gs_fwd_cells;
+    D.gs_bptr = H.ldu->d_gs_bwd_ptr;
+    D.gs_bcells = H.ldu->d_gs_bwd_cells;
+    D.gs_nfl = (int32_t)H.ldu->gs_fwd_ptr.size() - 1;
+    D.gs_nbl = (int32_t)H.ldu->gs_bwd_ptr.size() - 1;
     t.packed[k] = (double *)take(sizeof(double) * (size_t)std::max<int64_t>(H.ldu->nslots, 1));
     D.packed = t.packed[k];
     D.inv = (double *)take(sizeof(double) * n);
@@ -360,6 +395,8 @@ int pfm_gamg_create(pfm_ldu fine, const pfm_gamg_params *prm, pfm_gamg *out) {
               "unknown agglomeration %d", prm->agglomeration);
   PFM_REQUIRE(prm->coarse == PFM_COARSE_REFERENCE || prm->coarse == PFM_COARSE_GALERKIN,
               "unknown coarse operator %d", prm->coarse);
+  PFM_REQUIRE(prm->smoother == PFM_SMOOTHER_JACOBI || prm->smoother == PFM_SMOOTHER_GAUSS_SEIDEL,
+              "unknown smoother %d", prm->smoother);
   *out = nullptr;
   pfm_gamg g = new pfm_gamg_s();
   g->fine = fine;
@@ -547,7 +584,21 @@ int pfm_gamg_solve(pfm_gamg g, const pfm_solve_params *prm, const double *diag, 
   while (tail_k > 0 && g->lv[tail_k - 1].n <= kTailCells) --tail_k;
   int cur0 = 0;
   const int G = red_grid(g->fine);
+  const bool gs = g->prm.smoother == PFM_SMOOTHER_GAUSS_SEIDEL;
   auto jac = [&](int k, int &cur) {
+    if (gs) {  // one symmetric sweep, level by level, in place
+      const LevelDev &D = w.lv[k];
+      const pfm_ldu h = g->lv[k].ldu;
+      for (int pass = 0; pass < 2; ++pass) {
+        const std::vector<int32_t> &ptr = pass ? h->gs_bwd_ptr : h->gs_fwd_ptr;
+        const int32_t *cells = pass ? h->d_gs_bwd_cells : h->d_gs_fwd_cells;
+        for (size_t l = 0; l + 1 < ptr.size(); ++l) {
+          const int cnt = ptr[l + 1] - ptr[l];
+          k_gs_level<<<grid1d(cnt), kBlock, 0, s>>>(D, cur, cells + ptr[l], cnt, w.st);
+        }
+      }
+      return;
+    }
     k_jacobi<<<blocks_for_slices(w.lv[k].nslices), kBlock, 0, s>>>(w.lv[k], cur, w.st);
     cur ^= 1;
   };
@@ -563,8 +614,8 @@ int pfm_gamg_solve(pfm_gamg g, const pfm_solve_params *prm, const double *diag, 
     }
     // levels tail_k .. L-1 in one workgroup
     k_tail<<<1, kTailThreads, 0, s>>>(w.d_levels, tail_k, L, cur[tail_k], npre, npost, ncoarse,
-                                      w.st);
-    cur[tail_k] ^= (tail_k == L - 1 ? ncoarse : npre + npost) & 1;
+                                      gs ? 1 : 0, w.st);
+    if (!gs) cur[tail_k] ^= (tail_k == L - 1 ? ncoarse : npre + npost) & 1;
     for (k = tail_k - 1; k >= 0; --k) {  // ascend
       k_prolong<<<grid1d(w.lv[k].n), kBlock, 0, s>>>(w.lv[k], cur[k], w.lv[k + 1].x[cur[k + 1]],
                                                     w.st);

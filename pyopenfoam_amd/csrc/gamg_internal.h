@@ -29,6 +29,10 @@ struct LevelDev {
   const int32_t *agg_ptr;
   const int32_t *agg_cells;
   int32_t nc;
+  // Gauss-Seidel smoothing: slot -> face map and the sweep level schedules
+  const int32_t *map;
+  const int32_t *gs_fptr, *gs_fcells, *gs_bptr, *gs_bcells;
+  int32_t gs_nfl, gs_nbl;
 };
 
 }  // namespace pfm

@@ -10,45 +10,6 @@ namespace {
 
 inline int grid1d(int64_t n) { return (int)((n + kBlock - 1) / kBlock); }
 
-// One Gauss-Seidel cell update (solvers/smoothers.py:113-149).  The row's
-// couplings sit in two groups of its slice (owner-faces, column = neighbour,
-// then neighbour-faces, column = owner), each in face order; merging them by
-// face index visits the faces in the reference's face order.
-__device__ __forceinline__ void gs_cell(int c, const SliceInfo *__restrict__ sl,
-                                        const int32_t *__restrict__ col,
-                                        const int32_t *__restrict__ map,
-                                        const double *__restrict__ val,
-                                        const double *__restrict__ diag,
-                                        const double *__restrict__ b, double *x, double omega) {
-  const SliceInfo si = load_slice_lane(sl, c / kSlice);  // lanes hold cells of any slice
-  const int lane = c % kSlice;
-  const int bl = si.ptr + lane, bu = si.ptr + si.wl * kSlice + lane;
-  const double d = diag[c];
-  const double xc = x[c];
-  double row = d * xc;
-  int jl = 0, ju = 0;
-  int cl = jl < si.wl ? col[bl] : -1;
-  int cu = ju < si.wu ? col[bu] : -1;
-  while (cl >= 0 || cu >= 0) {
-    bool take_l = cl >= 0;
-    if (cl >= 0 && cu >= 0) take_l = map[bl + jl * kSlice] < map[bu + ju * kSlice];
-    if (take_l) {
-      row += val[bl + jl * kSlice] * x[cl];
-      ++jl;
-      cl = jl < si.wl ? col[bl + jl * kSlice] : -1;
-    } else {
-      row += val[bu + ju * kSlice] * x[cu];
-      ++ju;
-      cu = ju < si.wu ? col[bu + ju * kSlice] : -1;
-    }
-  }
-  const double off = row - d * xc;
-  if (fabs(d) > 1e-30) {
-    const double nv = (b[c] - off) / d;
-    x[c] = xc + omega * (nv - xc);
-  }
-}
-
 // One sweep level: its cells are mutually independent.
 __global__ __launch_bounds__(kBlock) void k_gs_level(const int32_t *__restrict__ cells, int count,
                                                       const SliceInfo *__restrict__ sl,

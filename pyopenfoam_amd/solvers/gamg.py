@@ -14,7 +14,10 @@ Additive keywords (not in the reference): ``agglomeration="pair"`` builds the
 levels with PairGamgAgglomeration's pairing
 (``fv_agglomeration/pair_agglomeration.py``) and ``coarse="galerkin"`` adds the
 intra-aggregate couplings to the coarse diagonal (the Galerkin product of the
-piecewise-constant prolongation), which the reference drops.
+piecewise-constant prolongation), which the reference drops;
+``level_smoother="GaussSeidel"`` replaces every Jacobi smoothing step (pre,
+post and coarsest) by a symmetric Gauss-Seidel sweep with the arithmetic of
+``GaussSeidelSmoother`` (``solvers/smoothers.py:92-149``).
 
 The hierarchy topology is built once per mesh topology (host C++) and cached;
 coefficients are coarsened on the device at every solve, and the whole V-cycle
@@ -40,6 +43,8 @@ __all__ = ["GAMGSolver", "GamgHierarchy"]
 _AGGLOM = {"greedy": _lib.PFM_AGGLOM_GREEDY, "faceareapair": _lib.PFM_AGGLOM_PAIR,
            "pair": _lib.PFM_AGGLOM_PAIR}
 _COARSE = {"reference": _lib.PFM_COARSE_REFERENCE, "galerkin": _lib.PFM_COARSE_GALERKIN}
+_LEVEL_SMOOTHER = {"jacobi": _lib.PFM_SMOOTHER_JACOBI, "gaussseidel": _lib.PFM_SMOOTHER_GAUSS_SEIDEL,
+                   "gs": _lib.PFM_SMOOTHER_GAUSS_SEIDEL}
 
 
 class GamgHierarchy:
@@ -101,7 +106,8 @@ class GAMGSolver(LinearSolverBase):
                  min_iter: int = 0, verbose: bool = False, n_pre_smooth: int = 2,
                  n_post_smooth: int = 2, max_levels: int = 10, min_cells_coarse: int = 10,
                  smoother: str = "PCG", agglomeration: str = "greedy",
-                 coarse: str = "reference", n_coarsest: int = 50) -> None:
+                 coarse: str = "reference", n_coarsest: int = 50,
+                 level_smoother: str = "Jacobi") -> None:
         super().__init__(tolerance=tolerance, rel_tol=rel_tol, max_iter=max_iter,
                          min_iter=min_iter, verbose=verbose)
         self._n_pre_smooth = n_pre_smooth
@@ -114,8 +120,13 @@ class GAMGSolver(LinearSolverBase):
             raise ValueError(f"Unknown agglomeration '{agglomeration}'. Use 'greedy' or 'pair'.")
         if coarse.lower() not in _COARSE:
             raise ValueError(f"Unknown coarse operator '{coarse}'. Use 'reference' or 'galerkin'.")
+        sk = level_smoother.lower().replace("_", "").replace("-", "")
+        if sk not in _LEVEL_SMOOTHER:
+            raise ValueError(f"Unknown level smoother '{level_smoother}'. "
+                             "Use 'Jacobi' or 'GaussSeidel'.")
         self._params = _lib.GamgParams(max_levels, min_cells_coarse, n_pre_smooth, n_post_smooth,
-                                       n_coarsest, _AGGLOM[key], _COARSE[coarse.lower()], 0)
+                                       n_coarsest, _AGGLOM[key], _COARSE[coarse.lower()],
+                                       _LEVEL_SMOOTHER[sk])
         self._ws = KrylovWorkspace()
 
     def hierarchy(self, matrix: LduMatrix) -> GamgHierarchy:
