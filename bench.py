@@ -223,6 +223,8 @@ def main() -> int:
     ap.add_argument("--sigma", type=float, default=1.0)
     ap.add_argument("--history", default=None, help="write the residual history (.npy) here")
     ap.add_argument("--no-batch", action="store_true")
+    ap.add_argument("--no-gamg", action="store_true")
+    ap.add_argument("--gamg-max-iter", type=int, default=1000)
     ap.add_argument("--batch-systems", type=int, default=512)
     ap.add_argument("--batch-nmin", type=int, default=8)
     ap.add_argument("--batch-nmax", type=int, default=48)
@@ -384,6 +386,30 @@ def main() -> int:
         log(f"mixed: {mms:.1f} ms ({ms / mms:.2f}x), inner iters {mi.inner_iterations}, "
             f"steps {outer - 1}, true rel residual {rt:.3e}, err vs fp64 {err:.2e}")
 
+    # --- GAMG on the same system ------------------------------------------------
+    gamg = None
+    if not args.no_gamg:
+        from pyopenfoam_amd import GAMGSolver
+        gamg = {}
+        for name, kw in (("pair_galerkin_jacobi", dict(agglomeration="pair", coarse="galerkin",
+                                                        max_levels=20)),
+                         ("reference", dict())):
+            gs = GAMGSolver(tolerance=1e-300, rel_tol=1e-8, max_iter=args.gamg_max_iter, **kw)
+            hier = gs.hierarchy(mat)
+            mi = args.gamg_max_iter if name != "reference" else 100
+            mat.solve(gs, x0, 1e-300, 2)  # warm (setup of coarse handles and workspace)
+            torch.cuda.synchronize()
+            t0 = time.perf_counter()
+            xg, cyc, _ = mat.solve(gs, x0, 1e-300, mi)
+            torch.cuda.synchronize()
+            gms = (time.perf_counter() - t0) * 1e3
+            rt = (mat.source - mat.Ax(xg)).norm().item() / mat.source.norm().item()
+            gamg[name] = {"ms": round(gms, 2), "cycles": int(cyc), "ms_per_cycle": round(gms / cyc, 3),
+                          "converged": bool(gs.last_info.converged), "true_rel_residual": rt,
+                          "levels": hier.n_levels, **{k: v for k, v in kw.items()}}
+            log(f"gamg {name}: {gms:.1f} ms, {cyc} cycles ({gms / cyc:.2f} ms/cycle), converged "
+                f"{gs.last_info.converged}, true rel residual {rt:.2e}, {hier.n_levels} levels")
+
     # --- batching: many uneven systems in one graph-replayed PCG-DIC ---------
     batch = None
     if not args.no_batch:
@@ -429,6 +455,7 @@ def main() -> int:
         "mixed_precision": mixed,
         "adjoint": adjoint,
         "batch": batch,
+        "gamg": gamg,
         "roofline": {"bound": "hbm", "kernel": "k_spmv<double> (pfm_spmv)",
                      "achieved": round(spmv_gbs, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": round(spmv_gbs / HBM_PEAK_GBS, 4),
