@@ -35,10 +35,10 @@ sys.path.insert(0, ROOT)
 
 METRIC = "ms per 1e-8 Poisson solve @2.1M cells, SpMV GB/s vs HBM peak, adjoint/fwd"
 HBM_PEAK_GBS = 8000.0
-# HBM bytes per k_spmv launch on the 128^3 system, from rocprofv3 FETCH_SIZE and
+# Bytes per k_spmv launch on the 128^3 system leaving L2, from rocprofv3 FETCH_SIZE and
 # WRITE_SIZE runs of this script (FETCH_SIZE scaled by 2.0 and WRITE_SIZE by 1.0,
-# the factors a 512 MiB device copy reads): 2 x 92,979.8 KiB + 16,384 KiB.
-SPMV_TRAFFIC_128 = 2 * 92979.8 * 1024 + 16384 * 1024
+# the factors a 512 MiB device copy reads): 2 x 91,550.2 KiB + 16,384 KiB.
+SPMV_TRAFFIC_128 = 2 * 91550.2 * 1024 + 16384 * 1024
 
 
 def log(msg: str) -> None:
