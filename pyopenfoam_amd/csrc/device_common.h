@@ -52,7 +52,10 @@ __device__ __forceinline__ double sum_partials(const double *partials, int np, d
 // that their per-block partial sums (reduced by a one-workgroup finalize
 // kernel in a fixed order) stay few; each workgroup owns one contiguous range
 // of slices, its four waves interleaved over it.
-constexpr int kMaxRedBlocks = 2048;
+#ifndef PFM_MAX_RED_BLOCKS
+#define PFM_MAX_RED_BLOCKS 2048
+#endif
+constexpr int kMaxRedBlocks = PFM_MAX_RED_BLOCKS;
 
 struct SliceRange {
   int begin, end;

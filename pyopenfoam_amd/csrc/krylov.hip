@@ -134,6 +134,20 @@ __global__ __launch_bounds__(kBlock) void k_pcg_update(int n, int nslices, int i
   __shared__ double lds[kWavesPerBlock];
   if (st->done) return;
   const int G = gridDim.x;
+  // The first round's loads do not depend on alpha: issue them before the
+  // reduction so both latencies overlap.
+  const int lane = threadIdx.x & 63;
+  const int stride = G * kWavesPerBlock;
+  int s = xcd_remap(blockIdx.x, G) * kWavesPerBlock + (threadIdx.x >> 6);
+  int row = s * kSlice + lane;
+  double xp = 0.0, pp = 0.0, rp = 0.0, ap = 0.0, dp = 1.0;
+  if (s < nslices && row < n) {
+    xp = x[row];
+    pp = p[row];
+    rp = r[row];
+    ap = Ap[row];
+    if (rD) dp = rD[row];
+  }
   const double PAP = reduce_partials(part, G, lds);
   if (fabs(PAP) < 1e-30) {
     if (blockIdx.x == 0 && threadIdx.x == 0) finish(st, PFM_SOLVE_BREAKDOWN);
@@ -141,16 +155,24 @@ __global__ __launch_bounds__(kBlock) void k_pcg_update(int n, int nslices, int i
   }
   const double alpha = st->rzs[(it - 1) & 1] / PAP;
   double acc[2] = {0.0, 0.0};
-  FOR_ROWS({
+  for (; s < nslices; s += stride, row = s * kSlice + lane) {
     if (row < n) {
-      x[row] = x[row] + alpha * p[row];
-      const double ri = r[row] - alpha * Ap[row];
+      x[row] = xp + alpha * pp;
+      const double ri = rp - alpha * ap;
       r[row] = ri;
-      const double zi = rD ? ri / rD[row] : ri;
+      const double zi = rD ? ri / dp : ri;
       acc[0] += ri * ri;
       acc[1] += ri * zi;
     }
-  })
+    const int sn = s + stride, rn = sn * kSlice + lane;
+    if (sn < nslices && rn < n) {
+      xp = x[rn];
+      pp = p[rn];
+      rp = r[rn];
+      ap = Ap[rn];
+      if (rD) dp = rD[rn];
+    }
+  }
   store_partials<2>(acc, part + G, lds);
 }
 
@@ -164,6 +186,18 @@ __global__ __launch_bounds__(kBlock) void k_pcg_pupdate(int n, int nslices, int 
   __shared__ double lds[kWavesPerBlock];
   if (st->done) return;
   const int G = gridDim.x;
+  // First round's r, rD, p loaded before the reductions (they do not depend on
+  // beta), so their latency overlaps the partial sums'.
+  const int lane = threadIdx.x & 63;
+  const int stride = G * kWavesPerBlock;
+  int s = xcd_remap(blockIdx.x, G) * kWavesPerBlock + (threadIdx.x >> 6);
+  int row = s * kSlice + lane;
+  double rp = 0.0, dp = 1.0, pp = 0.0;
+  if (s < nslices && row < n) {
+    rp = r[row];
+    if (rD) dp = rD[row];
+    pp = p[row];
+  }
   const double RR = reduce_partials(part + G, G, lds);
   const double RZ = reduce_partials(part + 2 * G, G, lds);
   const double res = sqrt(RR);
@@ -180,13 +214,18 @@ __global__ __launch_bounds__(kBlock) void k_pcg_pupdate(int n, int nslices, int 
   }
   if (conv || it >= st->max_iter) return;
   const double beta = RZ / rz_old;
-  FOR_ROWS({
+  for (; s < nslices; s += stride, row = s * kSlice + lane) {
     if (row < n) {
-      const double ri = r[row];
-      const double zi = rD ? ri / rD[row] : ri;
-      p[row] = zi + beta * p[row];
+      const double zi = rD ? rp / dp : rp;
+      p[row] = zi + beta * pp;
     }
-  })
+    const int sn = s + stride, rn = sn * kSlice + lane;
+    if (sn < nslices && rn < n) {
+      rp = r[rn];
+      if (rD) dp = rD[rn];
+      pp = p[rn];
+    }
+  }
 }
 
 // ---------------------------------------------------------- PBiCGStab ------
