@@ -267,7 +267,7 @@ __global__ __launch_bounds__(kBlock) void k_bs_init_fin(const double *__restrict
       finish(st, PFM_SOLVE_CONVERGED);
     else if (st->max_iter < 1)
       finish(st, PFM_SOLVE_MAXITER);
-    st->rho = RR;  // rho = rt . r with rt == r
+    st->rhos[0] = RR;  // rho = rt . r with rt == r
     if (!st->done && fabs(RR) < 1e-30) finish(st, PFM_SOLVE_BREAKDOWN);
   }
 }
@@ -296,28 +296,25 @@ __global__ __launch_bounds__(kBlock) void k_bs_spmv1(int n, int nslices, const S
   store_partials<1>(acc, part, lds);
 }
 
-__global__ __launch_bounds__(kBlock) void k_bs_alpha(const double *__restrict__ part, int G,
-                                                      KState *st) {
-  __shared__ double lds[kWavesPerBlock];
-  if (st->done) return;
-  const double RTV = sum_partials(part, G, lds);
-  if (threadIdx.x == 0) {
-    if (fabs(RTV) < 1e-30)
-      finish(st, PFM_SOLVE_BREAKDOWN);
-    else
-      st->alpha = st->rho / RTV;
-  }
-}
-
-// s = r - alpha v; z = M^-1 s; partial s.s.
-__global__ __launch_bounds__(kBlock) void k_bs_s(int n, int nslices, const double *__restrict__ rD,
+// Every workgroup: rt.v from region A, breakdown test, alpha = rho / rt.v;
+// s = r - alpha v; z = M^-1 s; partial s.s (region B).  Workgroup 0 records
+// alpha for the later kernels of the iteration.
+__global__ __launch_bounds__(kBlock) void k_bs_s(int n, int nslices, int it,
+                                                  const double *__restrict__ rD,
                                                   const double *__restrict__ r,
                                                   const double *__restrict__ v,
                                                   double *__restrict__ sv, double *__restrict__ z,
-                                                  double *__restrict__ part, const KState *st) {
+                                                  double *__restrict__ part, KState *st) {
   __shared__ double lds[kWavesPerBlock];
   if (st->done) return;
-  const double alpha = st->alpha;
+  const int G = gridDim.x;
+  const double RTV = reduce_partials(part, G, lds);
+  if (fabs(RTV) < 1e-30) {
+    if (blockIdx.x == 0 && threadIdx.x == 0) finish(st, PFM_SOLVE_BREAKDOWN);
+    return;
+  }
+  const double alpha = st->rhos[(it - 1) & 1] / RTV;
+  if (blockIdx.x == 0 && threadIdx.x == 0) st->alpha = alpha;
   double acc[1] = {0.0};
   FOR_ROWS({
     if (row < n) {
@@ -327,32 +324,32 @@ __global__ __launch_bounds__(kBlock) void k_bs_s(int n, int nslices, const doubl
       acc[0] += si * si;
     }
   })
-  store_partials<1>(acc, part, lds);
+  store_partials<1>(acc, part + G, lds);
 }
 
-// monitor(||s||, i): converged -> branch 2 (x += alpha y, then stop).
-__global__ __launch_bounds__(kBlock) void k_bs_s_fin(const double *__restrict__ part, int G,
-                                                      KState *st, double *__restrict__ hist) {
-  __shared__ double lds[kWavesPerBlock];
-  if (st->done) return;
-  const double SS = sum_partials(part, G, lds);
-  if (threadIdx.x == 0) {
-    const int it = st->iter + 1;
-    st->iter = it;
-    st->branch = monitor_update(st, hist, sqrt(SS), it) ? 2 : 0;
-  }
-}
-
-// t = A z; partials t.t and t.z.
-__global__ __launch_bounds__(kBlock) void k_bs_spmv2(int n, int nslices, const SliceInfo *__restrict__ sl,
+// Every workgroup: monitor decision on ||s|| (converged -> branch 2, the
+// half-step exit); t = A z; partials t.t, t.z (regions C).  Workgroup 0
+// records the monitor and the branch.
+__global__ __launch_bounds__(kBlock) void k_bs_spmv2(int n, int nslices, int it,
+                                                      const SliceInfo *__restrict__ sl,
                                                       const int32_t *__restrict__ col,
                                                       const double *__restrict__ val,
                                                       const double *__restrict__ diag,
                                                       const double *__restrict__ z,
                                                       double *__restrict__ t,
-                                                      double *__restrict__ part, const KState *st) {
+                                                      double *__restrict__ part, KState *st,
+                                                      double *__restrict__ hist) {
   __shared__ double lds[kWavesPerBlock];
-  if (st->done || st->branch == 2) return;
+  if (st->done) return;
+  const int G = gridDim.x;
+  const double res = sqrt(reduce_partials(part + G, G, lds));
+  const bool conv = monitor_decision(st, res, it);
+  if (blockIdx.x == 0 && threadIdx.x == 0) {
+    monitor_update(st, hist, res, it);
+    st->iter = it;
+    st->branch = conv ? 2 : 0;
+  }
+  if (conv) return;
   double acc[2] = {0.0, 0.0};
   FOR_ROWS({
     const SliceInfo si = load_slice(sl, s);
@@ -365,28 +362,11 @@ __global__ __launch_bounds__(kBlock) void k_bs_spmv2(int n, int nslices, const S
       acc[1] += at * zi;
     }
   })
-  store_partials<2>(acc, part, lds);
+  store_partials<2>(acc, part + 2 * G, lds);
 }
 
-// omega = t.z / t.t, or the t ~ 0 branch.
-__global__ __launch_bounds__(kBlock) void k_bs_omega(const double *__restrict__ part, int G,
-                                                      KState *st) {
-  __shared__ double lds[kWavesPerBlock];
-  if (st->done || st->branch == 2) return;
-  const double TT = sum_partials(part, G, lds);
-  const double TZ = sum_partials(part + G, G, lds);
-  if (threadIdx.x == 0) {
-    if (fabs(TT) < 1e-30) {
-      st->branch = 1;
-    } else {
-      st->branch = 0;
-      st->omega = TZ / TT;
-    }
-  }
-}
-
-// Solution/residual update of iteration i (all three branches); partials
-// r.r and rt.r.
+// Every workgroup: omega = t.z / t.t or the t ~ 0 branch (region C); the
+// solution/residual update of the branch; partials r.r and rt.r (region D).
 __global__ __launch_bounds__(kBlock) void k_bs_update(int n, int nslices, const double *__restrict__ y,
                                                        const double *__restrict__ z,
                                                        const double *__restrict__ sv,
@@ -394,83 +374,90 @@ __global__ __launch_bounds__(kBlock) void k_bs_update(int n, int nslices, const 
                                                        const double *__restrict__ rt,
                                                        double *__restrict__ x,
                                                        double *__restrict__ r,
-                                                       double *__restrict__ part, const KState *st) {
+                                                       double *__restrict__ part, KState *st) {
   __shared__ double lds[kWavesPerBlock];
   if (st->done) return;
-  const int branch = st->branch;
-  const double alpha = st->alpha, omega = st->omega;
+  const int G = gridDim.x;
+  const double alpha = st->alpha;
+  if (st->branch == 2) {  // converged on s: x += alpha y and stop
+    FOR_ROWS({
+      if (row < n) x[row] = x[row] + alpha * y[row];
+    })
+    return;
+  }
+  const double TT = reduce_partials(part + 2 * G, G, lds);
+  const double TZ = reduce_partials(part + 3 * G, G, lds);
+  const int branch = fabs(TT) < 1e-30 ? 1 : 0;
+  const double omega = branch ? 0.0 : TZ / TT;
+  if (blockIdx.x == 0 && threadIdx.x == 0) {
+    st->branch = branch;
+    if (!branch) st->omega = omega;
+  }
   double acc[2] = {0.0, 0.0};
   FOR_ROWS({
     if (row < n) {
-      if (branch == 2) {
+      double ri;
+      if (branch == 1) {
         x[row] = x[row] + alpha * y[row];
+        ri = sv[row];
       } else {
-        double ri;
-        if (branch == 1) {
-          x[row] = x[row] + alpha * y[row];
-          ri = sv[row];
-        } else {
-          x[row] = (x[row] + alpha * y[row]) + omega * z[row];
-          ri = sv[row] - omega * t[row];
-        }
-        r[row] = ri;
-        acc[0] += ri * ri;
-        acc[1] += rt[row] * ri;
+        x[row] = (x[row] + alpha * y[row]) + omega * z[row];
+        ri = sv[row] - omega * t[row];
       }
+      r[row] = ri;
+      acc[0] += ri * ri;
+      acc[1] += rt[row] * ri;
     }
   })
-  if (branch != 2) store_partials<2>(acc, part, lds);
+  store_partials<2>(acc, part + 4 * G, lds);
 }
 
-// monitor(||r||, i); rho_new = rt.r; omega breakdown; beta.
-__global__ __launch_bounds__(kBlock) void k_bs_beta(const double *__restrict__ part, int G,
-                                                     KState *st, double *__restrict__ hist) {
-  __shared__ double lds[kWavesPerBlock];
-  if (st->done) return;
-  if (st->branch == 2) {
-    if (threadIdx.x == 0) finish(st, PFM_SOLVE_CONVERGED);
-    return;
-  }
-  const double RR = sum_partials(part, G, lds);
-  const double RTR = sum_partials(part + G, G, lds);
-  if (threadIdx.x == 0) {
-    const int it = st->iter;
-    const int branch = st->branch;
-    const double alpha = st->alpha, omega = st->omega;
-    if (monitor_update(st, hist, sqrt(RR), it)) {
-      finish(st, PFM_SOLVE_CONVERGED);
-      return;
-    }
-    if (branch == 1) {
-      st->beta = (RTR / st->rho) * alpha;
-      st->rho = RTR;
-    } else {
-      if (fabs(omega) < 1e-30) {
-        finish(st, PFM_SOLVE_BREAKDOWN);
-        return;
-      }
-      st->beta = (RTR / st->rho) * (alpha / omega);
-      st->rho = RTR;
-    }
-    if (it >= st->max_iter) {
-      finish(st, PFM_SOLVE_MAXITER);
-      return;
-    }
-    // breakdown test at the top of the next iteration (pbicgstab.py:152-156)
-    if (fabs(st->rho) < 1e-30) finish(st, PFM_SOLVE_BREAKDOWN);
-  }
-}
-
-// p = r + beta (p - omega v)   (branch 1: p = r + beta (p - v)); y = M^-1 p.
-__global__ __launch_bounds__(kBlock) void k_bs_pupdate(int n, int nslices, const double *__restrict__ rD,
+// Every workgroup: monitor decision on ||r||, rho_new = rt.r, beta and the
+// breakdown / max-iteration tests (region D); p = r + beta (p - omega v)
+// (branch 1: p = r + beta (p - v)); y = M^-1 p.  Workgroup 0 records the
+// monitor, rho (by iteration parity) and the stop.
+__global__ __launch_bounds__(kBlock) void k_bs_pupdate(int n, int nslices, int it,
+                                                        const double *__restrict__ rD,
                                                         const double *__restrict__ r,
                                                         const double *__restrict__ v,
                                                         double *__restrict__ p,
                                                         double *__restrict__ y,
-                                                        const KState *st) {
+                                                        const double *__restrict__ part, KState *st,
+                                                        double *__restrict__ hist) {
+  __shared__ double lds[kWavesPerBlock];
   if (st->done) return;
-  const double beta = st->beta, omega = st->omega;
   const int branch = st->branch;
+  if (branch == 2) {
+    if (blockIdx.x == 0 && threadIdx.x == 0) finish(st, PFM_SOLVE_CONVERGED);
+    return;
+  }
+  const int G = gridDim.x;
+  const double RR = reduce_partials(part + 4 * G, G, lds);
+  const double RTR = reduce_partials(part + 5 * G, G, lds);
+  const double res = sqrt(RR);
+  const double alpha = st->alpha, omega = st->omega;
+  const double rho_old = st->rhos[(it - 1) & 1];
+  int status = 0;
+  double beta = 0.0;
+  if (monitor_decision(st, res, it)) {
+    status = PFM_SOLVE_CONVERGED;
+  } else if (branch == 1) {
+    beta = (RTR / rho_old) * alpha;
+  } else if (fabs(omega) < 1e-30) {
+    status = PFM_SOLVE_BREAKDOWN;
+  } else {
+    beta = (RTR / rho_old) * (alpha / omega);
+  }
+  if (!status && it >= st->max_iter) status = PFM_SOLVE_MAXITER;
+  // breakdown test at the top of the next iteration (pbicgstab.py:152-156)
+  if (!status && fabs(RTR) < 1e-30) status = PFM_SOLVE_BREAKDOWN;
+  if (blockIdx.x == 0 && threadIdx.x == 0) {
+    monitor_update(st, hist, res, it);
+    if (status != PFM_SOLVE_CONVERGED && !(branch == 0 && fabs(omega) < 1e-30))
+      st->rhos[it & 1] = RTR;
+    if (status) finish(st, status);
+  }
+  if (status) return;
   FOR_ROWS({
     if (row < n) {
       const double vi = branch == 1 ? v[row] : omega * v[row];
@@ -505,7 +492,7 @@ size_t work_layout(const pfm_ldu_s *h, int max_iter, int nvec, Work *w, char *ba
   };
   Work tmp;
   tmp.st = (KState *)take(sizeof(KState));
-  tmp.part = (double *)take(sizeof(double) * 3 * (size_t)G);
+  tmp.part = (double *)take(sizeof(double) * 6 * (size_t)G);
   tmp.hist = (double *)take(sizeof(double) * (2 * (size_t)std::max(max_iter, 0) + 2));
   for (int k = 0; k < nvec; ++k) tmp.vec[k] = (double *)take(sizeof(double) * (size_t)h->n);
   if (w) *w = tmp;
@@ -661,18 +648,17 @@ int pfm_pbicgstab_solve(pfm_ldu h, const pfm_solve_params *prm, const double *di
                                  y, w.part);
   k_bs_init_fin<<<1, kBlock, 0, s>>>(w.part, G, w.st, w.hist);
   PFM_CHECK_HIP(hipGetLastError());
-  rc = drive(h, w.st, prm->max_iter, s, [&](int) {
+  // Per iteration: v = A y | alpha, s, z | monitor(s), t = A z | omega, x, r |
+  // monitor(r), rho, beta, p, y.  Partial regions: A = [0, G) rt.v, B = [G, 2G)
+  // s.s, C = [2G, 4G) t.t t.z, D = [4G, 6G) r.r rt.r.
+  rc = drive(h, w.st, prm->max_iter, s, [&](int it) {
     k_bs_spmv1<<<G, kBlock, 0, s>>>(n, ns, h->d_slices, h->d_col, packed, diag, y, rt, v, w.part,
                                     w.st);
-    k_bs_alpha<<<1, kBlock, 0, s>>>(w.part, G, w.st);
-    k_bs_s<<<G, kBlock, 0, s>>>(n, ns, pd, r, v, sv, z, w.part, w.st);
-    k_bs_s_fin<<<1, kBlock, 0, s>>>(w.part, G, w.st, w.hist);
-    k_bs_spmv2<<<G, kBlock, 0, s>>>(n, ns, h->d_slices, h->d_col, packed, diag, z, t, w.part,
-                                    w.st);
-    k_bs_omega<<<1, kBlock, 0, s>>>(w.part, G, w.st);
+    k_bs_s<<<G, kBlock, 0, s>>>(n, ns, it, pd, r, v, sv, z, w.part, w.st);
+    k_bs_spmv2<<<G, kBlock, 0, s>>>(n, ns, it, h->d_slices, h->d_col, packed, diag, z, t, w.part,
+                                    w.st, w.hist);
     k_bs_update<<<G, kBlock, 0, s>>>(n, ns, y, z, sv, t, rt, x, r, w.part, w.st);
-    k_bs_beta<<<1, kBlock, 0, s>>>(w.part, G, w.st, w.hist);
-    k_bs_pupdate<<<G, kBlock, 0, s>>>(n, ns, pd, r, v, p, y, w.st);
+    k_bs_pupdate<<<G, kBlock, 0, s>>>(n, ns, it, pd, r, v, p, y, w.part, w.st, w.hist);
   });
   if (rc != PFM_OK) return rc;
   PFM_CHECK_HIP(hipGetLastError());

@@ -172,6 +172,19 @@ __global__ __launch_bounds__(kBlock) void k_pcg32_update(int n, int nslices, int
   __shared__ double lds[kWavesPerBlock];
   if (in->done) return;
   const int G = gridDim.x;
+  // first round's loads issued before the reduction (independent of alpha)
+  const int lane = threadIdx.x & 63;
+  const int stride = G * kWavesPerBlock;
+  int s = xcd_remap(blockIdx.x, G) * kWavesPerBlock + (threadIdx.x >> 6);
+  int row = s * kSlice + lane;
+  float ep = 0.0f, pp = 0.0f, rp = 0.0f, ap = 0.0f, dp = 1.0f;
+  if (s < nslices && row < n) {
+    ep = e[row];
+    pp = p[row];
+    rp = r[row];
+    ap = Ap[row];
+    if (rD) dp = rD[row];
+  }
   const double PAP = reduce_partials(part, G, lds);
   if (fabs(PAP) < 1e-30) {
     if (blockIdx.x == 0 && threadIdx.x == 0) in->done = 1;
@@ -179,16 +192,24 @@ __global__ __launch_bounds__(kBlock) void k_pcg32_update(int n, int nslices, int
   }
   const float alpha = (float)(in->rzs[(it - 1) & 1] / PAP);
   double acc[2] = {0.0, 0.0};
-  FOR_ROWS({
+  for (; s < nslices; s += stride, row = s * kSlice + lane) {
     if (row < n) {
-      e[row] = e[row] + alpha * p[row];
-      const float ri = r[row] - alpha * Ap[row];
+      e[row] = ep + alpha * pp;
+      const float ri = rp - alpha * ap;
       r[row] = ri;
-      const float zi = rD ? ri / rD[row] : ri;
+      const float zi = rD ? ri / dp : ri;
       acc[0] += (double)ri * (double)ri;
       acc[1] += (double)ri * (double)zi;
     }
-  })
+    const int sn = s + stride, rn = sn * kSlice + lane;
+    if (sn < nslices && rn < n) {
+      ep = e[rn];
+      pp = p[rn];
+      rp = r[rn];
+      ap = Ap[rn];
+      if (rD) dp = rD[rn];
+    }
+  }
   store_partials<2>(acc, part + G, lds);
 }
 
@@ -202,6 +223,16 @@ __global__ __launch_bounds__(kBlock) void k_pcg32_pupdate(int n, int nslices, in
   __shared__ double lds[kWavesPerBlock];
   if (in->done) return;
   const int G = gridDim.x;
+  const int lane = threadIdx.x & 63;
+  const int stride = G * kWavesPerBlock;
+  int s = xcd_remap(blockIdx.x, G) * kWavesPerBlock + (threadIdx.x >> 6);
+  int row = s * kSlice + lane;
+  float rp = 0.0f, dp = 1.0f, pp = 0.0f;
+  if (s < nslices && row < n) {
+    rp = r[row];
+    if (rD) dp = rD[row];
+    pp = p[row];
+  }
   const double RR = reduce_partials(part + G, G, lds);
   const double RZ = reduce_partials(part + 2 * G, G, lds);
   const bool stop = sqrt(RR) < in->target || it >= in->max_iter;
@@ -213,13 +244,18 @@ __global__ __launch_bounds__(kBlock) void k_pcg32_pupdate(int n, int nslices, in
   }
   if (stop) return;
   const float beta = (float)(RZ / rz_old);
-  FOR_ROWS({
+  for (; s < nslices; s += stride, row = s * kSlice + lane) {
     if (row < n) {
-      const float ri = r[row];
-      const float zi = rD ? ri / rD[row] : ri;
-      p[row] = zi + beta * p[row];
+      const float zi = rD ? rp / dp : rp;
+      p[row] = zi + beta * pp;
     }
-  })
+    const int sn = s + stride, rn = sn * kSlice + lane;
+    if (sn < nslices && rn < n) {
+      rp = r[rn];
+      if (rD) dp = rD[rn];
+      pp = p[rn];
+    }
+  }
 }
 
 // x += (double) e

@@ -14,7 +14,8 @@ struct alignas(256) KState {
   double res0, res;
   double rz, alpha, beta;    // PCG
   double rzs[2];             // PCG r.z by iteration parity (written while read)
-  double rho, omega;         // PBiCGStab
+  double rho, omega;         // PBiCGStab (rho kept per iteration parity in rhos)
+  double rhos[2];
   int32_t max_iter, min_iter;
   int32_t iter;              // current outer iteration i
   int32_t nhist;             // residual history length
