@@ -254,7 +254,8 @@ def main() -> int:
     torch.cuda.set_device(dev)
     n = args.n
     t0 = time.perf_counter()
-    case = poisson_case(n, jitter=args.jitter, sigma=args.sigma, seed=0)
+    case = poisson_case(n, jitter=args.jitter, sigma=args.sigma, seed=0,
+                        progress=log if n > 128 else None)
     t_case = time.perf_counter() - t0
     N, F = case.n_cells, case.mesh.n_internal_faces
     log(f"case {n}^3: {N} cells, {F} faces, built in {t_case:.1f} s")
@@ -268,6 +269,8 @@ def main() -> int:
         f"DILU levels {info.n_dilu_levels}")
 
     max_iter = args.max_iter
+    if n > 128:  # long solves: report from the solver loop
+        mat.addressing.set_progress(lambda it: log(f"  solver loop: {it} iterations enqueued"))
     solver = PCGSolver(tolerance=1e-300, rel_tol=1e-8, max_iter=max_iter)
     x0 = torch.zeros(N, dtype=torch.float64, device=dev)
 
@@ -370,7 +373,10 @@ def main() -> int:
     if not args.no_mixed:
         msolver = PCGSolver(tolerance=1e-300, rel_tol=1e-8, max_iter=max_iter, precision="mixed",
                             inner_rel_tol=args.inner_rel_tol)
-        mat.solve(msolver, x0, 1e-300, max_iter)  # warm (packs the FP32 copy)
+        if n <= 128:
+            mat.solve(msolver, x0, 1e-300, max_iter)  # warm (packs the FP32 copy)
+        else:  # long solves: pack the FP32 copy only
+            mat.packed32(False)
         torch.cuda.synchronize()
         t0 = time.perf_counter()
         xm, outer, _ = mat.solve(msolver, x0, 1e-300, max_iter)

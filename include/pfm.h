@@ -122,6 +122,15 @@ int pfm_ldu_create(int32_t n_cells, int32_t n_faces, const int32_t *owner_h,
 int pfm_ldu_destroy(pfm_ldu h);
 int pfm_ldu_get_info(pfm_ldu h, pfm_ldu_info *out);
 
+/* Progress reports of long solves (no reference counterpart: the reference
+ * logs residuals per iteration on the host).  When set, the host loops that
+ * drive pfm_pcg_solve, pfm_pbicgstab_solve, pfm_pcg_mixed_solve and
+ * pfm_gamg_solve (hierarchies built on this handle) call fn(iterations_enqueued, user) at most every interval_s seconds
+ * (from the calling thread, between polls of the device stop flag).  fn =
+ * NULL turns it off. */
+typedef void (*pfm_progress_fn)(int32_t iterations, void *user);
+int pfm_ldu_set_progress(pfm_ldu h, pfm_progress_fn fn, void *user, double interval_s);
+
 /* Pack face coefficients into row-slice order: packed[n_slots].
  * transpose = 0 packs A, transpose = 1 packs A^T (lower and upper swapped,
  * as LduMatrix.Ax_T ldu_matrix.py:219-258 and the adjoint matrix of

@@ -82,6 +82,7 @@ _SIGS = {
     "pfm_spmv_f32": (INT, [P, P, P, P, P, P]),
     "pfm_spmv_ldu": (INT, [P, INT, P, P, P, P, P, P]),
     "pfm_residual": (INT, [P, P, P, P, P, P, P]),
+    "pfm_ldu_set_progress": (INT, [P, P, P, C.c_double]),
     "pfm_gs_smooth": (INT, [P, P, P, P, P, I32, C.c_double, P]),
     "pfm_jacobi_smooth": (INT, [P, P, P, P, P, I32, C.c_double, P, P]),
     "pfm_dic_factor": (INT, [P, P, P, P, P]),
@@ -116,6 +117,9 @@ _SIGS = {
 }
 
 EXPORTED_SYMBOLS = tuple(_SIGS)
+
+# pfm_progress_fn: void (*)(int32_t iterations, void *user)
+PROGRESS_FN = C.CFUNCTYPE(None, C.c_int32, C.c_void_p)
 
 _lock = threading.Lock()
 _lib = None

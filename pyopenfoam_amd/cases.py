@@ -66,20 +66,25 @@ def rcm_permutation(n_cells: int, owner, neighbour) -> np.ndarray:
 
 def poisson_case(n: int, *, jitter: float = 0.0, sigma: float = 0.0, seed: int = 0,
                  permute: bool = True, renumber: bool = True,
-                 dirichlet_patch: str = "x0") -> PoissonCase:
+                 dirichlet_patch: str = "x0", progress=None) -> PoissonCase:
+    """``progress(msg)``, if given, is called after each construction stage."""
+    note = progress or (lambda msg: None)
     rng = np.random.default_rng(seed)
     mesh = structured_hex(n)
+    note(f"case {n}^3: structured mesh")
     if jitter > 0.0:
         mesh = jitter_points(mesh, n, jitter, rng)
     if permute:
         mesh = relabel_cells(mesh, rng.permutation(mesh.n_cells))
+        note("case: cells permuted")
     if renumber:
         nfi = mesh.n_internal_faces
         perm = rcm_permutation(mesh.n_cells, mesh.owner[:nfi], mesh.neighbour)
         inv = np.empty_like(perm)
         inv[perm] = np.arange(mesh.n_cells)
         mesh = relabel_cells(mesh, inv)
-    geo = compute_geometry(mesh)
+        note("case: RCM renumbering")
+    geo = compute_geometry(mesh, progress)
     if sigma > 0.0:
         gamma_all = rng.lognormal(0.0, sigma, size=mesh.n_faces)
     else:

@@ -49,6 +49,18 @@ class LduAddressing:
     def handle(self) -> C.c_void_p:
         return self._h
 
+    def set_progress(self, fn=None, interval_s: float = 20.0) -> None:
+        """Report long device solves on this topology: ``fn(iterations)`` is
+        called from the solving thread at most every ``interval_s`` seconds.
+        ``fn=None`` turns reports off."""
+        if fn is None:
+            self._progress = None
+            _lib.check(_lib.lib().pfm_ldu_set_progress(self._h, None, None, 0.0))
+            return
+        self._progress = _lib.PROGRESS_FN(lambda it, user: fn(int(it)))  # kept alive here
+        _lib.check(_lib.lib().pfm_ldu_set_progress(self._h, self._progress, None,
+                                                   float(interval_s)))
+
     def __del__(self):
         h = getattr(self, "_h", None)
         if h is not None and h.value:

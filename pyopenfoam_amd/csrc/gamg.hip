@@ -638,8 +638,10 @@ int pfm_gamg_solve(pfm_gamg g, const pfm_solve_params *prm, const double *diag, 
   // device, so track the value after every completed cycle separately
   std::vector<int> cur_after;
   cur_after.reserve(prm->max_iter + 1);
+  double last_report = now_seconds();
   while (true) {
     bool queued = false;
+    report_progress(g->fine, enq, last_report);
     if (enq < prm->max_iter) {
       const int c = std::min(chunk, prm->max_iter - enq);
       for (int k = 0; k < c; ++k) {

@@ -522,8 +522,10 @@ int drive(pfm_ldu h, const KState *st_dev, int max_iter, hipStream_t s, Body bod
   PFM_CHECK_HIP(hipEventCreateWithFlags(&ev[1], hipEventDisableTiming));
   int enq = 0, chunk = 4, slot = 0, rc = PFM_OK;
   bool pending = false;
+  double last_report = now_seconds();
   while (true) {
     bool queued = false;
+    report_progress(h, enq, last_report);
     if (enq < max_iter) {
       const int c = std::min(chunk, max_iter - enq);
       for (int k = 0; k < c; ++k) body(enq + k + 1);

@@ -427,6 +427,15 @@ int pfm_ldu_create(int32_t n, int32_t nf, const int32_t *own, const int32_t *nei
   return PFM_OK;
 }
 
+int pfm_ldu_set_progress(pfm_ldu h, pfm_progress_fn fn, void *user, double interval_s) {
+  PFM_REQUIRE(h, "null handle");
+  PFM_REQUIRE(interval_s >= 0.0, "negative progress interval");
+  h->progress = fn;
+  h->progress_user = user;
+  h->progress_interval = interval_s;
+  return PFM_OK;
+}
+
 int pfm_ldu_get_info(pfm_ldu h, pfm_ldu_info *out) {
   PFM_REQUIRE(h && out, "null handle or output");
   out->n_cells = h->n;

@@ -361,8 +361,10 @@ int pfm_pcg_mixed_solve(pfm_ldu h, const pfm_solve_params *prm, double inner_rel
     PFM_CHECK_HIP(hipEventCreateWithFlags(&ev[1], hipEventDisableTiming));
     int enq = 0, chunk = 4, slot = 0;
     bool pending = false;
+    double last_report = now_seconds();
     while (true) {
       bool queued = false;
+      report_progress(h, enq, last_report);
       if (enq < budget) {
         const int c = std::min(chunk, budget - enq);
         for (int k = 0; k < c; ++k) {

@@ -104,7 +104,16 @@ struct pfm_ldu_s {
   int32_t *d_gs_fwd_ptr = nullptr, *d_gs_fwd_cells = nullptr;
   int32_t *d_gs_bwd_ptr = nullptr, *d_gs_bwd_cells = nullptr;
   int32_t *h_flag = nullptr;            // pinned: solver progress polling
+  pfm_progress_fn progress = nullptr;   // optional host progress report
+  void *progress_user = nullptr;
+  double progress_interval = 0.0;
 };
+
+namespace pfm {
+// Calls the handle's progress report when its interval has passed since `last`.
+void report_progress(const pfm_ldu_s *h, int32_t iterations, double &last);
+double now_seconds();
+}  // namespace pfm
 
 namespace pfm {
 // Grid helpers --------------------------------------------------------------

@@ -1,3 +1,4 @@
+#include <chrono>
 // Host-side setup of libpfm: error state, RCM ordering, dependency levels of
 // the DIC/DILU recurrences and the row-slice layout used by every SpMV.
 // Sequential graph algorithms stay on the host (they are setup, done once per
@@ -144,6 +145,18 @@ int build_host_layout(int32_t n, int32_t nf, const int32_t *own,
   group(gf, ngf, L.gs_fwd_ptr, L.gs_fwd_cells);
   group(gb, ngb, L.gs_bwd_ptr, L.gs_bwd_cells);
   return PFM_OK;
+}
+
+double now_seconds() {
+  return std::chrono::duration<double>(std::chrono::steady_clock::now().time_since_epoch()).count();
+}
+
+void report_progress(const pfm_ldu_s *h, int32_t iterations, double &last) {
+  if (!h->progress) return;
+  const double t = now_seconds();
+  if (t - last < h->progress_interval) return;
+  last = t;
+  h->progress(iterations, h->progress_user);
 }
 
 }  // namespace pfm

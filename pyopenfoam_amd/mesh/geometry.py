@@ -48,7 +48,10 @@ def _dot(a, b):
     return a[..., 0] * b[..., 0] + a[..., 1] * b[..., 1] + a[..., 2] * b[..., 2]
 
 
-def compute_geometry(mesh: HexMesh) -> MeshGeometry:
+def compute_geometry(mesh: HexMesh, progress=None) -> MeshGeometry:
+    """``progress(msg)``, if given, is called after each stage (large meshes
+    take minutes on the host)."""
+    note = progress or (lambda msg: None)
     pts = mesh.points
     fv = pts[mesh.faces]                        # (nf, 4, 3)
     nv = fv.shape[1]
@@ -58,6 +61,7 @@ def compute_geometry(mesh: HexMesh) -> MeshGeometry:
         area += _cross(fv[:, j], fv[:, (j + 1) % nv])
     area *= 0.5
     mag = np.sqrt(_dot(area, area))
+    note("geometry: face centres and area vectors")
 
     n_cells = mesh.n_cells
     nfi = mesh.n_internal_faces
@@ -80,6 +84,7 @@ def compute_geometry(mesh: HexMesh) -> MeshGeometry:
     first[:, 1:] = cverts[:, 1:] != cverts[:, :-1]
     n_unique = first.sum(axis=1)
     ref = (pts[cverts] * first[..., None]).sum(axis=1) / n_unique[:, None]
+    note("geometry: cell reference points")
 
     vol = np.zeros(n_cells)
     wc = np.zeros((n_cells, 3))
@@ -96,6 +101,7 @@ def compute_geometry(mesh: HexMesh) -> MeshGeometry:
         cen = (refb + fcc + va + vb) / 4.0
         vol += v.sum(axis=1)
         wc += (v[..., None] * cen).sum(axis=1)
+        note(f"geometry: tetrahedra of face edge {j + 1}/{nv}")
     centres = wc / np.where(vol > 1e-30, vol, 1.0)[:, None]
 
     delta = np.zeros(mesh.n_faces)
