@@ -167,18 +167,19 @@ __device__ __forceinline__ SliceInfo load_slice_lane(const SliceInfo *sl, int s)
 // couplings sit in two groups of its slice (owner-faces, column = neighbour,
 // then neighbour-faces, column = owner), each in face order; merging them by
 // face index visits the faces in the reference's face order.
+template <typename T>
 __device__ __forceinline__ void gs_cell(int c, const SliceInfo *__restrict__ sl,
                                         const int32_t *__restrict__ col,
                                         const int32_t *__restrict__ map,
-                                        const double *__restrict__ val,
-                                        const double *__restrict__ diag,
-                                        const double *__restrict__ b, double *x, double omega) {
+                                        const T *__restrict__ val,
+                                        const T *__restrict__ diag,
+                                        const T *__restrict__ b, T *x, T omega) {
   const SliceInfo si = load_slice_lane(sl, c / kSlice);  // lanes hold cells of any slice
   const int lane = c % kSlice;
   const int bl = si.ptr + lane, bu = si.ptr + si.wl * kSlice + lane;
-  const double d = diag[c];
-  const double xc = x[c];
-  double row = d * xc;
+  const T d = diag[c];
+  const T xc = x[c];
+  T row = d * xc;
   int jl = 0, ju = 0;
   int cl = jl < si.wl ? col[bl] : -1;
   int cu = ju < si.wu ? col[bu] : -1;
@@ -195,9 +196,9 @@ __device__ __forceinline__ void gs_cell(int c, const SliceInfo *__restrict__ sl,
       cu = ju < si.wu ? col[bu + ju * kSlice] : -1;
     }
   }
-  const double off = row - d * xc;
-  if (fabs(d) > 1e-30) {
-    const double nv = (b[c] - off) / d;
+  const T off = row - d * xc;
+  if (fabs(d) > T(1e-30)) {
+    const T nv = (b[c] - off) / d;
     x[c] = xc + omega * (nv - xc);
   }
 }

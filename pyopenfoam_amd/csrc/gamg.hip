@@ -85,62 +85,71 @@ __global__ __launch_bounds__(kBlock) void k_inv_diag(int n, const double *__rest
 }
 
 // ------------------------------------------------------------- V-cycle -----
-__device__ __forceinline__ double level_ax(const LevelDev &L, int s, int lane, int row,
-                                          const double *x) {
+template <typename T>
+__device__ __forceinline__ T level_ax(const LevelDevT<T> &L, int s, int lane, int row, const T *x) {
   const SliceInfo si = load_slice(L.slices, s);
-  const double dx = row < L.n ? L.diag[row] * x[row] : 0.0;
-  return row_spmv<double>(si, lane, dx, L.col, L.packed, x);
+  const T dx = row < L.n ? L.diag[row] * x[row] : T(0);
+  return row_spmv<T>(si, lane, dx, L.col, L.packed, x);
 }
 
-__global__ __launch_bounds__(kBlock) void k_jacobi(LevelDev L, int cur, const KState *st) {
+template <typename T>
+__global__ __launch_bounds__(kBlock) void k_jacobi(LevelDevT<T> L, int cur, const KState *st) {
   if (st->done) return;
   const int s = xcd_remap(blockIdx.x, gridDim.x) * kWavesPerBlock + (threadIdx.x >> 6);
   if (s >= L.nslices) return;
   const int lane = threadIdx.x & 63, row = s * kSlice + lane;
-  const double *x = L.x[cur];
-  const double ax = level_ax(L, s, lane, row, x);
+  const T *x = L.x[cur];
+  const T ax = level_ax(L, s, lane, row, x);
   if (row < L.n) L.x[cur ^ 1][row] = x[row] + L.inv[row] * (L.b[row] - ax);
 }
 
 // One Gauss-Seidel sweep level of a level's x[cur], in place.
-__global__ __launch_bounds__(kBlock) void k_gs_level(LevelDev L, int cur,
+template <typename T>
+__global__ __launch_bounds__(kBlock) void k_gs_level(LevelDevT<T> L, int cur,
                                                       const int32_t *__restrict__ cells, int count,
                                                       const KState *st) {
   if (st->done) return;
   const int t = blockIdx.x * kBlock + threadIdx.x;
-  if (t < count) gs_cell(cells[t], L.slices, L.col, L.map, L.packed, L.diag, L.b, L.x[cur], 1.0);
+  if (t < count)
+    gs_cell<T>(cells[t], L.slices, L.col, L.map, L.packed, L.diag, L.b, L.x[cur], T(1));
 }
 
-__global__ __launch_bounds__(kBlock) void k_resid(LevelDev L, int cur, const KState *st) {
+template <typename T>
+__global__ __launch_bounds__(kBlock) void k_resid(LevelDevT<T> L, int cur, const KState *st) {
   if (st->done) return;
   const int s = xcd_remap(blockIdx.x, gridDim.x) * kWavesPerBlock + (threadIdx.x >> 6);
   if (s >= L.nslices) return;
   const int lane = threadIdx.x & 63, row = s * kSlice + lane;
-  const double ax = level_ax(L, s, lane, row, L.x[cur]);
+  const T ax = level_ax(L, s, lane, row, L.x[cur]);
   if (row < L.n) L.r[row] = L.b[row] - ax;
 }
 
-// b_c = sum of r over each aggregate (fine-cell order); x_c[0] = 0.
-__global__ __launch_bounds__(kBlock) void k_restrict(LevelDev F, LevelDev Cc, const KState *st) {
+// b_c = sum of r over each aggregate (fine-cell order, accumulated in the
+// coarse level's type); x_c[0] = 0.
+template <typename TF, typename TC>
+__global__ __launch_bounds__(kBlock) void k_restrict(LevelDevT<TF> F, LevelDevT<TC> Cc,
+                                                      const KState *st) {
   if (st->done) return;
   const int c = blockIdx.x * kBlock + threadIdx.x;
   if (c >= F.nc) return;
-  double s = 0.0;
-  for (int k = F.agg_ptr[c]; k < F.agg_ptr[c + 1]; ++k) s += F.r[F.agg_cells[k]];
+  TC s = TC(0);
+  for (int k = F.agg_ptr[c]; k < F.agg_ptr[c + 1]; ++k) s += (TC)F.r[F.agg_cells[k]];
   Cc.b[c] = s;
-  Cc.x[0][c] = 0.0;
+  Cc.x[0][c] = TC(0);
 }
 
-__global__ __launch_bounds__(kBlock) void k_prolong(LevelDev F, int cur, const double *__restrict__ xc,
+template <typename TF, typename TC>
+__global__ __launch_bounds__(kBlock) void k_prolong(LevelDevT<TF> F, int cur, const TC *__restrict__ xc,
                                                      const KState *st) {
   if (st->done) return;
   const int i = blockIdx.x * kBlock + threadIdx.x;
   if (i >= F.n) return;
-  F.x[cur][i] = F.x[cur][i] + xc[F.f2c[i]];
+  F.x[cur][i] = F.x[cur][i] + (TF)xc[F.f2c[i]];
 }
 
-// Single-workgroup V-cycle of levels k0..nlev-1 (small levels).  Waves map
-// onto whole slices (rows t, t + 1024, ... with 64-aligned wave offsets).
+// Single-workgroup V-cycle of levels k0..nlev-1 (small levels, always FP64).
+// Waves map onto whole slices (rows t, t + 1024, ... with 64-aligned wave
+// offsets).
 __device__ void tail_jacobi(const LevelDev &L, int cur) {
   for (int row0 = threadIdx.x; row0 < L.nslices * kSlice; row0 += kTailThreads) {
     const int s = row0 >> 6, lane = row0 & 63;
@@ -159,7 +168,7 @@ __device__ void tail_gs(const LevelDev &L, int cur) {
     const int nl = pass ? L.gs_nbl : L.gs_nfl;
     for (int l = 0; l < nl; ++l) {
       for (int k = ptr[l] + (int)threadIdx.x; k < ptr[l + 1]; k += kTailThreads)
-        gs_cell(cells[k], L.slices, L.col, L.map, L.packed, L.diag, L.b, L.x[cur], 1.0);
+        gs_cell<double>(cells[k], L.slices, L.col, L.map, L.packed, L.diag, L.b, L.x[cur], 1.0);
       __syncthreads();
     }
   }
@@ -213,9 +222,10 @@ __global__ __launch_bounds__(kTailThreads) void k_tail(const LevelDev *__restric
   }
 }
 
-// r = b - A x on level 0 and the partial sums of r.r.
-__global__ __launch_bounds__(kBlock) void k_res_norm(LevelDev L, int cur, double *__restrict__ part,
-                                                      const KState *st) {
+// r = b - A x on level 0 and the partial sums of r.r (accumulated in FP64).
+template <typename T>
+__global__ __launch_bounds__(kBlock) void k_res_norm(LevelDevT<T> L, int cur,
+                                                      double *__restrict__ part, const KState *st) {
   __shared__ double lds[kWavesPerBlock];
   if (st->done) return;
   double acc = 0.0;
@@ -223,9 +233,9 @@ __global__ __launch_bounds__(kBlock) void k_res_norm(LevelDev L, int cur, double
   const int lane = threadIdx.x & 63;
   for (int s = sr.begin + (threadIdx.x >> 6); s < sr.end; s += kWavesPerBlock) {
     const int row = s * kSlice + lane;
-    const double ax = level_ax(L, s, lane, row, L.x[cur]);
+    const T ax = level_ax(L, s, lane, row, L.x[cur]);
     if (row < L.n) {
-      const double ri = L.b[row] - ax;
+      const double ri = (double)(L.b[row] - ax);
       acc += ri * ri;
     }
   }
@@ -246,6 +256,85 @@ __global__ __launch_bounds__(kBlock) void k_gamg_fin(const double *__restrict__ 
     else if (it >= st->max_iter)
       finish(st, PFM_SOLVE_MAXITER);
   }
+}
+
+// ------------------------------------------------- mixed precision -----
+// Outer FP64 residual r = b - A x of level 0 (the caller's x and b) and its
+// partial sums of r.r.
+__global__ __launch_bounds__(kBlock) void k_gm_resid(LevelDev L, double *__restrict__ r,
+                                                      double *__restrict__ part, const KState *st) {
+  __shared__ double lds[kWavesPerBlock];
+  if (st->done) return;
+  double acc = 0.0;
+  const SliceRange sr = block_slices(L.nslices);
+  const int lane = threadIdx.x & 63;
+  for (int s = sr.begin + (threadIdx.x >> 6); s < sr.end; s += kWavesPerBlock) {
+    const int row = s * kSlice + lane;
+    const double ax = level_ax(L, s, lane, row, L.x[0]);
+    if (row < L.n) {
+      const double ri = L.b[row] - ax;
+      r[row] = ri;
+      acc += ri * ri;
+    }
+  }
+  acc = block_sum(acc, lds);
+  if (threadIdx.x == 0) part[blockIdx.x] = acc;
+}
+
+// Outer monitor (the reference rule on the FP64 residual, the initial one
+// included) and the inner state: stop when the inner residual falls below
+// tau times this outer residual or the remaining cycle budget is spent.
+__global__ __launch_bounds__(kBlock) void k_gm_outer_fin(const double *__restrict__ part, int G,
+                                                          KState *st, double *__restrict__ hist,
+                                                          int outer, int max_outer, int budget,
+                                                          double tau, KState *in) {
+  __shared__ double lds[kWavesPerBlock];
+  if (st->done) return;
+  const double RR = sum_partials(part, G, lds);
+  if (threadIdx.x == 0) {
+    const double res = sqrt(RR);
+    st->iter = outer;
+    if (monitor_update(st, hist, res, outer))
+      finish(st, PFM_SOLVE_CONVERGED);
+    else if (outer >= max_outer || budget <= 0)
+      finish(st, PFM_SOLVE_MAXITER);
+    in->tol = -1.0;
+    in->rel_tol = tau;
+    in->res0 = res;
+    in->res = res;
+    in->nhist = 1;
+    in->iter = 0;
+    in->max_iter = budget;
+    in->min_iter = 0;
+    in->hist_cap = 0;
+    in->status = 0;
+    in->done = st->done;
+  }
+}
+
+template <typename T>
+__global__ __launch_bounds__(kBlock) void k_gm_cast(int n, const double *__restrict__ r,
+                                                     T *__restrict__ b, T *__restrict__ x,
+                                                     const KState *st) {
+  if (st->done) return;
+  const int i = blockIdx.x * kBlock + threadIdx.x;
+  if (i >= n) return;
+  b[i] = (T)r[i];
+  x[i] = T(0);
+}
+
+template <typename T>
+__global__ __launch_bounds__(kBlock) void k_gm_correct(int n, const T *__restrict__ e,
+                                                        double *__restrict__ x, const KState *st) {
+  if (st->done) return;
+  const int i = blockIdx.x * kBlock + threadIdx.x;
+  if (i < n) x[i] = x[i] + (double)e[i];
+}
+
+__global__ __launch_bounds__(kBlock) void k_to_f32(int n, const double *__restrict__ a,
+                                                    float *__restrict__ b) {
+  const int i = blockIdx.x * kBlock + threadIdx.x;
+  if (i < n) b[i] = (float)a[i];
 }
 
 // ----------------------------------------------------------- workspace -----
@@ -323,6 +412,85 @@ size_t gamg_layout(const pfm_gamg_s *g, int max_iter, GamgWork *w, char *base) {
   return off;
 }
 
+// Mixed-precision workspace: the FP64 layout (every level's coefficients,
+// FP64 tail levels, outer state and history), then FP32 copies of levels
+// [0, tail_k), the inner state, the outer residual and FP64 level-0 inner
+// buffers (used when every level is a tail level).
+struct GamgMixedWork {
+  GamgWork w64;
+  std::vector<LevelDevT<float>> lv32;
+  std::vector<float *> packed32, diag32, inv32;
+  KState *in;
+  double *r64, *e64, *b64;
+  LevelDev *d_small;
+};
+
+// First level handled by the single-workgroup tail kernel.
+inline int tail_level(const pfm_gamg_s *g) {
+  int k = (int)g->lv.size() - 1;
+  while (k > 0 && g->lv[k - 1].n <= kTailCells) --k;
+  return k;
+}
+
+size_t gamg_mixed_layout(const pfm_gamg_s *g, int max_outer, GamgMixedWork *m, char *base) {
+  GamgWork w64;
+  size_t off = gamg_layout(g, max_outer, &w64, base);
+  auto take = [&](size_t bytes) -> void * {
+    size_t o = off;
+    off += align256(bytes);
+    return base ? (void *)(base + o) : nullptr;
+  };
+  const int tail_k = tail_level(g);
+  GamgMixedWork t;
+  t.lv32.resize(tail_k);
+  t.packed32.resize(tail_k);
+  t.diag32.resize(tail_k);
+  t.inv32.resize(tail_k);
+  for (int k = 0; k < tail_k; ++k) {
+    const auto &H = g->lv[k];
+    const size_t n = (size_t)H.n;
+    const LevelDev &D64 = w64.lv[k];
+    LevelDevT<float> &D = t.lv32[k];
+    memset(&D, 0, sizeof(D));
+    D.n = D64.n;
+    D.nslices = D64.nslices;
+    D.slices = D64.slices;
+    D.col = D64.col;
+    D.map = D64.map;
+    D.gs_fptr = D64.gs_fptr;
+    D.gs_fcells = D64.gs_fcells;
+    D.gs_bptr = D64.gs_bptr;
+    D.gs_bcells = D64.gs_bcells;
+    D.gs_nfl = D64.gs_nfl;
+    D.gs_nbl = D64.gs_nbl;
+    D.f2c = D64.f2c;
+    D.agg_ptr = D64.agg_ptr;
+    D.agg_cells = D64.agg_cells;
+    D.nc = D64.nc;
+    t.packed32[k] = (float *)take(sizeof(float) * (size_t)std::max<int64_t>(H.ldu->nslots, 1));
+    t.diag32[k] = (float *)take(sizeof(float) * n);
+    t.inv32[k] = (float *)take(sizeof(float) * n);
+    D.packed = t.packed32[k];
+    D.diag = t.diag32[k];
+    D.inv = t.inv32[k];
+    D.x[0] = (float *)take(sizeof(float) * n);
+    D.x[1] = (float *)take(sizeof(float) * n);
+    D.b = (float *)take(sizeof(float) * n);
+    D.r = (float *)take(sizeof(float) * n);
+  }
+  const size_t n0 = (size_t)g->lv[0].n;
+  t.in = (KState *)take(sizeof(KState));
+  t.r64 = (double *)take(sizeof(double) * n0);
+  t.e64 = (double *)take(sizeof(double) * n0);
+  t.b64 = (double *)take(sizeof(double) * n0);
+  t.d_small = (LevelDev *)take(sizeof(LevelDev) * g->lv.size());
+  if (m) {
+    t.w64 = w64;
+    *m = t;
+  }
+  return off;
+}
+
 template <typename T>
 int upload(T **dst, const std::vector<T> &src) {
   *dst = nullptr;
@@ -376,6 +544,118 @@ void free_level(GamgLevelHost &H) {
 }
 
 }  // namespace
+
+// The levels one V-cycle runs on: levels [0, tail_k) with values of type T,
+// the tail levels always FP64 (host descriptors and their device copy).
+template <typename T>
+struct CycleLevels {
+  std::vector<LevelDevT<T>> big;
+  std::vector<LevelDev> small;
+  const LevelDev *d_small = nullptr;
+  int tail_k = 0;
+};
+
+// Enqueue one V-cycle (solvers/gamg.py:437-496); cur0 is the parity of
+// level 0's Jacobi buffer before and after the cycle.
+template <typename T>
+void enqueue_cycle(const pfm_gamg_s *g, const CycleLevels<T> &C, const KState *st, int &cur0,
+                   hipStream_t s) {
+  const int L = (int)g->lv.size();
+  const int npre = g->prm.n_pre, npost = g->prm.n_post, ncoarse = g->prm.n_coarsest;
+  const bool gs = g->prm.smoother == PFM_SMOOTHER_GAUSS_SEIDEL;
+  const int tail_k = C.tail_k;
+  auto smooth = [&](int k, int &cur) {
+    const LevelDevT<T> &D = C.big[k];
+    if (gs) {  // one symmetric sweep, level by level, in place
+      const pfm_ldu h = g->lv[k].ldu;
+      for (int pass = 0; pass < 2; ++pass) {
+        const std::vector<int32_t> &ptr = pass ? h->gs_bwd_ptr : h->gs_fwd_ptr;
+        const int32_t *cells = pass ? h->d_gs_bwd_cells : h->d_gs_fwd_cells;
+        for (size_t l = 0; l + 1 < ptr.size(); ++l) {
+          const int cnt = ptr[l + 1] - ptr[l];
+          k_gs_level<T><<<grid1d(cnt), kBlock, 0, s>>>(D, cur, cells + ptr[l], cnt, st);
+        }
+      }
+      return;
+    }
+    k_jacobi<T><<<blocks_for_slices(D.nslices), kBlock, 0, s>>>(D, cur, st);
+    cur ^= 1;
+  };
+  int cur[kMaxLevels];
+  cur[0] = cur0;
+  for (int k = 0; k < tail_k; ++k) {  // descend through the large levels
+    for (int it = 0; it < npre; ++it) smooth(k, cur[k]);
+    k_resid<T><<<blocks_for_slices(C.big[k].nslices), kBlock, 0, s>>>(C.big[k], cur[k], st);
+    if (k + 1 < tail_k)
+      k_restrict<T, T><<<grid1d(C.big[k].nc), kBlock, 0, s>>>(C.big[k], C.big[k + 1], st);
+    else
+      k_restrict<T, double><<<grid1d(C.big[k].nc), kBlock, 0, s>>>(C.big[k], C.small[k + 1], st);
+    cur[k + 1] = 0;
+  }
+  // levels tail_k .. L-1 in one workgroup
+  k_tail<<<1, kTailThreads, 0, s>>>(C.d_small, tail_k, L, cur[tail_k], npre, npost, ncoarse,
+                                    gs ? 1 : 0, st);
+  if (!gs) cur[tail_k] ^= (tail_k == L - 1 ? ncoarse : npre + npost) & 1;
+  for (int k = tail_k - 1; k >= 0; --k) {  // ascend
+    if (k + 1 < tail_k)
+      k_prolong<T, T><<<grid1d(C.big[k].n), kBlock, 0, s>>>(C.big[k], cur[k],
+                                                            C.big[k + 1].x[cur[k + 1]], st);
+    else
+      k_prolong<T, double><<<grid1d(C.big[k].n), kBlock, 0, s>>>(C.big[k], cur[k],
+                                                                 C.small[k + 1].x[cur[k + 1]], st);
+    for (int it = 0; it < npost; ++it) smooth(k, cur[k]);
+  }
+  cur0 = cur[0];
+}
+
+// Chunked enqueue of V-cycles until the device stop flag of `st` is seen (one
+// chunk queued ahead of the poll).  cycle() enqueues one cycle and its
+// residual check; cur_after[i] is level 0's buffer parity after cycle i+1.
+template <typename Cycle>
+int drive_cycles(const pfm_gamg_s *g, const KState *st, int max_cycles, hipStream_t s,
+                 std::vector<int> &cur_after, int &cur0, Cycle cycle) {
+  int32_t *flag = g->h_flag;
+  hipEvent_t ev[2];
+  PFM_CHECK_HIP(hipEventCreateWithFlags(&ev[0], hipEventDisableTiming));
+  PFM_CHECK_HIP(hipEventCreateWithFlags(&ev[1], hipEventDisableTiming));
+  int enq = 0, chunk = 2, slot = 0, rc = PFM_OK;
+  bool pending = false;
+  double last_report = now_seconds();
+  while (true) {
+    bool queued = false;
+    report_progress(g->fine, enq, last_report);
+    if (enq < max_cycles) {
+      const int c = std::min(chunk, max_cycles - enq);
+      for (int k = 0; k < c; ++k) {
+        cycle();
+        cur_after.push_back(cur0);
+      }
+      enq += c;
+      chunk = std::min(chunk * 2, 16);
+      if (hipMemcpyAsync(&flag[slot], &st->done, sizeof(int32_t), hipMemcpyDeviceToHost, s) !=
+              hipSuccess ||
+          hipEventRecord(ev[slot], s) != hipSuccess) {
+        rc = PFM_ERR_HIP;
+        break;
+      }
+      queued = true;
+    }
+    if (pending) {
+      if (hipEventSynchronize(ev[slot ^ 1]) != hipSuccess) {
+        rc = PFM_ERR_HIP;
+        break;
+      }
+      if (flag[slot ^ 1]) break;
+    }
+    if (!queued) break;
+    pending = true;
+    slot ^= 1;
+  }
+  (void)hipEventDestroy(ev[0]);
+  (void)hipEventDestroy(ev[1]);
+  if (rc != PFM_OK) set_error("HIP error while driving the GAMG cycles");
+  return rc;
+}
 
 extern "C" {
 
@@ -579,102 +859,25 @@ int pfm_gamg_solve(pfm_gamg g, const pfm_solve_params *prm, const double *diag, 
   const int hist_cap = prm->max_iter + 1;
   k_state_init<<<1, 1, 0, s>>>(w.st, prm->tolerance, prm->rel_tol, prm->max_iter, prm->min_iter,
                                hist_cap);
-  const int npre = g->prm.n_pre, npost = g->prm.n_post, ncoarse = g->prm.n_coarsest;
-  int tail_k = L - 1;  // first level handled by the single-workgroup kernel
-  while (tail_k > 0 && g->lv[tail_k - 1].n <= kTailCells) --tail_k;
-  int cur0 = 0;
+  CycleLevels<double> C;
+  C.tail_k = tail_level(g);
+  C.big.assign(w.lv.begin(), w.lv.begin() + C.tail_k);
+  C.small = w.lv;
+  C.d_small = w.d_levels;
   const int G = red_grid(g->fine);
-  const bool gs = g->prm.smoother == PFM_SMOOTHER_GAUSS_SEIDEL;
-  auto jac = [&](int k, int &cur) {
-    if (gs) {  // one symmetric sweep, level by level, in place
-      const LevelDev &D = w.lv[k];
-      const pfm_ldu h = g->lv[k].ldu;
-      for (int pass = 0; pass < 2; ++pass) {
-        const std::vector<int32_t> &ptr = pass ? h->gs_bwd_ptr : h->gs_fwd_ptr;
-        const int32_t *cells = pass ? h->d_gs_bwd_cells : h->d_gs_fwd_cells;
-        for (size_t l = 0; l + 1 < ptr.size(); ++l) {
-          const int cnt = ptr[l + 1] - ptr[l];
-          k_gs_level<<<grid1d(cnt), kBlock, 0, s>>>(D, cur, cells + ptr[l], cnt, w.st);
-        }
-      }
-      return;
-    }
-    k_jacobi<<<blocks_for_slices(w.lv[k].nslices), kBlock, 0, s>>>(w.lv[k], cur, w.st);
-    cur ^= 1;
-  };
-  auto cycle = [&]() {
-    int cur[kMaxLevels];
-    cur[0] = cur0;
-    int k = 0;
-    for (; k < tail_k; ++k) {  // descend through the large levels
-      for (int it = 0; it < npre; ++it) jac(k, cur[k]);
-      k_resid<<<blocks_for_slices(w.lv[k].nslices), kBlock, 0, s>>>(w.lv[k], cur[k], w.st);
-      k_restrict<<<grid1d(w.lv[k].nc), kBlock, 0, s>>>(w.lv[k], w.lv[k + 1], w.st);
-      cur[k + 1] = 0;
-    }
-    // levels tail_k .. L-1 in one workgroup
-    k_tail<<<1, kTailThreads, 0, s>>>(w.d_levels, tail_k, L, cur[tail_k], npre, npost, ncoarse,
-                                      gs ? 1 : 0, w.st);
-    if (!gs) cur[tail_k] ^= (tail_k == L - 1 ? ncoarse : npre + npost) & 1;
-    for (k = tail_k - 1; k >= 0; --k) {  // ascend
-      k_prolong<<<grid1d(w.lv[k].n), kBlock, 0, s>>>(w.lv[k], cur[k], w.lv[k + 1].x[cur[k + 1]],
-                                                    w.st);
-      for (int it = 0; it < npost; ++it) jac(k, cur[k]);
-    }
-    cur0 = cur[0];
-    k_res_norm<<<G, kBlock, 0, s>>>(w.lv[0], cur0, w.part, w.st);
-    k_gamg_fin<<<1, kBlock, 0, s>>>(w.part, G, w.st, w.hist);
-  };
+  int cur0 = 0;
   // the buffer parity of level 0 advances by the same amount every cycle, so
-  // enqueueing cycles ahead of the stop flag never changes which buffer holds x
-  int32_t *flag = g->h_flag;
-  hipEvent_t ev[2];
-  PFM_CHECK_HIP(hipEventCreateWithFlags(&ev[0], hipEventDisableTiming));
-  PFM_CHECK_HIP(hipEventCreateWithFlags(&ev[1], hipEventDisableTiming));
-  int enq = 0, chunk = 2, slot = 0;
-  bool pending = false;
-  int rc = PFM_OK;
-  // cur0 after a cycle that did not run (stop flag set) is unchanged on the
-  // device, so track the value after every completed cycle separately
+  // enqueueing cycles ahead of the stop flag never changes which buffer holds
+  // x; cur0 after a cycle that did not run is unchanged on the device, so the
+  // value after every enqueued cycle is kept
   std::vector<int> cur_after;
   cur_after.reserve(prm->max_iter + 1);
-  double last_report = now_seconds();
-  while (true) {
-    bool queued = false;
-    report_progress(g->fine, enq, last_report);
-    if (enq < prm->max_iter) {
-      const int c = std::min(chunk, prm->max_iter - enq);
-      for (int k = 0; k < c; ++k) {
-        cycle();
-        cur_after.push_back(cur0);
-      }
-      enq += c;
-      chunk = std::min(chunk * 2, 16);
-      if (hipMemcpyAsync(&flag[slot], &w.st->done, sizeof(int32_t), hipMemcpyDeviceToHost, s) !=
-              hipSuccess ||
-          hipEventRecord(ev[slot], s) != hipSuccess) {
-        rc = PFM_ERR_HIP;
-        break;
-      }
-      queued = true;
-    }
-    if (pending) {
-      if (hipEventSynchronize(ev[slot ^ 1]) != hipSuccess) {
-        rc = PFM_ERR_HIP;
-        break;
-      }
-      if (flag[slot ^ 1]) break;
-    }
-    if (!queued) break;
-    pending = true;
-    slot ^= 1;
-  }
-  (void)hipEventDestroy(ev[0]);
-  (void)hipEventDestroy(ev[1]);
-  if (rc != PFM_OK) {
-    set_error("HIP error while driving the GAMG loop");
-    return rc;
-  }
+  int rc = drive_cycles(g, w.st, prm->max_iter, s, cur_after, cur0, [&]() {
+    enqueue_cycle<double>(g, C, w.st, cur0, s);
+    k_res_norm<double><<<G, kBlock, 0, s>>>(w.lv[0], cur0, w.part, w.st);
+    k_gamg_fin<<<1, kBlock, 0, s>>>(w.part, G, w.st, w.hist);
+  });
+  if (rc != PFM_OK) return rc;
   KState hs;
   PFM_CHECK_HIP(hipMemcpyAsync(&hs, w.st, sizeof(KState), hipMemcpyDeviceToHost, s));
   PFM_CHECK_HIP(hipStreamSynchronize(s));
@@ -692,6 +895,119 @@ int pfm_gamg_solve(pfm_gamg g, const pfm_solve_params *prm, const double *diag, 
   res->converged = res->status == PFM_SOLVE_CONVERGED;
   res->initial_residual = hs.nhist > 0 ? hs.res0 : 0.0;
   res->final_residual = hs.nhist > 0 ? hs.res : 0.0;
+  return PFM_OK;
+}
+
+int64_t pfm_gamg_mixed_workspace_bytes(pfm_gamg g, int32_t max_outer) {
+  if (!g || max_outer < 0) return -1;
+  return (int64_t)gamg_mixed_layout(g, max_outer, nullptr, nullptr);
+}
+
+int pfm_gamg_mixed_solve(pfm_gamg g, const pfm_solve_params *prm, double inner_rel_tol,
+                         int32_t max_outer, const double *diag, const double *lower,
+                         const double *upper, const double *b, double *x, void *work,
+                         double *history_h, pfm_solve_result *res, int32_t *inner_cycles,
+                         void *stream) {
+  PFM_REQUIRE(g && prm && res && work, "null argument");
+  PFM_REQUIRE(prm->max_iter >= 0 && max_outer >= 0, "negative iteration limit");
+  PFM_REQUIRE(inner_rel_tol > 0.0 && inner_rel_tol < 1.0, "inner_rel_tol must be in (0, 1)");
+  const int n0 = g->lv[0].n;
+  PFM_REQUIRE(n0 == 0 || (diag && b && x), "null device pointer");
+  memset(res, 0, sizeof(*res));
+  if (inner_cycles) *inner_cycles = 0;
+  if (n0 == 0) {
+    res->status = PFM_SOLVE_MAXITER;
+    return PFM_OK;
+  }
+  hipStream_t s = (hipStream_t)stream;
+  int st = pfm_gamg_setup_values(g, diag, lower, upper, work, stream);
+  if (st != PFM_OK) return st;
+  GamgMixedWork m;
+  gamg_mixed_layout(g, max_outer, &m, (char *)work);
+  GamgWork &w = m.w64;
+  const int L = (int)g->lv.size();
+  const int tail_k = tail_level(g);
+  w.lv[0].diag = diag;
+  w.lv[0].x[0] = x;
+  w.lv[0].b = const_cast<double *>(b);
+  // FP32 values of the large levels: coefficients packed from the FP64
+  // (coarsened in the reference order) ones, diagonal and 1/|diag| cast
+  for (int k = 0; k < tail_k; ++k) {
+    const auto &H = g->lv[k];
+    const double *lo = k == 0 ? lower : w.lower[k], *up = k == 0 ? upper : w.upper[k];
+    const double *d64 = k == 0 ? diag : w.diag[k];
+    if (H.nf && (st = pfm_ldu_pack_f32(H.ldu, 0, lo, up, m.packed32[k], stream)) != PFM_OK)
+      return st;
+    k_to_f32<<<grid1d(H.n), kBlock, 0, s>>>(H.n, d64, m.diag32[k]);
+    k_to_f32<<<grid1d(H.n), kBlock, 0, s>>>(H.n, w.lv[k].inv, m.inv32[k]);
+  }
+  // FP64 tail descriptors; with no large level the inner cycle starts in the
+  // tail on level 0, whose solution and right-hand side are the inner buffers
+  std::vector<LevelDev> small = w.lv;
+  small[0].x[0] = m.e64;
+  small[0].b = m.b64;
+  PFM_CHECK_HIP(hipMemcpyAsync(m.d_small, small.data(), sizeof(LevelDev) * L,
+                               hipMemcpyHostToDevice, s));
+  CycleLevels<float> C;
+  C.tail_k = tail_k;
+  C.big = m.lv32;
+  C.small = small;
+  C.d_small = m.d_small;
+  const int hist_cap = max_outer + 1;
+  k_state_init<<<1, 1, 0, s>>>(w.st, prm->tolerance, prm->rel_tol, prm->max_iter, prm->min_iter,
+                               hist_cap);
+  PFM_CHECK_HIP(hipGetLastError());
+  const int G = red_grid(g->fine);
+  int inner_total = 0;
+  std::vector<int> cur_after;
+  for (int outer = 0;; ++outer) {
+    k_gm_resid<<<G, kBlock, 0, s>>>(w.lv[0], m.r64, w.part, w.st);
+    const int budget = prm->max_iter - inner_total;
+    k_gm_outer_fin<<<1, kBlock, 0, s>>>(w.part, G, w.st, w.hist, outer, max_outer, budget,
+                                        inner_rel_tol, m.in);
+    int32_t done = 0;
+    PFM_CHECK_HIP(hipMemcpyAsync(&done, &w.st->done, sizeof(int32_t), hipMemcpyDeviceToHost, s));
+    PFM_CHECK_HIP(hipStreamSynchronize(s));
+    if (done) break;
+    if (tail_k > 0)
+      k_gm_cast<float><<<grid1d(n0), kBlock, 0, s>>>(n0, m.r64, m.lv32[0].b, m.lv32[0].x[0], w.st);
+    else
+      k_gm_cast<double><<<grid1d(n0), kBlock, 0, s>>>(n0, m.r64, m.b64, m.e64, w.st);
+    int cur0 = 0;
+    cur_after.clear();
+    int rc = drive_cycles(g, m.in, budget, s, cur_after, cur0, [&]() {
+      enqueue_cycle<float>(g, C, m.in, cur0, s);
+      if (tail_k > 0)
+        k_res_norm<float><<<G, kBlock, 0, s>>>(m.lv32[0], cur0, w.part, m.in);
+      else
+        k_res_norm<double><<<G, kBlock, 0, s>>>(small[0], cur0, w.part, m.in);
+      k_gamg_fin<<<1, kBlock, 0, s>>>(w.part, G, m.in, nullptr);
+    });
+    if (rc != PFM_OK) return rc;
+    int32_t executed = 0;
+    PFM_CHECK_HIP(hipMemcpyAsync(&executed, &m.in->iter, sizeof(int32_t), hipMemcpyDeviceToHost, s));
+    PFM_CHECK_HIP(hipStreamSynchronize(s));
+    const int par = executed > 0 ? cur_after[executed - 1] : 0;
+    if (tail_k > 0)
+      k_gm_correct<float><<<grid1d(n0), kBlock, 0, s>>>(n0, m.lv32[0].x[par], x, w.st);
+    else
+      k_gm_correct<double><<<grid1d(n0), kBlock, 0, s>>>(n0, par ? small[0].x[1] : m.e64, x,
+                                                        w.st);
+    inner_total += executed;
+  }
+  KState hs;
+  PFM_CHECK_HIP(hipMemcpyAsync(&hs, w.st, sizeof(KState), hipMemcpyDeviceToHost, s));
+  PFM_CHECK_HIP(hipStreamSynchronize(s));
+  const int nh = std::min(hs.nhist, hist_cap);
+  if (history_h && nh > 0)
+    PFM_CHECK_HIP(hipMemcpyAsync(history_h, w.hist, sizeof(double) * nh, hipMemcpyDeviceToHost, s));
+  PFM_CHECK_HIP(hipStreamSynchronize(s));
+  res->iterations = hs.nhist;
+  res->status = hs.status == 0 ? PFM_SOLVE_MAXITER : hs.status;
+  res->converged = res->status == PFM_SOLVE_CONVERGED;
+  res->initial_residual = hs.nhist > 0 ? hs.res0 : 0.0;
+  res->final_residual = hs.nhist > 0 ? hs.res : 0.0;
+  if (inner_cycles) *inner_cycles = inner_total;
   return PFM_OK;
 }
 

@@ -13,17 +13,19 @@ int gamg_coarse_faces(int32_t nf, const int32_t *own, const int32_t *nei,
 int pair_level(int32_t n_fine, int32_t n_cur, int32_t nf, const int32_t *own, const int32_t *nei,
                const std::vector<int32_t> &cell_map, std::vector<int32_t> &out, int32_t &n_new);
 
-// Device pointers of one level, as the kernels see them.
-struct LevelDev {
+// Device pointers of one level, as the kernels see them (T = double, or
+// float for the inner cycles of the mixed-precision solve).
+template <typename T>
+struct LevelDevT {
   int32_t n, nslices;
   const SliceInfo *slices;
   const int32_t *col;
-  const double *packed;
-  const double *diag;
-  const double *inv;      // 1 / max(|diag|, 1e-30)
-  double *x[2];           // Jacobi double buffer
-  double *b;
-  double *r;
+  const T *packed;
+  const T *diag;
+  const T *inv;           // 1 / max(|diag|, 1e-30)
+  T *x[2];                // Jacobi double buffer
+  T *b;
+  T *r;
   // restriction to the next coarser level (unused on the coarsest)
   const int32_t *f2c;
   const int32_t *agg_ptr;
@@ -34,6 +36,7 @@ struct LevelDev {
   const int32_t *gs_fptr, *gs_fcells, *gs_bptr, *gs_bcells;
   int32_t gs_nfl, gs_nbl;
 };
+using LevelDev = LevelDevT<double>;
 
 }  // namespace pfm
 

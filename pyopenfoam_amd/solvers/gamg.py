@@ -17,7 +17,13 @@ intra-aggregate couplings to the coarse diagonal (the Galerkin product of the
 piecewise-constant prolongation), which the reference drops;
 ``level_smoother="GaussSeidel"`` replaces every Jacobi smoothing step (pre,
 post and coarsest) by a symmetric Gauss-Seidel sweep with the arithmetic of
-``GaussSeidelSmoother`` (``solvers/smoothers.py:92-149``).
+``GaussSeidelSmoother`` (``solvers/smoothers.py:92-149``);
+``precision="mixed"`` solves in FP64 iterative refinement whose inner V-cycles
+run in FP32 on the large levels (FP64 on the small levels, the coarsest-level
+solve included) to ``inner_rel_tol``; the residual monitor then runs on the
+FP64 true residual after each refinement step (the initial residual
+included), ``max_iter`` bounds the total V-cycles and
+``last_info.inner_iterations`` counts them.
 
 The hierarchy topology is built once per mesh topology (host C++) and cached;
 coefficients are coarsened on the device at every solve, and the whole V-cycle
@@ -107,7 +113,8 @@ class GAMGSolver(LinearSolverBase):
                  n_post_smooth: int = 2, max_levels: int = 10, min_cells_coarse: int = 10,
                  smoother: str = "PCG", agglomeration: str = "greedy",
                  coarse: str = "reference", n_coarsest: int = 50,
-                 level_smoother: str = "Jacobi") -> None:
+                 level_smoother: str = "Jacobi", precision: str = "double",
+                 inner_rel_tol: float = 1e-3, max_outer: int = 50) -> None:
         super().__init__(tolerance=tolerance, rel_tol=rel_tol, max_iter=max_iter,
                          min_iter=min_iter, verbose=verbose)
         self._n_pre_smooth = n_pre_smooth
@@ -127,6 +134,11 @@ class GAMGSolver(LinearSolverBase):
         self._params = _lib.GamgParams(max_levels, min_cells_coarse, n_pre_smooth, n_post_smooth,
                                        n_coarsest, _AGGLOM[key], _COARSE[coarse.lower()],
                                        _LEVEL_SMOOTHER[sk])
+        if precision not in ("double", "mixed"):
+            raise ValueError(f"Unknown precision '{precision}'. Use 'double' or 'mixed'.")
+        self._precision = precision
+        self._inner_rel_tol = float(inner_rel_tol)
+        self._max_outer = int(max_outer)
         self._ws = KrylovWorkspace()
 
     def hierarchy(self, matrix: LduMatrix) -> GamgHierarchy:
@@ -148,10 +160,12 @@ class GAMGSolver(LinearSolverBase):
             return x, monitor.build_info(False)
         hier = self.hierarchy(matrix)
         L = _lib.lib()
-        work = self._ws.get(int(L.pfm_gamg_workspace_bytes(hier.handle, max_iter)), dev)
         prm = _lib.SolveParams(float(monitor.tolerance), float(monitor.rel_tol), max_iter,
                                int(monitor.min_iter), 0, 0)
         res = _lib.SolveResult()
+        if self._precision == "mixed":
+            return self._solve_mixed(matrix, hier, prm, b, x, monitor)
+        work = self._ws.get(int(L.pfm_gamg_workspace_bytes(hier.handle, max_iter)), dev)
         hist = np.zeros(max_iter + 1)
         _lib.check(L.pfm_gamg_solve(hier.handle, C.byref(prm),
                                     _lib.ptr(matrix.diag.detach().contiguous()),
@@ -162,6 +176,27 @@ class GAMGSolver(LinearSolverBase):
         monitor.load_history(hist[: min(res.iterations, max_iter + 1)].tolist())
         info = monitor.build_info(converged=bool(res.converged))
         info.status = int(res.status)
+        return x, info
+
+    def _solve_mixed(self, matrix, hier, prm, b, x, monitor):
+        L = _lib.lib()
+        dev = matrix.device
+        mo = self._max_outer
+        work = self._ws.get(int(L.pfm_gamg_mixed_workspace_bytes(hier.handle, mo)), dev)
+        res = _lib.SolveResult()
+        hist = np.zeros(mo + 1)
+        inner = C.c_int32()
+        _lib.check(L.pfm_gamg_mixed_solve(hier.handle, C.byref(prm), self._inner_rel_tol, mo,
+                                          _lib.ptr(matrix.diag.detach().contiguous()),
+                                          _lib.ptr(matrix.lower.detach().contiguous()),
+                                          _lib.ptr(matrix.upper.detach().contiguous()),
+                                          _lib.ptr(b), _lib.ptr(x), _lib.ptr(work),
+                                          hist.ctypes.data, C.byref(res), C.byref(inner),
+                                          current_stream(dev)), "pfm_gamg_mixed_solve")
+        monitor.load_history(hist[: min(res.iterations, mo + 1)].tolist())
+        info = monitor.build_info(converged=bool(res.converged))
+        info.status = int(res.status)
+        info.inner_iterations = int(inner.value)
         return x, info
 
     def coarse_values(self, matrix: LduMatrix, level: int):

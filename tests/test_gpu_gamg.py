@@ -106,3 +106,55 @@ def test_gamg_gauss_seidel_bit_exact(gpu_device, n, jit, sig):
     assert it == io == ncyc
     assert bits_equal(x.cpu().numpy(), xo)
     np.testing.assert_allclose(solver.last_info.residual_history, info.residual_history, rtol=1e-12)
+
+
+def _pair(c, d, lo, up, dev, **kw):
+    m = _matrix(c.n_cells, c.owner, c.neighbour, d, lo, up, dev)
+    base = dict(tolerance=1e-300, agglomeration="pair", coarse="galerkin", max_levels=12)
+    base.update(kw)
+    return m, base
+
+
+@pytest.mark.parametrize("n", [8, 24])
+def test_gamg_mixed_one_cycle_matches_fp64_cycle(gpu_device, n):
+    """One refinement step with one inner V-cycle from x0 = 0: x = e, the
+    V-cycle applied to r = b.  8^3 (every level in the FP64 tail): bit-exact
+    with one FP64 cycle.  24^3 (level 0 in FP32, an odd number of Jacobi
+    sweeps per level so the buffer parity flips): the FP64 cycle to FP32
+    rounding (rel 1e-5)."""
+    c, d, lo, up = system(n, 0.2, 1.0)
+    m, kw = _pair(c, d, lo, up, gpu_device, n_pre_smooth=1)
+    b = torch.from_numpy(c.b)
+    z = torch.zeros(c.n_cells, dtype=torch.float64)
+    x64, *_ = GAMGSolver(rel_tol=1e-300, max_iter=1, **kw)(m, b, z)
+    sm = GAMGSolver(rel_tol=1e-300, max_iter=1, precision="mixed", inner_rel_tol=0.999,
+                    max_outer=1, **kw)
+    xm, *_ = sm(m, b, z)
+    assert sm.last_info.inner_iterations == 1
+    a, e = x64.cpu().numpy(), xm.cpu().numpy()
+    if n == 8:
+        assert bits_equal(e, a)
+    else:
+        assert np.linalg.norm(e - a) <= 1e-5 * np.linalg.norm(a)
+
+
+def test_gamg_mixed_converges(gpu_device):
+    """FP64 refinement around FP32 V-cycles reaches the FP64 tolerance: the
+    true residual of the result is below rel_tol times the initial one, and
+    the solution agrees with the FP64 GAMG solution to the tolerance's
+    order."""
+    c, d, lo, up = system(24, 0.2, 1.0)
+    m, kw = _pair(c, d, lo, up, gpu_device)
+    b = torch.from_numpy(c.b)
+    z = torch.zeros(c.n_cells, dtype=torch.float64)
+    sm = GAMGSolver(rel_tol=1e-8, max_iter=5000, precision="mixed", inner_rel_tol=1e-3, **kw)
+    xm, outer, _ = sm(m, b, z)
+    assert sm.last_info.converged
+    assert outer >= 3 and sm.last_info.inner_iterations > 0
+    from oracle import ldu as oldu
+    xr = xm.cpu().numpy()
+    true_rel = np.linalg.norm(c.b - oldu.ax(d, lo, up, c.owner, c.neighbour, xr)) / np.linalg.norm(c.b)
+    assert true_rel < 1e-8, true_rel
+    x64, *_ = GAMGSolver(rel_tol=1e-10, max_iter=5000, **kw)(m, b, z)
+    xa = x64.cpu().numpy()
+    assert np.linalg.norm(xr - xa) <= 1e-6 * np.linalg.norm(xa)
