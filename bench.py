@@ -399,6 +399,9 @@ def main() -> int:
         gamg = {}
         for name, kw in (("pair_galerkin_jacobi", dict(agglomeration="pair", coarse="galerkin",
                                                         max_levels=20)),
+                         ("pair_galerkin_jacobi_mixed", dict(agglomeration="pair", coarse="galerkin",
+                                                              max_levels=20, precision="mixed",
+                                                              inner_rel_tol=1e-3)),
                          ("reference", dict())):
             gs = GAMGSolver(tolerance=1e-300, rel_tol=1e-8, max_iter=args.gamg_max_iter, **kw)
             hier = gs.hierarchy(mat)
@@ -410,9 +413,19 @@ def main() -> int:
             torch.cuda.synchronize()
             gms = (time.perf_counter() - t0) * 1e3
             rt = (mat.source - mat.Ax(xg)).norm().item() / mat.source.norm().item()
+            extra = {}
+            if kw.get("precision") == "mixed":  # cyc counts refinement steps
+                extra["refinement_steps"] = int(cyc) - 1
+                cyc = gs.last_info.inner_iterations
+                ref = gamg.get("pair_galerkin_jacobi")
+                if ref is not None:
+                    extra["speedup_vs_fp64"] = round(ref["ms"] / gms, 3)
+                    extra["rel_err_vs_fp64_solution"] = ((xg - x_fp64).norm() / x_fp64.norm()).item()
+            else:
+                x_fp64 = xg
             gamg[name] = {"ms": round(gms, 2), "cycles": int(cyc), "ms_per_cycle": round(gms / cyc, 3),
                           "converged": bool(gs.last_info.converged), "true_rel_residual": rt,
-                          "levels": hier.n_levels, **{k: v for k, v in kw.items()}}
+                          "levels": hier.n_levels, **extra, **{k: v for k, v in kw.items()}}
             log(f"gamg {name}: {gms:.1f} ms, {cyc} cycles ({gms / cyc:.2f} ms/cycle), converged "
                 f"{gs.last_info.converged}, true rel residual {rt:.2e}, {hier.n_levels} levels")
 

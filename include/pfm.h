@@ -339,16 +339,18 @@ int pfm_gamg_solve(pfm_gamg g, const pfm_solve_params *prm, const double *diag,
 /* Mixed-precision GAMG (no reference counterpart: the precision pattern on
  * GAMGSolver._solve, gamg.py:541-585).  FP64 iterative refinement: r = b - A x
  * in FP64 and the residual monitor on it (the initial residual included, as
- * PCG); each refinement step solves A e = r with V-cycles whose large levels
- * (more than 4096 cells) run in FP32 (coefficients coarsened in FP64 in the
+ * PCG); each refinement step solves A e = r with V-cycles whose finest
+ * fp32_levels levels run in FP32 (coefficients coarsened in FP64 in the
  * reference order, then cast; FP32 SpMV, smoothing, restriction and
- * prolongation) and whose small levels, the coarsest-level solve included,
- * run in FP64, until the inner residual drops below inner_rel_tol times the
- * outer one; then x += e in FP64.  prm->max_iter bounds the total V-cycles,
- * max_outer the refinement steps; *inner_cycles returns the cycles run. */
-int64_t pfm_gamg_mixed_workspace_bytes(pfm_gamg g, int32_t max_outer);
+ * prolongation) and whose coarser levels, the coarsest-level solve included,
+ * run in FP64 (levels of at most 4096 cells are always FP64), until the inner
+ * residual drops below inner_rel_tol times the outer one; then x += e in
+ * FP64.  prm->max_iter bounds the total V-cycles, max_outer the refinement
+ * steps; *inner_cycles returns the cycles run. */
+int64_t pfm_gamg_mixed_workspace_bytes(pfm_gamg g, int32_t max_outer, int32_t fp32_levels);
 int pfm_gamg_mixed_solve(pfm_gamg g, const pfm_solve_params *prm, double inner_rel_tol,
-                         int32_t max_outer, const double *diag, const double *lower,
+                         int32_t max_outer, int32_t fp32_levels, const double *diag,
+                         const double *lower,
                          const double *upper, const double *b, double *x, void *work,
                          double *history_h, pfm_solve_result *res, int32_t *inner_cycles,
                          void *stream);

@@ -22,6 +22,7 @@
 
 #include <cmath>
 #include <cstring>
+#include <type_traits>
 
 #include "gamg_internal.h"
 #include "solver_state.h"
@@ -432,7 +433,13 @@ inline int tail_level(const pfm_gamg_s *g) {
   return k;
 }
 
-size_t gamg_mixed_layout(const pfm_gamg_s *g, int max_outer, GamgMixedWork *m, char *base) {
+// Levels computed in FP32: the first fp32_levels, and never a tail level.
+inline int fp32_count(const pfm_gamg_s *g, int fp32_levels) {
+  return std::min(fp32_levels, tail_level(g));
+}
+
+size_t gamg_mixed_layout(const pfm_gamg_s *g, int max_outer, int fp32_levels, GamgMixedWork *m,
+                         char *base) {
   GamgWork w64;
   size_t off = gamg_layout(g, max_outer, &w64, base);
   auto take = [&](size_t bytes) -> void * {
@@ -440,13 +447,13 @@ size_t gamg_mixed_layout(const pfm_gamg_s *g, int max_outer, GamgMixedWork *m, c
     off += align256(bytes);
     return base ? (void *)(base + o) : nullptr;
   };
-  const int tail_k = tail_level(g);
+  const int nbig = fp32_count(g, fp32_levels);
   GamgMixedWork t;
-  t.lv32.resize(tail_k);
-  t.packed32.resize(tail_k);
-  t.diag32.resize(tail_k);
-  t.inv32.resize(tail_k);
-  for (int k = 0; k < tail_k; ++k) {
+  t.lv32.resize(nbig);
+  t.packed32.resize(nbig);
+  t.diag32.resize(nbig);
+  t.inv32.resize(nbig);
+  for (int k = 0; k < nbig; ++k) {
     const auto &H = g->lv[k];
     const size_t n = (size_t)H.n;
     const LevelDev &D64 = w64.lv[k];
@@ -545,14 +552,16 @@ void free_level(GamgLevelHost &H) {
 
 }  // namespace
 
-// The levels one V-cycle runs on: levels [0, tail_k) with values of type T,
-// the tail levels always FP64 (host descriptors and their device copy).
+// The levels one V-cycle runs on: levels [0, nbig) with values of type T,
+// the others in FP64: [nbig, tail_k) by per-level launches, [tail_k, L) in
+// the one-workgroup tail (host descriptors and their device copy).
 template <typename T>
 struct CycleLevels {
-  std::vector<LevelDevT<T>> big;
-  std::vector<LevelDev> small;
+  std::vector<LevelDevT<T>> big;  // levels [0, nbig) in T
+  std::vector<LevelDev> small;    // FP64 descriptors of every level: [nbig, L) use them
   const LevelDev *d_small = nullptr;
-  int tail_k = 0;
+  int nbig = 0;                   // levels in T (nbig <= tail_k)
+  int tail_k = 0;                 // levels [tail_k, L) run in the one-workgroup tail
 };
 
 // Enqueue one V-cycle (solvers/gamg.py:437-496); cur0 is the parity of
@@ -563,9 +572,9 @@ void enqueue_cycle(const pfm_gamg_s *g, const CycleLevels<T> &C, const KState *s
   const int L = (int)g->lv.size();
   const int npre = g->prm.n_pre, npost = g->prm.n_post, ncoarse = g->prm.n_coarsest;
   const bool gs = g->prm.smoother == PFM_SMOOTHER_GAUSS_SEIDEL;
-  const int tail_k = C.tail_k;
-  auto smooth = [&](int k, int &cur) {
-    const LevelDevT<T> &D = C.big[k];
+  const int tail_k = C.tail_k, nbig = C.nbig;
+  auto smooth_level = [&](const auto &D, int k, int &cur) {
+    using V = typename std::remove_const<typename std::remove_pointer<decltype(D.packed)>::type>::type;
     if (gs) {  // one symmetric sweep, level by level, in place
       const pfm_ldu h = g->lv[k].ldu;
       for (int pass = 0; pass < 2; ++pass) {
@@ -573,23 +582,36 @@ void enqueue_cycle(const pfm_gamg_s *g, const CycleLevels<T> &C, const KState *s
         const int32_t *cells = pass ? h->d_gs_bwd_cells : h->d_gs_fwd_cells;
         for (size_t l = 0; l + 1 < ptr.size(); ++l) {
           const int cnt = ptr[l + 1] - ptr[l];
-          k_gs_level<T><<<grid1d(cnt), kBlock, 0, s>>>(D, cur, cells + ptr[l], cnt, st);
+          k_gs_level<V><<<grid1d(cnt), kBlock, 0, s>>>(D, cur, cells + ptr[l], cnt, st);
         }
       }
       return;
     }
-    k_jacobi<T><<<blocks_for_slices(D.nslices), kBlock, 0, s>>>(D, cur, st);
+    k_jacobi<V><<<blocks_for_slices(D.nslices), kBlock, 0, s>>>(D, cur, st);
     cur ^= 1;
+  };
+  auto smooth = [&](int k, int &cur) {
+    if (k < nbig)
+      smooth_level(C.big[k], k, cur);
+    else
+      smooth_level(C.small[k], k, cur);
   };
   int cur[kMaxLevels];
   cur[0] = cur0;
   for (int k = 0; k < tail_k; ++k) {  // descend through the large levels
     for (int it = 0; it < npre; ++it) smooth(k, cur[k]);
-    k_resid<T><<<blocks_for_slices(C.big[k].nslices), kBlock, 0, s>>>(C.big[k], cur[k], st);
-    if (k + 1 < tail_k)
-      k_restrict<T, T><<<grid1d(C.big[k].nc), kBlock, 0, s>>>(C.big[k], C.big[k + 1], st);
-    else
-      k_restrict<T, double><<<grid1d(C.big[k].nc), kBlock, 0, s>>>(C.big[k], C.small[k + 1], st);
+    if (k < nbig) {
+      const LevelDevT<T> &F = C.big[k];
+      k_resid<T><<<blocks_for_slices(F.nslices), kBlock, 0, s>>>(F, cur[k], st);
+      if (k + 1 < nbig)
+        k_restrict<T, T><<<grid1d(F.nc), kBlock, 0, s>>>(F, C.big[k + 1], st);
+      else
+        k_restrict<T, double><<<grid1d(F.nc), kBlock, 0, s>>>(F, C.small[k + 1], st);
+    } else {
+      const LevelDev &F = C.small[k];
+      k_resid<double><<<blocks_for_slices(F.nslices), kBlock, 0, s>>>(F, cur[k], st);
+      k_restrict<double, double><<<grid1d(F.nc), kBlock, 0, s>>>(F, C.small[k + 1], st);
+    }
     cur[k + 1] = 0;
   }
   // levels tail_k .. L-1 in one workgroup
@@ -597,12 +619,18 @@ void enqueue_cycle(const pfm_gamg_s *g, const CycleLevels<T> &C, const KState *s
                                     gs ? 1 : 0, st);
   if (!gs) cur[tail_k] ^= (tail_k == L - 1 ? ncoarse : npre + npost) & 1;
   for (int k = tail_k - 1; k >= 0; --k) {  // ascend
-    if (k + 1 < tail_k)
-      k_prolong<T, T><<<grid1d(C.big[k].n), kBlock, 0, s>>>(C.big[k], cur[k],
-                                                            C.big[k + 1].x[cur[k + 1]], st);
-    else
-      k_prolong<T, double><<<grid1d(C.big[k].n), kBlock, 0, s>>>(C.big[k], cur[k],
-                                                                 C.small[k + 1].x[cur[k + 1]], st);
+    if (k < nbig) {
+      const LevelDevT<T> &F = C.big[k];
+      if (k + 1 < nbig)
+        k_prolong<T, T><<<grid1d(F.n), kBlock, 0, s>>>(F, cur[k], C.big[k + 1].x[cur[k + 1]], st);
+      else
+        k_prolong<T, double><<<grid1d(F.n), kBlock, 0, s>>>(F, cur[k],
+                                                           C.small[k + 1].x[cur[k + 1]], st);
+    } else {
+      const LevelDev &F = C.small[k];
+      k_prolong<double, double><<<grid1d(F.n), kBlock, 0, s>>>(F, cur[k],
+                                                              C.small[k + 1].x[cur[k + 1]], st);
+    }
     for (int it = 0; it < npost; ++it) smooth(k, cur[k]);
   }
   cur0 = cur[0];
@@ -861,6 +889,7 @@ int pfm_gamg_solve(pfm_gamg g, const pfm_solve_params *prm, const double *diag, 
                                hist_cap);
   CycleLevels<double> C;
   C.tail_k = tail_level(g);
+  C.nbig = C.tail_k;
   C.big.assign(w.lv.begin(), w.lv.begin() + C.tail_k);
   C.small = w.lv;
   C.d_small = w.d_levels;
@@ -898,18 +927,20 @@ int pfm_gamg_solve(pfm_gamg g, const pfm_solve_params *prm, const double *diag, 
   return PFM_OK;
 }
 
-int64_t pfm_gamg_mixed_workspace_bytes(pfm_gamg g, int32_t max_outer) {
-  if (!g || max_outer < 0) return -1;
-  return (int64_t)gamg_mixed_layout(g, max_outer, nullptr, nullptr);
+int64_t pfm_gamg_mixed_workspace_bytes(pfm_gamg g, int32_t max_outer, int32_t fp32_levels) {
+  if (!g || max_outer < 0 || fp32_levels < 0) return -1;
+  return (int64_t)gamg_mixed_layout(g, max_outer, fp32_levels, nullptr, nullptr);
 }
 
 int pfm_gamg_mixed_solve(pfm_gamg g, const pfm_solve_params *prm, double inner_rel_tol,
-                         int32_t max_outer, const double *diag, const double *lower,
+                         int32_t max_outer, int32_t fp32_levels, const double *diag,
+                         const double *lower,
                          const double *upper, const double *b, double *x, void *work,
                          double *history_h, pfm_solve_result *res, int32_t *inner_cycles,
                          void *stream) {
   PFM_REQUIRE(g && prm && res && work, "null argument");
-  PFM_REQUIRE(prm->max_iter >= 0 && max_outer >= 0, "negative iteration limit");
+  PFM_REQUIRE(prm->max_iter >= 0 && max_outer >= 0 && fp32_levels >= 0,
+              "negative iteration or level count");
   PFM_REQUIRE(inner_rel_tol > 0.0 && inner_rel_tol < 1.0, "inner_rel_tol must be in (0, 1)");
   const int n0 = g->lv[0].n;
   PFM_REQUIRE(n0 == 0 || (diag && b && x), "null device pointer");
@@ -923,16 +954,17 @@ int pfm_gamg_mixed_solve(pfm_gamg g, const pfm_solve_params *prm, double inner_r
   int st = pfm_gamg_setup_values(g, diag, lower, upper, work, stream);
   if (st != PFM_OK) return st;
   GamgMixedWork m;
-  gamg_mixed_layout(g, max_outer, &m, (char *)work);
+  gamg_mixed_layout(g, max_outer, fp32_levels, &m, (char *)work);
   GamgWork &w = m.w64;
   const int L = (int)g->lv.size();
   const int tail_k = tail_level(g);
+  const int nbig = fp32_count(g, fp32_levels);
   w.lv[0].diag = diag;
   w.lv[0].x[0] = x;
   w.lv[0].b = const_cast<double *>(b);
   // FP32 values of the large levels: coefficients packed from the FP64
   // (coarsened in the reference order) ones, diagonal and 1/|diag| cast
-  for (int k = 0; k < tail_k; ++k) {
+  for (int k = 0; k < nbig; ++k) {
     const auto &H = g->lv[k];
     const double *lo = k == 0 ? lower : w.lower[k], *up = k == 0 ? upper : w.upper[k];
     const double *d64 = k == 0 ? diag : w.diag[k];
@@ -941,8 +973,8 @@ int pfm_gamg_mixed_solve(pfm_gamg g, const pfm_solve_params *prm, double inner_r
     k_to_f32<<<grid1d(H.n), kBlock, 0, s>>>(H.n, d64, m.diag32[k]);
     k_to_f32<<<grid1d(H.n), kBlock, 0, s>>>(H.n, w.lv[k].inv, m.inv32[k]);
   }
-  // FP64 tail descriptors; with no large level the inner cycle starts in the
-  // tail on level 0, whose solution and right-hand side are the inner buffers
+  // FP64 descriptors; with no FP32 level the inner cycle starts on the FP64
+  // level 0, whose solution and right-hand side are the inner buffers
   std::vector<LevelDev> small = w.lv;
   small[0].x[0] = m.e64;
   small[0].b = m.b64;
@@ -950,6 +982,7 @@ int pfm_gamg_mixed_solve(pfm_gamg g, const pfm_solve_params *prm, double inner_r
                                hipMemcpyHostToDevice, s));
   CycleLevels<float> C;
   C.tail_k = tail_k;
+  C.nbig = nbig;
   C.big = m.lv32;
   C.small = small;
   C.d_small = m.d_small;
@@ -969,7 +1002,7 @@ int pfm_gamg_mixed_solve(pfm_gamg g, const pfm_solve_params *prm, double inner_r
     PFM_CHECK_HIP(hipMemcpyAsync(&done, &w.st->done, sizeof(int32_t), hipMemcpyDeviceToHost, s));
     PFM_CHECK_HIP(hipStreamSynchronize(s));
     if (done) break;
-    if (tail_k > 0)
+    if (nbig > 0)
       k_gm_cast<float><<<grid1d(n0), kBlock, 0, s>>>(n0, m.r64, m.lv32[0].b, m.lv32[0].x[0], w.st);
     else
       k_gm_cast<double><<<grid1d(n0), kBlock, 0, s>>>(n0, m.r64, m.b64, m.e64, w.st);
@@ -977,7 +1010,7 @@ int pfm_gamg_mixed_solve(pfm_gamg g, const pfm_solve_params *prm, double inner_r
     cur_after.clear();
     int rc = drive_cycles(g, m.in, budget, s, cur_after, cur0, [&]() {
       enqueue_cycle<float>(g, C, m.in, cur0, s);
-      if (tail_k > 0)
+      if (nbig > 0)
         k_res_norm<float><<<G, kBlock, 0, s>>>(m.lv32[0], cur0, w.part, m.in);
       else
         k_res_norm<double><<<G, kBlock, 0, s>>>(small[0], cur0, w.part, m.in);
@@ -988,7 +1021,7 @@ int pfm_gamg_mixed_solve(pfm_gamg g, const pfm_solve_params *prm, double inner_r
     PFM_CHECK_HIP(hipMemcpyAsync(&executed, &m.in->iter, sizeof(int32_t), hipMemcpyDeviceToHost, s));
     PFM_CHECK_HIP(hipStreamSynchronize(s));
     const int par = executed > 0 ? cur_after[executed - 1] : 0;
-    if (tail_k > 0)
+    if (nbig > 0)
       k_gm_correct<float><<<grid1d(n0), kBlock, 0, s>>>(n0, m.lv32[0].x[par], x, w.st);
     else
       k_gm_correct<double><<<grid1d(n0), kBlock, 0, s>>>(n0, par ? small[0].x[1] : m.e64, x,

@@ -19,8 +19,8 @@ piecewise-constant prolongation), which the reference drops;
 post and coarsest) by a symmetric Gauss-Seidel sweep with the arithmetic of
 ``GaussSeidelSmoother`` (``solvers/smoothers.py:92-149``);
 ``precision="mixed"`` solves in FP64 iterative refinement whose inner V-cycles
-run in FP32 on the large levels (FP64 on the small levels, the coarsest-level
-solve included) to ``inner_rel_tol``; the residual monitor then runs on the
+run in FP32 on the finest ``fp32_levels`` levels (FP64 on the coarser ones,
+the coarsest-level solve included) to ``inner_rel_tol``; the residual monitor then runs on the
 FP64 true residual after each refinement step (the initial residual
 included), ``max_iter`` bounds the total V-cycles and
 ``last_info.inner_iterations`` counts them.
@@ -114,7 +114,7 @@ class GAMGSolver(LinearSolverBase):
                  smoother: str = "PCG", agglomeration: str = "greedy",
                  coarse: str = "reference", n_coarsest: int = 50,
                  level_smoother: str = "Jacobi", precision: str = "double",
-                 inner_rel_tol: float = 1e-3, max_outer: int = 50) -> None:
+                 inner_rel_tol: float = 1e-3, max_outer: int = 50, fp32_levels: int = 2) -> None:
         super().__init__(tolerance=tolerance, rel_tol=rel_tol, max_iter=max_iter,
                          min_iter=min_iter, verbose=verbose)
         self._n_pre_smooth = n_pre_smooth
@@ -139,6 +139,7 @@ class GAMGSolver(LinearSolverBase):
         self._precision = precision
         self._inner_rel_tol = float(inner_rel_tol)
         self._max_outer = int(max_outer)
+        self._fp32_levels = int(fp32_levels)
         self._ws = KrylovWorkspace()
 
     def hierarchy(self, matrix: LduMatrix) -> GamgHierarchy:
@@ -182,11 +183,13 @@ class GAMGSolver(LinearSolverBase):
         L = _lib.lib()
         dev = matrix.device
         mo = self._max_outer
-        work = self._ws.get(int(L.pfm_gamg_mixed_workspace_bytes(hier.handle, mo)), dev)
+        work = self._ws.get(int(L.pfm_gamg_mixed_workspace_bytes(hier.handle, mo,
+                                                                 self._fp32_levels)), dev)
         res = _lib.SolveResult()
         hist = np.zeros(mo + 1)
         inner = C.c_int32()
         _lib.check(L.pfm_gamg_mixed_solve(hier.handle, C.byref(prm), self._inner_rel_tol, mo,
+                                          self._fp32_levels,
                                           _lib.ptr(matrix.diag.detach().contiguous()),
                                           _lib.ptr(matrix.lower.detach().contiguous()),
                                           _lib.ptr(matrix.upper.detach().contiguous()),
