@@ -402,9 +402,17 @@ def main() -> int:
                          ("pair_galerkin_jacobi_mixed", dict(agglomeration="pair", coarse="galerkin",
                                                               max_levels=20, precision="mixed",
                                                               inner_rel_tol=1e-3)),
-                         ("reference", dict())):
-            gs = GAMGSolver(tolerance=1e-300, rel_tol=1e-8, max_iter=args.gamg_max_iter, **kw)
-            hier = gs.hierarchy(mat)
+                         ("reference", dict()),
+                         ("pcg_gamg_precond", dict(pcg=True)),
+                         ("pcg_gamg_precond_mixed", dict(pcg=True, precision="mixed"))):
+            if kw.get("pcg"):  # PCG with one pair/Galerkin V-cycle as the preconditioner
+                kwp = {k: v for k, v in kw.items() if k != "pcg"}
+                gs = PCGSolver(tolerance=1e-300, rel_tol=1e-8, max_iter=args.gamg_max_iter,
+                               preconditioner="GAMG", **kwp)
+                hier = gs._gamg.hierarchy(mat)
+            else:
+                gs = GAMGSolver(tolerance=1e-300, rel_tol=1e-8, max_iter=args.gamg_max_iter, **kw)
+                hier = gs.hierarchy(mat)
             mi = args.gamg_max_iter if name != "reference" else 100
             mat.solve(gs, x0, 1e-300, 2)  # warm (setup of coarse handles and workspace)
             torch.cuda.synchronize()
@@ -414,15 +422,22 @@ def main() -> int:
             gms = (time.perf_counter() - t0) * 1e3
             rt = (mat.source - mat.Ax(xg)).norm().item() / mat.source.norm().item()
             extra = {}
-            if kw.get("precision") == "mixed":  # cyc counts refinement steps
+            if kw.get("precision") == "mixed" and not kw.get("pcg"):  # cyc counts refinement steps
                 extra["refinement_steps"] = int(cyc) - 1
                 cyc = gs.last_info.inner_iterations
                 ref = gamg.get("pair_galerkin_jacobi")
                 if ref is not None:
                     extra["speedup_vs_fp64"] = round(ref["ms"] / gms, 3)
                     extra["rel_err_vs_fp64_solution"] = ((xg - x_fp64).norm() / x_fp64.norm()).item()
-            else:
+            elif not kw.get("pcg"):
                 x_fp64 = xg
+            if kw.get("pcg") and kw.get("precision") == "mixed":
+                ref = gamg.get("pcg_gamg_precond")
+                if ref is not None:
+                    extra["speedup_vs_fp64"] = round(ref["ms"] / gms, 3)
+                    extra["rel_err_vs_fp64_solution"] = ((xg - x_pcg64).norm() / x_pcg64.norm()).item()
+            elif kw.get("pcg"):
+                x_pcg64 = xg
             gamg[name] = {"ms": round(gms, 2), "cycles": int(cyc), "ms_per_cycle": round(gms / cyc, 3),
                           "converged": bool(gs.last_info.converged), "true_rel_residual": rt,
                           "levels": hier.n_levels, **extra, **{k: v for k, v in kw.items()}}

@@ -355,6 +355,20 @@ int pfm_gamg_mixed_solve(pfm_gamg g, const pfm_solve_params *prm, double inner_r
                          double *history_h, pfm_solve_result *res, int32_t *inner_cycles,
                          void *stream);
 
+/* PCG (solvers/pcg.py:97-187, the reference's loop and stopping rule) with a
+ * GAMG V-cycle as the preconditioner: z = M^-1 r is one cycle of this
+ * hierarchy from zero on A z = r (no reference counterpart: the reference
+ * PCG knows DIC, DILU and none; OpenFOAM's "PCG with GAMG preconditioner").
+ * fp32_levels > 0 runs the cycle's finest levels in FP32 (mixed precision:
+ * FP64 Krylov vectors, dot products, SpMV of the outer loop and coarse
+ * levels).  pfm_gamg_pcg_workspace_bytes(g, max_iter, fp32_levels) bytes. */
+int64_t pfm_gamg_pcg_workspace_bytes(pfm_gamg g, int32_t max_iter, int32_t fp32_levels);
+int pfm_gamg_pcg_solve(pfm_gamg g, const pfm_solve_params *prm, int32_t fp32_levels,
+                       const double *diag, const double *lower, const double *upper,
+                       const double *b, double *x, void *work, double *history_h,
+                       pfm_solve_result *res, void *stream);
+
+
 
 /* ------------------------------------------------------------------ */
 /* Assembly and adjoint                                               */

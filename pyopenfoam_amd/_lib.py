@@ -111,6 +111,9 @@ _SIGS = {
     "pfm_gamg_level_values": (INT, [P, P, I32, P, P, P, P]),
     "pfm_gamg_solve": (INT, [P, C.POINTER(SolveParams), P, P, P, P, P, P, P, C.POINTER(SolveResult), P]),
     "pfm_gamg_mixed_workspace_bytes": (I64, [P, I32, I32]),
+    "pfm_gamg_pcg_workspace_bytes": (I64, [P, I32, I32]),
+    "pfm_gamg_pcg_solve": (INT, [P, C.POINTER(SolveParams), I32, P, P, P, P, P, P, P,
+                                 C.POINTER(SolveResult), P]),
     "pfm_gamg_mixed_solve": (INT, [P, C.POINTER(SolveParams), C.c_double, I32, I32, P, P, P, P, P,
                                    P, P, C.POINTER(SolveResult), P, P]),
     "pfm_laplacian_assemble": (INT, [P, P, P, P, P, I32, P, P, P, P, P, P, P, P]),

@@ -338,6 +338,140 @@ __global__ __launch_bounds__(kBlock) void k_to_f32(int n, const double *__restri
   if (i < n) b[i] = (float)a[i];
 }
 
+// ----------------------------------------- PCG with a V-cycle preconditioner ---
+// The PCG of solvers/pcg.py:97-187 with z = M^-1 r one V-cycle from zero on
+// A z = r (the hierarchy's levels, FP64 or FP32 on the finest ones).  Partial
+// regions: A = [0, G) p.Ap, B = [G, 2G) r.r, C = [2G, 3G) r.z.
+
+// Ap = A p on level 0; partial p.Ap (region A).
+__global__ __launch_bounds__(kBlock) void k_gp_spmv(LevelDev L, const double *__restrict__ p,
+                                                     double *__restrict__ Ap,
+                                                     double *__restrict__ part, const KState *st) {
+  __shared__ double lds[kWavesPerBlock];
+  if (st->done) return;
+  double acc = 0.0;
+  const SliceRange sr = block_slices(L.nslices);
+  const int lane = threadIdx.x & 63;
+  for (int s = sr.begin + (threadIdx.x >> 6); s < sr.end; s += kWavesPerBlock) {
+    const int row = s * kSlice + lane;
+    const double pi = row < L.n ? p[row] : 0.0;
+    const double v = level_ax(L, s, lane, row, p);
+    if (row < L.n) {
+      Ap[row] = v;
+      acc += pi * v;
+    }
+  }
+  acc = block_sum(acc, lds);
+  if (threadIdx.x == 0) part[blockIdx.x] = acc;
+}
+
+// alpha = rz / pAp (breakdown if |pAp| < 1e-30); x += alpha p; r -= alpha Ap;
+// partial r.r (region B).
+__global__ __launch_bounds__(kBlock) void k_gp_update(int n, int nslices, int it,
+                                                       const double *__restrict__ p,
+                                                       const double *__restrict__ Ap,
+                                                       double *__restrict__ x,
+                                                       double *__restrict__ r,
+                                                       double *__restrict__ part, KState *st) {
+  __shared__ double lds[kWavesPerBlock];
+  if (st->done) return;
+  const int G = gridDim.x;
+  const double PAP = reduce_partials(part, G, lds);
+  if (fabs(PAP) < 1e-30) {
+    if (blockIdx.x == 0 && threadIdx.x == 0) finish(st, PFM_SOLVE_BREAKDOWN);
+    return;
+  }
+  const double alpha = st->rzs[(it - 1) & 1] / PAP;
+  double acc = 0.0;
+  const SliceRange sr = block_slices(nslices);
+  const int lane = threadIdx.x & 63;
+  for (int s = sr.begin + (threadIdx.x >> 6); s < sr.end; s += kWavesPerBlock) {
+    const int row = s * kSlice + lane;
+    if (row < n) {
+      x[row] = x[row] + alpha * p[row];
+      const double ri = r[row] - alpha * Ap[row];
+      r[row] = ri;
+      acc += ri * ri;
+    }
+  }
+  acc = block_sum(acc, lds);
+  if (threadIdx.x == 0) part[G + blockIdx.x] = acc;
+}
+
+// Residual monitor of iteration it (region B); stop before the preconditioner.
+__global__ __launch_bounds__(kBlock) void k_gp_decide(const double *__restrict__ part, int G,
+                                                       int it, KState *st,
+                                                       double *__restrict__ hist) {
+  __shared__ double lds[kWavesPerBlock];
+  if (st->done) return;
+  const double RR = reduce_partials(part + G, G, lds);
+  if (threadIdx.x == 0) {
+    st->iter = it;
+    if (monitor_update(st, hist, sqrt(RR), it))
+      finish(st, PFM_SOLVE_CONVERGED);
+    else if (it >= st->max_iter)
+      finish(st, PFM_SOLVE_MAXITER);
+  }
+}
+
+// z = M^-1 r from the cycle's level-0 solution; partial r.z (region C).
+template <typename T>
+__global__ __launch_bounds__(kBlock) void k_gp_z(int n, int nslices, const T *__restrict__ e,
+                                                  const double *__restrict__ r,
+                                                  double *__restrict__ z, double *__restrict__ part,
+                                                  const KState *st) {
+  __shared__ double lds[kWavesPerBlock];
+  if (st->done) return;
+  const int G = gridDim.x;
+  double acc = 0.0;
+  const SliceRange sr = block_slices(nslices);
+  const int lane = threadIdx.x & 63;
+  for (int s = sr.begin + (threadIdx.x >> 6); s < sr.end; s += kWavesPerBlock) {
+    const int row = s * kSlice + lane;
+    if (row < n) {
+      const double zi = (double)e[row];
+      z[row] = zi;
+      acc += r[row] * zi;
+    }
+  }
+  acc = block_sum(acc, lds);
+  if (threadIdx.x == 0) part[2 * G + blockIdx.x] = acc;
+}
+
+// rz' = r.z (region C); beta = rz' / rz (0 on the first pass: p = z);
+// p = z + beta p.  Workgroup 0 records rz' by iteration parity.
+__global__ __launch_bounds__(kBlock) void k_gp_pupdate(int n, int nslices, int it,
+                                                        const double *__restrict__ z,
+                                                        double *__restrict__ p,
+                                                        const double *__restrict__ part,
+                                                        KState *st) {
+  __shared__ double lds[kWavesPerBlock];
+  if (st->done) return;
+  const int G = gridDim.x;
+  const double RZ = reduce_partials(part + 2 * G, G, lds);
+  if (blockIdx.x == 0 && threadIdx.x == 0) st->rzs[it & 1] = RZ;
+  const double beta = it == 0 ? 0.0 : RZ / st->rzs[(it - 1) & 1];
+  const SliceRange sr = block_slices(nslices);
+  const int lane = threadIdx.x & 63;
+  for (int s = sr.begin + (threadIdx.x >> 6); s < sr.end; s += kWavesPerBlock) {
+    const int row = s * kSlice + lane;
+    if (row < n) p[row] = it == 0 ? z[row] : z[row] + beta * p[row];
+  }
+}
+
+// Initial monitor (r = b - A x0 from k_gm_resid, region A of its partials).
+__global__ __launch_bounds__(kBlock) void k_gp_init_fin(const double *__restrict__ part, int G,
+                                                         KState *st, double *__restrict__ hist) {
+  __shared__ double lds[kWavesPerBlock];
+  const double RR = sum_partials(part, G, lds);
+  if (threadIdx.x == 0) {
+    if (monitor_update(st, hist, sqrt(RR), 0))
+      finish(st, PFM_SOLVE_CONVERGED);
+    else if (st->max_iter < 1)
+      finish(st, PFM_SOLVE_MAXITER);
+  }
+}
+
 // ----------------------------------------------------------- workspace -----
 inline size_t align256(size_t b) { return (b + 255) & ~size_t(255); }
 
@@ -1041,6 +1175,140 @@ int pfm_gamg_mixed_solve(pfm_gamg g, const pfm_solve_params *prm, double inner_r
   res->initial_residual = hs.nhist > 0 ? hs.res0 : 0.0;
   res->final_residual = hs.nhist > 0 ? hs.res : 0.0;
   if (inner_cycles) *inner_cycles = inner_total;
+  return PFM_OK;
+}
+
+namespace {
+struct GamgPcgWork {
+  GamgMixedWork m;
+  double *r, *p, *Ap, *z, *part;
+};
+
+size_t gamg_pcg_layout(const pfm_gamg_s *g, int max_iter, int fp32_levels, GamgPcgWork *w,
+                       char *base) {
+  GamgMixedWork m;
+  size_t off = gamg_mixed_layout(g, max_iter, fp32_levels, &m, base);
+  auto take = [&](size_t bytes) -> void * {
+    size_t o = off;
+    off += align256(bytes);
+    return base ? (void *)(base + o) : nullptr;
+  };
+  GamgPcgWork t;
+  const size_t n0 = (size_t)g->lv[0].n;
+  t.r = (double *)take(sizeof(double) * n0);
+  t.p = (double *)take(sizeof(double) * n0);
+  t.Ap = (double *)take(sizeof(double) * n0);
+  t.z = (double *)take(sizeof(double) * n0);
+  t.part = (double *)take(sizeof(double) * 3 * (size_t)red_grid(g->fine));
+  if (w) {
+    t.m = m;
+    *w = t;
+  }
+  return off;
+}
+}  // namespace
+
+int64_t pfm_gamg_pcg_workspace_bytes(pfm_gamg g, int32_t max_iter, int32_t fp32_levels) {
+  if (!g || max_iter < 0 || fp32_levels < 0) return -1;
+  return (int64_t)gamg_pcg_layout(g, max_iter, fp32_levels, nullptr, nullptr);
+}
+
+int pfm_gamg_pcg_solve(pfm_gamg g, const pfm_solve_params *prm, int32_t fp32_levels,
+                       const double *diag, const double *lower, const double *upper,
+                       const double *b, double *x, void *work, double *history_h,
+                       pfm_solve_result *res, void *stream) {
+  PFM_REQUIRE(g && prm && res && work, "null argument");
+  PFM_REQUIRE(prm->max_iter >= 0 && fp32_levels >= 0, "negative iteration or level count");
+  const int n0 = g->lv[0].n;
+  PFM_REQUIRE(n0 == 0 || (diag && b && x), "null device pointer");
+  memset(res, 0, sizeof(*res));
+  if (n0 == 0) {
+    res->status = PFM_SOLVE_CONVERGED;
+    res->converged = 1;
+    return PFM_OK;
+  }
+  hipStream_t s = (hipStream_t)stream;
+  int st = pfm_gamg_setup_values(g, diag, lower, upper, work, stream);
+  if (st != PFM_OK) return st;
+  GamgPcgWork W;
+  gamg_pcg_layout(g, prm->max_iter, fp32_levels, &W, (char *)work);
+  GamgMixedWork &m = W.m;
+  GamgWork &w = m.w64;
+  const int L = (int)g->lv.size();
+  const int tail_k = tail_level(g);
+  const int nbig = fp32_count(g, fp32_levels);
+  w.lv[0].diag = diag;
+  w.lv[0].x[0] = x;
+  w.lv[0].b = const_cast<double *>(b);
+  for (int k = 0; k < nbig; ++k) {  // FP32 copies of the finest levels
+    const auto &H = g->lv[k];
+    const double *lo = k == 0 ? lower : w.lower[k], *up = k == 0 ? upper : w.upper[k];
+    const double *d64 = k == 0 ? diag : w.diag[k];
+    if (H.nf && (st = pfm_ldu_pack_f32(H.ldu, 0, lo, up, m.packed32[k], stream)) != PFM_OK)
+      return st;
+    k_to_f32<<<grid1d(H.n), kBlock, 0, s>>>(H.n, d64, m.diag32[k]);
+    k_to_f32<<<grid1d(H.n), kBlock, 0, s>>>(H.n, w.lv[k].inv, m.inv32[k]);
+  }
+  // the preconditioner's level 0 solves for e64 (FP64 path) with rhs b64
+  std::vector<LevelDev> small = w.lv;
+  small[0].x[0] = m.e64;
+  small[0].b = m.b64;
+  PFM_CHECK_HIP(hipMemcpyAsync(m.d_small, small.data(), sizeof(LevelDev) * L,
+                               hipMemcpyHostToDevice, s));
+  CycleLevels<float> C;
+  C.tail_k = tail_k;
+  C.nbig = nbig;
+  C.big = m.lv32;
+  C.small = small;
+  C.d_small = m.d_small;
+  const int hist_cap = prm->max_iter + 1;
+  k_state_init<<<1, 1, 0, s>>>(w.st, prm->tolerance, prm->rel_tol, prm->max_iter, prm->min_iter,
+                               hist_cap);
+  const int G = red_grid(g->fine);
+  const int ns0 = g->fine->nslices;
+  // z = M^-1 r: level-0 rhs = r, solution from zero, one cycle
+  int zpar = 0;
+  auto precondition = [&](int it) {
+    if (nbig > 0)
+      k_gm_cast<float><<<grid1d(n0), kBlock, 0, s>>>(n0, W.r, m.lv32[0].b, m.lv32[0].x[0], w.st);
+    else
+      k_gm_cast<double><<<grid1d(n0), kBlock, 0, s>>>(n0, W.r, m.b64, m.e64, w.st);
+    int cur0 = 0;
+    enqueue_cycle<float>(g, C, w.st, cur0, s);
+    zpar = cur0;
+    if (nbig > 0)
+      k_gp_z<float><<<G, kBlock, 0, s>>>(n0, ns0, m.lv32[0].x[zpar], W.r, W.z, W.part, w.st);
+    else
+      k_gp_z<double><<<G, kBlock, 0, s>>>(n0, ns0, zpar ? small[0].x[1] : m.e64, W.r, W.z, W.part,
+                                          w.st);
+    k_gp_pupdate<<<G, kBlock, 0, s>>>(n0, ns0, it, W.z, W.p, W.part, w.st);
+  };
+  k_gm_resid<<<G, kBlock, 0, s>>>(w.lv[0], W.r, W.part, w.st);
+  k_gp_init_fin<<<1, kBlock, 0, s>>>(W.part, G, w.st, w.hist);
+  precondition(0);
+  PFM_CHECK_HIP(hipGetLastError());
+  std::vector<int> unused;
+  int cur_unused = 0, it = 0;
+  int rc = drive_cycles(g, w.st, prm->max_iter, s, unused, cur_unused, [&]() {
+    ++it;
+    k_gp_spmv<<<G, kBlock, 0, s>>>(w.lv[0], W.p, W.Ap, W.part, w.st);
+    k_gp_update<<<G, kBlock, 0, s>>>(n0, ns0, it, W.p, W.Ap, x, W.r, W.part, w.st);
+    k_gp_decide<<<1, kBlock, 0, s>>>(W.part, G, it, w.st, w.hist);
+    precondition(it);
+  });
+  if (rc != PFM_OK) return rc;
+  KState hs;
+  PFM_CHECK_HIP(hipMemcpyAsync(&hs, w.st, sizeof(KState), hipMemcpyDeviceToHost, s));
+  PFM_CHECK_HIP(hipStreamSynchronize(s));
+  const int nh = std::min(hs.nhist, hist_cap);
+  if (history_h && nh > 0)
+    PFM_CHECK_HIP(hipMemcpyAsync(history_h, w.hist, sizeof(double) * nh, hipMemcpyDeviceToHost, s));
+  PFM_CHECK_HIP(hipStreamSynchronize(s));
+  res->iterations = hs.nhist;
+  res->status = hs.status == 0 ? PFM_SOLVE_MAXITER : hs.status;
+  res->converged = res->status == PFM_SOLVE_CONVERGED;
+  res->initial_residual = hs.nhist > 0 ? hs.res0 : 0.0;
+  res->final_residual = hs.nhist > 0 ? hs.res : 0.0;
   return PFM_OK;
 }
 

@@ -202,6 +202,30 @@ class GAMGSolver(LinearSolverBase):
         info.inner_iterations = int(inner.value)
         return x, info
 
+    def pcg_solve(self, matrix: LduMatrix, b: torch.Tensor, x: torch.Tensor,
+                  monitor: ResidualMonitor, max_iter: int, fp32_levels: int):
+        """PCG preconditioned by one V-cycle of this solver's hierarchy
+        (``pfm_gamg_pcg_solve``); x is updated in place."""
+        hier = self.hierarchy(matrix)
+        L = _lib.lib()
+        dev = matrix.device
+        work = self._ws.get(int(L.pfm_gamg_pcg_workspace_bytes(hier.handle, max_iter,
+                                                               fp32_levels)), dev)
+        prm = _lib.SolveParams(float(monitor.tolerance), float(monitor.rel_tol), max_iter,
+                               int(monitor.min_iter), 0, 0)
+        res = _lib.SolveResult()
+        hist = np.zeros(max_iter + 1)
+        _lib.check(L.pfm_gamg_pcg_solve(hier.handle, C.byref(prm), int(fp32_levels),
+                                        _lib.ptr(matrix.diag.detach().contiguous()),
+                                        _lib.ptr(matrix.lower.detach().contiguous()),
+                                        _lib.ptr(matrix.upper.detach().contiguous()), _lib.ptr(b),
+                                        _lib.ptr(x), _lib.ptr(work), hist.ctypes.data,
+                                        C.byref(res), current_stream(dev)), "pfm_gamg_pcg_solve")
+        monitor.load_history(hist[: min(res.iterations, max_iter + 1)].tolist())
+        info = monitor.build_info(converged=bool(res.converged))
+        info.status = int(res.status)
+        return x, info
+
     def coarse_values(self, matrix: LduMatrix, level: int):
         """(diag, lower, upper) of a coarse level as the solver builds them."""
         hier = self.hierarchy(matrix)

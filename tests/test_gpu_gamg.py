@@ -158,3 +158,59 @@ def test_gamg_mixed_converges(gpu_device):
     x64, *_ = GAMGSolver(rel_tol=1e-10, max_iter=5000, **kw)(m, b, z)
     xa = x64.cpu().numpy()
     assert np.linalg.norm(xr - xa) <= 1e-6 * np.linalg.norm(xa)
+
+
+@pytest.mark.parametrize("n,jit,sig", [(12, 0.0, 1.0), (24, 0.0, 1.0), (16, 0.2, 1.0)])
+def test_pcg_gamg_preconditioner_matches_oracle(gpu_device, n, jit, sig):
+    """PCG with one reference-hierarchy V-cycle as the preconditioner against
+    the oracle PCG with the oracle's V-cycle (bit-exact on its own, see the
+    GAMG tests): on symmetric systems iterations within one and the solution
+    within 1e-9 (or ten times the oracle's summation-order spread); on the
+    jittered one inside the oracle's spread."""
+    from oracle import krylov as okry
+    from pyopenfoam_amd import PCGSolver
+    c, d, lo, up = system(n, jit, sig)
+    print(f"pcg-gamg n={n} jitter={jit}", flush=True)
+    m = _matrix(c.n_cells, c.owner, c.neighbour, d, lo, up, gpu_device)
+    solver = PCGSolver(tolerance=1e-300, rel_tol=1e-8, max_iter=500, preconditioner="GAMG",
+                       gamg_options=dict(agglomeration="greedy", coarse="reference"))
+    x, it, _ = solver(m, torch.from_numpy(c.b), torch.zeros(c.n_cells, dtype=torch.float64))
+    levels, aggs = ogamg.build_hierarchy(ogamg.Level(c.n_cells, d, lo, up, c.owner, c.neighbour))
+    pre = lambda r: ogamg.v_cycle(levels, aggs, 0, np.zeros_like(r), r, 2, 2)
+    from oracle import ldu as oldu
+    mv = lambda v: oldu.ax(d, lo, up, c.owner, c.neighbour, v)
+    runs = {k: okry.pcg(mv, pre, c.b, np.zeros(c.n_cells), tolerance=1e-300, rel_tol=1e-8,
+                        max_iter=500, dot=dot) for k, dot in okry.DOTS.items()}
+    xo, io, ro, info = runs["blas"]
+    its = [r[1] for r in runs.values()]
+    assert info.converged and solver.last_info.converged
+    assert min(its) - 1 <= it <= max(its) + 1, (it, its)
+    np.testing.assert_allclose(solver.last_info.residual_history[:3], info.residual_history[:3],
+                               rtol=1e-9)
+    xg = x.cpu().numpy()
+    nx = np.linalg.norm(xo)
+    spread = max(np.linalg.norm(r[0] - xo) for r in runs.values()) / nx
+    assert np.linalg.norm(xg - xo) / nx <= max(10 * spread, 1e-9), spread
+
+
+def test_pcg_gamg_mixed_converges(gpu_device):
+    """FP32 finest levels in the V-cycle preconditioner: same stopping rule on
+    the FP64 residual, converged to the tolerance, iterations close to the FP64
+    preconditioner's."""
+    from pyopenfoam_amd import PCGSolver
+    c, d, lo, up = system(32, 0.2, 1.0)
+    m = _matrix(c.n_cells, c.owner, c.neighbour, d, lo, up, gpu_device)
+    b = torch.from_numpy(c.b)
+    z = torch.zeros(c.n_cells, dtype=torch.float64)
+    s64 = PCGSolver(tolerance=1e-300, rel_tol=1e-8, max_iter=1000, preconditioner="GAMG")
+    x64, it64, _ = s64(m, b, z)
+    s32 = PCGSolver(tolerance=1e-300, rel_tol=1e-8, max_iter=1000, preconditioner="GAMG",
+                    precision="mixed")
+    x32, it32, _ = s32(m, b, z)
+    assert s64.last_info.converged and s32.last_info.converged
+    assert abs(it32 - it64) <= max(3, it64 // 10), (it32, it64)
+    from oracle import ldu as oldu
+    for xx in (x64, x32):
+        xr = xx.cpu().numpy()
+        tr = np.linalg.norm(c.b - oldu.ax(d, lo, up, c.owner, c.neighbour, xr)) / np.linalg.norm(c.b)
+        assert tr < 1.5e-8, tr
