@@ -401,7 +401,7 @@ def main() -> int:
                                                         max_levels=20)),
                          ("pair_galerkin_jacobi_mixed", dict(agglomeration="pair", coarse="galerkin",
                                                               max_levels=20, precision="mixed",
-                                                              inner_rel_tol=1e-3)),
+                                                              inner_rel_tol=1e-2)),
                          ("reference", dict()),
                          ("pcg_gamg_precond", dict(pcg=True)),
                          ("pcg_gamg_precond_mixed", dict(pcg=True, precision="mixed"))):

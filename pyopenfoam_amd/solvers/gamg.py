@@ -19,8 +19,11 @@ piecewise-constant prolongation), which the reference drops;
 post and coarsest) by a symmetric Gauss-Seidel sweep with the arithmetic of
 ``GaussSeidelSmoother`` (``solvers/smoothers.py:92-149``);
 ``precision="mixed"`` solves in FP64 iterative refinement whose inner V-cycles
-run in FP32 on the finest ``fp32_levels`` levels (FP64 on the coarser ones,
-the coarsest-level solve included) to ``inner_rel_tol``; the residual monitor then runs on the
+run in FP32 on the finest ``fp32_levels`` levels (default: every level
+above 4096 cells; FP64 on the coarser ones, the coarsest-level solve
+included) to ``inner_rel_tol`` (default 1e-2: the FP32 residual of a
+stationary cycle stalls at its rounding floor, which a tighter inner
+tolerance would not see past); the residual monitor then runs on the
 FP64 true residual after each refinement step (the initial residual
 included), ``max_iter`` bounds the total V-cycles and
 ``last_info.inner_iterations`` counts them.
@@ -114,7 +117,7 @@ class GAMGSolver(LinearSolverBase):
                  smoother: str = "PCG", agglomeration: str = "greedy",
                  coarse: str = "reference", n_coarsest: int = 50,
                  level_smoother: str = "Jacobi", precision: str = "double",
-                 inner_rel_tol: float = 1e-3, max_outer: int = 50, fp32_levels: int = 2) -> None:
+                 inner_rel_tol: float = 1e-2, max_outer: int = 50, fp32_levels: int = 64) -> None:
         super().__init__(tolerance=tolerance, rel_tol=rel_tol, max_iter=max_iter,
                          min_iter=min_iter, verbose=verbose)
         self._n_pre_smooth = n_pre_smooth
