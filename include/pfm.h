@@ -277,6 +277,36 @@ int pfm_gs_smooth(pfm_ldu h, const double *diag, const double *packed, const dou
 int pfm_jacobi_smooth(pfm_ldu h, const double *diag, const double *packed, const double *b,
                       double *x, int32_t n_iter, double omega, double *work, void *stream);
 
+/* DICPreconditioner.apply_full / DILUPreconditioner.apply_full
+ * (solvers/preconditioners.py:153-201, :290-337): z = M^-1 r with the forward
+ * face sweep z[n] -= (upper_f z[p]) / rD[p], z /= rD and the reverse face
+ * sweep z[p] -= (c_f z[n]) / rD[n], c = upper (DIC, faces with owner <
+ * neighbour only) or lower (DILU).  rD from pfm_dic_factor / pfm_dilu_factor.
+ * Faces oriented owner < neighbour with non-decreasing owners run cell-level
+ * schedules (one launch per level); any other face order one workgroup over
+ * the face levels.  z may alias r. */
+int pfm_precond_apply_full(pfm_ldu h, int dilu, const double *rD, const double *lower,
+                           const double *upper, const double *r, double *z, void *stream);
+
+/* DiagonalSolver._solve (solvers/diagonal.py:57-85): x = b / max(|diag|,
+ * 1e-30); *residual_h = ||b - diag x||.  work holds n_cells/256 + 2 doubles. */
+int pfm_diagonal_solve(pfm_ldu h, const double *diag, const double *b, double *x,
+                       double *work, double *residual_h, void *stream);
+
+#define PFM_SMOOTH_GAUSS_SEIDEL 0 /* forward sweeps, x[c] = (b - off) / diag */
+#define PFM_SMOOTH_DIC 1          /* Richardson: x += omega DIC.apply_full(b - A x) */
+#define PFM_SMOOTH_DILU 2         /* Richardson with DILU.apply_full */
+
+/* SmoothSolver._solve (solvers/smooth_solver.py:96-232): monitor on b - A x0;
+ * each iteration n_sweeps smoothing passes, then the monitor on b - A x.
+ * packed = pfm_ldu_pack(h, 0, lower, upper); rD for DIC/DILU. */
+int64_t pfm_smooth_workspace_bytes(pfm_ldu h, int32_t max_iter);
+int pfm_smooth_solve(pfm_ldu h, const pfm_solve_params *prm, int smoother, int32_t n_sweeps,
+                     double omega, const double *diag, const double *lower, const double *upper,
+                     const double *packed, const double *rD, const double *b, double *x,
+                     void *work, double *history_h, pfm_solve_result *res, void *stream);
+
+
 /* ------------------------------------------------------------------ */
 /* GAMG                                                               */
 /* ------------------------------------------------------------------ */

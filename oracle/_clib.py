@@ -22,6 +22,7 @@ _SIGS = {
     "orc_dic_apply_full": (C.c_int, [I64, I64, P, P, P, P, P, P]),
     "orc_dilu_apply_full": (C.c_int, [I64, I64, P, P, P, P, P, P, P]),
     "orc_gauss_seidel": (C.c_int, [I64, I64, P, P, P, P, P, P, D, I64, P]),
+    "orc_smooth_gs": (C.c_int, [I64, I64, P, P, P, P, P, P, I64, P]),
     "orc_gamg_aggregate": (I64, [I64, I64, P, P, I64, P]),
     "orc_gamg_coarse_faces": (I64, [I64, P, P, P, P, P, P, P, P, P, P]),
     "orc_pair_level": (I64, [I64, I64, I64, P, P, P, P]),

@@ -156,6 +156,48 @@ int orc_gauss_seidel(int64_t n, int64_t nf, const int64_t *own, const int64_t *n
   return 0;
 }
 
+/* SmoothSolver._solve_gauss_seidel sweep (solvers/smooth_solver.py:150-180):
+ * cells 0..n-1 in order, row sum over the cell's faces in face order with the
+ * latest x (diag*x first), off = row - diag*x, x[c] = (b - off)/diag when
+ * |diag| > 1e-30 (direct assignment, forward only). */
+int orc_smooth_gs(int64_t n, int64_t nf, const int64_t *own, const int64_t *nei,
+                  const double *diag, const double *lower, const double *upper,
+                  const double *b, int64_t sweeps, double *x) {
+  int64_t *ptr = calloc(n + 1, sizeof(int64_t));
+  int64_t *lst = malloc(sizeof(int64_t) * (2 * nf + 1));
+  int64_t *c = malloc(sizeof(int64_t) * (n + 1));
+  if (!ptr || !lst || !c) return 1;
+  for (int64_t f = 0; f < nf; ++f) {
+    ptr[own[f] + 1]++;
+    ptr[nei[f] + 1]++;
+  }
+  for (int64_t i = 0; i < n; ++i) ptr[i + 1] += ptr[i];
+  memcpy(c, ptr, sizeof(int64_t) * n);
+  for (int64_t f = 0; f < nf; ++f) {
+    lst[c[own[f]]++] = f;
+    lst[c[nei[f]]++] = f;
+  }
+  for (int64_t s = 0; s < sweeps; ++s) {
+    for (int64_t cc = 0; cc < n; ++cc) {
+      const double d = diag[cc];
+      double row = d * x[cc];
+      for (int64_t k = ptr[cc]; k < ptr[cc + 1]; ++k) {
+        const int64_t f = lst[k];
+        if (own[f] == cc)
+          row += lower[f] * x[nei[f]];
+        else
+          row += upper[f] * x[own[f]];
+      }
+      const double off = row - d * x[cc];
+      if (fabs(d) > 1e-30) x[cc] = (b[cc] - off) / d;
+    }
+  }
+  free(ptr);
+  free(lst);
+  free(c);
+  return 0;
+}
+
 /* GAMGSolver._build_aggregates (solvers/gamg.py:164-239).  adj[c] lists the
  * other cell of every face touching c, in face order. */
 int64_t orc_gamg_aggregate(int64_t n, int64_t nf, const int64_t *own, const int64_t *nei,

@@ -48,6 +48,7 @@ class GamgParams(C.Structure):
 
 PFM_AGGLOM_GREEDY, PFM_AGGLOM_PAIR = 0, 1
 PFM_SMOOTHER_JACOBI, PFM_SMOOTHER_GAUSS_SEIDEL = 0, 1
+PFM_SMOOTH_GAUSS_SEIDEL, PFM_SMOOTH_DIC, PFM_SMOOTH_DILU = 0, 1, 2
 PFM_COARSE_REFERENCE, PFM_COARSE_GALERKIN = 0, 1
 
 
@@ -84,6 +85,11 @@ _SIGS = {
     "pfm_residual": (INT, [P, P, P, P, P, P, P]),
     "pfm_ldu_set_progress": (INT, [P, P, P, C.c_double]),
     "pfm_gs_smooth": (INT, [P, P, P, P, P, I32, C.c_double, P]),
+    "pfm_precond_apply_full": (INT, [P, INT, P, P, P, P, P, P]),
+    "pfm_diagonal_solve": (INT, [P, P, P, P, P, P, P]),
+    "pfm_smooth_workspace_bytes": (I64, [P, I32]),
+    "pfm_smooth_solve": (INT, [P, C.POINTER(SolveParams), INT, I32, C.c_double, P, P, P, P, P, P,
+                               P, P, P, C.POINTER(SolveResult), P]),
     "pfm_jacobi_smooth": (INT, [P, P, P, P, P, I32, C.c_double, P, P]),
     "pfm_dic_factor": (INT, [P, P, P, P, P]),
     "pfm_dilu_factor": (INT, [P, INT, P, P, P, P, P]),

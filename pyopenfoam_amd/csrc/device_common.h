@@ -167,7 +167,9 @@ __device__ __forceinline__ SliceInfo load_slice_lane(const SliceInfo *sl, int s)
 // couplings sit in two groups of its slice (owner-faces, column = neighbour,
 // then neighbour-faces, column = owner), each in face order; merging them by
 // face index visits the faces in the reference's face order.
-template <typename T>
+// kAssign: x[c] = new value (SmoothSolver's sweep, smooth_solver.py:170-180)
+// instead of x[c] + omega (new - x[c]) (GaussSeidelSmoother).
+template <typename T, bool kAssign = false>
 __device__ __forceinline__ void gs_cell(int c, const SliceInfo *__restrict__ sl,
                                         const int32_t *__restrict__ col,
                                         const int32_t *__restrict__ map,
@@ -199,7 +201,10 @@ __device__ __forceinline__ void gs_cell(int c, const SliceInfo *__restrict__ sl,
   const T off = row - d * xc;
   if (fabs(d) > T(1e-30)) {
     const T nv = (b[c] - off) / d;
-    x[c] = xc + omega * (nv - xc);
+    if (kAssign)
+      x[c] = nv;
+    else
+      x[c] = xc + omega * (nv - xc);
   }
 }
 

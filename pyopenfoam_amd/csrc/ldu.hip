@@ -398,6 +398,9 @@ int pfm_ldu_create(int32_t n, int32_t nf, const int32_t *own, const int32_t *nei
   h->dilu_level_ptr = L.dilu_level_ptr;
   h->gs_fwd_ptr = L.gs_fwd_ptr;
   h->gs_bwd_ptr = L.gs_bwd_ptr;
+  h->oriented_sorted = true;
+  for (int32_t f = 0; f < nf && h->oriented_sorted; ++f)
+    h->oriented_sorted = own[f] < nei[f] && (f == 0 || own[f - 1] <= own[f]);
   hipError_t e = hipGetDevice(&h->device);
   if (e != hipSuccess) {
     delete h;

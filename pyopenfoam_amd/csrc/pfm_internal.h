@@ -104,6 +104,7 @@ struct pfm_ldu_s {
   int32_t *d_gs_fwd_ptr = nullptr, *d_gs_fwd_cells = nullptr;
   int32_t *d_gs_bwd_ptr = nullptr, *d_gs_bwd_cells = nullptr;
   int32_t *h_flag = nullptr;            // pinned: solver progress polling
+  bool oriented_sorted = false;         // every owner < neighbour, owners non-decreasing
   pfm_progress_fn progress = nullptr;   // optional host progress report
   void *progress_user = nullptr;
   double progress_interval = 0.0;

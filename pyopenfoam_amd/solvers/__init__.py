@@ -6,9 +6,11 @@ from .pcg import PCGSolver
 from .gamg import GAMGSolver
 from .preconditioners import DICPreconditioner, DILUPreconditioner, Preconditioner
 from .residual import ConvergenceInfo, ResidualMonitor
-from .smoothers import GaussSeidelSmoother, JacobiSmoother, Smoother
+from .smoothers import DICGSmoother, GaussSeidelSmoother, JacobiSmoother, Smoother
+from .smooth_solver import SmoothSolver
+from .diagonal import DiagonalSolver
 
 __all__ = ["LinearSolverBase", "create_solver", "solver_from_dict", "PCGSolver",
            "PBiCGSTABSolver", "GAMGSolver", "DICPreconditioner", "DILUPreconditioner",
            "Preconditioner", "ConvergenceInfo", "ResidualMonitor", "GaussSeidelSmoother",
-           "JacobiSmoother", "Smoother"]
+           "JacobiSmoother", "DICGSmoother", "Smoother", "SmoothSolver", "DiagonalSolver"]

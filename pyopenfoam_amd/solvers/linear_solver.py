@@ -81,12 +81,16 @@ def _register(name: str, cls: type[LinearSolverBase]) -> None:
 def _ensure_registry() -> None:
     if _SOLVER_REGISTRY:
         return
+    from .diagonal import DiagonalSolver
     from .gamg import GAMGSolver
     from .pbicgstab import PBiCGSTABSolver
     from .pcg import PCGSolver
+    from .smooth_solver import SmoothSolver
 
     _register("PCG", PCGSolver)
     _register("PBICGSTAB", PBiCGSTABSolver)
+    _register("SMOOTHSOLVER", SmoothSolver)
+    _register("DIAGONAL", DiagonalSolver)
     _register("GAMG", GAMGSolver)
 
 
@@ -105,6 +109,11 @@ def create_solver(name: str, *, tolerance: float = 1e-6, rel_tol: float = 0.01,
 def solver_from_dict(solver_name: str, solver_dict: dict[str, Any]) -> LinearSolverBase:
     extra: dict[str, Any] = {}
     key = solver_name.upper().replace("-", "").replace("_", "")
+    if key == "SMOOTHSOLVER":
+        if "smoother" in solver_dict:
+            extra["smoother"] = solver_dict["smoother"]
+        if "nSweeps" in solver_dict:
+            extra["n_sweeps"] = int(solver_dict["nSweeps"])
     if key == "GAMG":
         for src, dst in (("nPreSweeps", "n_pre_smooth"), ("nPostSweeps", "n_post_smooth"),
                          ("nCellsInCoarsestLevel", "min_cells_coarse")):

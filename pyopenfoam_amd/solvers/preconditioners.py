@@ -5,7 +5,10 @@ the diagonal with the reference's sequential recurrences, evaluated on the
 device by libpfm in dependency order (``pfm_dic_factor``: one launch per cell
 level; ``pfm_dilu_factor``: one persistent workgroup over face levels).  The
 result is bit-identical to the sequential loops.  ``apply`` is the reference's
-diagonal apply ``r / d`` (the Krylov solvers fuse it into their kernels).
+diagonal apply ``r / d`` (the Krylov solvers fuse it into their kernels);
+``apply_full`` the triangular solve of the factored M (forward face sweep,
+diagonal scaling, reverse face sweep; ``pfm_precond_apply_full``, level-scheduled
+and bit-identical to the sequential sweeps).
 """
 
 from __future__ import annotations
@@ -48,6 +51,26 @@ class _DiagonalFactor:
         assert_floating(r, "r")
         r = r.to(device=self._device, dtype=self._dtype)
         return r / self._precond_diag
+
+    def apply_full(self, r: torch.Tensor) -> torch.Tensor:
+        """z = M^-1 r with M = (D + L) D^-1 (D + U) (reference :153-201, :290-337)."""
+        assert_floating(r, "r")
+        m = self._matrix
+        r = r.to(device=self._device, dtype=self._dtype).contiguous()
+        if r.shape[0] != m.n_cells:
+            raise ValueError(f"r has {r.shape[0]} elements, expected {m.n_cells}")
+        z = torch.empty_like(r)
+        if m.n_cells == 0:
+            return z
+        lo, up = m.lower.detach().contiguous(), m.upper.detach().contiguous()
+        if self._transpose:
+            lo, up = up, lo
+        _lib.check(_lib.lib().pfm_precond_apply_full(m.addressing.handle, int(self.kind == "DILU"),
+                                                     _lib.ptr(self._precond_diag), _lib.ptr(lo),
+                                                     _lib.ptr(up), _lib.ptr(r), _lib.ptr(z),
+                                                     current_stream(self._device)),
+                   "pfm_precond_apply_full")
+        return z
 
 
 class DICPreconditioner(_DiagonalFactor):

@@ -10,7 +10,10 @@ the sequential sweeps' exact arithmetic.
 ``JacobiSmoother(matrix, omega=2/3).smooth(x, b, n_iterations)`` (:152-200):
 x = x + (omega / max(|diag|, 1e-30)) * (b - A x), ``pfm_jacobi_smooth``.
 
-Both return a new tensor and leave ``x`` unchanged, as the reference does.
+``DICGSmoother(matrix, omega=1.0).smooth(x, b, n_iterations)`` (:203-260):
+x = x + omega * DIC.apply_full(b - A x) per iteration.
+
+All return a new tensor and leave ``x`` unchanged, as the reference does.
 """
 
 from __future__ import annotations
@@ -21,7 +24,7 @@ from .. import _lib
 from ..core.device import current_stream, require_gpu
 from ..core.ldu_matrix import LduMatrix
 
-__all__ = ["GaussSeidelSmoother", "JacobiSmoother", "Smoother"]
+__all__ = ["GaussSeidelSmoother", "JacobiSmoother", "DICGSmoother", "Smoother"]
 
 
 class Smoother:
@@ -81,4 +84,18 @@ class JacobiSmoother(_DeviceSmoother):
                                                 _lib.ptr(xx), int(n_iterations), self._omega,
                                                 _lib.ptr(work), current_stream(self._device)),
                    "pfm_jacobi_smooth")
+        return xx
+
+
+class DICGSmoother(_DeviceSmoother):
+    def __init__(self, matrix: LduMatrix, omega: float = 1.0) -> None:
+        super().__init__(matrix, omega)
+        from .preconditioners import DICPreconditioner
+        self._dic = DICPreconditioner(matrix)
+
+    def smooth(self, x: torch.Tensor, b: torch.Tensor, n_iterations: int) -> torch.Tensor:
+        xx, bb = self._prep(x, b)
+        for _ in range(max(int(n_iterations), 0)):
+            r = bb - self._matrix.Ax(xx)
+            xx = xx + self._omega * self._dic.apply_full(r)
         return xx

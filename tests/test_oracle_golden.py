@@ -266,3 +266,54 @@ def test_jacobi_oracle_known_answers():
     x = ogamg.jacobi(np.array([4.0, 5.0]), np.array([-1.0]), np.array([-1.0]), np.array([0]),
                      np.array([1]), np.array([3.0, 1.0]), xs, 3, omega=0.7)
     np.testing.assert_allclose(x, xs, rtol=1e-14)
+
+
+def test_smooth_solver_oracle_known_answers():
+    """SmoothSolver's Gauss-Seidel sweep (smooth_solver.py:150-180): a 2-cell
+    system swept twice by hand (d = (4, 4), l = u = -1, b = (2, 7), x0 = 0:
+    (0.5, 1.875) then (0.96875, 1.9921875), exact in binary); identity -> x = b;
+    against a dense forward Gauss-Seidel on a jittered 4^3 system (1e-13);
+    and the reference tests' identity/2-cell behaviours through the full loop
+    (tests/unit/solvers/test_smooth_solver.py:15-60, 86-150)."""
+    from oracle import smooth as osm
+    one = lambda k: osm.smooth_gs_sweeps(np.array([4.0, 4.0]), np.array([-1.0]), np.array([-1.0]),
+                                         np.array([0]), np.array([1]), np.array([2.0, 7.0]),
+                                         np.zeros(2), k)
+    assert one(1).tolist() == [0.5, 1.875]
+    assert one(2).tolist() == [0.96875, 1.9921875]
+    e = np.zeros(0, np.int64)
+    x = osm.smooth_gs_sweeps(np.ones(3), np.zeros(0), np.zeros(0), e, e, np.array([2.0, 3.0, 4.0]),
+                             np.zeros(3), 1)
+    assert x.tolist() == [2.0, 3.0, 4.0]
+    from tests.helpers import system
+    c, d, lo, up = system(4, 0.2, 1.0)
+    A = oldu.dense(d, lo, up, c.owner, c.neighbour)
+    xs = np.random.default_rng(3).standard_normal(c.n_cells)
+    xd = xs.copy()
+    for _ in range(3):
+        for i in range(c.n_cells):
+            xd[i] = (c.b[i] - (A[i] @ xd - A[i, i] * xd[i])) / A[i, i]
+    xo = osm.smooth_gs_sweeps(d, lo, up, c.owner, c.neighbour, c.b, xs, 3)
+    np.testing.assert_allclose(xo, xd, rtol=1e-13, atol=1e-13 * np.abs(xd).max())
+    # full loops: the reference tests' residual bars for DIC and DILU on 2x2 systems
+    own, nei = np.array([0]), np.array([1])
+    xs, it, res, info = osm.smooth_solver(np.array([4.0, 4.0]), np.array([-1.0]), np.array([-1.0]),
+                                          own, nei, np.array([2.0, 7.0]), np.zeros(2),
+                                          smoother="DIC", tolerance=1e-10, max_iter=100)
+    A2 = np.array([[4.0, -1.0], [-1.0, 4.0]])
+    assert info.converged and np.linalg.norm(np.array([2.0, 7.0]) - A2 @ xs) < 1e-4
+    xs, it, res, info = osm.smooth_solver(np.array([4.0, 5.0]), np.array([-1.0]), np.array([-2.0]),
+                                          own, nei, np.array([3.0, 3.0]), np.zeros(2),
+                                          smoother="DILU", omega=0.5, tolerance=1e-10, max_iter=200)
+    A2 = np.array([[4.0, -1.0], [-2.0, 5.0]])  # the reference's bars (rel_tol 0.01)
+    assert info.converged and np.linalg.norm(np.array([3.0, 3.0]) - A2 @ xs) < 0.1
+
+
+def test_diagonal_solver_oracle_known_answers():
+    """DiagonalSolver (tests/unit/solvers/test_diagonal.py:15-126): x = b/d,
+    zero RHS gives zero, a near-zero diagonal is clamped to 1e-30 (finite)."""
+    from oracle import smooth as osm
+    x, r = osm.diagonal_solver(np.array([2.0, 4.0, 5.0]), np.array([4.0, 8.0, 10.0]))
+    assert x.tolist() == [2.0, 2.0, 2.0] and r == 0.0
+    x, r = osm.diagonal_solver(np.array([1e-35, 2.0]), np.array([1.0, 2.0]))
+    assert np.isfinite(x).all() and x[1] == 1.0 and x[0] == 1.0 / 1e-30
