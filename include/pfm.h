@@ -208,6 +208,16 @@ int pfm_pbicgstab_solve(pfm_ldu h, const pfm_solve_params *prm,
                         void *work, double *history_h, pfm_solve_result *res,
                         void *stream);
 
+/* PBiCGSolver._solve (solvers/pbicg.py:107-220): BiCG with the shadow system
+ * on A^T and the same diagonal preconditioner (the reference's M^-T = M^-1);
+ * packed_t = pfm_ldu_pack(h, 1, lower, upper) (A^T).  Breakdown when
+ * |rho| or |z~.v| < 1e-30 (not converged). */
+int64_t pfm_pbicg_workspace_bytes(pfm_ldu h, int32_t max_iter);
+int pfm_pbicg_solve(pfm_ldu h, const pfm_solve_params *prm, const double *diag,
+                    const double *packed, const double *packed_t, const double *rD,
+                    const double *b, double *x, void *work, double *history_h,
+                    pfm_solve_result *res, void *stream);
+
 /* Mixed precision (extension of PCGSolver, solvers/pcg.py:97-187): FP32
  * coefficients (pfm_ldu_pack_f32), FP32 SpMV and FP32 diagonal DIC apply in the
  * inner PCG loop (every dot product and norm accumulated in FP64), inside FP64

@@ -4,6 +4,10 @@ Restates, in numpy float64 with the reference's op order:
   * ``Monitor``   -- ResidualMonitor.update/build_info (solvers/residual.py:112-168)
   * ``pcg``       -- PCGSolver._solve (solvers/pcg.py:97-187)
   * ``pbicgstab`` -- PBiCGSTABSolver._solve (solvers/pbicgstab.py:103-239)
+  * ``pbicg``     -- PBiCGSolver._solve (solvers/pbicg.py:107-220); its
+                     ``matvec_t`` is the transposed product (oracle.ldu.ax_t);
+                     like the reference it updates r~ with A^T p (one search
+                     direction), which is BiCG only for symmetric A
 ``matvec(x)`` is the LDU product (oracle.ldu.ax) and ``precond(r)`` the
 diagonal apply r / rD, or None for no preconditioner.  ``dot`` selects the
 summation order of the dot products and norms (the reference leaves it to
@@ -98,6 +102,41 @@ def pcg(matvec, precond, b, x0, *, tolerance=1e-6, rel_tol=0.01, max_iter=1000, 
         rz_new = dot(r, z)
         beta = rz_new / rz
         rz = rz_new
+        p = z + beta * p
+    return _ret(x, mon, False)
+
+
+def pbicg(matvec, matvec_t, precond, b, x0, *, tolerance=1e-6, rel_tol=0.01, max_iter=1000,
+          min_iter=0, dot=dot_blas):
+    mon = Monitor(tolerance, rel_tol, min_iter, dot)
+    x = np.array(x0, dtype=np.float64, copy=True)
+    r = b - matvec(x)
+    if mon.update(r, 0):
+        return _ret(x, mon, True)
+    rt = r.copy()
+    ap = (lambda v: precond(v)) if precond is not None else (lambda v: v.copy())
+    z = ap(r)
+    zt = ap(rt)
+    p = z.copy()
+    rho = dot(zt, r)
+    for i in range(1, max_iter + 1):
+        if abs(float(rho)) < 1e-30:
+            return _ret(x, mon, False)
+        v = matvec(p)
+        ztv = dot(zt, v)
+        if abs(float(ztv)) < 1e-30:
+            return _ret(x, mon, False)
+        alpha = rho / ztv
+        x = x + alpha * p
+        r = r - alpha * v
+        if mon.update(r, i):
+            return _ret(x, mon, True)
+        z = ap(r)
+        rt = rt - alpha * matvec_t(p)
+        zt = ap(rt)
+        rho_new = dot(zt, r)
+        beta = rho_new / rho
+        rho = rho_new
         p = z + beta * p
     return _ret(x, mon, False)
 

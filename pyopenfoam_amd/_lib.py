@@ -98,6 +98,9 @@ _SIGS = {
     "pfm_pcg_solve": (INT, [P, C.POINTER(SolveParams), P, P, P, P, P, P, P, C.POINTER(SolveResult), P]),
     "pfm_pbicgstab_solve": (INT, [P, C.POINTER(SolveParams), P, P, P, P, P, P, P,
                                   C.POINTER(SolveResult), P]),
+    "pfm_pbicg_workspace_bytes": (I64, [P, I32]),
+    "pfm_pbicg_solve": (INT, [P, C.POINTER(SolveParams), P, P, P, P, P, P, P, P,
+                              C.POINTER(SolveResult), P]),
     "pfm_pcg_mixed_workspace_bytes": (I64, [P, I32]),
     "pfm_pcg_mixed_solve": (INT, [P, C.POINTER(SolveParams), C.c_double, I32, P, P, P, P, P, P, P,
                                   P, C.POINTER(SolveResult), P, P]),

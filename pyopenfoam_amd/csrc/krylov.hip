@@ -468,6 +468,165 @@ __global__ __launch_bounds__(kBlock) void k_bs_pupdate(int n, int nslices, int i
   })
 }
 
+// ------------------------------------------------------------ PBiCG --------
+// solvers/pbicg.py:107-220.  Per iteration: v = A p and t = A^T p with
+// z~.v partials (region A) | rho/z~.v breakdown, alpha, x, r, r~ and the
+// partials r.r, z~.r (region B) | monitor, rho_new, beta, p = z + beta p.
+// z = r / rD and z~ = r~ / rD are formed where they are read.
+
+// r = b - A x; r~ = r; p = z = r / rD; partials r.r and z~.r (= z.r).
+__global__ __launch_bounds__(kBlock) void k_bicg_init(int n, int nslices,
+                                                       const SliceInfo *__restrict__ sl,
+                                                       const int32_t *__restrict__ col,
+                                                       const double *__restrict__ val,
+                                                       const double *__restrict__ diag,
+                                                       const double *__restrict__ rD,
+                                                       const double *__restrict__ b,
+                                                       const double *__restrict__ x,
+                                                       double *__restrict__ r,
+                                                       double *__restrict__ rt,
+                                                       double *__restrict__ p,
+                                                       double *__restrict__ part) {
+  __shared__ double lds[kWavesPerBlock];
+  double acc[2] = {0.0, 0.0};
+  FOR_ROWS({
+    const SliceInfo si = load_slice(sl, s);
+    const double dx = row < n ? diag[row] * x[row] : 0.0;
+    const double ax = row_spmv<double>(si, lane, dx, col, val, x);
+    if (row < n) {
+      const double ri = b[row] - ax;
+      const double zi = rD ? ri / rD[row] : ri;
+      r[row] = ri;
+      rt[row] = ri;
+      p[row] = zi;
+      acc[0] += ri * ri;
+      acc[1] += zi * ri;
+    }
+  })
+  store_partials<2>(acc, part + gridDim.x, lds);
+}
+
+__global__ __launch_bounds__(kBlock) void k_bicg_init_fin(const double *__restrict__ part, int G,
+                                                           KState *st, double *__restrict__ hist) {
+  __shared__ double lds[kWavesPerBlock];
+  const double RR = reduce_partials(part + G, G, lds);
+  const double RZ = reduce_partials(part + 2 * G, G, lds);
+  if (threadIdx.x == 0) {
+    st->rhos[0] = RZ;
+    if (monitor_update(st, hist, sqrt(RR), 0))
+      finish(st, PFM_SOLVE_CONVERGED);
+    else if (st->max_iter < 1)
+      finish(st, PFM_SOLVE_MAXITER);
+    else if (fabs(RZ) < 1e-30)  // the rho test at the top of iteration 1
+      finish(st, PFM_SOLVE_BREAKDOWN);
+  }
+}
+
+// v = A p, t = A^T p; partial z~.v (region A).
+__global__ __launch_bounds__(kBlock) void k_bicg_spmv(int n, int nslices,
+                                                       const SliceInfo *__restrict__ sl,
+                                                       const int32_t *__restrict__ col,
+                                                       const double *__restrict__ val,
+                                                       const double *__restrict__ val_t,
+                                                       const double *__restrict__ diag,
+                                                       const double *__restrict__ rD,
+                                                       const double *__restrict__ p,
+                                                       const double *__restrict__ rt,
+                                                       double *__restrict__ v,
+                                                       double *__restrict__ t,
+                                                       double *__restrict__ part, const KState *st) {
+  __shared__ double lds[kWavesPerBlock];
+  if (st->done) return;
+  double acc[1] = {0.0};
+  FOR_ROWS({
+    const SliceInfo si = load_slice(sl, s);
+    const double dx = row < n ? diag[row] * p[row] : 0.0;
+    const double av = row_spmv<double>(si, lane, dx, col, val, p);
+    const double at = row_spmv<double>(si, lane, dx, col, val_t, p);
+    if (row < n) {
+      v[row] = av;
+      t[row] = at;
+      const double zt = rD ? rt[row] / rD[row] : rt[row];
+      acc[0] += zt * av;
+    }
+  })
+  store_partials<1>(acc, part, lds);
+}
+
+// alpha = rho / z~.v (breakdown if |z~.v| < 1e-30); x += alpha p;
+// r -= alpha v; r~ -= alpha t; partials r.r, z~.r (region B).
+__global__ __launch_bounds__(kBlock) void k_bicg_update(int n, int nslices, int it,
+                                                         const double *__restrict__ rD,
+                                                         const double *__restrict__ p,
+                                                         const double *__restrict__ v,
+                                                         const double *__restrict__ t,
+                                                         double *__restrict__ x,
+                                                         double *__restrict__ r,
+                                                         double *__restrict__ rt,
+                                                         double *__restrict__ part, KState *st) {
+  __shared__ double lds[kWavesPerBlock];
+  if (st->done) return;
+  const int G = gridDim.x;
+  const double ZTV = reduce_partials(part, G, lds);
+  if (fabs(ZTV) < 1e-30) {
+    if (blockIdx.x == 0 && threadIdx.x == 0) finish(st, PFM_SOLVE_BREAKDOWN);
+    return;
+  }
+  const double alpha = st->rhos[(it - 1) & 1] / ZTV;
+  double acc[2] = {0.0, 0.0};
+  FOR_ROWS({
+    if (row < n) {
+      x[row] = x[row] + alpha * p[row];
+      const double ri = r[row] - alpha * v[row];
+      r[row] = ri;
+      const double rti = rt[row] - alpha * t[row];
+      rt[row] = rti;
+      const double zt = rD ? rti / rD[row] : rti;
+      acc[0] += ri * ri;
+      acc[1] += zt * ri;
+    }
+  })
+  store_partials<2>(acc, part + G, lds);
+}
+
+// monitor(||r||, it); rho_new = z~.r; beta = rho_new / rho; p = z + beta p.
+__global__ __launch_bounds__(kBlock) void k_bicg_pupdate(int n, int nslices, int it,
+                                                          const double *__restrict__ rD,
+                                                          const double *__restrict__ r,
+                                                          double *__restrict__ p,
+                                                          const double *__restrict__ part,
+                                                          KState *st, double *__restrict__ hist) {
+  __shared__ double lds[kWavesPerBlock];
+  if (st->done) return;
+  const int G = gridDim.x;
+  const double RR = reduce_partials(part + G, G, lds);
+  const double RZ = reduce_partials(part + 2 * G, G, lds);
+  const double res = sqrt(RR);
+  const double rho_old = st->rhos[(it - 1) & 1];
+  int status = 0;
+  if (monitor_decision(st, res, it))
+    status = PFM_SOLVE_CONVERGED;
+  else if (it >= st->max_iter)
+    status = PFM_SOLVE_MAXITER;
+  else if (fabs(RZ) < 1e-30)  // the rho test at the top of the next iteration
+    status = PFM_SOLVE_BREAKDOWN;
+  if (blockIdx.x == 0 && threadIdx.x == 0) {
+    monitor_update(st, hist, res, it);
+    st->iter = it;
+    st->rhos[it & 1] = RZ;
+    if (status) finish(st, status);
+  }
+  if (status) return;
+  const double beta = RZ / rho_old;
+  FOR_ROWS({
+    if (row < n) {
+      const double ri = r[row];
+      const double zi = rD ? ri / rD[row] : ri;
+      p[row] = zi + beta * p[row];
+    }
+  })
+}
+
 // ------------------------------------------------------------ workspace ----
 struct Work {
   KState *st;
@@ -661,6 +820,50 @@ int pfm_pbicgstab_solve(pfm_ldu h, const pfm_solve_params *prm, const double *di
                                     w.st, w.hist);
     k_bs_update<<<G, kBlock, 0, s>>>(n, ns, y, z, sv, t, rt, x, r, w.part, w.st);
     k_bs_pupdate<<<G, kBlock, 0, s>>>(n, ns, it, pd, r, v, p, y, w.part, w.st, w.hist);
+  });
+  if (rc != PFM_OK) return rc;
+  PFM_CHECK_HIP(hipGetLastError());
+  return collect(w.st, w.hist, history_h, hist_cap, res, s);
+}
+
+
+int64_t pfm_pbicg_workspace_bytes(pfm_ldu h, int32_t max_iter) {
+  if (!h || max_iter < 0) return -1;
+  return (int64_t)work_layout(h, max_iter, 5, nullptr, nullptr);
+}
+
+int pfm_pbicg_solve(pfm_ldu h, const pfm_solve_params *prm, const double *diag,
+                    const double *packed, const double *packed_t, const double *rD,
+                    const double *b, double *x, void *work, double *history_h,
+                    pfm_solve_result *res, void *stream) {
+  int rc = check_solve_args(h, prm, diag, packed, rD, b, x, work, res);
+  if (rc != PFM_OK) return rc;
+  PFM_REQUIRE(h->nslots == 0 || packed_t, "null transposed packed coefficients");
+  memset(res, 0, sizeof(*res));
+  if (h->n == 0) {
+    res->status = PFM_SOLVE_CONVERGED;
+    res->converged = 1;
+    return PFM_OK;
+  }
+  hipStream_t s = (hipStream_t)stream;
+  Work w;
+  work_layout(h, prm->max_iter, 5, &w, (char *)work);
+  double *r = w.vec[0], *rt = w.vec[1], *p = w.vec[2], *v = w.vec[3], *t = w.vec[4];
+  const double *pd = prm->precond == PFM_PRECOND_DIAG ? rD : nullptr;
+  const int hist_cap = 2 * prm->max_iter + 2;
+  const int G = red_grid(h);
+  const int n = h->n, ns = h->nslices;
+  k_state_init<<<1, 1, 0, s>>>(w.st, prm->tolerance, prm->rel_tol, prm->max_iter, prm->min_iter,
+                               hist_cap);
+  k_bicg_init<<<G, kBlock, 0, s>>>(n, ns, h->d_slices, h->d_col, packed, diag, pd, b, x, r, rt, p,
+                                   w.part);
+  k_bicg_init_fin<<<1, kBlock, 0, s>>>(w.part, G, w.st, w.hist);
+  PFM_CHECK_HIP(hipGetLastError());
+  rc = drive(h, w.st, prm->max_iter, s, [&](int it) {
+    k_bicg_spmv<<<G, kBlock, 0, s>>>(n, ns, h->d_slices, h->d_col, packed, packed_t, diag, pd, p,
+                                     rt, v, t, w.part, w.st);
+    k_bicg_update<<<G, kBlock, 0, s>>>(n, ns, it, pd, p, v, t, x, r, rt, w.part, w.st);
+    k_bicg_pupdate<<<G, kBlock, 0, s>>>(n, ns, it, pd, r, p, w.part, w.st, w.hist);
   });
   if (rc != PFM_OK) return rc;
   PFM_CHECK_HIP(hipGetLastError());

@@ -9,8 +9,9 @@ from .residual import ConvergenceInfo, ResidualMonitor
 from .smoothers import DICGSmoother, GaussSeidelSmoother, JacobiSmoother, Smoother
 from .smooth_solver import SmoothSolver
 from .diagonal import DiagonalSolver
+from .pbicg import PBiCGSolver
 
 __all__ = ["LinearSolverBase", "create_solver", "solver_from_dict", "PCGSolver",
            "PBiCGSTABSolver", "GAMGSolver", "DICPreconditioner", "DILUPreconditioner",
            "Preconditioner", "ConvergenceInfo", "ResidualMonitor", "GaussSeidelSmoother",
-           "JacobiSmoother", "DICGSmoother", "Smoother", "SmoothSolver", "DiagonalSolver"]
+           "JacobiSmoother", "DICGSmoother", "Smoother", "SmoothSolver", "DiagonalSolver", "PBiCGSolver"]

@@ -83,12 +83,14 @@ def _ensure_registry() -> None:
         return
     from .diagonal import DiagonalSolver
     from .gamg import GAMGSolver
+    from .pbicg import PBiCGSolver
     from .pbicgstab import PBiCGSTABSolver
     from .pcg import PCGSolver
     from .smooth_solver import SmoothSolver
 
     _register("PCG", PCGSolver)
     _register("PBICGSTAB", PBiCGSTABSolver)
+    _register("PBICG", PBiCGSolver)
     _register("SMOOTHSOLVER", SmoothSolver)
     _register("DIAGONAL", DiagonalSolver)
     _register("GAMG", GAMGSolver)

@@ -15,6 +15,7 @@ from oracle import krylov as okry
 from oracle import ldu as oldu
 from oracle import precond as opre
 from pyopenfoam_amd import LduMatrix, PBiCGSTABSolver, PCGSolver, create_solver
+from pyopenfoam_amd.solvers import PBiCGSolver
 from tests.helpers import system
 
 pytestmark = pytest.mark.gpu
@@ -131,6 +132,72 @@ def test_pbicgstab_matches_oracle(gpu_device, n, jit, sig, pre):
         xg = x.cpu().numpy()
         spread = max(np.linalg.norm(r[0] - xo) / np.linalg.norm(xo) for r in runs.values())
         assert np.linalg.norm(xg - xo) / np.linalg.norm(xo) <= max(10 * spread, 1e-9)
+
+
+@pytest.mark.parametrize("n,jit,sig", [(8, 0.0, 0.0), (16, 0.2, 1.0), (32, 0.2, 1.0)])
+@pytest.mark.parametrize("pre", ["DILU", "DIC", "none"])
+def test_pbicg_matches_oracle(gpu_device, n, jit, sig, pre):
+    """PBiCG (solvers/pbicg.py:107-220): the same decision, early history
+    inside 10x the oracle's summation-order spread (floor 1e-9), iterations
+    within the spread +-1 and the solution within 10x the spread (floor 1e-9).
+    The transposed products run on the second packed layout."""
+    c, d, lo, up = system(n, jit, sig)
+    print(f"pbicg n={n} jitter={jit} sigma={sig} precond={pre}", flush=True)
+    m = _matrix(c, d, lo, up, gpu_device)
+    solver = PBiCGSolver(tolerance=1e-300, rel_tol=1e-8, max_iter=3000, preconditioner=pre)
+    x, iters, res = solver(m, torch.from_numpy(c.b), torch.zeros(c.n_cells, dtype=torch.float64))
+    rD = {"DILU": lambda: opre.dilu_build(d, lo, up, c.owner, c.neighbour),
+          "DIC": lambda: opre.dic_build(d, up, c.owner, c.neighbour),
+          "none": lambda: None}[pre]()
+    mv = lambda v: oldu.ax(d, lo, up, c.owner, c.neighbour, v)
+    mvt = lambda v: oldu.ax_t(d, lo, up, c.owner, c.neighbour, v)
+    lo_it, hi_it, runs = _oracle_spread(lambda dot: okry.pbicg(
+        mv, mvt, (lambda r: r / rD) if rD is not None else None, c.b, np.zeros(c.n_cells),
+        tolerance=1e-300, rel_tol=1e-8, max_iter=3000, dot=dot))
+    xo, io, ro, info = runs["blas"]
+    assert solver.last_info.converged == info.converged
+    h = np.array(solver.last_info.residual_history)
+    k = min(20, len(h), *(len(r[3].residual_history) for r in runs.values()))
+    hb = np.array(info.residual_history[:k])
+    spread = np.max([np.abs(np.array(r[3].residual_history[:k]) / hb - 1) for r in runs.values()],
+                    axis=0)
+    spread = np.maximum.accumulate(spread)
+    assert np.all(np.abs(h[:k] / hb - 1) <= np.maximum(10 * spread, 1e-9)), (h[:k], hb, spread)
+    if info.converged:
+        assert lo_it - 1 <= iters <= hi_it + 1, (iters, lo_it, hi_it)
+        xg = x.cpu().numpy()
+        spread = max(np.linalg.norm(r[0] - xo) / np.linalg.norm(xo) for r in runs.values())
+        assert np.linalg.norm(xg - xo) / np.linalg.norm(xo) <= max(10 * spread, 1e-9)
+    if info.converged:
+        r = c.b - mv(x.cpu().numpy())
+        assert np.sqrt(r @ r) <= 1e-6 * h[0]
+
+
+def test_pbicg_from_dict_and_flipped_faces(gpu_device):
+    """``solver: PBiCG`` through the dictionary factory; a face set with half the
+    faces flipped (owner > neighbour) gives the same solve as the oriented one.
+    Unjittered (symmetric) system: the reference's PBiCG carries one search
+    direction for both the primal and the shadow recurrences, which is BiCG
+    only when A = A^T; on the jittered, non-symmetric systems it stalls (the
+    oracle does too, and the parity test above checks that decision)."""
+    c, d, lo, up = system(12, 0.0, 1.0)
+    m = _matrix(c, d, lo, up, gpu_device)
+    s1 = create_solver("PBiCG", tolerance=1e-300, rel_tol=1e-8, max_iter=2000)
+    assert isinstance(s1, PBiCGSolver)
+    x1, i1, _ = s1(m, torch.from_numpy(c.b), torch.zeros(c.n_cells, dtype=torch.float64))
+    assert s1.last_info.converged
+    msk = np.random.default_rng(3).random(len(lo)) < 0.5
+    own = np.where(msk, c.neighbour, c.owner)
+    nei = np.where(msk, c.owner, c.neighbour)
+    m2 = LduMatrix(c.n_cells, torch.from_numpy(own), torch.from_numpy(nei), device=gpu_device)
+    m2.diag = torch.from_numpy(d)
+    m2.lower = torch.from_numpy(np.where(msk, up, lo))
+    m2.upper = torch.from_numpy(np.where(msk, lo, up))
+    s2 = PBiCGSolver(tolerance=1e-300, rel_tol=1e-8, max_iter=2000, preconditioner="DILU")
+    x2, i2, _ = s2(m2, torch.from_numpy(c.b), torch.zeros(c.n_cells, dtype=torch.float64))
+    assert s2.last_info.converged and abs(i2 - i1) <= 2
+    xa, xb = x1.cpu().numpy(), x2.cpu().numpy()
+    assert np.linalg.norm(xa - xb) / np.linalg.norm(xa) < 1e-7
 
 
 @pytest.mark.parametrize("n,jit,sig", [(16, 0.0, 0.0), (32, 0.0, 1.0), (32, 0.2, 1.0)])

@@ -317,3 +317,37 @@ def test_diagonal_solver_oracle_known_answers():
     assert x.tolist() == [2.0, 2.0, 2.0] and r == 0.0
     x, r = osm.diagonal_solver(np.array([1e-35, 2.0]), np.array([1.0, 2.0]))
     assert np.isfinite(x).all() and x[1] == 1.0 and x[0] == 1.0 / 1e-30
+
+
+def test_pbicg_oracle_pins():
+    """PBiCG (solvers/pbicg.py:107-220): with a symmetric matrix, a symmetric
+    diagonal preconditioner and r~0 = r0, BiCG's iterates are CG's (the shadow
+    recurrence is the primal one), so the oracle's PBiCG must follow its PCG:
+    the same histories to rounding over the first 20 iterations (later ones
+    part as rounding grows, as two CG codes do) and the same iteration count
+    within 2; on a 2x2 non-symmetric
+    system it reaches the exact solution in two iterations (finite
+    termination); the identity system converges in one."""
+    from tests.helpers import system
+    c, d, lo, up = system(6, 0.0, 1.0)
+    rD = opre.dic_build(d, up, c.owner, c.neighbour)
+    mv = lambda v: oldu.ax(d, lo, up, c.owner, c.neighbour, v)
+    mvt = lambda v: oldu.ax_t(d, lo, up, c.owner, c.neighbour, v)
+    pre = lambda r: r / rD
+    xb, ib, rb, infob = okry.pbicg(mv, mvt, pre, c.b, np.zeros(c.n_cells), tolerance=1e-300,
+                                   rel_tol=1e-8, max_iter=500)
+    xc, ic, rc, infoc = okry.pcg(mv, pre, c.b, np.zeros(c.n_cells), tolerance=1e-300,
+                                 rel_tol=1e-8, max_iter=500)
+    assert infob.converged and abs(ib - ic) <= 2
+    np.testing.assert_allclose(infob.residual_history[:20], infoc.residual_history[:20],
+                               rtol=1e-12)
+    np.testing.assert_allclose(xb, xc, rtol=0, atol=1e-6 * np.abs(xc).max())
+    A = np.array([[4.0, -1.0], [-2.0, 5.0]])
+    own, nei = np.array([0]), np.array([1])
+    d2, l2, u2 = np.array([4.0, 5.0]), np.array([-1.0]), np.array([-2.0])
+    x2, i2, r2, info2 = okry.pbicg(lambda v: oldu.ax(d2, l2, u2, own, nei, v),
+                                   lambda v: oldu.ax_t(d2, l2, u2, own, nei, v), None,
+                                   np.array([3.0, 3.0]), np.zeros(2), tolerance=1e-12,
+                                   rel_tol=1e-14, max_iter=10)
+    assert info2.converged and i2 <= 3
+    np.testing.assert_allclose(x2, np.linalg.solve(A, [3.0, 3.0]), rtol=1e-12)
