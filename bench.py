@@ -207,6 +207,63 @@ def bench_batch(args, dev):
             "setup_s": {"cases_host": round(t_gen, 1), "handle_autotune": round(t_create, 1)}}
 
 
+def bench_geometry(case, dev, no_cpu: bool):
+    """Device mesh geometry (pfm_mesh_geometry) on the case's mesh: kernel
+    time from HIP events on the launch stream (5 launches), the whole call
+    with host CSR and transfers, and the oracle's vectorised host geometry on
+    a 40^3 sample scaled per cell."""
+    import numpy as np
+    import torch
+
+    from pyopenfoam_amd import _lib
+    from pyopenfoam_amd.core.device import current_stream
+    from pyopenfoam_amd.mesh.geometry import cell_faces, compute_geometry, face_csr
+
+    m = case.mesh
+    t0 = time.perf_counter()
+    compute_geometry(m, device=dev)
+    torch.cuda.synchronize()
+    t_call = time.perf_counter() - t0
+    fptr, fv = face_csr(m.faces)
+    cptr, cf, mx = cell_faces(m.n_cells, m.owner, m.neighbour, fptr)
+    up = lambda a, dt: torch.from_numpy(np.ascontiguousarray(a)).to(device=dev, dtype=dt)
+    pts = up(m.points, torch.float64)
+    d = [up(a, torch.int32) for a in (fptr, fv, m.owner, m.neighbour, cptr, cf)]
+    nf, nfi, nc = m.n_faces, m.n_internal_faces, m.n_cells
+    f64 = dict(dtype=torch.float64, device=dev)
+    outs = [torch.empty(nf, 3, **f64), torch.empty(nf, 3, **f64), torch.empty(nf, **f64),
+            torch.empty(nc, **f64), torch.empty(nc, 3, **f64), torch.empty(nf, **f64)]
+    p = _lib.ptr
+    L = _lib.lib()
+    s = current_stream(dev)
+    call = lambda: _lib.check(L.pfm_mesh_geometry(
+        int(m.points.shape[0]), p(pts), nf, p(d[0]), p(d[1]), nfi, p(d[2]), p(d[3]), nc, p(d[4]),
+        p(d[5]), mx, *[p(o) for o in outs], s), "pfm_mesh_geometry")
+    call()
+    e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+    e0.record()
+    for _ in range(5):
+        call()
+    e1.record()
+    torch.cuda.synchronize()
+    k_ms = e0.elapsed_time(e1) / 5
+    out = {"n_cells": nc, "n_faces": nf, "kernels_ms": round(k_ms, 3),
+           "call_with_transfers_s": round(t_call, 3)}
+    log(f"geometry: kernels {k_ms:.2f} ms, whole call {t_call:.2f} s")
+    if not no_cpu:
+        from oracle import mesh as omesh
+        from pyopenfoam_amd.mesh.hexmesh import relabel_cells, structured_hex
+        ms = structured_hex(40)
+        ms = relabel_cells(ms, np.random.default_rng(0).permutation(ms.n_cells))
+        t0 = time.perf_counter()
+        omesh.geometry_vec(ms.points, ms.faces, ms.owner, ms.neighbour, ms.n_cells)
+        t_o = time.perf_counter() - t0
+        out["cpu_oracle"] = {"sample": "oracle geometry_vec (numpy, vectorised) on a 40^3 mesh",
+                             "s": round(t_o, 2), "s_scaled_to_case": round(t_o * nc / ms.n_cells, 1)}
+        log(f"geometry oracle 40^3: {t_o:.2f} s")
+    return out
+
+
 def main() -> int:
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -259,6 +316,7 @@ def main() -> int:
     t_case = time.perf_counter() - t0
     N, F = case.n_cells, case.mesh.n_internal_faces
     log(f"case {n}^3: {N} cells, {F} faces, built in {t_case:.1f} s")
+    geometry = bench_geometry(case, dev, args.no_cpu)
     mesh = FvMesh.from_case(case, dev)
     t0 = time.perf_counter()
     mat = assemble(case, mesh)
@@ -497,6 +555,7 @@ def main() -> int:
                      "algorithmic_bytes": spmv_bytes},
         "cpu_baseline": cpu,
         "setup_s": {"case_host": round(t_case, 2), "handle_and_assembly": round(t_asm, 2)},
+        "geometry": geometry,
     }
     print(json.dumps(line), flush=True)
     return 0

@@ -73,6 +73,30 @@ int pfm_dilu_levels(int32_t n_cells, int32_t n_faces, const int32_t *owner_h,
                     const int32_t *neighbour_h, int32_t *level_out_h,
                     int32_t *n_levels);
 
+/* Cell -> face lists of a polyMesh, the order of mesh/mesh_geometry.py:188-194:
+ * for each cell the faces it owns in ascending face order, then the internal
+ * faces it neighbours in ascending order; a neighbour-side entry is stored as
+ * -(f + 1).  face_ptr_h (n_faces + 1) delimits each face's vertex loop;
+ * *max_cell_verts_out = the largest number of vertex references of one cell
+ * (the bound pfm_mesh_geometry checks).  cell_ptr_out_h: n_cells + 1,
+ * cell_faces_out_h: n_faces + n_internal. */
+int pfm_cell_faces(int32_t n_cells, int32_t n_faces, int32_t n_internal,
+                   const int32_t *owner_h, const int32_t *neighbour_h,
+                   const int32_t *face_ptr_h, int32_t *cell_ptr_out_h,
+                   int32_t *cell_faces_out_h, int32_t *max_cell_verts_out);
+
+/* Applies a cell permutation to the internal faces, the step
+ * tools/renumber_mesh.py:60-97 leaves to its caller: owner and neighbour are
+ * relabelled (new_of_old[old] = new), a face whose new owner exceeds its new
+ * neighbour is swapped (flip_out[k] = 1: reverse its vertex loop), and the
+ * faces are sorted by (owner, neighbour), ties in face order, which gives the
+ * owner < neighbour orientation checked by mesh/topology.py:81-87.
+ * order_out[k] = old index of new face k. */
+int pfm_relabel_faces(int32_t n_cells, int32_t n_internal, const int32_t *owner_h,
+                      const int32_t *neighbour_h, const int32_t *new_of_old_h,
+                      int32_t *order_out_h, uint8_t *flip_out_h, int32_t *owner_out_h,
+                      int32_t *neighbour_out_h);
+
 /* Sweep levels of the symmetric Gauss-Seidel smoother
  * (solvers/smoothers.py:92-149): in the forward sweep cell i reads the
  * updated x of every neighbour j < i, so fwd_level[i] = 1 + max over those;
@@ -413,6 +437,26 @@ int pfm_gamg_pcg_solve(pfm_gamg g, const pfm_solve_params *prm, int32_t fp32_lev
 /* ------------------------------------------------------------------ */
 /* Assembly and adjoint                                               */
 /* ------------------------------------------------------------------ */
+
+/* Mesh geometry on the device, mesh/mesh_geometry.py:35-395: face centres
+ * (n_faces x 3, vertex mean), area vectors (0.5 sum v_j x v_{j+1}; zero for
+ * faces of fewer than 3 vertices) and |S_f|; cell volumes and centres by the
+ * reference's tet decomposition around the mean of the cell's distinct
+ * vertices (faces in pfm_cell_faces order, loops reversed on the neighbour
+ * side, |t| < 1e-30 skipped, degenerate cells: V = 0 and the mean of their
+ * face centres); delta = 1/|d . n_hat| on internal faces, 0 on boundary
+ * faces (guards at 1e-30 as in :386-393).  Faces: CSR face_ptr /
+ * face_verts (int32 vertex ids in [0, n_points)); points and all vector
+ * outputs are row-major (n, 3) FP64.  All arrays are device arrays;
+ * max_cell_verts (from pfm_cell_faces) must be <= 64. */
+int pfm_mesh_geometry(int32_t n_points, const double *points, int32_t n_faces,
+                      const int32_t *face_ptr, const int32_t *face_verts,
+                      int32_t n_internal, const int32_t *owner,
+                      const int32_t *neighbour, int32_t n_cells,
+                      const int32_t *cell_ptr, const int32_t *cell_faces,
+                      int32_t max_cell_verts, double *face_centres,
+                      double *face_areas, double *mag_sf, double *cell_volumes,
+                      double *cell_centres, double *delta, void *stream);
 
 /* fvm.laplacian (discretisation/operators.py:490-517) plus the
  * laplacianFoam Dirichlet diagonal term (applications/laplacian_foam.py:342-366):

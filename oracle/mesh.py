@@ -5,7 +5,15 @@
   ascending column order stably sorted by degree, start = first unvisited
   index of minimum degree, FIFO BFS, reversed; perm[new] = old.
 * ``geometry`` restates mesh/mesh_geometry.py:35-395 with the reference's
-  per-face / per-cell loops (small meshes only).
+  per-face / per-cell loops (small meshes only, any polygon).
+* ``geometry_vec`` is the same computation vectorised over faces and cells
+  for meshes whose faces all have n_v vertices and whose cells all have the
+  same number of faces, in the reference's operation order: sums from 0 in
+  vertex / list order, cells face by face and edge by edge, the tet centroid
+  ((ref + fc) + v_j + v_{j+1}) / 4 weighted by |t| (mesh_geometry.py:129-136,
+  213-247), the distinct vertices summed in ascending vertex order.  It is the
+  checker of the device geometry; both loop and vectorised forms are pinned to
+  the reference's own output (golden) at 1e-13.
 """
 
 from __future__ import annotations
@@ -105,6 +113,93 @@ def geometry(points, faces, owner, neighbour, n_cells):
         delta[f] = 1.0 / (dn if dn > 1e-30 else 1.0)
     return {"face_centres": fc, "face_areas": area, "cell_volumes": vol, "cell_centres": cen,
             "delta": delta}
+
+
+def _cross(a, b):
+    return np.stack([a[..., 1] * b[..., 2] - a[..., 2] * b[..., 1],
+                     a[..., 2] * b[..., 0] - a[..., 0] * b[..., 2],
+                     a[..., 0] * b[..., 1] - a[..., 1] * b[..., 0]], axis=-1)
+
+
+def _dot(a, b):
+    return (a[..., 0] * b[..., 0] + a[..., 1] * b[..., 1]) + a[..., 2] * b[..., 2]
+
+
+def geometry_vec(points, faces, owner, neighbour, n_cells):
+    """Vectorised ``geometry``; faces is an (n_faces, n_v) array.  Returns the
+    same dict plus ``mag_sf`` = sqrt(S.S)."""
+    pts = np.asarray(points, dtype=np.float64)
+    faces = np.asarray(faces, dtype=np.int64)
+    owner = np.asarray(owner, dtype=np.int64)
+    neighbour = np.asarray(neighbour, dtype=np.int64)
+    nf, nv = faces.shape
+    nfi = neighbour.shape[0]
+    fv = pts[faces]                                   # (nf, nv, 3)
+    fc = np.zeros((nf, 3))
+    for j in range(nv):
+        fc = fc + fv[:, j]
+    fc = fc / float(nv)
+    area = np.zeros((nf, 3))
+    if nv >= 3:
+        for j in range(nv):
+            area = area + _cross(fv[:, j], fv[:, (j + 1) % nv])
+    area = area * 0.5
+    mag = np.sqrt(_dot(area, area))
+
+    fidx = np.concatenate([np.arange(nf), np.arange(nfi)])
+    cell = np.concatenate([owner, neighbour])
+    flip = np.concatenate([np.zeros(nf, bool), np.ones(nfi, bool)])
+    order = np.argsort(cell, kind="stable")
+    counts = np.bincount(cell, minlength=n_cells)
+    if counts.min() != counts.max():
+        raise ValueError("geometry_vec expects cells with equal face counts")
+    nfc = int(counts[0])
+    fidx = fidx[order].reshape(n_cells, nfc)
+    flip = flip[order].reshape(n_cells, nfc)
+
+    cverts = np.sort(faces[fidx].reshape(n_cells, nfc * nv), axis=1)
+    first = np.ones_like(cverts, dtype=bool)
+    first[:, 1:] = cverts[:, 1:] != cverts[:, :-1]
+    ref = np.zeros((n_cells, 3))
+    for k in range(nfc * nv):
+        ref = np.where(first[:, k, None], ref + pts[cverts[:, k]], ref)
+    ref = ref / first.sum(axis=1)[:, None].astype(np.float64)
+
+    tv = np.zeros(n_cells)
+    wc = np.zeros((n_cells, 3))
+    loops = faces[fidx]                                # (n_cells, nfc, nv)
+    loops = np.where(flip[..., None], loops[..., ::-1], loops)
+    if nv >= 3:
+        for k in range(nfc):
+            fck = fc[fidx[:, k]]
+            d = fck - ref
+            rf = ref + fck
+            for j in range(nv):
+                va = pts[loops[:, k, j]]
+                vb = pts[loops[:, k, (j + 1) % nv]]
+                t = _dot(d, _cross(va - ref, vb - ref)) / 6.0
+                keep = ~(np.abs(t) < 1e-30)
+                t = np.abs(t)
+                cen = ((rf + va) + vb) / 4.0
+                tv = np.where(keep, tv + t, tv)
+                wc = np.where(keep[:, None], wc + cen * t[:, None], wc)
+    good = np.abs(tv) > 1e-30
+    vol = np.where(good, np.abs(tv), 0.0)
+    mean_fc = np.zeros((n_cells, 3))
+    for k in range(nfc):
+        mean_fc = mean_fc + fc[fidx[:, k]]
+    mean_fc = mean_fc / float(nfc)
+    cen = np.where(good[:, None], wc / np.where(good, tv, 1.0)[:, None], mean_fc)
+
+    delta = np.zeros(nf)
+    if nfi:
+        d = cen[neighbour] - cen[owner[:nfi]]
+        m = mag[:nfi]
+        nh = area[:nfi] / np.where(m > 1e-30, m, 1.0)[:, None]
+        dn = np.abs(_dot(d, nh))
+        delta[:nfi] = 1.0 / np.where(dn > 1e-30, dn, 1.0)
+    return {"face_centres": fc, "face_areas": area, "mag_sf": mag, "cell_volumes": vol,
+            "cell_centres": cen, "delta": delta}
 
 
 def dirichlet_delta(face_centres, face_areas, cell_centres, owner, face_ids):

@@ -69,6 +69,9 @@ _SIGS = {
     "pfm_last_error": (C.c_char_p, []),
     "pfm_abi_version": (INT, []),
     "pfm_rcm": (INT, [I32, I32, P, P, P]),
+    "pfm_cell_faces": (INT, [I32, I32, I32, P, P, P, P, P, P]),
+    "pfm_relabel_faces": (INT, [I32, I32, P, P, P, P, P, P, P]),
+    "pfm_mesh_geometry": (INT, [I32, P, I32, P, P, I32, P, P, I32, P, P, I32, P, P, P, P, P, P, P]),
     "pfm_dic_levels": (INT, [I32, I32, P, P, P, P]),
     "pfm_dilu_levels": (INT, [I32, I32, P, P, P, P]),
     "pfm_gs_levels": (INT, [I32, I32, P, P, P, P, P, P]),

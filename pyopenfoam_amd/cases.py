@@ -64,11 +64,9 @@ def rcm_permutation(n_cells: int, owner, neighbour) -> np.ndarray:
     return perm.astype(np.int64)
 
 
-def poisson_case(n: int, *, jitter: float = 0.0, sigma: float = 0.0, seed: int = 0,
-                 permute: bool = True, renumber: bool = True,
-                 dirichlet_patch: str = "x0", progress=None) -> PoissonCase:
-    """``progress(msg)``, if given, is called after each construction stage."""
-    note = progress or (lambda msg: None)
+def _case_mesh(n: int, jitter: float, seed: int, permute: bool, renumber: bool, note):
+    """Mesh stages: structured block, jitter, permutation, RCM.  Returns the
+    mesh and the generator after its mesh draws."""
     rng = np.random.default_rng(seed)
     mesh = structured_hex(n)
     note(f"case {n}^3: structured mesh")
@@ -84,7 +82,10 @@ def poisson_case(n: int, *, jitter: float = 0.0, sigma: float = 0.0, seed: int =
         inv[perm] = np.arange(mesh.n_cells)
         mesh = relabel_cells(mesh, inv)
         note("case: RCM renumbering")
-    geo = compute_geometry(mesh, progress)
+    return mesh, rng
+
+
+def _finish_case(n, mesh, rng, sigma, seed, dirichlet_patch, geo) -> PoissonCase:
     if sigma > 0.0:
         gamma_all = rng.lognormal(0.0, sigma, size=mesh.n_faces)
     else:
@@ -101,6 +102,18 @@ def poisson_case(n: int, *, jitter: float = 0.0, sigma: float = 0.0, seed: int =
         n=n, mesh=mesh, geo=geo, gamma=gamma_all[:nfi].copy(), dir_faces=faces, dir_cells=cells,
         dir_gamma=gamma_all[faces].copy(), dir_mag=geo.mag_sf[faces].copy(),
         dir_delta=dirichlet_delta(mesh, geo, faces), b=b, seed=seed)
+
+
+def poisson_case(n: int, *, jitter: float = 0.0, sigma: float = 0.0, seed: int = 0,
+                 permute: bool = True, renumber: bool = True,
+                 dirichlet_patch: str = "x0", progress=None, geometry=None) -> PoissonCase:
+    """``progress(msg)``, if given, is called after each construction stage.
+    ``geometry(mesh, progress)`` computes the mesh geometry (default: the
+    device kernels, ``compute_geometry``)."""
+    note = progress or (lambda msg: None)
+    mesh, rng = _case_mesh(n, jitter, seed, permute, renumber, note)
+    geo = (geometry or compute_geometry)(mesh, progress)
+    return _finish_case(n, mesh, rng, sigma, seed, dirichlet_patch, geo)
 
 
 @dataclass
@@ -133,25 +146,32 @@ class BatchCase:
         return int(self.cell_offsets[-1])
 
 
-def _batch_member(args):
+def _batch_mesh(args):
+    """Mesh stages of one batch member (host; runs in worker processes)."""
     n, seed, jitter, sigma = args
-    return poisson_case(n, jitter=jitter, sigma=sigma, seed=seed)
+    return _case_mesh(n, jitter, seed, True, True, lambda msg: None)
 
 
 def poisson_batch(n_systems: int = 512, n_min: int = 8, n_max: int = 48, *, seed: int = 0,
-                  jitter: float = 0.2, sigma: float = 1.0, workers: int = 8) -> BatchCase:
+                  jitter: float = 0.2, sigma: float = 1.0, workers: int = 8,
+                  geometry=None) -> BatchCase:
     """Batch of independent cube systems: side n_k ~ U{n_min..n_max} from
     ``default_rng(seed)``; system k is ``poisson_case(n_k, seed=seed*100003 + k)``
-    (jittered, permuted, RCM-renumbered, lognormal diffusivity, zero-mean RHS)."""
+    (jittered, permuted, RCM-renumbered, lognormal diffusivity, zero-mean RHS).
+    The meshes are built in ``workers`` host processes; the geometry
+    (``geometry``, default the device kernels) runs in this process."""
     rng = np.random.default_rng(seed)
     sizes = rng.integers(n_min, n_max + 1, size=n_systems)
     jobs = [(int(n), seed * 100003 + k, jitter, sigma) for k, n in enumerate(sizes)]
     if workers > 1 and n_systems > 1:
         from concurrent.futures import ProcessPoolExecutor
         with ProcessPoolExecutor(max_workers=workers) as ex:
-            cases = list(ex.map(_batch_member, jobs, chunksize=4))
+            meshes = list(ex.map(_batch_mesh, jobs, chunksize=4))
     else:
-        cases = [_batch_member(j) for j in jobs]
+        meshes = [_batch_mesh(j) for j in jobs]
+    geometry = geometry or compute_geometry
+    cases = [_finish_case(n, mesh, rng, sigma, sd, "x0", geometry(mesh, None))
+             for (n, sd, _, _), (mesh, rng) in zip(jobs, meshes)]
     cell_off = np.zeros(n_systems + 1, np.int64)
     face_off = np.zeros(n_systems + 1, np.int64)
     for k, c in enumerate(cases):

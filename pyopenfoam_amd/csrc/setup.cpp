@@ -5,9 +5,11 @@
 // topology); their results are uploaded by pfm_ldu_create.
 
 #include <algorithm>
+#include <cstdint>
 #include <cstdarg>
 #include <cstdio>
 #include <numeric>
+#include <vector>
 
 #include "pfm_internal.h"
 
@@ -316,6 +318,95 @@ int pfm_dilu_levels(int32_t n, int32_t nf, const int32_t *own, const int32_t *ne
     nl = std::max(nl, l + 1);
   }
   if (n_levels) *n_levels = nl;
+  return PFM_OK;
+}
+
+int pfm_cell_faces(int32_t n_cells, int32_t n_faces, int32_t n_internal, const int32_t *owner,
+                   const int32_t *neighbour, const int32_t *face_ptr, int32_t *cell_ptr_out,
+                   int32_t *cell_faces_out, int32_t *max_cell_verts_out) {
+  PFM_REQUIRE(n_cells >= 0 && n_faces >= 0 && n_internal >= 0 && n_internal <= n_faces,
+              "bad sizes (cells %d, faces %d, internal %d)", n_cells, n_faces, n_internal);
+  PFM_REQUIRE(cell_ptr_out && max_cell_verts_out, "null output");
+  PFM_REQUIRE(n_faces == 0 || (owner && face_ptr && cell_faces_out), "null array");
+  PFM_REQUIRE(n_internal == 0 || neighbour, "null neighbour array");
+  for (int32_t f = 0; f < n_faces; ++f) {
+    PFM_REQUIRE(owner[f] >= 0 && owner[f] < n_cells, "face %d: owner %d outside [0, %d)", f,
+                owner[f], n_cells);
+    PFM_REQUIRE(face_ptr[f + 1] >= face_ptr[f], "face_ptr decreases at face %d", f);
+  }
+  for (int32_t f = 0; f < n_internal; ++f)
+    PFM_REQUIRE(neighbour[f] >= 0 && neighbour[f] < n_cells && neighbour[f] != owner[f],
+                "face %d: neighbour %d invalid", f, neighbour[f]);
+  // Counting sort: owned faces in face order, then neighboured faces in face
+  // order (mesh_geometry.py:188-194).
+  std::fill(cell_ptr_out, cell_ptr_out + n_cells + 1, 0);
+  for (int32_t f = 0; f < n_faces; ++f) cell_ptr_out[owner[f] + 1]++;
+  for (int32_t f = 0; f < n_internal; ++f) cell_ptr_out[neighbour[f] + 1]++;
+  for (int32_t c = 0; c < n_cells; ++c) cell_ptr_out[c + 1] += cell_ptr_out[c];
+  std::vector<int32_t> fill(cell_ptr_out, cell_ptr_out + n_cells);
+  for (int32_t f = 0; f < n_faces; ++f) cell_faces_out[fill[owner[f]]++] = f;
+  for (int32_t f = 0; f < n_internal; ++f) cell_faces_out[fill[neighbour[f]]++] = -f - 1;
+  int32_t mx = 0;
+  for (int32_t c = 0; c < n_cells; ++c) {
+    int64_t nv = 0;
+    for (int32_t e = cell_ptr_out[c]; e < cell_ptr_out[c + 1]; ++e) {
+      const int32_t ent = cell_faces_out[e];
+      const int32_t f = ent >= 0 ? ent : -ent - 1;
+      nv += face_ptr[f + 1] - face_ptr[f];
+    }
+    if (nv > mx) mx = nv > INT32_MAX ? INT32_MAX : (int32_t)nv;
+  }
+  *max_cell_verts_out = mx;
+  return PFM_OK;
+}
+
+int pfm_relabel_faces(int32_t n_cells, int32_t n_internal, const int32_t *owner,
+                      const int32_t *neighbour, const int32_t *new_of_old, int32_t *order_out,
+                      uint8_t *flip_out, int32_t *owner_out, int32_t *neighbour_out) {
+  PFM_REQUIRE(n_cells >= 0 && n_internal >= 0, "negative size");
+  PFM_REQUIRE(n_cells == 0 || new_of_old, "null permutation");
+  PFM_REQUIRE(n_internal == 0 || (owner && neighbour && order_out && flip_out && owner_out &&
+                                  neighbour_out),
+              "null array");
+  {
+    std::vector<uint8_t> seen(n_cells, 0);
+    for (int32_t c = 0; c < n_cells; ++c) {
+      const int32_t q = new_of_old[c];
+      PFM_REQUIRE(q >= 0 && q < n_cells && !seen[q], "new_of_old is not a permutation (%d)", c);
+      seen[q] = 1;
+    }
+  }
+  std::vector<int32_t> lo(n_internal), hi(n_internal);
+  for (int32_t f = 0; f < n_internal; ++f) {
+    PFM_REQUIRE(owner[f] >= 0 && owner[f] < n_cells && neighbour[f] >= 0 &&
+                    neighbour[f] < n_cells,
+                "face %d addresses a cell outside [0, %d)", f, n_cells);
+    const int32_t a = new_of_old[owner[f]], b = new_of_old[neighbour[f]];
+    flip_out[f] = a > b;
+    lo[f] = a > b ? b : a;
+    hi[f] = a > b ? a : b;
+  }
+  // Two stable counting sorts (by hi, then by lo) give the (lo, hi) order
+  // with ties in face order.
+  std::vector<int32_t> cnt(n_cells + 1), tmp(n_internal);
+  auto pass = [&](const std::vector<int32_t> &key, const int32_t *in, int32_t *out) {
+    std::fill(cnt.begin(), cnt.end(), 0);
+    for (int32_t k = 0; k < n_internal; ++k) cnt[key[in ? in[k] : k] + 1]++;
+    for (int32_t c = 0; c < n_cells; ++c) cnt[c + 1] += cnt[c];
+    for (int32_t k = 0; k < n_internal; ++k) {
+      const int32_t f = in ? in[k] : k;
+      out[cnt[key[f]]++] = f;
+    }
+  };
+  pass(hi, nullptr, tmp.data());
+  pass(lo, tmp.data(), order_out);
+  std::vector<uint8_t> fl(flip_out, flip_out + n_internal);
+  for (int32_t k = 0; k < n_internal; ++k) {
+    const int32_t f = order_out[k];
+    owner_out[k] = lo[f];
+    neighbour_out[k] = hi[f];
+    flip_out[k] = fl[f];
+  }
   return PFM_OK;
 }
 

@@ -164,24 +164,32 @@ def relabel_cells(mesh: HexMesh, new_of_old: np.ndarray) -> HexMesh:
     """Apply a cell permutation (``new_of_old[old] = new``).
 
     Internal faces are re-oriented to owner < neighbour (vertex loop reversed
-    when swapped) and sorted by (owner, neighbour); boundary faces keep their
-    patch grouping and order and only get the new owner label.
+    when swapped) and sorted by (owner, neighbour) with ties in face order
+    (``pfm_relabel_faces``, two counting sorts on the host); boundary faces
+    keep their patch grouping and order and only get the new owner label.
     """
+    from .. import _lib
+
     new_of_old = np.asarray(new_of_old, dtype=np.int64)
     if new_of_old.shape != (mesh.n_cells,):
         raise ValueError("permutation has the wrong length")
-    nf_int = mesh.n_internal_faces
-    own = new_of_old[mesh.owner]
-    nei = new_of_old[mesh.neighbour]
+    nfi = mesh.n_internal_faces
+    o = _lib.host_i32(mesh.owner[:nfi])
+    q = _lib.host_i32(mesh.neighbour)
+    perm = _lib.host_i32(new_of_old)
+    order = np.empty(nfi, np.int32)
+    flip = np.empty(nfi, np.uint8)
+    own_i = np.empty(nfi, np.int32)
+    nei = np.empty(nfi, np.int32)
+    p = _lib.ptr
+    _lib.check(_lib.lib().pfm_relabel_faces(mesh.n_cells, nfi, p(o), p(q), p(perm), p(order),
+                                            p(flip), p(own_i), p(nei)), "pfm_relabel_faces")
     faces = mesh.faces.copy()
-    o_i = own[:nf_int].copy()
-    swap = o_i > nei
-    faces_int = faces[:nf_int]
-    faces_int[swap] = faces_int[swap][:, ::-1]
-    lo = np.where(swap, nei, o_i)
-    hi = np.where(swap, o_i, nei)
-    order = np.lexsort((hi, lo))
-    faces[:nf_int] = faces_int[order]
-    own[:nf_int] = lo[order]
-    return HexMesh(points=mesh.points, faces=faces, owner=own, neighbour=hi[order],
+    fi = mesh.faces[:nfi][order]
+    sw = flip.astype(bool)
+    fi[sw] = fi[sw][:, ::-1]
+    faces[:nfi] = fi
+    own = new_of_old[mesh.owner]
+    own[:nfi] = own_i
+    return HexMesh(points=mesh.points, faces=faces, owner=own, neighbour=nei.astype(np.int64),
                    n_cells=mesh.n_cells, patches=[dict(q) for q in mesh.patches])

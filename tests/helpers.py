@@ -7,12 +7,27 @@ import functools
 import numpy as np
 
 from oracle import ldu as oldu
+from oracle import mesh as omesh
 from pyopenfoam_amd.cases import poisson_case
+from pyopenfoam_amd.mesh.geometry import MeshGeometry
+
+
+def oracle_geometry(mesh, progress=None) -> MeshGeometry:
+    """Mesh geometry from the oracle's vectorised restatement (host)."""
+    g = omesh.geometry_vec(mesh.points, mesh.faces, mesh.owner, mesh.neighbour, mesh.n_cells)
+    return MeshGeometry(face_centres=g["face_centres"], face_areas=g["face_areas"],
+                        mag_sf=g["mag_sf"], cell_volumes=g["cell_volumes"],
+                        cell_centres=g["cell_centres"], delta=g["delta"])
+
+
+def make_case(n: int, **kw):
+    """``poisson_case`` with the oracle's geometry (no device needed)."""
+    return poisson_case(n, geometry=oracle_geometry, **kw)
 
 
 @functools.lru_cache(maxsize=16)
 def case(n: int, jitter: float = 0.0, sigma: float = 0.0, seed: int = 0):
-    return poisson_case(n, jitter=jitter, sigma=sigma, seed=seed)
+    return make_case(n, jitter=jitter, sigma=sigma, seed=seed)
 
 
 @functools.lru_cache(maxsize=16)

@@ -11,7 +11,8 @@ import pytest
 
 from oracle import mesh as omesh
 from pyopenfoam_amd import _lib
-from pyopenfoam_amd.cases import poisson_case, rcm_permutation
+from pyopenfoam_amd.cases import rcm_permutation
+from tests.helpers import make_case
 from pyopenfoam_amd.mesh.hexmesh import relabel_cells, structured_hex
 
 ROOT = Path(__file__).resolve().parent.parent
@@ -72,7 +73,7 @@ def _seq_dic_ok(n, own, nei, level):
 
 def test_dic_and_dilu_levels_respect_sequential_order():
     L = _lib.lib()
-    c = poisson_case(6, seed=4)
+    c = make_case(6, seed=4)
     own = _lib.host_i32(c.owner)
     nei = _lib.host_i32(c.neighbour)
     import ctypes as C
@@ -119,7 +120,7 @@ def test_gs_levels_are_the_sequential_dependency_depth(n, renumber, swap):
     """Forward level of i = 1 + max forward level of its neighbours j < i
     (0 if none); backward level likewise over neighbours j > i; for faces of
     either orientation."""
-    c = poisson_case(n, jitter=0.1, seed=n, renumber=renumber)
+    c = make_case(n, jitter=0.1, seed=n, renumber=renumber)
     own, nei = c.owner.astype(np.int32), c.neighbour.astype(np.int32)
     if swap:  # mixed orientation, as on agglomerated levels
         m = np.random.default_rng(0).random(len(own)) < 0.5
@@ -143,3 +144,66 @@ def test_gs_levels_are_the_sequential_dependency_depth(n, renumber, swap):
         eb[i] = max([eb[j] + 1 for j in adj[i] if j > i], default=0)
     assert np.array_equal(fl, ef) and np.array_equal(bl, eb)
     assert nf.value == ef.max() + 1 and nb.value == eb.max() + 1
+
+
+def _relabel_numpy(mesh, new_of_old):
+    """Restatement: relabel, swap so that owner < neighbour (reversing the
+    loop), stable sort by (owner, neighbour)."""
+    nfi = mesh.n_internal_faces
+    own = new_of_old[mesh.owner]
+    nei = new_of_old[mesh.neighbour]
+    faces = mesh.faces.copy()
+    o = own[:nfi].copy()
+    sw = o > nei
+    fi = faces[:nfi]
+    fi[sw] = fi[sw][:, ::-1]
+    lo, hi = np.where(sw, nei, o), np.where(sw, o, nei)
+    order = np.lexsort((hi, lo))
+    faces[:nfi] = fi[order]
+    own[:nfi] = lo[order]
+    return faces, own, hi[order]
+
+
+@pytest.mark.parametrize("n,seed", [(3, 0), (6, 1), (9, 2)])
+def test_relabel_faces_matches_restatement(n, seed):
+    m = structured_hex(n)
+    perm = np.random.default_rng(seed).permutation(m.n_cells)
+    r = relabel_cells(m, perm)
+    f, o, q = _relabel_numpy(m, perm)
+    np.testing.assert_array_equal(r.faces, f)
+    np.testing.assert_array_equal(r.owner, o)
+    np.testing.assert_array_equal(r.neighbour, q)
+    assert np.all(r.owner[: r.n_internal_faces] < r.neighbour)
+    # relabelling twice composes
+    p2 = np.random.default_rng(seed + 7).permutation(m.n_cells)
+    r2 = relabel_cells(r, p2)
+    f2, o2, q2 = _relabel_numpy(m, p2[perm])
+    np.testing.assert_array_equal(r2.owner, o2)
+    np.testing.assert_array_equal(r2.neighbour, q2)
+    np.testing.assert_array_equal(np.sort(r2.faces, axis=1), np.sort(f2, axis=1))
+
+
+def test_relabel_faces_rejects_bad_permutation():
+    m = structured_hex(2)
+    with pytest.raises(ValueError, match="permutation"):
+        relabel_cells(m, np.zeros(m.n_cells, np.int64))
+
+
+def test_cell_faces_order_and_bound():
+    from pyopenfoam_amd.mesh.geometry import cell_faces, face_csr
+    m = structured_hex(4)
+    m = relabel_cells(m, np.random.default_rng(3).permutation(m.n_cells))
+    fptr, fv = face_csr(m.faces)
+    cptr, cf, mx = cell_faces(m.n_cells, m.owner, m.neighbour, fptr)
+    nfi = m.n_internal_faces
+    fidx = np.concatenate([np.arange(m.n_faces), -np.arange(nfi) - 1])
+    cell = np.concatenate([m.owner, m.neighbour])
+    order = np.argsort(cell, kind="stable")
+    np.testing.assert_array_equal(cf, fidx[order])
+    np.testing.assert_array_equal(cptr, np.concatenate([[0], np.cumsum(np.bincount(cell))]))
+    assert mx == 24
+    poly = [list(f) for f in m.faces]
+    poly[0] = poly[0] + [int(m.points.shape[0]) - 1]  # one pentagon
+    fptr2, _ = face_csr(poly)
+    _, _, mx2 = cell_faces(m.n_cells, m.owner, m.neighbour, fptr2)
+    assert mx2 == 25

@@ -17,9 +17,8 @@ import torch
 from oracle import gamg as ogamg
 from oracle import ldu as oldu
 from pyopenfoam_amd import LduMatrix
-from pyopenfoam_amd.cases import poisson_case
 from pyopenfoam_amd.solvers import GaussSeidelSmoother, JacobiSmoother
-from tests.helpers import bits_equal, system
+from tests.helpers import bits_equal, make_case, system
 
 pytestmark = pytest.mark.gpu
 G = Path(__file__).resolve().parent / "golden"
@@ -47,7 +46,7 @@ def test_gauss_seidel_reference_golden(gpu_device):
 
 
 def _gs_case(n, renumber, swap, seed):
-    c = poisson_case(n, jitter=0.2, sigma=1.0, seed=seed, renumber=renumber)
+    c = make_case(n, jitter=0.2, sigma=1.0, seed=seed, renumber=renumber)
     nfi = c.mesh.n_internal_faces
     d, lo, up = oldu.laplacian(c.gamma, c.geo.mag_sf[:nfi], c.geo.delta[:nfi],
                                c.geo.cell_volumes, c.owner, c.neighbour)

@@ -9,6 +9,8 @@ norms) or to Python's set ordering (cell reference points of the geometry).
 
 from pathlib import Path
 
+from types import SimpleNamespace
+
 import numpy as np
 import pytest
 
@@ -19,7 +21,6 @@ from oracle import ldu as oldu
 from oracle import mesh as omesh
 from oracle import precond as opre
 from pyopenfoam_amd.cases import rcm_permutation
-from pyopenfoam_amd.mesh.geometry import compute_geometry
 from pyopenfoam_amd.mesh.hexmesh import HexMesh
 from tests.helpers import bits_equal
 
@@ -53,9 +54,8 @@ def test_geometry_golden(sysg):
         geo = omesh.geometry(g["points"], list(g["faces"]), g["owner"], g["neighbour"], n)
         for k in ("face_centres", "face_areas", "cell_volumes", "cell_centres", "delta"):
             np.testing.assert_allclose(geo[k], g[k], rtol=1e-13, atol=1e-16)
-    m = HexMesh(points=g["points"], faces=g["faces"], owner=g["owner"], neighbour=g["neighbour"],
-                n_cells=n)
-    pg = compute_geometry(m)
+    pg = SimpleNamespace(**omesh.geometry_vec(g["points"], g["faces"], g["owner"], g["neighbour"],
+                                              n))
     np.testing.assert_allclose(pg.face_centres, g["face_centres"], rtol=1e-14, atol=1e-16)
     # components that cancel to ~0 are compared against the face area scale
     np.testing.assert_allclose(pg.face_areas, g["face_areas"], rtol=1e-13,
@@ -351,3 +351,24 @@ def test_pbicg_oracle_pins():
                                    rel_tol=1e-14, max_iter=10)
     assert info2.converged and i2 <= 3
     np.testing.assert_allclose(x2, np.linalg.solve(A, [3.0, 3.0]), rtol=1e-12)
+
+
+@pytest.mark.parametrize("n,jit", [(3, 0.0), (4, 0.3), (5, 0.2)])
+def test_geometry_vec_matches_loop_restatement(n, jit):
+    """The vectorised geometry against the per-face / per-cell loops (both in
+    the reference's order except the centroid weighting, t*(S/4) against
+    (t*S)/4, a last-bit difference)."""
+    from pyopenfoam_amd.mesh.hexmesh import jitter_points, relabel_cells, structured_hex
+    rng = np.random.default_rng(n)
+    m = structured_hex(n)
+    if jit:
+        m = jitter_points(m, n, jit, rng)
+    m = relabel_cells(m, rng.permutation(m.n_cells))
+    a = omesh.geometry(m.points, list(m.faces), m.owner, m.neighbour, m.n_cells)
+    b = omesh.geometry_vec(m.points, m.faces, m.owner, m.neighbour, m.n_cells)
+    for k in ("face_centres", "face_areas", "cell_volumes", "cell_centres", "delta"):
+        scale = np.abs(a[k]).max()
+        np.testing.assert_allclose(b[k], a[k], rtol=1e-13, atol=1e-14 * scale, err_msg=k)
+    np.testing.assert_array_equal(b["face_centres"], a["face_centres"])
+    assert np.all(b["cell_volumes"] > 0)
+    np.testing.assert_allclose(b["cell_volumes"].sum(), 1.0, rtol=1e-12)  # unit cube
