@@ -458,6 +458,63 @@ int pfm_mesh_geometry(int32_t n_points, const double *points, int32_t n_faces,
                       double *face_areas, double *mag_sf, double *cell_volumes,
                       double *cell_centres, double *delta, void *stream);
 
+/* Linear-interpolation face weights, mesh/mesh_geometry.py:262-327:
+ * w = |x_f - C_N| / (|x_f - C_P| + |x_f - C_N|) on internal faces (norms in
+ * the order of torch's norm, the sum guarded at 1e-30), 1 on boundary faces.
+ * Device arrays; face_centres / cell_centres row-major (n, 3). */
+int pfm_face_weights(int32_t n_faces, int32_t n_internal, const int32_t *owner,
+                     const int32_t *neighbour, const double *face_centres,
+                     const double *cell_centres, double *weights, void *stream);
+
+/* ------------------------------------------------------------------ */
+/* The pressure-equation step around the solve                         */
+/* (solvers/pressure_equation.py).  Face arrays hold the handle's       */
+/* internal faces first, then n_faces - n_internal boundary faces;      */
+/* bnd_ptr (n_cells + 1) / bnd_face list, per cell and in face order,   */
+/* the boundary faces it owns (index b means face n_internal + b).      */
+/* 1/A_p is 1 / clamp(|A_p|, 1e-30); face sums run in the order of the  */
+/* reference's scatter_add / index_add_ (owner faces, neighbour faces,  */
+/* boundary faces, each ascending).                                     */
+/* ------------------------------------------------------------------ */
+
+/* assemble_pressure_equation (pressure_equation.py:49-152): lower/upper =
+ * -(1/A_p)_f |S_f| delta_f / V_{P,N} with (1/A_p)_f = w inv_P + (1-w) inv_N,
+ * diag = the sum of their negatives per row, source = -div(phiHbyA) / V
+ * (boundary faces included).  weights, mag_sf, delta: internal faces. */
+int pfm_pressure_assemble(pfm_ldu h, int32_t n_faces, const int32_t *bnd_ptr,
+                          const int32_t *bnd_face, const double *phi_hbya,
+                          const double *a_p, const double *weights,
+                          const double *mag_sf, const double *delta,
+                          const double *vol, double *diag, double *lower,
+                          double *upper, double *source, void *stream);
+
+/* correct_face_flux (pressure_equation.py:270-332): phi_out = phi +
+ * (1/A_p)_f (p_P - p_N) |S_f| delta_f on internal faces, phi elsewhere. */
+int pfm_correct_face_flux(pfm_ldu h, int32_t n_faces, const double *phi,
+                          const double *p, const double *a_p,
+                          const double *weights, const double *mag_sf,
+                          const double *delta, double *phi_out, void *stream);
+
+/* correct_velocity (pressure_equation.py:187-267): Gauss gradient of p with
+ * linearly interpolated face values (boundary faces: the owner value),
+ * divided by clamp(V, 1e-30); u = hbya - (alpha/A_p) grad p, rescaled to
+ * |u| = 1e4 where larger.  hbya, u_out, face_areas: row-major (n, 3). */
+int pfm_correct_velocity(pfm_ldu h, int32_t n_faces, const int32_t *bnd_ptr,
+                         const int32_t *bnd_face, const double *hbya,
+                         const double *p, const double *a_p,
+                         const double *weights, const double *face_areas,
+                         const double *vol, double alpha, double *u_out,
+                         void *stream);
+
+/* adjust_phi (pressure_equation.py:335-377): when closed and the net
+ * boundary flux is at least 1e-30 in magnitude, subtract net / n_boundary
+ * from every boundary face of phi (in place).  work: one device double.
+ * Synchronises the stream; *adjusted_out_h = 1 when phi changed,
+ * *net_out_h (optional) = the net flux (summed in a fixed device order). */
+int pfm_adjust_phi(int32_t n_faces, int32_t n_internal, double *phi,
+                   int closed, double *work, double *net_out_h,
+                   int32_t *adjusted_out_h, void *stream);
+
 /* fvm.laplacian (discretisation/operators.py:490-517) plus the
  * laplacianFoam Dirichlet diagonal term (applications/laplacian_foam.py:342-366):
  *   c_f = gamma_f*|S_f|*delta_f, lower_f = -c_f/V_P, upper_f = -c_f/V_N,

@@ -127,7 +127,7 @@ def _dot(a, b):
 
 def geometry_vec(points, faces, owner, neighbour, n_cells):
     """Vectorised ``geometry``; faces is an (n_faces, n_v) array.  Returns the
-    same dict plus ``mag_sf`` = sqrt(S.S)."""
+    same dict plus ``mag_sf`` = |S_f| in the order of torch's norm (FMA)."""
     pts = np.asarray(points, dtype=np.float64)
     faces = np.asarray(faces, dtype=np.int64)
     owner = np.asarray(owner, dtype=np.int64)
@@ -144,7 +144,8 @@ def geometry_vec(points, faces, owner, neighbour, n_cells):
         for j in range(nv):
             area = area + _cross(fv[:, j], fv[:, (j + 1) % nv])
     area = area * 0.5
-    mag = np.sqrt(_dot(area, area))
+    from .ldu import face_mag
+    mag = face_mag(area)  # the reference's torch norm: sqrt(fma(z,z,fma(y,y,x*x)))
 
     fidx = np.concatenate([np.arange(nf), np.arange(nfi)])
     cell = np.concatenate([owner, neighbour])

@@ -71,6 +71,11 @@ _SIGS = {
     "pfm_rcm": (INT, [I32, I32, P, P, P]),
     "pfm_cell_faces": (INT, [I32, I32, I32, P, P, P, P, P, P]),
     "pfm_relabel_faces": (INT, [I32, I32, P, P, P, P, P, P, P]),
+    "pfm_face_weights": (INT, [I32, I32, P, P, P, P, P, P]),
+    "pfm_pressure_assemble": (INT, [P, I32, P, P, P, P, P, P, P, P, P, P, P, P, P]),
+    "pfm_correct_face_flux": (INT, [P, I32, P, P, P, P, P, P, P, P]),
+    "pfm_correct_velocity": (INT, [P, I32, P, P, P, P, P, P, P, P, C.c_double, P, P]),
+    "pfm_adjust_phi": (INT, [I32, I32, P, INT, P, P, P, P]),
     "pfm_mesh_geometry": (INT, [I32, P, I32, P, P, I32, P, P, I32, P, P, I32, P, P, P, P, P, P, P]),
     "pfm_dic_levels": (INT, [I32, I32, P, P, P, P]),
     "pfm_dilu_levels": (INT, [I32, I32, P, P, P, P]),

@@ -8,6 +8,9 @@
 // operation (the file is built with -ffp-contract=off):
 //   face centre     sum of the vertices from 0, then / n_v          (:35-63)
 //   area vector     0.5 * sum_j v_j x v_{j+1}; zero for n_v < 3     (:66-108)
+//   |S_f|           sqrt(fma(z,z,fma(y,y,x*x))), the order of the torch
+//                   norm the reference applies to S_f (delta :384,
+//                   fvm.laplacian operators.py:490)
 //   cell reference  sum of the cell's distinct vertices in ascending
 //                   vertex order, / their count                     (:199-211)
 //   volume, centre  per face (in list order) and per edge j, the tet
@@ -54,6 +57,8 @@ __device__ inline V3 cross(V3 a, V3 b) {
   return {a.y * b.z - a.z * b.y, a.z * b.x - a.x * b.z, a.x * b.y - a.y * b.x};
 }
 __device__ inline double dot(V3 a, V3 b) { return a.x * b.x + a.y * b.y + a.z * b.z; }
+// |v| as torch's CPU norm over a row of 3 evaluates it: sqrt(fma(z,z,fma(y,y,x*x))).
+__device__ inline double norm_fma(V3 v) { return sqrt(fma(v.z, v.z, fma(v.y, v.y, v.x * v.x))); }
 
 __global__ __launch_bounds__(kGeoBlock) void k_face_geom(int32_t nf,
                                                           const int32_t *__restrict__ fptr,
@@ -81,7 +86,7 @@ __global__ __launch_bounds__(kGeoBlock) void k_face_geom(int32_t nf,
   }
   st3(fc, f, c);
   st3(sf, f, a);
-  mag[f] = sqrt(dot(a, a));
+  mag[f] = norm_fma(a);
 }
 
 __global__ __launch_bounds__(kGeoBlock) void k_cell_geom(int32_t nc,
@@ -188,6 +193,27 @@ __global__ __launch_bounds__(kGeoBlock) void k_face_delta(int32_t nf, int32_t nf
   delta[f] = 1.0 / (dn > 1e-30 ? dn : 1.0);
 }
 
+// w = |x_f - C_N| / (|x_f - C_P| + |x_f - C_N|) on internal faces (norms in
+// torch's order), 1 on boundary faces (:262-327).
+__global__ __launch_bounds__(kGeoBlock) void k_face_weights(int32_t nf, int32_t nfi,
+                                                             const int32_t *__restrict__ own,
+                                                             const int32_t *__restrict__ nei,
+                                                             const double *__restrict__ fc,
+                                                             const double *__restrict__ cc,
+                                                             double *__restrict__ w) {
+  const int32_t f = blockIdx.x * kGeoBlock + threadIdx.x;
+  if (f >= nf) return;
+  if (f >= nfi) {
+    w[f] = 1.0;
+    return;
+  }
+  const V3 x = ld3(fc, f);
+  const double dp = norm_fma(sub(x, ld3(cc, own[f])));
+  const double dn = norm_fma(sub(x, ld3(cc, nei[f])));
+  const double den = dp + dn;
+  w[f] = dn / (den > 1e-30 ? den : 1.0);
+}
+
 inline unsigned blocks(int64_t n) { return (unsigned)((n + kGeoBlock - 1) / kGeoBlock); }
 
 }  // namespace
@@ -236,6 +262,25 @@ int pfm_mesh_geometry(int32_t n_points, const double *points, int32_t n_faces,
   if (n_faces > 0)
     k_face_delta<<<blocks(n_faces), kGeoBlock, 0, s>>>(n_faces, n_internal, owner, neighbour,
                                                         face_areas, mag_sf, cell_centres, delta);
+  PFM_CHECK_HIP(hipGetLastError());
+  return PFM_OK;
+}
+
+int pfm_face_weights(int32_t n_faces, int32_t n_internal, const int32_t *owner,
+                     const int32_t *neighbour, const double *face_centres,
+                     const double *cell_centres, double *weights, void *stream) {
+  PFM_REQUIRE(n_faces >= 0 && n_internal >= 0 && n_internal <= n_faces,
+              "bad sizes (faces %d, internal %d)", n_faces, n_internal);
+  if (n_faces == 0) return PFM_OK;
+  PFM_NONNULL(weights);
+  if (n_internal > 0) {
+    PFM_NONNULL(owner);
+    PFM_NONNULL(neighbour);
+    PFM_NONNULL(face_centres);
+    PFM_NONNULL(cell_centres);
+  }
+  k_face_weights<<<blocks(n_faces), kGeoBlock, 0, (hipStream_t)stream>>>(
+      n_faces, n_internal, owner, neighbour, face_centres, cell_centres, weights);
   PFM_CHECK_HIP(hipGetLastError());
   return PFM_OK;
 }

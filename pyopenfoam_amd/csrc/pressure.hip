@@ -1,0 +1,323 @@
+// The pressure-equation step around the solve (solvers/pressure_equation.py):
+// assembly of laplacian(1/A_p, p) = div(phiHbyA) per unit volume, the face
+// flux and cell velocity corrections, and the closed-domain flux adjustment.
+//
+// Face sums follow the reference's scatter order: torch's scatter_add and
+// index_add_ on the CPU add a face's contribution to its cell in face order,
+// so a cell's sum runs over its owner faces (ascending), then its neighbour
+// faces (ascending), then its boundary faces (ascending) -- exactly the
+// lower slots, the upper slots and the boundary CSR of a row here.  Built
+// with -ffp-contract=off; 1/A_p = 1 / clamp(|A_p|, 1e-30) everywhere.
+
+#include "device_common.h"
+#include "pfm_internal.h"
+
+using namespace pfm;
+
+namespace {
+
+inline int grid1d(int64_t n) { return (int)((n + kBlock - 1) / kBlock); }
+
+// torch's clamp(min=lo) keeps NaN.
+__device__ inline double clamp_min(double x, double lo) { return x < lo ? lo : x; }
+__device__ inline double inv_ap(double a) { return 1.0 / clamp_min(fabs(a), 1e-30); }
+
+// Face-interpolated 1/A_p: w*inv_P + (1 - w)*inv_N (pressure_equation.py:102-111, 313).
+__device__ inline double inv_face(const double *__restrict__ ap, const double *__restrict__ w,
+                                  const int32_t *__restrict__ own, const int32_t *__restrict__ nei,
+                                  int f) {
+  const double wf = w[f];
+  return wf * inv_ap(ap[own[f]]) + (1.0 - wf) * inv_ap(ap[nei[f]]);
+}
+
+// lower = -c/V_P, upper = -c/V_N with c = (1/A_p)_f |S_f| delta_f (:117-129).
+__global__ __launch_bounds__(kBlock) void k_pe_faces(int nf, const int32_t *__restrict__ own,
+                                                      const int32_t *__restrict__ nei,
+                                                      const double *__restrict__ ap,
+                                                      const double *__restrict__ w,
+                                                      const double *__restrict__ mag,
+                                                      const double *__restrict__ delta,
+                                                      const double *__restrict__ vol,
+                                                      double *__restrict__ lower,
+                                                      double *__restrict__ upper) {
+  const int f = blockIdx.x * kBlock + threadIdx.x;
+  if (f >= nf) return;
+  const double c = (inv_face(ap, w, own, nei, f) * mag[f]) * delta[f];
+  lower[f] = (-c) / vol[own[f]];
+  upper[f] = (-c) / vol[nei[f]];
+}
+
+// Per row: diag = sum(c/V_P over owner faces) + sum(c/V_N over neighbour
+// faces) (c/V == -lower / -upper exactly) (:131-135); source =
+// ((0 + sum(-phi, owner faces)) + sum(phi, neighbour faces)
+//   + sum(-phi, boundary faces)) / V (:137-150).
+__global__ __launch_bounds__(kBlock) void k_pe_rows(int n, int nf, int nslices,
+                                                     const SliceInfo *__restrict__ sl,
+                                                     const int32_t *__restrict__ map,
+                                                     const double *__restrict__ lower,
+                                                     const double *__restrict__ upper,
+                                                     const double *__restrict__ phi,
+                                                     const int32_t *__restrict__ bptr,
+                                                     const int32_t *__restrict__ bface,
+                                                     const double *__restrict__ vol,
+                                                     double *__restrict__ diag,
+                                                     double *__restrict__ source) {
+  const int s = xcd_remap(blockIdx.x, gridDim.x) * kWavesPerBlock + (threadIdx.x >> 6);
+  if (s >= nslices) return;
+  const SliceInfo si = load_slice(sl, s);
+  const int lane = threadIdx.x & 63;
+  const int row = s * kSlice + lane;
+  double d1 = 0.0, d2 = 0.0, t1 = 0.0, t2 = 0.0, t3 = 0.0;
+  int base = si.ptr + lane;
+  for (int j = 0; j < si.wl; ++j, base += kSlice) {
+    const int f = map[base];
+    if (f >= 0) {
+      d1 += -lower[f];
+      t1 += -phi[f];
+    }
+  }
+  for (int j = 0; j < si.wu; ++j, base += kSlice) {
+    const int f = map[base];
+    if (f >= 0) {
+      d2 += -upper[f];
+      t2 += phi[f];
+    }
+  }
+  if (row < n) {
+    for (int k = bptr[row]; k < bptr[row + 1]; ++k) t3 += -phi[nf + bface[k]];
+    diag[row] = d1 + d2;
+    source[row] = (((0.0 + t1) + t2) + t3) / vol[row];
+  }
+}
+
+// phi_out = phi + ((((1/A_p)_f * (p_P - p_N)) * |S_f|) * delta_f) on internal
+// faces, phi on boundary faces (:305-330).
+__global__ __launch_bounds__(kBlock) void k_cff(int nf_all, int nf,
+                                                 const int32_t *__restrict__ own,
+                                                 const int32_t *__restrict__ nei,
+                                                 const double *__restrict__ phi,
+                                                 const double *__restrict__ p,
+                                                 const double *__restrict__ ap,
+                                                 const double *__restrict__ w,
+                                                 const double *__restrict__ mag,
+                                                 const double *__restrict__ delta,
+                                                 double *__restrict__ out) {
+  const int f = blockIdx.x * kBlock + threadIdx.x;
+  if (f >= nf_all) return;
+  if (f >= nf) {
+    out[f] = phi[f];
+    return;
+  }
+  const double dp = p[own[f]] - p[nei[f]];
+  out[f] = phi[f] + (((inv_face(ap, w, own, nei, f) * dp) * mag[f]) * delta[f]);
+}
+
+// Gauss gradient with linearly interpolated face pressure (boundary faces
+// take the owner value), one running sum per cell in index_add_ order; then
+// U = HbyA - (alpha/A_p) grad p, limited to |U| <= 1e4 (:228-265).
+__global__ __launch_bounds__(kBlock) void k_cvel(int n, int nf, int nslices,
+                                                  const SliceInfo *__restrict__ sl,
+                                                  const int32_t *__restrict__ map,
+                                                  const int32_t *__restrict__ own,
+                                                  const int32_t *__restrict__ nei,
+                                                  const double *__restrict__ w,
+                                                  const double *__restrict__ p,
+                                                  const double *__restrict__ sf,
+                                                  const int32_t *__restrict__ bptr,
+                                                  const int32_t *__restrict__ bface,
+                                                  const double *__restrict__ vol,
+                                                  const double *__restrict__ ap,
+                                                  const double *__restrict__ hbya, double alpha,
+                                                  double *__restrict__ u) {
+  const int s = xcd_remap(blockIdx.x, gridDim.x) * kWavesPerBlock + (threadIdx.x >> 6);
+  if (s >= nslices) return;
+  const SliceInfo si = load_slice(sl, s);
+  const int lane = threadIdx.x & 63;
+  const int row = s * kSlice + lane;
+  double gx = 0.0, gy = 0.0, gz = 0.0;
+  int base = si.ptr + lane;
+  for (int j = 0; j < si.wl + si.wu; ++j, base += kSlice) {
+    const int f = map[base];
+    if (f < 0) continue;
+    const double wf = w[f];
+    const double pf = wf * p[own[f]] + (1.0 - wf) * p[nei[f]];
+    const double cx = pf * sf[3 * f], cy = pf * sf[3 * f + 1], cz = pf * sf[3 * f + 2];
+    if (j < si.wl) {  // owner side: + p_f S_f
+      gx += cx;
+      gy += cy;
+      gz += cz;
+    } else {          // neighbour side: + (-p_f S_f)
+      gx += -cx;
+      gy += -cy;
+      gz += -cz;
+    }
+  }
+  if (row >= n) return;
+  const double pp = p[row];
+  for (int k = bptr[row]; k < bptr[row + 1]; ++k) {
+    const int64_t b = (int64_t)nf + bface[k];
+    gx += pp * sf[3 * b];
+    gy += pp * sf[3 * b + 1];
+    gz += pp * sf[3 * b + 2];
+  }
+  const double v = clamp_min(vol[row], 1e-30);
+  gx = gx / v;
+  gy = gy / v;
+  gz = gz / v;
+  const double a = alpha * inv_ap(ap[row]);
+  double ux = hbya[3 * row] - a * gx;
+  double uy = hbya[3 * row + 1] - a * gy;
+  double uz = hbya[3 * row + 2] - a * gz;
+  const double m = clamp_min(sqrt(fma(uz, uz, fma(uy, uy, ux * ux))), 1e-30);
+  if (m > 1e4) {
+    const double sc = 1e4 / m;
+    ux = ux * sc;
+    uy = uy * sc;
+    uz = uz * sc;
+  }
+  u[3 * row] = ux;
+  u[3 * row + 1] = uy;
+  u[3 * row + 2] = uz;
+}
+
+constexpr int kAdjThreads = 1024;
+
+// Net boundary flux, one workgroup: fixed strided partial sums, then a fixed
+// tree (deterministic; the reference leaves the order to torch's sum).
+__global__ __launch_bounds__(kAdjThreads) void k_adj_sum(int nb, const double *__restrict__ phib,
+                                                          double *__restrict__ net) {
+  __shared__ double red[kAdjThreads];
+  double s = 0.0;
+  for (int k = threadIdx.x; k < nb; k += kAdjThreads) s += phib[k];
+  red[threadIdx.x] = s;
+  __syncthreads();
+  for (int h = kAdjThreads / 2; h > 0; h >>= 1) {
+    if ((int)threadIdx.x < h) red[threadIdx.x] += red[threadIdx.x + h];
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) net[0] = red[0];
+}
+
+// phi_b -= net / n_b when |net| >= 1e-30 (:366-375).
+__global__ __launch_bounds__(kBlock) void k_adj_apply(int nb, double *__restrict__ phib,
+                                                       const double *__restrict__ net) {
+  const int k = blockIdx.x * kBlock + threadIdx.x;
+  const double t = net[0];
+  if (k >= nb || fabs(t) < 1e-30) return;
+  phib[k] = phib[k] - t / (double)nb;
+}
+
+int check_bnd(const pfm_ldu_s *h, int32_t n_faces, const int32_t *bptr, const int32_t *bface) {
+  PFM_REQUIRE(n_faces >= h->nf, "n_faces (%d) < internal faces of the handle (%d)", n_faces,
+              h->nf);
+  PFM_REQUIRE(h->n == 0 || bptr, "null boundary pointer array");
+  PFM_REQUIRE(n_faces == h->nf || bface, "null boundary face list");
+  return PFM_OK;
+}
+
+}  // namespace
+
+extern "C" {
+
+int pfm_pressure_assemble(pfm_ldu h, int32_t n_faces, const int32_t *bnd_ptr,
+                          const int32_t *bnd_face, const double *phi_hbya, const double *a_p,
+                          const double *weights, const double *mag_sf, const double *delta,
+                          const double *vol, double *diag, double *lower, double *upper,
+                          double *source, void *stream) {
+  PFM_NONNULL(h);
+  int rc = check_bnd(h, n_faces, bnd_ptr, bnd_face);
+  if (rc != PFM_OK) return rc;
+  if (h->n == 0) return PFM_OK;
+  PFM_NONNULL(phi_hbya);
+  PFM_NONNULL(a_p);
+  PFM_NONNULL(vol);
+  PFM_NONNULL(diag);
+  PFM_NONNULL(source);
+  if (h->nf > 0) {
+    PFM_NONNULL(weights);
+    PFM_NONNULL(mag_sf);
+    PFM_NONNULL(delta);
+    PFM_NONNULL(lower);
+    PFM_NONNULL(upper);
+  }
+  hipStream_t s = (hipStream_t)stream;
+  if (h->nf > 0)
+    k_pe_faces<<<grid1d(h->nf), kBlock, 0, s>>>(h->nf, h->d_own, h->d_nei, a_p, weights, mag_sf,
+                                                 delta, vol, lower, upper);
+  k_pe_rows<<<blocks_for_slices(h->nslices), kBlock, 0, s>>>(
+      h->n, h->nf, h->nslices, h->d_slices, h->d_map, lower, upper, phi_hbya, bnd_ptr, bnd_face,
+      vol, diag, source);
+  PFM_CHECK_HIP(hipGetLastError());
+  return PFM_OK;
+}
+
+int pfm_correct_face_flux(pfm_ldu h, int32_t n_faces, const double *phi, const double *p,
+                          const double *a_p, const double *weights, const double *mag_sf,
+                          const double *delta, double *phi_out, void *stream) {
+  PFM_NONNULL(h);
+  PFM_REQUIRE(n_faces >= h->nf, "n_faces (%d) < internal faces of the handle (%d)", n_faces,
+              h->nf);
+  if (n_faces == 0) return PFM_OK;
+  PFM_NONNULL(phi);
+  PFM_NONNULL(phi_out);
+  if (h->nf > 0) {
+    PFM_NONNULL(p);
+    PFM_NONNULL(a_p);
+    PFM_NONNULL(weights);
+    PFM_NONNULL(mag_sf);
+    PFM_NONNULL(delta);
+  }
+  hipStream_t s = (hipStream_t)stream;
+  k_cff<<<grid1d(n_faces), kBlock, 0, s>>>(n_faces, h->nf, h->d_own, h->d_nei, phi, p, a_p,
+                                            weights, mag_sf, delta, phi_out);
+  PFM_CHECK_HIP(hipGetLastError());
+  return PFM_OK;
+}
+
+int pfm_correct_velocity(pfm_ldu h, int32_t n_faces, const int32_t *bnd_ptr,
+                         const int32_t *bnd_face, const double *hbya, const double *p,
+                         const double *a_p, const double *weights, const double *face_areas,
+                         const double *vol, double alpha, double *u_out, void *stream) {
+  PFM_NONNULL(h);
+  int rc = check_bnd(h, n_faces, bnd_ptr, bnd_face);
+  if (rc != PFM_OK) return rc;
+  if (h->n == 0) return PFM_OK;
+  PFM_NONNULL(hbya);
+  PFM_NONNULL(p);
+  PFM_NONNULL(a_p);
+  PFM_NONNULL(vol);
+  PFM_NONNULL(u_out);
+  if (n_faces > 0) PFM_NONNULL(face_areas);
+  if (h->nf > 0) PFM_NONNULL(weights);
+  hipStream_t s = (hipStream_t)stream;
+  k_cvel<<<blocks_for_slices(h->nslices), kBlock, 0, s>>>(
+      h->n, h->nf, h->nslices, h->d_slices, h->d_map, h->d_own, h->d_nei, weights, p,
+      face_areas, bnd_ptr, bnd_face, vol, a_p, hbya, alpha, u_out);
+  PFM_CHECK_HIP(hipGetLastError());
+  return PFM_OK;
+}
+
+int pfm_adjust_phi(int32_t n_faces, int32_t n_internal, double *phi, int closed, double *work,
+                   double *net_out_h, int32_t *adjusted_out_h, void *stream) {
+  PFM_REQUIRE(n_faces >= 0 && n_internal >= 0 && n_internal <= n_faces,
+              "bad sizes (faces %d, internal %d)", n_faces, n_internal);
+  PFM_REQUIRE(adjusted_out_h, "null output");
+  *adjusted_out_h = 0;
+  if (net_out_h) *net_out_h = 0.0;
+  const int nb = n_faces - n_internal;
+  if (nb == 0 || !closed) return PFM_OK;
+  PFM_NONNULL(phi);
+  PFM_NONNULL(work);
+  hipStream_t s = (hipStream_t)stream;
+  k_adj_sum<<<1, kAdjThreads, 0, s>>>(nb, phi + n_internal, work);
+  k_adj_apply<<<grid1d(nb), kBlock, 0, s>>>(nb, phi + n_internal, work);
+  PFM_CHECK_HIP(hipGetLastError());
+  double net = 0.0;
+  PFM_CHECK_HIP(hipMemcpyAsync(&net, work, sizeof(double), hipMemcpyDeviceToHost, s));
+  PFM_CHECK_HIP(hipStreamSynchronize(s));
+  if (net_out_h) *net_out_h = net;
+  *adjusted_out_h = fabs(net) < 1e-30 ? 0 : 1;
+  return PFM_OK;
+}
+
+}  // extern "C"

@@ -103,6 +103,56 @@ class FvMesh:
     def delta_coefficients(self) -> torch.Tensor:
         return self._tensor("delta", lambda: self.geometry.delta, CFD_DTYPE)
 
+    @property
+    def face_weights(self) -> torch.Tensor:
+        """Linear-interpolation weights (n_faces,), ``pfm_face_weights``
+        (mesh_geometry.py:262-327): |x_f - C_N| / (|x_f - C_P| + |x_f - C_N|),
+        1 on boundary faces."""
+        if "w" not in self._t:
+            from .. import _lib
+            from ..core.device import current_stream
+            nf, nfi = self.n_faces, self.n_internal_faces
+            w = torch.empty(nf, dtype=CFD_DTYPE, device=self._device)
+            own, nei = self.owner_i32, self.neighbour_i32
+            _lib.check(_lib.lib().pfm_face_weights(
+                nf, nfi, _lib.ptr(own), _lib.ptr(nei), _lib.ptr(self.face_centres.contiguous()),
+                _lib.ptr(self.cell_centres.contiguous()), _lib.ptr(w),
+                current_stream(self._device)), "pfm_face_weights")
+            self._t["w"] = w
+        return self._t["w"]
+
+    @property
+    def owner_i32(self) -> torch.Tensor:
+        return self._tensor("own32", lambda: self._mesh.owner, torch.int32)
+
+    @property
+    def neighbour_i32(self) -> torch.Tensor:
+        return self._tensor("nei32", lambda: self._mesh.neighbour, torch.int32)
+
+    @property
+    def boundary_csr(self) -> tuple[torch.Tensor, torch.Tensor]:
+        """(bnd_ptr (n_cells + 1), bnd_face): per cell, in face order, the
+        boundary faces it owns as offsets past the internal faces (int32)."""
+        if "bcsr" not in self._t:
+            nfi = self.n_internal_faces
+            bo = np.asarray(self._mesh.owner[nfi:], dtype=np.int64)
+            order = np.argsort(bo, kind="stable").astype(np.int32)
+            ptr = np.zeros(self.n_cells + 1, np.int64)
+            np.cumsum(np.bincount(bo, minlength=self.n_cells), out=ptr[1:])
+            self._t["bcsr"] = (torch.from_numpy(ptr.astype(np.int32)).to(self._device),
+                               torch.from_numpy(order).to(self._device))
+        return self._t["bcsr"]
+
+    def ldu_addressing(self):
+        """The native LDU topology of the internal faces, shared by every
+        matrix assembled on this mesh."""
+        if not hasattr(self, "_ldu_addr"):
+            from ..core.addressing import addressing_for
+            nfi = self.n_internal_faces
+            self._ldu_addr = addressing_for(self.n_cells, self.owner[:nfi], self.neighbour,
+                                            self._device)
+        return self._ldu_addr
+
     def patch_faces(self, name: str) -> np.ndarray:
         p = self._mesh.patch(name)
         return np.arange(p["start"], p["start"] + p["size"])

@@ -372,3 +372,106 @@ def test_geometry_vec_matches_loop_restatement(n, jit):
     np.testing.assert_array_equal(b["face_centres"], a["face_centres"])
     assert np.all(b["cell_volumes"] > 0)
     np.testing.assert_allclose(b["cell_volumes"].sum(), 1.0, rtol=1e-12)  # unit cube
+
+
+def _pe_case(n, jit, seed=0):
+    from tests.helpers import make_case
+    c = make_case(n, jitter=jit, sigma=0.0, seed=seed)
+    nfi = c.mesh.n_internal_faces
+    return c, nfi, c.geo
+
+
+def test_pressure_oracle_matrix_pins():
+    """w = 0.5 and A_p = 1 reduce the pressure matrix to the Laplacian with
+    D = 1 bit for bit; w = 1 (owner only) to D = 1/|A_P|: a swapped
+    interpolation direction or a missing abs fails these."""
+    from oracle import pressure as opr
+    c, nfi, g = _pe_case(6, 0.2)
+    own, nei = c.owner, c.neighbour
+    mag, dl, vol = g.mag_sf[:nfi], g.delta[:nfi], g.cell_volumes
+    phi = np.zeros(c.mesh.n_faces)
+    d0, l0, u0 = oldu.laplacian(np.ones(nfi), mag, dl, vol, own, nei)
+    d, lo, up, src = opr.assemble(phi, np.ones(c.n_cells), np.full(c.mesh.n_faces, 0.5), mag, dl,
+                                  vol, c.mesh.owner, nei)
+    assert bits_equal(lo, l0) and bits_equal(up, u0) and bits_equal(d, d0)
+    assert not src.any()
+    ap = np.random.default_rng(1).uniform(-3.0, 3.0, c.n_cells)
+    d1, l1, u1 = oldu.laplacian(1.0 / np.abs(ap)[own], mag, dl, vol, own, nei)
+    d, lo, up, _ = opr.assemble(phi, ap, np.ones(c.mesh.n_faces), mag, dl, vol, c.mesh.owner, nei)
+    assert bits_equal(lo, l1) and bits_equal(up, u1) and bits_equal(d, d1)
+
+
+def test_pressure_oracle_source_pins():
+    from oracle import pressure as opr
+    c, nfi, g = _pe_case(5, 0.2)
+    nfa = c.mesh.n_faces
+    own_all, nei = c.mesh.owner, c.neighbour
+    args = (np.ones(c.n_cells), np.full(nfa, 0.5), g.mag_sf[:nfi], g.delta[:nfi], g.cell_volumes,
+            own_all, nei)
+    # a uniform velocity has zero divergence: sum over each closed cell of S.U0
+    u0 = np.array([0.3, -1.1, 0.7])
+    phi = g.face_areas @ u0
+    src = opr.assemble(phi, *args)[3]
+    assert np.abs(src).max() <= 1e-12 * np.abs(phi).max() / g.cell_volumes.min()
+    # unit flux on one internal face leaves its owner and enters its neighbour
+    f = nfi // 3
+    phi = np.zeros(nfa)
+    phi[f] = 1.0
+    src = opr.assemble(phi, *args)[3]
+    exp = np.zeros(c.n_cells)
+    exp[own_all[f]] = -1.0 / g.cell_volumes[own_all[f]]
+    exp[nei[f]] = 1.0 / g.cell_volumes[nei[f]]
+    np.testing.assert_array_equal(src, exp)
+    # ... and on a boundary face it leaves its owner
+    phi = np.zeros(nfa)
+    phi[nfa - 2] = 2.0
+    src = opr.assemble(phi, *args)[3]
+    assert src[own_all[nfa - 2]] == -2.0 / g.cell_volumes[own_all[nfa - 2]]
+    assert np.count_nonzero(src) == 1
+
+
+def test_pressure_oracle_corrections_pins():
+    """Uniform orthogonal mesh, p = g.x: the flux correction is -(1/A_p) g.S_f
+    on internal faces; the Gauss gradient is g in interior cells, so U =
+    HbyA - (alpha/A_p) g there; the limiter caps |U| at 1e4."""
+    from oracle import pressure as opr
+    c, nfi, geo = _pe_case(6, 0.0)
+    own_all, nei = c.mesh.owner, c.neighbour
+    w = opr.face_weights(geo.face_centres, geo.cell_centres, own_all, nei)
+    np.testing.assert_allclose(w[:nfi], 0.5, rtol=1e-14)
+    assert np.all(w[nfi:] == 1.0)
+    gvec = np.array([0.4, -1.3, 2.1])
+    p = geo.cell_centres @ gvec
+    ap = np.full(c.n_cells, 4.0)
+    phi = np.random.default_rng(2).normal(size=c.mesh.n_faces)
+    out = opr.correct_face_flux(phi, p, ap, w, geo.mag_sf[:nfi], geo.delta[:nfi], own_all, nei)
+    corr = -(geo.face_areas[:nfi] @ gvec) / 4.0
+    np.testing.assert_allclose(out[:nfi] - phi[:nfi], corr, rtol=1e-9, atol=1e-13)
+    assert np.array_equal(out[nfi:], phi[nfi:])
+    hb = np.random.default_rng(3).normal(size=(c.n_cells, 3))
+    u = opr.correct_velocity(hb, p, ap, w, geo.face_areas, geo.cell_volumes, own_all, nei,
+                             alpha=0.5)
+    cc = geo.cell_centres
+    h = 1.0 / 6
+    interior = np.all((cc > h) & (cc < 1 - h), axis=1)
+    assert interior.sum() == 64
+    np.testing.assert_allclose(u[interior], hb[interior] - 0.5 / 4.0 * gvec, rtol=1e-10,
+                               atol=1e-12)
+    big = np.zeros((c.n_cells, 3))
+    big[:, 0] = 3e4
+    u = opr.correct_velocity(big, np.zeros(c.n_cells), ap, w, geo.face_areas, geo.cell_volumes,
+                             own_all, nei)
+    np.testing.assert_allclose(u[:, 0], 1e4, rtol=1e-15)
+
+
+def test_pressure_oracle_adjust_phi_pins():
+    from oracle import pressure as opr
+    rng = np.random.default_rng(4)
+    phi = rng.normal(size=40)
+    adj, out = opr.adjust_phi(phi, 30)
+    assert adj and abs(out[30:].sum()) < 1e-14 and np.array_equal(out[:30], phi[:30])
+    phi2 = phi.copy()
+    phi2[30:] = 0.0
+    assert opr.adjust_phi(phi2, 30)[0] is False
+    assert opr.adjust_phi(phi, 30, closed=False)[0] is False
+    assert opr.adjust_phi(phi[:30], 30)[0] is False
