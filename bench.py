@@ -264,6 +264,68 @@ def bench_geometry(case, dev, no_cpu: bool):
     return out
 
 
+def bench_pressure(case, mesh, dev):
+    """The pressure-equation step around the solve at the case's size: HIP
+    event times (5 launches each) and GB/s on algorithmic bytes (every input
+    read once, every output written once):
+      assemble      56 B/internal face (own, nei, w, |S|, delta, phi; lower,
+                    upper) + 32 B/cell (A_p, V; diag, source) + 12 B/boundary face
+      face flux     48 B/face + 16 B/cell
+      velocity      40 B/face (own, nei, w, S) + 72 B/cell (p, A_p, V, HbyA, U)
+                    + 28 B/boundary face"""
+    import numpy as np
+    import torch
+
+    from pyopenfoam_amd import _lib
+    from pyopenfoam_amd.core.device import current_stream
+    from pyopenfoam_amd.solvers import (assemble_pressure_equation, correct_face_flux,
+                                        correct_velocity)
+
+    n, nfi, nfa = mesh.n_cells, mesh.n_internal_faces, mesh.n_faces
+    nb = nfa - nfi
+    rng = np.random.default_rng(11)
+    t = lambda a: torch.from_numpy(a).to(dev)
+    ap, phi, p = t(rng.uniform(1, 2, n)), t(rng.normal(size=nfa)), t(rng.normal(size=n))
+    hb = t(rng.normal(size=(n, 3)))
+    mesh.face_weights  # built once, outside the timings
+    mat = assemble_pressure_equation(phi, ap, mesh)
+    L, q = _lib.lib(), _lib.ptr
+    h = mesh.ldu_addressing().handle
+    w = mesh.face_weights[:nfi].contiguous()
+    mag = mesh.face_areas_magnitude[:nfi].contiguous()
+    dl = mesh.delta_coefficients[:nfi].contiguous()
+    sf = mesh.face_areas.contiguous()
+    vol = mesh.cell_volumes
+    bptr, bface = mesh.boundary_csr
+    s = current_stream(dev)
+    out_f = torch.empty_like(phi)
+    out_u = torch.empty_like(hb)
+    calls = {
+        "assemble": (lambda: L.pfm_pressure_assemble(
+            h, nfa, q(bptr), q(bface), q(phi), q(ap), q(w), q(mag), q(dl), q(vol), q(mat.diag),
+            q(mat.lower), q(mat.upper), q(mat.source), s), 56 * nfi + 32 * n + 12 * nb),
+        "correct_face_flux": (lambda: L.pfm_correct_face_flux(
+            h, nfa, q(phi), q(p), q(ap), q(w), q(mag), q(dl), q(out_f), s), 48 * nfi + 16 * n),
+        "correct_velocity": (lambda: L.pfm_correct_velocity(
+            h, nfa, q(bptr), q(bface), q(hb), q(p), q(ap), q(w), q(sf), q(vol), 0.7, q(out_u), s),
+            40 * nfi + 72 * n + 28 * nb),
+    }
+    out = {}
+    for name, (fn, nbytes) in calls.items():
+        _lib.check(fn(), name)
+        e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+        e0.record()
+        for _ in range(5):
+            fn()
+        e1.record()
+        torch.cuda.synchronize()
+        us = e0.elapsed_time(e1) / 5 * 1e3
+        out[name] = {"us": round(us, 2), "algorithmic_bytes": int(nbytes),
+                     "gbs": round(nbytes / (us * 1e-6) / 1e9, 1)}
+        log(f"pressure step {name}: {us:.1f} us, {nbytes / (us * 1e-6) / 1e9:.0f} GB/s")
+    return out
+
+
 def main() -> int:
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -281,6 +343,7 @@ def main() -> int:
     ap.add_argument("--history", default=None, help="write the residual history (.npy) here")
     ap.add_argument("--no-batch", action="store_true")
     ap.add_argument("--no-gamg", action="store_true")
+    ap.add_argument("--no-pressure", action="store_true")
     ap.add_argument("--gamg-max-iter", type=int, default=1000)
     ap.add_argument("--batch-systems", type=int, default=512)
     ap.add_argument("--batch-nmin", type=int, default=8)
@@ -318,6 +381,7 @@ def main() -> int:
     log(f"case {n}^3: {N} cells, {F} faces, built in {t_case:.1f} s")
     geometry = bench_geometry(case, dev, args.no_cpu)
     mesh = FvMesh.from_case(case, dev)
+    pressure = bench_pressure(case, mesh, dev) if not args.no_pressure else None
     t0 = time.perf_counter()
     mat = assemble(case, mesh)
     torch.cuda.synchronize()
@@ -556,6 +620,7 @@ def main() -> int:
         "cpu_baseline": cpu,
         "setup_s": {"case_host": round(t_case, 2), "handle_and_assembly": round(t_asm, 2)},
         "geometry": geometry,
+        "pressure_step": pressure,
     }
     print(json.dumps(line), flush=True)
     return 0

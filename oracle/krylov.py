@@ -52,6 +52,68 @@ def dot_pairwise(a, b):
 DOTS = {"blas": dot_blas, "exact": dot_exact, "pairwise": dot_pairwise}
 
 
+def dot_device(n, *, max_red_blocks=2048, block=256, slice_rows=64, xcds=8):
+    """The summation order of the device solvers' fused dot products, so that
+    an oracle solve can be compared with a device solve bit for bit.  This is
+    one more summation order (the reference leaves the order to torch), set
+    by the device kernels' structure:
+
+    * G = min(ceil(n_slices / waves), max_red_blocks) workgroups of 4 waves;
+      physical workgroup b works as logical block lb = (b % 8) * (G // 8) +
+      b // 8 (b < 8 * (G // 8), else lb = b);
+    * wave w of logical block lb walks slices lb*4 + w + k*4G, k = 0, 1, ...;
+      lane l adds the product of row 64 s + l (rows < n) to its running sum
+      from 0;
+    * each wave is reduced by the butterfly v += v[l ^ off], off = 32 .. 1,
+      lane 0's value taken; the 4 waves are added in order (partial of b);
+    * the G partials: thread t sums part[t + 256 k], k = 0..7, from 0, then
+      the same wave butterfly and in-order wave sum over 256 threads.
+    """
+    waves = block // slice_rows
+    nsl = -(-n // slice_rows)
+    G = max(1, min(-(-nsl // waves), max_red_blocks))
+    per = G // xcds
+    b = np.arange(G)
+    lb = np.where(b < per * xcds, (b % xcds) * per + b // xcds, b)
+    steps = max(1, -(-nsl // (G * waves)))
+    s = (lb[:, None, None] * waves + np.arange(waves)[None, :, None]
+         + np.arange(steps)[None, None, :] * (G * waves))        # (G, waves, steps)
+    rows = s[:, :, None, :] * slice_rows + np.arange(slice_rows)[None, None, :, None]
+    valid = (s[:, :, None, :] < nsl) & (rows < n)
+    idx = np.where(valid, rows, n)                                 # n -> a 0.0 pad
+    lanes = np.arange(slice_rows)
+    kthreads = max_red_blocks // block
+    tl = np.arange(block)
+
+    def tree(v):  # (..., 64) -> (...,): the wave butterfly, lane 0
+        for off in (32, 16, 8, 4, 2, 1):
+            v = v + v[..., lanes ^ off]
+        return v[..., 0]
+
+    def dot(a, c):
+        pr = np.append(np.asarray(a, dtype=np.float64) * np.asarray(c, dtype=np.float64), 0.0)
+        vals = pr[idx]                                             # (G, waves, 64, steps)
+        acc = np.zeros(vals.shape[:3])
+        for k in range(steps):
+            acc = acc + vals[..., k]
+        wv = tree(acc)                                             # (G, waves)
+        part = wv[:, 0]
+        for w in range(1, waves):
+            part = part + wv[:, w]
+        pp = np.zeros(block * kthreads)
+        pp[:G] = part
+        t = np.zeros(block)
+        for k in range(kthreads):
+            t = t + pp[tl + k * block]
+        wt = tree(t.reshape(waves, slice_rows))
+        out = wt[0]
+        for w in range(1, waves):
+            out = out + wt[w]
+        return float(out)
+
+    return dot
+
+
 class Monitor:
     def __init__(self, tolerance=1e-6, rel_tol=0.01, min_iter=0, dot=dot_blas):
         self.tol, self.rel_tol, self.min_iter = tolerance, rel_tol, min_iter

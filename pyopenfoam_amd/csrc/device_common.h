@@ -76,11 +76,15 @@ __device__ __forceinline__ SliceRange block_slices(int nslices) {
 // (diag*x + sum_lower) + sum_upper, each sum in face order from 0, which is
 // the association of the reference's Ax (diag*x, then scatter of the owner
 // side, then scatter of the neighbour side).
-template <typename T>
-__device__ __forceinline__ T row_spmv(const SliceInfo si, int lane, T dx,
-                                      const int32_t *__restrict__ col,
-                                      const T *__restrict__ val,
-                                      const T *__restrict__ x) {
+// `gx(c)` supplies the x value of column c (a plain load, or a value formed
+// where it is read).  kUpperFirst returns (diag*x + sum_upper) + sum_lower,
+// the association of the reference's Ax_T (ldu_matrix.py:248-257: the
+// neighbour-side scatter first) when the packed values are the transposed
+// ones.
+template <typename T, typename Gather, bool kUpperFirst = false>
+__device__ __forceinline__ T row_spmv_g(const SliceInfo si, int lane, T dx,
+                                        const int32_t *__restrict__ col,
+                                        const T *__restrict__ val, Gather gx) {
   constexpr int U = 4;
   T accL = T(0), accU = T(0);
   const int wl = si.wl, wu = si.wu;
@@ -100,14 +104,14 @@ __device__ __forceinline__ T row_spmv(const SliceInfo si, int lane, T dx,
     }
     T xv[2 * U];
 #pragma unroll
-    for (int j = 0; j < 2 * U; ++j) xv[j] = c[j] >= 0 ? x[c[j]] : T(0);
+    for (int j = 0; j < 2 * U; ++j) xv[j] = c[j] >= 0 ? gx(c[j]) : T(0);
 #pragma unroll
     for (int j = 0; j < U; ++j)
       if (c[j] >= 0) accL += v[j] * xv[j];
 #pragma unroll
     for (int j = 0; j < U; ++j)
       if (c[U + j] >= 0) accU += v[U + j] * xv[U + j];
-    return (dx + accL) + accU;
+    return kUpperFirst ? (dx + accU) + accL : (dx + accL) + accU;
   }
   int base = bl;
   for (int j0 = 0; j0 < wl; j0 += U) {
@@ -120,7 +124,7 @@ __device__ __forceinline__ T row_spmv(const SliceInfo si, int lane, T dx,
     }
 #pragma unroll
     for (int j = 0; j < U; ++j)
-      if (c[j] >= 0) accL += v[j] * x[c[j]];
+      if (c[j] >= 0) accL += v[j] * gx(c[j]);
     base += U * kSlice;
   }
   base = bu;
@@ -134,10 +138,17 @@ __device__ __forceinline__ T row_spmv(const SliceInfo si, int lane, T dx,
     }
 #pragma unroll
     for (int j = 0; j < U; ++j)
-      if (c[j] >= 0) accU += v[j] * x[c[j]];
+      if (c[j] >= 0) accU += v[j] * gx(c[j]);
     base += U * kSlice;
   }
-  return (dx + accL) + accU;
+  return kUpperFirst ? (dx + accU) + accL : (dx + accL) + accU;
+}
+template <typename T>
+__device__ __forceinline__ T row_spmv(const SliceInfo si, int lane, T dx,
+                                      const int32_t *__restrict__ col,
+                                      const T *__restrict__ val,
+                                      const T *__restrict__ x) {
+  return row_spmv_g<T>(si, lane, dx, col, val, [x](int c) { return x[c]; });
 }
 
 // Slice descriptor for a slice index that is the same on every lane of the

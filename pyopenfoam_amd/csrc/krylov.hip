@@ -15,6 +15,7 @@
 // start after the stop flag is set return immediately.
 
 #include <cmath>
+#include <cstdlib>
 #include <cstring>
 
 #include "solver_state.h"
@@ -37,14 +38,14 @@ __device__ __forceinline__ void store_partials(double (&v)[K], double *part, dou
 // workgroups of one XCD (consecutive logical blocks after xcd_remap) cover one
 // contiguous chunk of rows per step, so neighbouring rows' x[] values are
 // served from that XCD's L2 while the chunk moves through the matrix.
-#define FOR_ROWS(body)                                                          \
+#define FOR_ROWS(...)                                                           \
   {                                                                             \
     const int lb_ = xcd_remap(blockIdx.x, gridDim.x);                           \
     const int lane = threadIdx.x & 63;                                          \
     for (int s = lb_ * kWavesPerBlock + (threadIdx.x >> 6); s < nslices;        \
          s += gridDim.x * kWavesPerBlock) {                                     \
       const int row = s * kSlice + lane;                                        \
-      body                                                                      \
+      __VA_ARGS__                                                               \
     }                                                                           \
   }
 
@@ -542,7 +543,9 @@ __global__ __launch_bounds__(kBlock) void k_bicg_spmv(int n, int nslices,
     const SliceInfo si = load_slice(sl, s);
     const double dx = row < n ? diag[row] * p[row] : 0.0;
     const double av = row_spmv<double>(si, lane, dx, col, val, p);
-    const double at = row_spmv<double>(si, lane, dx, col, val_t, p);
+    // A^T p in the association of the reference's Ax_T
+    auto ld = [p](int c) { return p[c]; };
+    const double at = row_spmv_g<double, decltype(ld), true>(si, lane, dx, col, val_t, ld);
     if (row < n) {
       v[row] = av;
       t[row] = at;
@@ -637,8 +640,19 @@ struct Work {
 
 inline size_t align256(size_t b) { return (b + 255) & ~size_t(255); }
 
+// Workgroups of the reduction grid: one per slice quartet, at most
+// kMaxRedBlocks, or fewer when PFM_RED_BLOCKS (8 .. kMaxRedBlocks) says so.
+inline int red_cap() {
+  static const int cap = [] {
+    const char *e = std::getenv("PFM_RED_BLOCKS");
+    const int v = e ? std::atoi(e) : kMaxRedBlocks;
+    return v >= 8 && v <= kMaxRedBlocks ? v : kMaxRedBlocks;
+  }();
+  return cap;
+}
+
 inline int red_grid(const pfm_ldu_s *h) {
-  return std::max(1, std::min(blocks_for_slices(h->nslices), kMaxRedBlocks));
+  return std::max(1, std::min(blocks_for_slices(h->nslices), red_cap()));
 }
 
 size_t work_layout(const pfm_ldu_s *h, int max_iter, int nvec, Work *w, char *base) {
