@@ -52,7 +52,7 @@ def dot_pairwise(a, b):
 DOTS = {"blas": dot_blas, "exact": dot_exact, "pairwise": dot_pairwise}
 
 
-def dot_device(n, *, max_red_blocks=2048, block=256, slice_rows=64, xcds=8):
+def dot_device(n, *, layout="strided", max_red_blocks=2048, block=256, slice_rows=64, xcds=8):
     """The summation order of the device solvers' fused dot products, so that
     an oracle solve can be compared with a device solve bit for bit.  This is
     one more summation order (the reference leaves the order to torch), set
@@ -61,9 +61,11 @@ def dot_device(n, *, max_red_blocks=2048, block=256, slice_rows=64, xcds=8):
     * G = min(ceil(n_slices / waves), max_red_blocks) workgroups of 4 waves;
       physical workgroup b works as logical block lb = (b % 8) * (G // 8) +
       b // 8 (b < 8 * (G // 8), else lb = b);
-    * wave w of logical block lb walks slices lb*4 + w + k*4G, k = 0, 1, ...;
-      lane l adds the product of row 64 s + l (rows < n) to its running sum
-      from 0;
+    * wave w of logical block lb walks slices lb*4 + w + k*4G, k = 0, 1, ...
+      (``layout="strided"``, the Krylov solvers), or, with ``"contiguous"``
+      (the GAMG-preconditioned PCG), slices lb*per + w + 4k below
+      min((lb+1)*per, n_slices), per = ceil(n_slices / G); lane l adds the
+      product of row 64 s + l (rows < n) to its running sum from 0;
     * each wave is reduced by the butterfly v += v[l ^ off], off = 32 .. 1,
       lane 0's value taken; the 4 waves are added in order (partial of b);
     * the G partials: thread t sums part[t + 256 k], k = 0..7, from 0, then
@@ -75,11 +77,21 @@ def dot_device(n, *, max_red_blocks=2048, block=256, slice_rows=64, xcds=8):
     per = G // xcds
     b = np.arange(G)
     lb = np.where(b < per * xcds, (b % xcds) * per + b // xcds, b)
-    steps = max(1, -(-nsl // (G * waves)))
-    s = (lb[:, None, None] * waves + np.arange(waves)[None, :, None]
-         + np.arange(steps)[None, None, :] * (G * waves))        # (G, waves, steps)
+    if layout == "strided":
+        steps = max(1, -(-nsl // (G * waves)))
+        s = (lb[:, None, None] * waves + np.arange(waves)[None, :, None]
+             + np.arange(steps)[None, None, :] * (G * waves))    # (G, waves, steps)
+        end = np.full(s.shape, nsl)
+    elif layout == "contiguous":
+        per = -(-nsl // G)
+        steps = max(1, -(-per // waves))
+        s = (lb[:, None, None] * per + np.arange(waves)[None, :, None]
+             + np.arange(steps)[None, None, :] * waves)
+        end = np.minimum((lb[:, None, None] + 1) * per, nsl) + 0 * s
+    else:
+        raise ValueError(layout)
     rows = s[:, :, None, :] * slice_rows + np.arange(slice_rows)[None, None, :, None]
-    valid = (s[:, :, None, :] < nsl) & (rows < n)
+    valid = (s[:, :, None, :] < end[:, :, None, :]) & (rows < n)
     idx = np.where(valid, rows, n)                                 # n -> a 0.0 pad
     lanes = np.arange(slice_rows)
     kthreads = max_red_blocks // block

@@ -164,9 +164,9 @@ def test_gamg_mixed_converges(gpu_device):
 def test_pcg_gamg_preconditioner_matches_oracle(gpu_device, n, jit, sig):
     """PCG with one reference-hierarchy V-cycle as the preconditioner against
     the oracle PCG with the oracle's V-cycle (bit-exact on its own, see the
-    GAMG tests): on symmetric systems iterations within one and the solution
-    within 1e-9 (or ten times the oracle's summation-order spread); on the
-    jittered one inside the oracle's spread."""
+    GAMG tests): bit-exact when the oracle sums in the device's order; in the
+    reference's orders iterations within one of their spread and the solution
+    within ten times it (floor 1e-9)."""
     from oracle import krylov as okry
     from pyopenfoam_amd import PCGSolver
     c, d, lo, up = system(n, jit, sig)
@@ -179,15 +179,22 @@ def test_pcg_gamg_preconditioner_matches_oracle(gpu_device, n, jit, sig):
     pre = lambda r: ogamg.v_cycle(levels, aggs, 0, np.zeros_like(r), r, 2, 2)
     from oracle import ldu as oldu
     mv = lambda v: oldu.ax(d, lo, up, c.owner, c.neighbour, v)
-    runs = {k: okry.pcg(mv, pre, c.b, np.zeros(c.n_cells), tolerance=1e-300, rel_tol=1e-8,
-                        max_iter=500, dot=dot) for k, dot in okry.DOTS.items()}
+    solve = lambda dot: okry.pcg(mv, pre, c.b, np.zeros(c.n_cells), tolerance=1e-300,
+                                 rel_tol=1e-8, max_iter=500, dot=dot)
+    # bit-exact against the oracle summing in the device's order
+    xd, idev, _, infod = solve(okry.dot_device(c.n_cells, layout="contiguous"))
+    xg = x.cpu().numpy()
+    assert it == idev and solver.last_info.converged == infod.converged, (it, idev)
+    assert bits_equal(np.array(solver.last_info.residual_history), infod.residual_history)
+    assert bits_equal(xg, xd)
+    # against the reference's own summation orders
+    runs = {k: solve(dot) for k, dot in okry.DOTS.items()}
     xo, io, ro, info = runs["blas"]
     its = [r[1] for r in runs.values()]
     assert info.converged and solver.last_info.converged
     assert min(its) - 1 <= it <= max(its) + 1, (it, its)
     np.testing.assert_allclose(solver.last_info.residual_history[:3], info.residual_history[:3],
                                rtol=1e-9)
-    xg = x.cpu().numpy()
     nx = np.linalg.norm(xo)
     spread = max(np.linalg.norm(r[0] - xo) for r in runs.values()) / nx
     assert np.linalg.norm(xg - xo) / nx <= max(10 * spread, 1e-9), spread
