@@ -97,6 +97,24 @@ int pfm_relabel_faces(int32_t n_cells, int32_t n_internal, const int32_t *owner_
                       int32_t *order_out_h, uint8_t *flip_out_h, int32_t *owner_out_h,
                       int32_t *neighbour_out_h);
 
+/* Parsers of the large ASCII lists of an OpenFOAM polyMesh, the inner loops
+ * of io/mesh_io.py:154-369 (_parse_ascii_points / _parse_ascii_faces /
+ * _parse_ascii_label_list).  `text` is what follows the list's opening
+ * parenthesis (len bytes, not null-terminated; reading stops at the list's
+ * closing parenthesis); n is the count the file states, and a different
+ * count is an error, as in the reference.  Points:
+ * one "(x y z)" group per point -> out[3n].  Faces: one "<k>(v ...)" group
+ * per face (k unchecked, as the reference's regex leaves it) -> face_ptr[n+1]
+ * and verts[verts_cap] (*n_verts = labels found; more than verts_cap is an
+ * error).  Labels: whitespace-separated integers -> out[n]. */
+int pfm_parse_ascii_points(const char *text, int64_t len, int64_t n,
+                           double *out, int64_t *n_read);
+int pfm_parse_ascii_faces(const char *text, int64_t len, int64_t n,
+                          int64_t verts_cap, int32_t *face_ptr, int32_t *verts,
+                          int64_t *n_read, int64_t *n_verts);
+int pfm_parse_ascii_labels(const char *text, int64_t len, int64_t n,
+                           int64_t *out, int64_t *n_read);
+
 /* Sweep levels of the symmetric Gauss-Seidel smoother
  * (solvers/smoothers.py:92-149): in the forward sweep cell i reads the
  * updated x of every neighbour j < i, so fwd_level[i] = 1 + max over those;

@@ -69,6 +69,9 @@ _SIGS = {
     "pfm_last_error": (C.c_char_p, []),
     "pfm_abi_version": (INT, []),
     "pfm_rcm": (INT, [I32, I32, P, P, P]),
+    "pfm_parse_ascii_points": (INT, [C.c_char_p, I64, I64, P, P]),
+    "pfm_parse_ascii_faces": (INT, [C.c_char_p, I64, I64, I64, P, P, P, P]),
+    "pfm_parse_ascii_labels": (INT, [C.c_char_p, I64, I64, P, P]),
     "pfm_cell_faces": (INT, [I32, I32, I32, P, P, P, P, P, P]),
     "pfm_relabel_faces": (INT, [I32, I32, P, P, P, P, P, P, P]),
     "pfm_face_weights": (INT, [I32, I32, P, P, P, P, P, P]),

@@ -47,8 +47,10 @@ class MeshGeometry:
 
 
 def face_csr(faces) -> tuple[np.ndarray, np.ndarray]:
-    """(face_ptr, face_verts) int32 from an (n_faces, n_v) array or a list of
-    vertex loops."""
+    """(face_ptr, face_verts) int32 from an (n_faces, n_v) array, a list of
+    vertex loops or a (face_ptr, face_verts) pair."""
+    if isinstance(faces, tuple) and len(faces) == 2:
+        return np.asarray(faces[0], np.int32), np.asarray(faces[1], np.int32)
     if isinstance(faces, np.ndarray) and faces.ndim == 2:
         nf, nv = faces.shape
         return (np.arange(nf + 1, dtype=np.int64) * nv).astype(np.int32), \
