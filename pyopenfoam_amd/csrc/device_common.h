@@ -72,6 +72,23 @@ __device__ __forceinline__ SliceRange block_slices(int nslices) {
   return r;
 }
 
+// Column of slot k of a row: the int32 column (-1 = padding slot), or an
+// int16 offset from the row (kColPad16 = padding slot), the compact form the
+// handle keeps when every offset fits (half the index bytes per slot).
+constexpr int kColPad16 = -32768;
+struct ColAbs {
+  const int32_t *__restrict__ c;
+  __device__ __forceinline__ int operator()(int k) const { return c[k]; }
+};
+struct ColRel {
+  const int16_t *__restrict__ d;
+  int row;
+  __device__ __forceinline__ int operator()(int k) const {
+    const int v = d[k];
+    return v == kColPad16 ? -1 : row + v;
+  }
+};
+
 // One row of y = A x on the packed row-slice layout.  Returns
 // (diag*x + sum_lower) + sum_upper, each sum in face order from 0, which is
 // the association of the reference's Ax (diag*x, then scatter of the owner
@@ -81,9 +98,8 @@ __device__ __forceinline__ SliceRange block_slices(int nslices) {
 // the association of the reference's Ax_T (ldu_matrix.py:248-257: the
 // neighbour-side scatter first) when the packed values are the transposed
 // ones.
-template <typename T, typename Gather, bool kUpperFirst = false>
-__device__ __forceinline__ T row_spmv_g(const SliceInfo si, int lane, T dx,
-                                        const int32_t *__restrict__ col,
+template <typename T, typename Gather, bool kUpperFirst = false, typename ColF = ColAbs>
+__device__ __forceinline__ T row_spmv_g(const SliceInfo si, int lane, T dx, ColF col,
                                         const T *__restrict__ val, Gather gx) {
   constexpr int U = 4;
   T accL = T(0), accU = T(0);
@@ -97,9 +113,9 @@ __device__ __forceinline__ T row_spmv_g(const SliceInfo si, int lane, T dx,
     T v[2 * U];
 #pragma unroll
     for (int j = 0; j < U; ++j) {
-      c[j] = j < wl ? col[bl + j * kSlice] : -1;
+      c[j] = j < wl ? col(bl + j * kSlice) : -1;
       v[j] = j < wl ? val[bl + j * kSlice] : T(0);
-      c[U + j] = j < wu ? col[bu + j * kSlice] : -1;
+      c[U + j] = j < wu ? col(bu + j * kSlice) : -1;
       v[U + j] = j < wu ? val[bu + j * kSlice] : T(0);
     }
     T xv[2 * U];
@@ -119,7 +135,7 @@ __device__ __forceinline__ T row_spmv_g(const SliceInfo si, int lane, T dx,
     T v[U];
 #pragma unroll
     for (int j = 0; j < U; ++j) {
-      c[j] = (j0 + j < wl) ? col[base + j * kSlice] : -1;
+      c[j] = (j0 + j < wl) ? col(base + j * kSlice) : -1;
       v[j] = (j0 + j < wl) ? val[base + j * kSlice] : T(0);
     }
 #pragma unroll
@@ -133,7 +149,7 @@ __device__ __forceinline__ T row_spmv_g(const SliceInfo si, int lane, T dx,
     T v[U];
 #pragma unroll
     for (int j = 0; j < U; ++j) {
-      c[j] = (j0 + j < wu) ? col[base + j * kSlice] : -1;
+      c[j] = (j0 + j < wu) ? col(base + j * kSlice) : -1;
       v[j] = (j0 + j < wu) ? val[base + j * kSlice] : T(0);
     }
 #pragma unroll
@@ -148,7 +164,7 @@ __device__ __forceinline__ T row_spmv(const SliceInfo si, int lane, T dx,
                                       const int32_t *__restrict__ col,
                                       const T *__restrict__ val,
                                       const T *__restrict__ x) {
-  return row_spmv_g<T>(si, lane, dx, col, val, [x](int c) { return x[c]; });
+  return row_spmv_g<T>(si, lane, dx, ColAbs{col}, val, [x](int c) { return x[c]; });
 }
 
 // Slice descriptor for a slice index that is the same on every lane of the

@@ -49,17 +49,39 @@ __device__ __forceinline__ void store_partials(double (&v)[K], double *part, dou
     }                                                                           \
   }
 
+// One row of A v with the handle's int16 row offsets (kRel) or int32 columns.
+template <bool kRel>
+__device__ __forceinline__ double row_av(const SliceInfo si, int lane, int row, double dx,
+                                         const int32_t *__restrict__ col,
+                                         const int16_t *__restrict__ col16,
+                                         const double *__restrict__ val,
+                                         const double *__restrict__ v) {
+  auto gx = [v](int c) { return v[c]; };
+  if constexpr (kRel)
+    return row_spmv_g<double, decltype(gx), false, ColRel>(si, lane, dx, ColRel{col16, row},
+                                                           val, gx);
+  else
+    return row_spmv_g<double>(si, lane, dx, ColAbs{col}, val, gx);
+}
+
 // ---------------------------------------------------------------- PCG ------
 // Per iteration i: k_pcg_spmv (Ap, p.Ap partials) -> k_pcg_update (every
-// workgroup reduces p.Ap itself: breakdown test, alpha; x, r; r.r and r.z
+// workgroup reduces p.Ap itself: breakdown test, alpha; r; r.r and r.z
 // partials) -> k_pcg_pupdate (every workgroup reduces r.r, r.z: monitor
-// decision, beta; p).  Workgroup 0 alone writes the state; scalars another
-// workgroup of the same kernel still reads (r.z) are kept per iteration
-// parity.  Partial regions: A = part[0, G) (p.Ap), B = part[G, 3G) (r.r, r.z).
+// decision, beta; x += alpha p, then p).  x is updated where p is read anyway
+// (one vector less per iteration); since every workgroup of k_pcg_pupdate
+// must update x even when iteration i converges, its stop decision is left in
+// `pending` and the next k_pcg_spmv turns it into `done` (or collect() reads
+// it after the last iteration).  Workgroup 0 alone writes the state; scalars
+// another workgroup of the same kernel still reads (r.z, alpha) are kept per
+// iteration parity.  Partial regions: A = part[0, G) (p.Ap), B = part[G, 3G)
+// (r.r, r.z).
 
 // r = b - A x; z = M^-1 r; p = z; partials r.r and r.z (region B).
+template <bool kRel>
 __global__ __launch_bounds__(kBlock) void k_pcg_init(int n, int nslices, const SliceInfo *__restrict__ sl,
                                                       const int32_t *__restrict__ col,
+                                                      const int16_t *__restrict__ col16,
                                                       const double *__restrict__ val,
                                                       const double *__restrict__ diag,
                                                       const double *__restrict__ rD,
@@ -72,7 +94,7 @@ __global__ __launch_bounds__(kBlock) void k_pcg_init(int n, int nslices, const S
   FOR_ROWS({
     const SliceInfo si = load_slice(sl, s);
     const double dx = row < n ? diag[row] * x[row] : 0.0;
-    const double ax = row_spmv<double>(si, lane, dx, col, val, x);
+    const double ax = row_av<kRel>(si, lane, row, dx, col, col16, val, x);
     if (row < n) {
       const double ri = b[row] - ax;
       const double zi = rD ? ri / rD[row] : ri;
@@ -99,22 +121,29 @@ __global__ __launch_bounds__(kBlock) void k_pcg_init_fin(const double *__restric
   }
 }
 
-// Ap = A p; partial p.Ap (region A).
+// Ap = A p; partial p.Ap (region A).  Ends the solve if the previous
+// iteration decided so.
+template <bool kRel>
 __global__ __launch_bounds__(kBlock) void k_pcg_spmv(int n, int nslices, const SliceInfo *__restrict__ sl,
                                                       const int32_t *__restrict__ col,
+                                                      const int16_t *__restrict__ col16,
                                                       const double *__restrict__ val,
                                                       const double *__restrict__ diag,
                                                       const double *__restrict__ p,
                                                       double *__restrict__ Ap,
-                                                      double *__restrict__ part, const KState *st) {
+                                                      double *__restrict__ part, KState *st) {
   __shared__ double lds[kWavesPerBlock];
   if (st->done) return;
+  if (st->pending) {  // the previous iteration ended the solve
+    if (blockIdx.x == 0 && threadIdx.x == 0) finish(st, st->pending);
+    return;
+  }
   double acc[1] = {0.0};
   FOR_ROWS({
     const SliceInfo si = load_slice(sl, s);
     const double pi = row < n ? p[row] : 0.0;
     const double dx = row < n ? diag[row] * pi : 0.0;
-    const double v = row_spmv<double>(si, lane, dx, col, val, p);
+    const double v = row_av<kRel>(si, lane, row, dx, col, col16, val, p);
     if (row < n) {
       Ap[row] = v;
       acc[0] += pi * v;
@@ -123,13 +152,11 @@ __global__ __launch_bounds__(kBlock) void k_pcg_spmv(int n, int nslices, const S
   store_partials<1>(acc, part, lds);
 }
 
-// alpha = rz / pAp (breakdown if |pAp| < 1e-30); x += alpha p; r -= alpha Ap;
-// partials r.r and r.z (region B).
+// alpha = rz / pAp (breakdown if |pAp| < 1e-30), kept for k_pcg_pupdate;
+// r -= alpha Ap; partials r.r and r.z (region B).
 __global__ __launch_bounds__(kBlock) void k_pcg_update(int n, int nslices, int it,
                                                         const double *__restrict__ rD,
-                                                        const double *__restrict__ p,
                                                         const double *__restrict__ Ap,
-                                                        double *__restrict__ x,
                                                         double *__restrict__ r,
                                                         double *__restrict__ part, KState *st) {
   __shared__ double lds[kWavesPerBlock];
@@ -141,10 +168,8 @@ __global__ __launch_bounds__(kBlock) void k_pcg_update(int n, int nslices, int i
   const int stride = G * kWavesPerBlock;
   int s = xcd_remap(blockIdx.x, G) * kWavesPerBlock + (threadIdx.x >> 6);
   int row = s * kSlice + lane;
-  double xp = 0.0, pp = 0.0, rp = 0.0, ap = 0.0, dp = 1.0;
+  double rp = 0.0, ap = 0.0, dp = 1.0;
   if (s < nslices && row < n) {
-    xp = x[row];
-    pp = p[row];
     rp = r[row];
     ap = Ap[row];
     if (rD) dp = rD[row];
@@ -155,10 +180,10 @@ __global__ __launch_bounds__(kBlock) void k_pcg_update(int n, int nslices, int i
     return;
   }
   const double alpha = st->rzs[(it - 1) & 1] / PAP;
+  if (blockIdx.x == 0 && threadIdx.x == 0) st->alphas[it & 1] = alpha;
   double acc[2] = {0.0, 0.0};
   for (; s < nslices; s += stride, row = s * kSlice + lane) {
     if (row < n) {
-      x[row] = xp + alpha * pp;
       const double ri = rp - alpha * ap;
       r[row] = ri;
       const double zi = rD ? ri / dp : ri;
@@ -167,8 +192,6 @@ __global__ __launch_bounds__(kBlock) void k_pcg_update(int n, int nslices, int i
     }
     const int sn = s + stride, rn = sn * kSlice + lane;
     if (sn < nslices && rn < n) {
-      xp = x[rn];
-      pp = p[rn];
       rp = r[rn];
       ap = Ap[rn];
       if (rD) dp = rD[rn];
@@ -177,54 +200,62 @@ __global__ __launch_bounds__(kBlock) void k_pcg_update(int n, int nslices, int i
   store_partials<2>(acc, part + G, lds);
 }
 
-// monitor(||r||, i); beta = r.z' / r.z; p = M^-1 r + beta p.
+// monitor(||r||, i); x += alpha p; beta = r.z' / r.z; p = M^-1 r + beta p
+// (not after the last iteration).
 __global__ __launch_bounds__(kBlock) void k_pcg_pupdate(int n, int nslices, int it,
                                                          const double *__restrict__ rD,
                                                          const double *__restrict__ r,
                                                          double *__restrict__ p,
+                                                         double *__restrict__ x,
                                                          const double *__restrict__ part, KState *st,
                                                          double *__restrict__ hist) {
   __shared__ double lds[kWavesPerBlock];
-  if (st->done) return;
+  if (st->done) return;  // set by an earlier kernel only (a breakdown in k_pcg_update)
   const int G = gridDim.x;
-  // First round's r, rD, p loaded before the reductions (they do not depend on
-  // beta), so their latency overlaps the partial sums'.
+  // First round's r, rD, p, x loaded before the reductions (they do not
+  // depend on beta), so their latency overlaps the partial sums'.
   const int lane = threadIdx.x & 63;
   const int stride = G * kWavesPerBlock;
   int s = xcd_remap(blockIdx.x, G) * kWavesPerBlock + (threadIdx.x >> 6);
   int row = s * kSlice + lane;
-  double rp = 0.0, dp = 1.0, pp = 0.0;
+  double rp = 0.0, dp = 1.0, pp = 0.0, xp = 0.0;
   if (s < nslices && row < n) {
     rp = r[row];
     if (rD) dp = rD[row];
     pp = p[row];
+    xp = x[row];
   }
   const double RR = reduce_partials(part + G, G, lds);
   const double RZ = reduce_partials(part + 2 * G, G, lds);
   const double res = sqrt(RR);
   const bool conv = monitor_decision(st, res, it);
+  const bool last = conv || it >= st->max_iter;
   const double rz_old = st->rzs[(it - 1) & 1];
+  const double alpha = st->alphas[it & 1];
   if (blockIdx.x == 0 && threadIdx.x == 0) {
     monitor_update(st, hist, res, it);
     st->iter = it;
     st->rzs[it & 1] = RZ;
     if (conv)
-      finish(st, PFM_SOLVE_CONVERGED);
+      st->pending = PFM_SOLVE_CONVERGED;
     else if (it >= st->max_iter)
-      finish(st, PFM_SOLVE_MAXITER);
+      st->pending = PFM_SOLVE_MAXITER;
   }
-  if (conv || it >= st->max_iter) return;
   const double beta = RZ / rz_old;
   for (; s < nslices; s += stride, row = s * kSlice + lane) {
     if (row < n) {
-      const double zi = rD ? rp / dp : rp;
-      p[row] = zi + beta * pp;
+      x[row] = xp + alpha * pp;
+      if (!last) {
+        const double zi = rD ? rp / dp : rp;
+        p[row] = zi + beta * pp;
+      }
     }
     const int sn = s + stride, rn = sn * kSlice + lane;
     if (sn < nslices && rn < n) {
       rp = r[rn];
       if (rD) dp = rD[rn];
       pp = p[rn];
+      xp = x[rn];
     }
   }
 }
@@ -545,7 +576,7 @@ __global__ __launch_bounds__(kBlock) void k_bicg_spmv(int n, int nslices,
     const double av = row_spmv<double>(si, lane, dx, col, val, p);
     // A^T p in the association of the reference's Ax_T
     auto ld = [p](int c) { return p[c]; };
-    const double at = row_spmv_g<double, decltype(ld), true>(si, lane, dx, col, val_t, ld);
+    const double at = row_spmv_g<double, decltype(ld), true>(si, lane, dx, ColAbs{col}, val_t, ld);
     if (row < n) {
       v[row] = av;
       t[row] = at;
@@ -740,7 +771,7 @@ int collect(const KState *st_dev, const double *hist_dev, double *history_h, int
     PFM_CHECK_HIP(hipStreamSynchronize(s));
   }
   res->iterations = st.nhist;
-  res->status = st.status == 0 ? PFM_SOLVE_MAXITER : st.status;
+  res->status = st.status != 0 ? st.status : st.pending != 0 ? st.pending : PFM_SOLVE_MAXITER;
   res->converged = res->status == PFM_SOLVE_CONVERGED;
   res->initial_residual = st.nhist > 0 ? st.res0 : 0.0;
   res->final_residual = st.nhist > 0 ? st.res : 0.0;
@@ -782,15 +813,24 @@ int pfm_pcg_solve(pfm_ldu h, const pfm_solve_params *prm, const double *diag, co
   const int n = h->n, ns = h->nslices;
   k_state_init<<<1, 1, 0, s>>>(w.st, prm->tolerance, prm->rel_tol, prm->max_iter, prm->min_iter,
                                hist_cap);
-  k_pcg_init<<<G, kBlock, 0, s>>>(n, ns, h->d_slices, h->d_col, packed, diag, pd, b, x, r, p,
-                                  w.part);
+  const int16_t *c16 = h->d_col16;
+  if (c16)
+    k_pcg_init<true><<<G, kBlock, 0, s>>>(n, ns, h->d_slices, h->d_col, c16, packed, diag, pd, b,
+                                          x, r, p, w.part);
+  else
+    k_pcg_init<false><<<G, kBlock, 0, s>>>(n, ns, h->d_slices, h->d_col, c16, packed, diag, pd, b,
+                                           x, r, p, w.part);
   k_pcg_init_fin<<<1, kBlock, 0, s>>>(w.part, G, w.st, w.hist);
   PFM_CHECK_HIP(hipGetLastError());
   rc = drive(h, w.st, prm->max_iter, s, [&](int it) {
-    k_pcg_spmv<<<G, kBlock, 0, s>>>(n, ns, h->d_slices, h->d_col, packed, diag, p, Ap, w.part,
-                                    w.st);
-    k_pcg_update<<<G, kBlock, 0, s>>>(n, ns, it, pd, p, Ap, x, r, w.part, w.st);
-    k_pcg_pupdate<<<G, kBlock, 0, s>>>(n, ns, it, pd, r, p, w.part, w.st, w.hist);
+    if (c16)
+      k_pcg_spmv<true><<<G, kBlock, 0, s>>>(n, ns, h->d_slices, h->d_col, c16, packed, diag, p,
+                                            Ap, w.part, w.st);
+    else
+      k_pcg_spmv<false><<<G, kBlock, 0, s>>>(n, ns, h->d_slices, h->d_col, c16, packed, diag, p,
+                                             Ap, w.part, w.st);
+    k_pcg_update<<<G, kBlock, 0, s>>>(n, ns, it, pd, Ap, r, w.part, w.st);
+    k_pcg_pupdate<<<G, kBlock, 0, s>>>(n, ns, it, pd, r, p, x, w.part, w.st, w.hist);
   });
   if (rc != PFM_OK) return rc;
   PFM_CHECK_HIP(hipGetLastError());

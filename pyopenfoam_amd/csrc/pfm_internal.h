@@ -92,6 +92,7 @@ struct pfm_ldu_s {
   std::vector<int32_t> dilu_level_ptr;
   pfm::SliceInfo *d_slices = nullptr;
   int32_t *d_col = nullptr;
+  int16_t *d_col16 = nullptr;           // column - row per slot, when all fit (else null)
   int32_t *d_map = nullptr;
   int32_t *d_own = nullptr;
   int32_t *d_nei = nullptr;

@@ -14,6 +14,7 @@ struct alignas(256) KState {
   double res0, res;
   double rz, alpha, beta;    // PCG
   double rzs[2];             // PCG r.z by iteration parity (written while read)
+  double alphas[2];          // PCG alpha by iteration parity
   double rho, omega;         // PBiCGStab (rho kept per iteration parity in rhos)
   double rhos[2];
   int32_t max_iter, min_iter;
@@ -23,6 +24,8 @@ struct alignas(256) KState {
   int32_t status;
   int32_t branch;            // PBiCGStab: 0 normal, 1 t.t ~ 0, 2 converged on s
   int32_t hist_cap;
+  int32_t pending;           // PCG: status decided by the last kernel of an
+                             // iteration, made `done` by the next kernel
 };
 
 // ResidualMonitor.update (residual.py:112-145), evaluated by one thread.
