@@ -23,14 +23,14 @@ using namespace pfm;
 
 namespace {
 
-#define FOR_ROWS(body)                                                          \
+#define FOR_ROWS(...)                                                           \
   {                                                                             \
     const int lb_ = xcd_remap(blockIdx.x, gridDim.x);                           \
     const int lane = threadIdx.x & 63;                                          \
     for (int s = lb_ * kWavesPerBlock + (threadIdx.x >> 6); s < nslices;        \
          s += gridDim.x * kWavesPerBlock) {                                     \
       const int row = s * kSlice + lane;                                        \
-      body                                                                      \
+      __VA_ARGS__                                                               \
     }                                                                           \
   }
 
@@ -47,10 +47,11 @@ __device__ __forceinline__ void store_partials(double (&v)[K], double *part, dou
 struct alignas(256) IState {
   double target;   // stop when ||r_in|| < target
   double rzs[2];
+  double alphas[2];  // alpha by iteration parity (k_pcg32_update -> k_pcg32_pupdate)
   int32_t done;
   int32_t iters;   // inner iterations run
   int32_t max_iter;
-  int32_t pad;
+  int32_t pending; // stop decided by k_pcg32_pupdate, made `done` by the next k_pcg32_spmv
 };
 
 __global__ __launch_bounds__(kBlock) void k_cast(int n, const double *__restrict__ a,
@@ -98,6 +99,7 @@ __global__ __launch_bounds__(kBlock) void k_ir_outer_fin(const double *__restric
       finish(st, PFM_SOLVE_MAXITER);
     in->target = tau * res;
     in->done = 0;
+    in->pending = 0;
     in->iters = 0;
     in->max_iter = inner_budget;
   }
@@ -137,22 +139,36 @@ __global__ __launch_bounds__(kBlock) void k_ir_inner_init_fin(const double *__re
   }
 }
 
-// Ap = A32 p (FP32); partial p.Ap in FP64 (region A).
+// Ap = A32 p (FP32); partial p.Ap in FP64 (region A).  kRel: columns from the
+// handle's int16 row offsets.  Ends the inner loop if the previous iteration
+// decided so.
+template <bool kRel>
 __global__ __launch_bounds__(kBlock) void k_pcg32_spmv(int n, int nslices, const SliceInfo *__restrict__ sl,
                                                         const int32_t *__restrict__ col,
+                                                        const int16_t *__restrict__ col16,
                                                         const float *__restrict__ val,
                                                         const float *__restrict__ diag,
                                                         const float *__restrict__ p,
                                                         float *__restrict__ Ap,
-                                                        double *__restrict__ part, const IState *in) {
+                                                        double *__restrict__ part, IState *in) {
   __shared__ double lds[kWavesPerBlock];
   if (in->done) return;
+  if (in->pending) {
+    if (blockIdx.x == 0 && threadIdx.x == 0) in->done = 1;
+    return;
+  }
   double acc[1] = {0.0};
+  auto gx = [p](int c) { return p[c]; };
   FOR_ROWS({
     const SliceInfo si = load_slice(sl, s);
     const float pi = row < n ? p[row] : 0.0f;
     const float dx = row < n ? diag[row] * pi : 0.0f;
-    const float v = row_spmv<float>(si, lane, dx, col, val, p);
+    float v;
+    if constexpr (kRel)
+      v = row_spmv_g<float, decltype(gx), false, ColRel>(si, lane, dx, ColRel{col16, row}, val,
+                                                         gx);
+    else
+      v = row_spmv_g<float>(si, lane, dx, ColAbs{col}, val, gx);
     if (row < n) {
       Ap[row] = v;
       acc[0] += (double)pi * (double)v;
@@ -161,12 +177,10 @@ __global__ __launch_bounds__(kBlock) void k_pcg32_spmv(int n, int nslices, const
   store_partials<1>(acc, part, lds);
 }
 
-// alpha; e += alpha p; r -= alpha Ap; partials r.r, r.z (region B).
+// alpha (kept for k_pcg32_pupdate); r -= alpha Ap; partials r.r, r.z (region B).
 __global__ __launch_bounds__(kBlock) void k_pcg32_update(int n, int nslices, int it,
                                                           const float *__restrict__ rD,
-                                                          const float *__restrict__ p,
                                                           const float *__restrict__ Ap,
-                                                          float *__restrict__ e,
                                                           float *__restrict__ r,
                                                           double *__restrict__ part, IState *in) {
   __shared__ double lds[kWavesPerBlock];
@@ -177,10 +191,8 @@ __global__ __launch_bounds__(kBlock) void k_pcg32_update(int n, int nslices, int
   const int stride = G * kWavesPerBlock;
   int s = xcd_remap(blockIdx.x, G) * kWavesPerBlock + (threadIdx.x >> 6);
   int row = s * kSlice + lane;
-  float ep = 0.0f, pp = 0.0f, rp = 0.0f, ap = 0.0f, dp = 1.0f;
+  float rp = 0.0f, ap = 0.0f, dp = 1.0f;
   if (s < nslices && row < n) {
-    ep = e[row];
-    pp = p[row];
     rp = r[row];
     ap = Ap[row];
     if (rD) dp = rD[row];
@@ -190,11 +202,12 @@ __global__ __launch_bounds__(kBlock) void k_pcg32_update(int n, int nslices, int
     if (blockIdx.x == 0 && threadIdx.x == 0) in->done = 1;
     return;
   }
-  const float alpha = (float)(in->rzs[(it - 1) & 1] / PAP);
+  const double alpha64 = in->rzs[(it - 1) & 1] / PAP;
+  if (blockIdx.x == 0 && threadIdx.x == 0) in->alphas[it & 1] = alpha64;
+  const float alpha = (float)alpha64;
   double acc[2] = {0.0, 0.0};
   for (; s < nslices; s += stride, row = s * kSlice + lane) {
     if (row < n) {
-      e[row] = ep + alpha * pp;
       const float ri = rp - alpha * ap;
       r[row] = ri;
       const float zi = rD ? ri / dp : ri;
@@ -203,8 +216,6 @@ __global__ __launch_bounds__(kBlock) void k_pcg32_update(int n, int nslices, int
     }
     const int sn = s + stride, rn = sn * kSlice + lane;
     if (sn < nslices && rn < n) {
-      ep = e[rn];
-      pp = p[rn];
       rp = r[rn];
       ap = Ap[rn];
       if (rD) dp = rD[rn];
@@ -213,47 +224,55 @@ __global__ __launch_bounds__(kBlock) void k_pcg32_update(int n, int nslices, int
   store_partials<2>(acc, part + G, lds);
 }
 
-// inner stop test; beta; p = z + beta p.
+// inner stop test; e += alpha p; beta; p = z + beta p (not after the last
+// inner iteration).  The stop is left in `pending`: every workgroup still
+// updates e in this kernel.
 __global__ __launch_bounds__(kBlock) void k_pcg32_pupdate(int n, int nslices, int it,
                                                            const float *__restrict__ rD,
                                                            const float *__restrict__ r,
                                                            float *__restrict__ p,
+                                                           float *__restrict__ e,
                                                            const double *__restrict__ part,
                                                            IState *in) {
   __shared__ double lds[kWavesPerBlock];
-  if (in->done) return;
+  if (in->done) return;  // set by an earlier kernel only (a breakdown in k_pcg32_update)
   const int G = gridDim.x;
   const int lane = threadIdx.x & 63;
   const int stride = G * kWavesPerBlock;
   int s = xcd_remap(blockIdx.x, G) * kWavesPerBlock + (threadIdx.x >> 6);
   int row = s * kSlice + lane;
-  float rp = 0.0f, dp = 1.0f, pp = 0.0f;
+  float rp = 0.0f, dp = 1.0f, pp = 0.0f, ep = 0.0f;
   if (s < nslices && row < n) {
     rp = r[row];
     if (rD) dp = rD[row];
     pp = p[row];
+    ep = e[row];
   }
   const double RR = reduce_partials(part + G, G, lds);
   const double RZ = reduce_partials(part + 2 * G, G, lds);
   const bool stop = sqrt(RR) < in->target || it >= in->max_iter;
   const double rz_old = in->rzs[(it - 1) & 1];
+  const float alpha = (float)in->alphas[it & 1];
   if (blockIdx.x == 0 && threadIdx.x == 0) {
     in->iters = it;
     in->rzs[it & 1] = RZ;
-    if (stop) in->done = 1;
+    if (stop) in->pending = 1;
   }
-  if (stop) return;
   const float beta = (float)(RZ / rz_old);
   for (; s < nslices; s += stride, row = s * kSlice + lane) {
     if (row < n) {
-      const float zi = rD ? rp / dp : rp;
-      p[row] = zi + beta * pp;
+      e[row] = ep + alpha * pp;
+      if (!stop) {
+        const float zi = rD ? rp / dp : rp;
+        p[row] = zi + beta * pp;
+      }
     }
     const int sn = s + stride, rn = sn * kSlice + lane;
     if (sn < nslices && rn < n) {
       rp = r[rn];
       if (rD) dp = rD[rn];
       pp = p[rn];
+      ep = e[rn];
     }
   }
 }
@@ -369,10 +388,14 @@ int pfm_pcg_mixed_solve(pfm_ldu h, const pfm_solve_params *prm, double inner_rel
         const int c = std::min(chunk, budget - enq);
         for (int k = 0; k < c; ++k) {
           const int it = enq + k + 1;
-          k_pcg32_spmv<<<G, kBlock, 0, s>>>(n, ns, h->d_slices, h->d_col, packed32, w.diag32, w.p,
-                                            w.Ap, w.part, w.in);
-          k_pcg32_update<<<G, kBlock, 0, s>>>(n, ns, it, pd, w.p, w.Ap, w.e, w.r, w.part, w.in);
-          k_pcg32_pupdate<<<G, kBlock, 0, s>>>(n, ns, it, pd, w.r, w.p, w.part, w.in);
+          if (h->d_col16)
+            k_pcg32_spmv<true><<<G, kBlock, 0, s>>>(n, ns, h->d_slices, h->d_col, h->d_col16,
+                                                    packed32, w.diag32, w.p, w.Ap, w.part, w.in);
+          else
+            k_pcg32_spmv<false><<<G, kBlock, 0, s>>>(n, ns, h->d_slices, h->d_col, nullptr,
+                                                     packed32, w.diag32, w.p, w.Ap, w.part, w.in);
+          k_pcg32_update<<<G, kBlock, 0, s>>>(n, ns, it, pd, w.Ap, w.r, w.part, w.in);
+          k_pcg32_pupdate<<<G, kBlock, 0, s>>>(n, ns, it, pd, w.r, w.p, w.e, w.part, w.in);
         }
         enq += c;
         chunk = std::min(chunk * 2, 32);
