@@ -326,6 +326,55 @@ def bench_pressure(case, mesh, dev):
     return out
 
 
+def bench_family(mat, dev):
+    """The rest of the solver family on the benchmark matrix: DILU / DIC factor
+    times (HIP events, 5 factors), fixed iteration counts of PBiCGStab (DILU),
+    PBiCG (DILU) and SmoothSolver (Gauss-Seidel, one sweep per iteration),
+    reported per iteration (tolerances 0; the reference's DILU PBiCGStab
+    breaks down on this system after a few dozen iterations, see the status)."""
+    import torch
+
+    from pyopenfoam_amd import PBiCGSTABSolver
+    from pyopenfoam_amd.solvers import (DICPreconditioner, DILUPreconditioner, PBiCGSolver,
+                                        SmoothSolver)
+
+    out = {}
+    for name, cls in (("dilu_factor_ms", DILUPreconditioner), ("dic_factor_ms", DICPreconditioner)):
+        cls(mat)
+        torch.cuda.synchronize()
+        e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+        e0.record()
+        for _ in range(5):
+            cls(mat)
+        e1.record()
+        torch.cuda.synchronize()
+        out[name] = round(e0.elapsed_time(e1) / 5, 3)
+    x0 = torch.zeros(mat.n_cells, dtype=torch.float64, device=dev)
+    runs = (("pbicgstab_dilu", lambda k: PBiCGSTABSolver(tolerance=0.0, rel_tol=0.0, max_iter=k,
+                                                         preconditioner="DILU"), 400),
+            ("pbicg_dilu", lambda k: PBiCGSolver(tolerance=0.0, rel_tol=0.0, max_iter=k,
+                                                 preconditioner="DILU"), 400),
+            ("smooth_gauss_seidel", lambda k: SmoothSolver(tolerance=0.0, rel_tol=0.0, max_iter=k,
+                                                           smoother="GaussSeidel", n_sweeps=1), 20))
+    for name, make, k in runs:
+        mat.solve(make(5), x0, 0.0, 5)  # warm
+        torch.cuda.synchronize()
+        solver = make(k)
+        t0 = time.perf_counter()
+        mat.solve(solver, x0, 0.0, k)
+        torch.cuda.synchronize()
+        ms = (time.perf_counter() - t0) * 1e3
+        it = max(1, solver.last_info.iterations - 1)
+        # the DILU solvers build their factor in the call: per iteration without it
+        per = (ms - (out["dilu_factor_ms"] if "dilu" in name else 0.0)) / it
+        out[name] = {"ms": round(ms, 2), "iterations": it, "ms_per_iteration": round(per, 4),
+                     "status": int(getattr(solver.last_info, "status", 0))}
+        log(f"family {name}: {ms:.1f} ms for {it} iterations ({per * 1e3:.1f} us each after "
+            f"the factor), status {out[name]['status']}")
+    log(f"family factors: DILU {out['dilu_factor_ms']} ms, DIC {out['dic_factor_ms']} ms")
+    return out
+
+
 def main() -> int:
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -344,6 +393,7 @@ def main() -> int:
     ap.add_argument("--no-batch", action="store_true")
     ap.add_argument("--no-gamg", action="store_true")
     ap.add_argument("--no-pressure", action="store_true")
+    ap.add_argument("--no-family", action="store_true")
     ap.add_argument("--gamg-max-iter", type=int, default=1000)
     ap.add_argument("--batch-systems", type=int, default=512)
     ap.add_argument("--batch-nmin", type=int, default=8)
@@ -447,6 +497,8 @@ def main() -> int:
             dst.copy_(src)
         torch.cuda.synchronize()
         log(f"copy probe: 5 x {args.copy_probe_mib} MiB device copies")
+
+    family = bench_family(mat, dev) if not args.no_family else None
 
     # --- DIC factor alone ---------------------------------------------------
     from pyopenfoam_amd.solvers import DICPreconditioner
@@ -621,6 +673,7 @@ def main() -> int:
         "setup_s": {"case_host": round(t_case, 2), "handle_and_assembly": round(t_asm, 2)},
         "geometry": geometry,
         "pressure_step": pressure,
+        "solver_family": family,
     }
     print(json.dumps(line), flush=True)
     return 0
