@@ -7,8 +7,8 @@ tolerance where the reference leaves the order to torch (dot products and
 norms) or to Python's set ordering (cell reference points of the geometry).
 """
 
+import math
 from pathlib import Path
-
 from types import SimpleNamespace
 
 import numpy as np
@@ -317,6 +317,30 @@ def test_diagonal_solver_oracle_known_answers():
     assert x.tolist() == [2.0, 2.0, 2.0] and r == 0.0
     x, r = osm.diagonal_solver(np.array([1e-35, 2.0]), np.array([1.0, 2.0]))
     assert np.isfinite(x).all() and x[1] == 1.0 and x[0] == 1.0 / 1e-30
+
+
+@pytest.mark.parametrize("layout", ["strided", "contiguous"])
+@pytest.mark.parametrize("n", [1, 63, 64, 65, 257, 6161, 131_073, 600_001])
+def test_dot_device_sums_every_product_once(n, layout):
+    """The device-order dot (the device solvers' reduction tree: grid, XCD
+    block remap, slice walk, wave butterflies, partials) adds every product
+    exactly once: integer-valued products sum exactly, a single non-zero
+    product comes back unchanged wherever it sits, and random data lie inside
+    the rounding bound of the exactly rounded sum.  The sizes reach a one-slice
+    grid, ragged last slices, a grid that is not a multiple of 8 (remap tail)
+    and the 2048-workgroup cap with several slices per wave."""
+    rng = np.random.default_rng(n)
+    dot = okry.dot_device(n, layout=layout)
+    a = rng.integers(-1000, 1000, n)
+    c = rng.integers(-1000, 1000, n)
+    assert dot(a.astype(float), c.astype(float)) == float(np.dot(a, c))
+    for i in sorted({0, n // 3, n - 1}):
+        e = np.zeros(n)
+        e[i] = 3.0
+        assert dot(e, np.full(n, 0.5)) == 1.5
+    x, y = rng.standard_normal(n), rng.standard_normal(n)
+    pr = x * y
+    assert abs(dot(x, y) - math.fsum(pr)) <= n * 2.0**-53 * np.abs(pr).sum()
 
 
 def test_pbicg_oracle_pins():
