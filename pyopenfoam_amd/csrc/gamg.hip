@@ -14,7 +14,12 @@
 //     level-scheduled like pfm_gs_smooth (one launch per sweep level on large
 //     levels, the single-workgroup tail walks the levels of small ones).
 // Levels at or below kTailCells run, together with everything coarser, in a
-// single-workgroup kernel (one launch instead of dozens of tiny ones).  After
+// single-workgroup kernel (one launch instead of dozens of tiny ones); inside
+// it, the levels of at most 64 cells are held by one wave in registers
+// (cross-lane reads instead of memory round trips and barriers).  Larger
+// levels run as grid launches: measured at 128^3, a 4096-cell level costs
+// less as seven ~4.6 us launches than as barrier-separated phases of one
+// workgroup, whose rows are chains of dependent loads.  After
 // each cycle r = b - A x is reduced and the residual monitor runs on the
 // device (GAMG records no initial residual, :574-582).  All arithmetic is
 // element-wise in a fixed order, so the solution is bit-identical to the
@@ -31,7 +36,8 @@ using namespace pfm;
 
 namespace {
 
-constexpr int kTailCells = 4096;
+constexpr int kTailCells = 256;
+constexpr int kFp32MinCells = 4096;
 constexpr int kTailThreads = 1024;
 constexpr int kMaxLevels = 64;
 
@@ -185,13 +191,139 @@ __device__ __forceinline__ void tail_smooth(const LevelDev &L, int &cur, bool gs
   }
 }
 
+// ---- levels of at most one slice, held by one wave ----
+// A level with n <= 64 cells is one slice: lane i is cell i, its couplings
+// sit in registers and the x value of column c is lane c's register
+// (__shfl), so Jacobi sweeps, the residual, restriction and prolongation
+// need neither memory round trips nor workgroup barriers.  Every sum keeps
+// the association of row_spmv_g and of the block path: (diag*x + sum_lower)
+// + sum_upper, each sum from 0 in slot order, padding slots skipped;
+// aggregate sums from 0 in fine-cell order.  Shuffles run on all 64 lanes
+// (loops with wave-uniform bounds); lanes >= n carry zeros.
+constexpr int kWaveSlots = 8;     // slots per group kept in registers
+constexpr int kMaxWaveLevels = 16;  // levels of the one-wave part (LDS rows kept)
+
+struct WaveRow {
+  int cl[kWaveSlots], cu[kWaveSlots];
+  double vl[kWaveSlots], vu[kWaveSlots];
+  double d, inv;
+  int n;
+};
+
+// Whether level L can be held by one wave.
+__device__ __forceinline__ bool wave_level(const LevelDev &L) {
+  if (L.n > kSlice) return false;
+  if (L.nslices == 0) return true;
+  const SliceInfo si = load_slice(L.slices, 0);
+  return si.wl <= kWaveSlots && si.wu <= kWaveSlots;
+}
+
+__device__ __forceinline__ void wave_load(const LevelDev &L, int lane, WaveRow &R) {
+  R.n = L.n;
+  SliceInfo si{0, 0, 0};
+  if (L.nslices > 0) si = load_slice(L.slices, 0);
+  const int bl = si.ptr + lane, bu = si.ptr + si.wl * kSlice + lane;
+#pragma unroll
+  for (int j = 0; j < kWaveSlots; ++j) {
+    R.cl[j] = j < si.wl ? L.col[bl + j * kSlice] : -1;
+    R.vl[j] = j < si.wl ? L.packed[bl + j * kSlice] : 0.0;
+    R.cu[j] = j < si.wu ? L.col[bu + j * kSlice] : -1;
+    R.vu[j] = j < si.wu ? L.packed[bu + j * kSlice] : 0.0;
+  }
+  R.d = lane < L.n ? L.diag[lane] : 0.0;
+  R.inv = lane < L.n ? L.inv[lane] : 0.0;
+}
+
+__device__ __forceinline__ double wave_ax(const WaveRow &R, int lane, double x) {
+  const double dx = lane < R.n ? R.d * x : 0.0;
+  double accL = 0.0, accU = 0.0;
+#pragma unroll
+  for (int j = 0; j < kWaveSlots; ++j) {
+    const double xv = __shfl(x, R.cl[j] >= 0 ? R.cl[j] : 0);
+    if (R.cl[j] >= 0) accL += R.vl[j] * xv;
+  }
+#pragma unroll
+  for (int j = 0; j < kWaveSlots; ++j) {
+    const double xv = __shfl(x, R.cu[j] >= 0 ? R.cu[j] : 0);
+    if (R.cu[j] >= 0) accU += R.vu[j] * xv;
+  }
+  return (dx + accL) + accU;
+}
+
+__device__ __forceinline__ double wave_jacobi(const WaveRow &R, int lane, double x, double b) {
+  const double ax = wave_ax(R, lane, x);
+  return lane < R.n ? x + R.inv * (b - ax) : 0.0;
+}
+
+__device__ __forceinline__ int wave_max(int v) {
+#pragma unroll
+  for (int off = 32; off >= 1; off >>= 1) v = max(v, __shfl_xor(v, off));
+  return v;
+}
+
+// The V-cycle below level kw (levels kw..nlev-1, all one-wave levels), run by
+// the 64 lanes of one wave.  Level kw's x is read from x[par_in] and its
+// result written to x[par_out]; b of level kw is read from memory, the other
+// levels' b and x stay in registers and in `save` (lane-private LDS rows,
+// indexed by k - kw).
+__device__ void wave_cycle(const LevelDev *__restrict__ Ls, int kw, int nlev, int par_in,
+                           int par_out, int n_pre, int n_post, int n_coarsest,
+                           double (*save)[2][kSlice]) {
+  const int lane = threadIdx.x;
+  WaveRow R;
+  double x = lane < Ls[kw].n ? Ls[kw].x[par_in][lane] : 0.0;
+  double b = lane < Ls[kw].n ? Ls[kw].b[lane] : 0.0;
+  for (int k = kw; k < nlev - 1; ++k) {
+    const LevelDev &L = Ls[k];
+    wave_load(L, lane, R);
+    for (int it = 0; it < n_pre; ++it) x = wave_jacobi(R, lane, x, b);
+    const double ax = wave_ax(R, lane, x);
+    const double r = lane < L.n ? b - ax : 0.0;
+    save[k - kw][0][lane] = x;
+    save[k - kw][1][lane] = b;
+    int a0 = 0, len = 0;
+    if (lane < L.nc) {
+      a0 = L.agg_ptr[lane];
+      len = L.agg_ptr[lane + 1] - a0;
+    }
+    const int maxlen = wave_max(len);
+    double s = 0.0;
+    for (int q = 0; q < maxlen; ++q) {
+      const double v = __shfl(r, q < len ? L.agg_cells[a0 + q] : 0);
+      if (q < len) s += v;
+    }
+    b = s;
+    x = 0.0;
+  }
+  wave_load(Ls[nlev - 1], lane, R);
+  for (int it = 0; it < n_coarsest; ++it) x = wave_jacobi(R, lane, x, b);
+  for (int k = nlev - 2; k >= kw; --k) {
+    const LevelDev &L = Ls[k];
+    const double xc = x;
+    const int f = lane < L.n ? L.f2c[lane] : 0;
+    const double e = __shfl(xc, f);
+    x = lane < L.n ? save[k - kw][0][lane] + e : 0.0;
+    b = save[k - kw][1][lane];
+    wave_load(L, lane, R);
+    for (int it = 0; it < n_post; ++it) x = wave_jacobi(R, lane, x, b);
+  }
+  if (lane < Ls[kw].n) Ls[kw].x[par_out][lane] = x;
+}
+
 __global__ __launch_bounds__(kTailThreads) void k_tail(const LevelDev *__restrict__ Ls, int k0,
                                                         int nlev, int cur0, int n_pre, int n_post,
                                                         int n_coarsest, int gs, const KState *st) {
+  __shared__ double save[kMaxWaveLevels][2][kSlice];
   if (st->done) return;
   int cur[kMaxLevels];
   cur[k0] = cur0;
-  for (int k = k0; k < nlev - 1; ++k) {
+  // kw: first level of the one-wave part (every level from kw on fits one
+  // wave); nlev when there is none.  Jacobi smoothing only.
+  int kw = nlev;
+  if (!gs)
+    while (kw > k0 && nlev - (kw - 1) <= kMaxWaveLevels && wave_level(Ls[kw - 1])) --kw;
+  const int kb = kw < nlev ? kw : nlev - 1;  // block-path levels: [k0, kb)
+  for (int k = k0; k < kb; ++k) {
     const LevelDev L = Ls[k];
     for (int it = 0; it < n_pre; ++it) tail_smooth(L, cur[k], gs);
     for (int row0 = threadIdx.x; row0 < L.nslices * kSlice; row0 += kTailThreads) {
@@ -209,11 +341,20 @@ __global__ __launch_bounds__(kTailThreads) void k_tail(const LevelDev *__restric
     cur[k + 1] = 0;
     __syncthreads();
   }
-  {
+  if (kw < nlev) {
+    // Level kw's buffer parity after its smoothing steps, as the block path
+    // would leave it (every Jacobi step flips it).
+    const int flips = kw == nlev - 1 ? n_coarsest : n_pre + n_post;
+    const int pin = cur[kw], pout = (pin + flips) & 1;
+    if (threadIdx.x < kSlice)
+      wave_cycle(Ls, kw, nlev, pin, pout, n_pre, n_post, n_coarsest, save);
+    cur[kw] = pout;
+    __syncthreads();
+  } else {
     const LevelDev L = Ls[nlev - 1];
     for (int it = 0; it < n_coarsest; ++it) tail_smooth(L, cur[nlev - 1], gs);
   }
-  for (int k = nlev - 2; k >= k0; --k) {
+  for (int k = kb - 1; k >= k0; --k) {
     const LevelDev L = Ls[k];
     const double *xc = Ls[k + 1].x[cur[k + 1]];
     for (int i = threadIdx.x; i < L.n; i += kTailThreads)
@@ -567,9 +708,18 @@ inline int tail_level(const pfm_gamg_s *g) {
   return k;
 }
 
-// Levels computed in FP32: the first fp32_levels, and never a tail level.
+// First level with at most kFp32MinCells cells (it and all coarser levels
+// stay FP64 in the mixed-precision cycles).
+inline int fp64_level(const pfm_gamg_s *g) {
+  int k = (int)g->lv.size() - 1;
+  while (k > 0 && g->lv[k - 1].n <= kFp32MinCells) --k;
+  return k;
+}
+
+// Levels computed in FP32: the first fp32_levels, and never one of at most
+// kFp32MinCells cells.
 inline int fp32_count(const pfm_gamg_s *g, int fp32_levels) {
-  return std::min(fp32_levels, tail_level(g));
+  return std::min(fp32_levels, fp64_level(g));
 }
 
 size_t gamg_mixed_layout(const pfm_gamg_s *g, int max_outer, int fp32_levels, GamgMixedWork *m,

@@ -108,6 +108,40 @@ def test_gamg_gauss_seidel_bit_exact(gpu_device, n, jit, sig):
     np.testing.assert_allclose(solver.last_info.residual_history, info.residual_history, rtol=1e-12)
 
 
+@pytest.mark.parametrize("n,jit,sig", [(16, 0.2, 1.0), (6, 0.2, 1.0)])
+def test_gamg_pair_hierarchy_bit_exact(gpu_device, n, jit, sig):
+    """Jacobi V-cycles over a deep pair hierarchy (down to a few cells) against
+    the oracle's V-cycle over the oracle's pair maps and coarse matrices,
+    bit-exact after 6 cycles.  At 16^3 one cycle crosses every execution
+    mode: grid launches (4096 .. 512 cells), the one-workgroup tail (256, 128)
+    and the one-wave levels (64 cells and fewer, registers and cross-lane
+    reads); at 6^3 the whole hierarchy is in the tail."""
+    c, d, lo, up = system(n, jit, sig)
+    print(f"gamg pair n={n}", flush=True)
+    m = _matrix(c.n_cells, c.owner, c.neighbour, d, lo, up, gpu_device)
+    ncyc = 6
+    solver = GAMGSolver(tolerance=1e-300, rel_tol=1e-300, max_iter=ncyc, agglomeration="pair",
+                        coarse="reference", max_levels=20)
+    sizes = solver.hierarchy(m).level_sizes()
+    x, it, _ = solver(m, torch.from_numpy(c.b), torch.zeros(c.n_cells, dtype=torch.float64))
+    lv = ogamg.pair_levels(c.n_cells, c.owner, c.neighbour, n_levels=len(sizes) - 1)
+    levels = [ogamg.Level(c.n_cells, d, lo, up, c.owner, c.neighbour)]
+    aggs, prev = [], np.arange(c.n_cells)
+    for L in lv:
+        f2c = np.empty(levels[-1].n, np.int64)
+        f2c[prev] = L["cell_map"]
+        aggs.append((L["n_cells_coarse"], f2c))
+        levels.append(ogamg.coarse_matrix(levels[-1], L["n_cells_coarse"], f2c)[0])
+        prev = L["cell_map"]
+    assert [(v.n, len(v.owner)) for v in levels] == [tuple(s) for s in sizes]
+    assert min(sizes)[0] <= 8 and len(sizes) >= 4
+    xo = np.zeros(c.n_cells)
+    for _ in range(ncyc):
+        xo = ogamg.v_cycle(levels, aggs, 0, xo, c.b, 2, 2)
+    assert it == ncyc
+    assert bits_equal(x.cpu().numpy(), xo)
+
+
 def _pair(c, d, lo, up, dev, **kw):
     m = _matrix(c.n_cells, c.owner, c.neighbour, d, lo, up, dev)
     base = dict(tolerance=1e-300, agglomeration="pair", coarse="galerkin", max_levels=12)
@@ -118,8 +152,8 @@ def _pair(c, d, lo, up, dev, **kw):
 @pytest.mark.parametrize("n", [8, 24])
 def test_gamg_mixed_one_cycle_matches_fp64_cycle(gpu_device, n):
     """One refinement step with one inner V-cycle from x0 = 0: x = e, the
-    V-cycle applied to r = b.  8^3 (every level in the FP64 tail): bit-exact
-    with one FP64 cycle.  24^3 (level 0 in FP32, an odd number of Jacobi
+    V-cycle applied to r = b.  8^3 (every level FP64: at most 4096 cells):
+    bit-exact with one FP64 cycle.  24^3 (level 0 in FP32, an odd number of Jacobi
     sweeps per level so the buffer parity flips): the FP64 cycle to FP32
     rounding (rel 1e-5)."""
     c, d, lo, up = system(n, 0.2, 1.0)
