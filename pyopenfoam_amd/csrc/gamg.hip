@@ -38,6 +38,7 @@ namespace {
 
 constexpr int kTailCells = 256;
 constexpr int kFp32MinCells = 4096;
+constexpr int kFuseCells = 65536;  // levels whose launches cost more than their traffic
 constexpr int kTailThreads = 1024;
 constexpr int kMaxLevels = 64;
 
@@ -152,6 +153,27 @@ __global__ __launch_bounds__(kBlock) void k_prolong(LevelDevT<TF> F, int cur, co
   const int i = blockIdx.x * kBlock + threadIdx.x;
   if (i >= F.n) return;
   F.x[cur][i] = F.x[cur][i] + (TF)xc[F.f2c[i]];
+}
+
+// Prolongation fused with the first post-smoothing Jacobi sweep: the sweep
+// reads x_j + e[f2c[j]] (k_prolong's value) wherever it reads x_j and writes
+// the swept values to x[cur ^ 1].
+template <typename TF, typename TC>
+__global__ __launch_bounds__(kBlock) void k_prolong_jacobi(LevelDevT<TF> F, int cur,
+                                                            const TC *__restrict__ xc,
+                                                            const KState *st) {
+  if (st->done) return;
+  const int s = xcd_remap(blockIdx.x, gridDim.x) * kWavesPerBlock + (threadIdx.x >> 6);
+  if (s >= F.nslices) return;
+  const int lane = threadIdx.x & 63, row = s * kSlice + lane;
+  const TF *x = F.x[cur];
+  const int32_t *f2c = F.f2c;
+  auto xp = [=](int j) { return x[j] + (TF)xc[f2c[j]]; };
+  const SliceInfo si = load_slice(F.slices, s);
+  const TF xi = row < F.n ? xp(row) : TF(0);
+  const TF dx = row < F.n ? F.diag[row] * xi : TF(0);
+  const TF ax = row_spmv_g<TF>(si, lane, dx, ColAbs{F.col}, F.packed, xp);
+  if (row < F.n) F.x[cur ^ 1][row] = xi + F.inv[row] * (F.b[row] - ax);
 }
 
 // Single-workgroup V-cycle of levels k0..nlev-1 (small levels, always FP64).
@@ -903,19 +925,29 @@ void enqueue_cycle(const pfm_gamg_s *g, const CycleLevels<T> &C, const KState *s
                                     gs ? 1 : 0, st);
   if (!gs) cur[tail_k] ^= (tail_k == L - 1 ? ncoarse : npre + npost) & 1;
   for (int k = tail_k - 1; k >= 0; --k) {  // ascend
+    // On levels of at most kFuseCells cells (latency-bound launches) the
+    // prolongation is folded into the first post-sweep.
+    const bool pj = !gs && npost >= 1 && g->lv[k].n <= kFuseCells;
+    auto prolong = [&](const auto &F, const auto *xc) {
+      using TF = typename std::remove_const<typename std::remove_pointer<decltype(F.packed)>::type>::type;
+      using TC = typename std::remove_const<typename std::remove_pointer<decltype(xc)>::type>::type;
+      if (pj) {
+        k_prolong_jacobi<TF, TC><<<blocks_for_slices(F.nslices), kBlock, 0, s>>>(F, cur[k], xc, st);
+        cur[k] ^= 1;
+      } else {
+        k_prolong<TF, TC><<<grid1d(F.n), kBlock, 0, s>>>(F, cur[k], xc, st);
+      }
+    };
     if (k < nbig) {
       const LevelDevT<T> &F = C.big[k];
       if (k + 1 < nbig)
-        k_prolong<T, T><<<grid1d(F.n), kBlock, 0, s>>>(F, cur[k], C.big[k + 1].x[cur[k + 1]], st);
+        prolong(F, C.big[k + 1].x[cur[k + 1]]);
       else
-        k_prolong<T, double><<<grid1d(F.n), kBlock, 0, s>>>(F, cur[k],
-                                                           C.small[k + 1].x[cur[k + 1]], st);
+        prolong(F, C.small[k + 1].x[cur[k + 1]]);
     } else {
-      const LevelDev &F = C.small[k];
-      k_prolong<double, double><<<grid1d(F.n), kBlock, 0, s>>>(F, cur[k],
-                                                              C.small[k + 1].x[cur[k + 1]], st);
+      prolong(C.small[k], C.small[k + 1].x[cur[k + 1]]);
     }
-    for (int it = 0; it < npost; ++it) smooth(k, cur[k]);
+    for (int it = pj ? 1 : 0; it < npost; ++it) smooth(k, cur[k]);
   }
   cur0 = cur[0];
 }
