@@ -395,6 +395,8 @@ def main() -> int:
     ap.add_argument("--no-pressure", action="store_true")
     ap.add_argument("--no-family", action="store_true")
     ap.add_argument("--gamg-max-iter", type=int, default=1000)
+    ap.add_argument("--gamg-variants", default="",
+                    help="comma-separated GAMG variants to run (default: all)")
     ap.add_argument("--batch-systems", type=int, default=512)
     ap.add_argument("--batch-nmin", type=int, default=8)
     ap.add_argument("--batch-nmax", type=int, default=48)
@@ -579,6 +581,8 @@ def main() -> int:
                          ("reference", dict()),
                          ("pcg_gamg_precond", dict(pcg=True)),
                          ("pcg_gamg_precond_mixed", dict(pcg=True, precision="mixed"))):
+            if args.gamg_variants and name not in args.gamg_variants.split(","):
+                continue
             if kw.get("pcg"):  # PCG with one pair/Galerkin V-cycle as the preconditioner
                 kwp = {k: v for k, v in kw.items() if k != "pcg"}
                 gs = PCGSolver(tolerance=1e-300, rel_tol=1e-8, max_iter=args.gamg_max_iter,
