@@ -378,8 +378,8 @@ def bench_family(mat, dev):
 def main() -> int:
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
-    ap.add_argument("--steps", type=int, default=3)
-    ap.add_argument("--warmup", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=5)
+    ap.add_argument("--warmup", type=int, default=2)
     ap.add_argument("--n", type=int, default=128)
     ap.add_argument("--cpu-budget", type=float, default=15.0, help="seconds of oracle CPU work")
     ap.add_argument("--no-cpu", action="store_true")
@@ -457,10 +457,13 @@ def main() -> int:
         torch.cuda.synchronize()
         log(f"warmup {w}: iters {iters}, converged {solver.last_info.converged}")
     torch.cuda.synchronize()
+    step_ms = []
     t0 = time.perf_counter()
     for s in range(args.steps):
+        ts = time.perf_counter()
         x, iters, res = step()
-    torch.cuda.synchronize()
+        torch.cuda.synchronize()
+        step_ms.append((time.perf_counter() - ts) * 1e3)
     dt = time.perf_counter() - t0
     ms = dt / args.steps * 1e3
     converged = bool(solver.last_info.converged)
@@ -645,6 +648,8 @@ def main() -> int:
         "steps": args.steps,
         "warmup": args.warmup,
         "ms_per_step": round(ms, 3),
+        "step_ms": [round(v, 3) for v in step_ms],
+        "step_ms_median": round(float(np.median(step_ms)), 3),
         "higher_is_better": False,
         "scaling": "weak",
         "vs_baseline": None,
