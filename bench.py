@@ -580,7 +580,7 @@ def main() -> int:
                                                         max_levels=20)),
                          ("pair_galerkin_jacobi_mixed", dict(agglomeration="pair", coarse="galerkin",
                                                               max_levels=20, precision="mixed",
-                                                              inner_rel_tol=1e-2)),
+                                                              inner_rel_tol=0.1)),
                          ("reference", dict()),
                          ("pcg_gamg_precond", dict(pcg=True)),
                          ("pcg_gamg_precond_mixed", dict(pcg=True, precision="mixed"))):
