@@ -15,11 +15,13 @@
 //     levels, the single-workgroup tail walks the levels of small ones).
 // Levels at or below kTailCells run, together with everything coarser, in a
 // single-workgroup kernel (one launch instead of dozens of tiny ones); inside
-// it, the levels of at most 64 cells are held by one wave in registers
-// (cross-lane reads instead of memory round trips and barriers).  Larger
-// levels run as grid launches: measured at 128^3, a 4096-cell level costs
-// less as seven ~4.6 us launches than as barrier-separated phases of one
-// workgroup, whose rows are chains of dependent loads.  After
+// it, each row is held by one thread in registers (x exchanged through LDS
+// between barriers), and the levels of at most 64 cells by one wave
+// (cross-lane reads, no barriers).  Larger levels run as grid launches:
+// measured at 128^3, a 4096-cell level costs less as seven ~4.6 us launches
+// than as barrier-separated phases of one workgroup walking its rows with
+// chains of dependent loads (the tail's fallback when a level does not fit
+// the registers, and the Gauss-Seidel path).  After
 // each cycle r = b - A x is reduced and the residual monitor runs on the
 // device (GAMG records no initial residual, :574-582).  All arithmetic is
 // element-wise in a fixed order, so the solution is bit-identical to the
@@ -36,7 +38,7 @@ using namespace pfm;
 
 namespace {
 
-constexpr int kTailCells = 256;
+constexpr int kTailCells = 1024;
 constexpr int kFp32MinCells = 4096;
 constexpr int kFuseCells = 65536;  // levels whose launches cost more than their traffic
 constexpr int kTailThreads = 1024;
@@ -223,7 +225,8 @@ __device__ __forceinline__ void tail_smooth(const LevelDev &L, int &cur, bool gs
 // aggregate sums from 0 in fine-cell order.  Shuffles run on all 64 lanes
 // (loops with wave-uniform bounds); lanes >= n carry zeros.
 constexpr int kWaveSlots = 8;     // slots per group kept in registers
-constexpr int kMaxWaveLevels = 16;  // levels of the one-wave part (LDS rows kept)
+constexpr int kMaxWaveLevels = 8;   // levels of the one-wave part (LDS rows kept)
+constexpr int kRegCells = 2048;     // cells of the one-row-per-thread levels (LDS rows kept)
 
 struct WaveRow {
   int cl[kWaveSlots], cu[kWaveSlots];
@@ -240,10 +243,12 @@ __device__ __forceinline__ bool wave_level(const LevelDev &L) {
   return si.wl <= kWaveSlots && si.wu <= kWaveSlots;
 }
 
-__device__ __forceinline__ void wave_load(const LevelDev &L, int lane, WaveRow &R) {
+// Row t's couplings (slice t / 64, lane t % 64; t is wave-uniform per slice).
+__device__ __forceinline__ void wave_load(const LevelDev &L, int t, WaveRow &R) {
   R.n = L.n;
   SliceInfo si{0, 0, 0};
-  if (L.nslices > 0) si = load_slice(L.slices, 0);
+  if ((t >> 6) < L.nslices) si = load_slice(L.slices, t >> 6);
+  const int lane = t & 63;
   const int bl = si.ptr + lane, bu = si.ptr + si.wl * kSlice + lane;
 #pragma unroll
   for (int j = 0; j < kWaveSlots; ++j) {
@@ -252,8 +257,8 @@ __device__ __forceinline__ void wave_load(const LevelDev &L, int lane, WaveRow &
     R.cu[j] = j < si.wu ? L.col[bu + j * kSlice] : -1;
     R.vu[j] = j < si.wu ? L.packed[bu + j * kSlice] : 0.0;
   }
-  R.d = lane < L.n ? L.diag[lane] : 0.0;
-  R.inv = lane < L.n ? L.inv[lane] : 0.0;
+  R.d = t < L.n ? L.diag[t] : 0.0;
+  R.inv = t < L.n ? L.inv[t] : 0.0;
 }
 
 __device__ __forceinline__ double wave_ax(const WaveRow &R, int lane, double x) {
@@ -284,17 +289,14 @@ __device__ __forceinline__ int wave_max(int v) {
 }
 
 // The V-cycle below level kw (levels kw..nlev-1, all one-wave levels), run by
-// the 64 lanes of one wave.  Level kw's x is read from x[par_in] and its
-// result written to x[par_out]; b of level kw is read from memory, the other
-// levels' b and x stay in registers and in `save` (lane-private LDS rows,
-// indexed by k - kw).
-__device__ void wave_cycle(const LevelDev *__restrict__ Ls, int kw, int nlev, int par_in,
-                           int par_out, int n_pre, int n_post, int n_coarsest,
-                           double (*save)[2][kSlice]) {
+// the 64 lanes of one wave: lane i enters with level kw's x and b of cell i
+// and returns its x after the cycle.  The other levels' b and x stay in
+// registers and in `save` (lane-private LDS rows, indexed by k - kw).
+__device__ double wave_cycle(const LevelDev *__restrict__ Ls, int kw, int nlev, double x,
+                             double b, int n_pre, int n_post, int n_coarsest,
+                             double (*save)[2][kSlice]) {
   const int lane = threadIdx.x;
   WaveRow R;
-  double x = lane < Ls[kw].n ? Ls[kw].x[par_in][lane] : 0.0;
-  double b = lane < Ls[kw].n ? Ls[kw].b[lane] : 0.0;
   for (int k = kw; k < nlev - 1; ++k) {
     const LevelDev &L = Ls[k];
     wave_load(L, lane, R);
@@ -329,22 +331,73 @@ __device__ void wave_cycle(const LevelDev *__restrict__ Ls, int kw, int nlev, in
     wave_load(L, lane, R);
     for (int it = 0; it < n_post; ++it) x = wave_jacobi(R, lane, x, b);
   }
-  if (lane < Ls[kw].n) Ls[kw].x[par_out][lane] = x;
+  return x;
+}
+
+// ---- levels of at most kTailThreads cells, one row per thread ----
+// Thread t holds row t's couplings in registers; the x values a sweep reads
+// are exchanged through LDS (xs) between two workgroup barriers, the
+// residual through rs.  Same association as everywhere: (diag*x + sum_lower)
+// + sum_upper, aggregate sums from 0 in fine-cell order.
+__device__ __forceinline__ double reg_ax(const WaveRow &R, int t, double x,
+                                         const double *__restrict__ xs) {
+  const double dx = t < R.n ? R.d * x : 0.0;
+  double accL = 0.0, accU = 0.0;
+#pragma unroll
+  for (int j = 0; j < kWaveSlots; ++j)
+    if (R.cl[j] >= 0) accL += R.vl[j] * xs[R.cl[j]];
+#pragma unroll
+  for (int j = 0; j < kWaveSlots; ++j)
+    if (R.cu[j] >= 0) accU += R.vu[j] * xs[R.cu[j]];
+  return (dx + accL) + accU;
+}
+
+__device__ __forceinline__ double reg_jacobi(const WaveRow &R, int t, double x, double b,
+                                             double *xs) {
+  xs[t] = x;
+  __syncthreads();
+  const double ax = reg_ax(R, t, x, xs);
+  __syncthreads();
+  return t < R.n ? x + R.inv * (b - ax) : 0.0;
+}
+
+// Whether level L can run one row per thread (every slice's groups fit the
+// registers).  Called by the whole workgroup.
+__device__ bool reg_level(const LevelDev &L) {
+  if (L.n > kTailThreads) return false;
+  int ok = 1;
+  for (int s = threadIdx.x; s < L.nslices; s += kTailThreads) {
+    const SliceInfo si = load_slice_lane(L.slices, s);
+    ok &= si.wl <= kWaveSlots && si.wu <= kWaveSlots;
+  }
+  return __syncthreads_and(ok) != 0;
 }
 
 __global__ __launch_bounds__(kTailThreads) void k_tail(const LevelDev *__restrict__ Ls, int k0,
                                                         int nlev, int cur0, int n_pre, int n_post,
                                                         int n_coarsest, int gs, const KState *st) {
   __shared__ double save[kMaxWaveLevels][2][kSlice];
+  __shared__ double rsave[2][kRegCells];
+  __shared__ double xs[kTailThreads], rs[kTailThreads];
   if (st->done) return;
   int cur[kMaxLevels];
   cur[k0] = cur0;
   // kw: first level of the one-wave part (every level from kw on fits one
-  // wave); nlev when there is none.  Jacobi smoothing only.
+  // wave); nlev when there is none.  kt: first level of the one-row-per-
+  // thread part [kt, kw) just above it (kt = kw when there is none).  Both
+  // for Jacobi smoothing only; levels [k0, kb) take the block path.
   int kw = nlev;
   if (!gs)
     while (kw > k0 && nlev - (kw - 1) <= kMaxWaveLevels && wave_level(Ls[kw - 1])) --kw;
-  const int kb = kw < nlev ? kw : nlev - 1;  // block-path levels: [k0, kb)
+  int kt = kw;
+  if (kw < nlev) {
+    int cells = 0;
+    while (kt > k0 && cells + Ls[kt - 1].n <= kRegCells && reg_level(Ls[kt - 1])) {
+      cells += Ls[kt - 1].n;
+      --kt;
+    }
+  }
+  const int kb = kw < nlev ? kt : nlev - 1;
   for (int k = k0; k < kb; ++k) {
     const LevelDev L = Ls[k];
     for (int it = 0; it < n_pre; ++it) tail_smooth(L, cur[k], gs);
@@ -364,13 +417,52 @@ __global__ __launch_bounds__(kTailThreads) void k_tail(const LevelDev *__restric
     __syncthreads();
   }
   if (kw < nlev) {
-    // Level kw's buffer parity after its smoothing steps, as the block path
-    // would leave it (every Jacobi step flips it).
-    const int flips = kw == nlev - 1 ? n_coarsest : n_pre + n_post;
-    const int pin = cur[kw], pout = (pin + flips) & 1;
-    if (threadIdx.x < kSlice)
-      wave_cycle(Ls, kw, nlev, pin, pout, n_pre, n_post, n_coarsest, save);
-    cur[kw] = pout;
+    // Levels kt..nlev-1 in registers.  Level kt enters from memory (x[cur],
+    // b) and leaves in x[pout], the buffer the block path would leave it in
+    // (every Jacobi step flips the parity).
+    const int t = threadIdx.x;
+    const int flips = kt == nlev - 1 ? n_coarsest : n_pre + n_post;
+    const int pout = (cur[kt] + flips) & 1;
+    double x = t < Ls[kt].n ? Ls[kt].x[cur[kt]][t] : 0.0;
+    double b = t < Ls[kt].n ? Ls[kt].b[t] : 0.0;
+    WaveRow R;
+    int off = 0;
+    for (int k = kt; k < kw; ++k) {  // one row per thread, descending
+      const LevelDev &L = Ls[k];
+      wave_load(L, t, R);
+      for (int it = 0; it < n_pre; ++it) x = reg_jacobi(R, t, x, b, xs);
+      xs[t] = x;
+      __syncthreads();
+      const double ax = reg_ax(R, t, x, xs);
+      rs[t] = t < L.n ? b - ax : 0.0;
+      if (t < L.n) {
+        rsave[0][off + t] = x;
+        rsave[1][off + t] = b;
+      }
+      __syncthreads();
+      double sb = 0.0;
+      if (t < L.nc)
+        for (int q = L.agg_ptr[t]; q < L.agg_ptr[t + 1]; ++q) sb += rs[L.agg_cells[q]];
+      b = sb;
+      x = 0.0;
+      off += L.n;
+      __syncthreads();
+    }
+    if (t < kSlice) x = wave_cycle(Ls, kw, nlev, x, b, n_pre, n_post, n_coarsest, save);
+    for (int k = kw - 1; k >= kt; --k) {  // ascending
+      const LevelDev &L = Ls[k];
+      off -= L.n;
+      xs[t] = t < Ls[k + 1].n ? x : 0.0;
+      wave_load(L, t, R);
+      const int f = t < L.n ? L.f2c[t] : 0;
+      __syncthreads();
+      x = t < L.n ? rsave[0][off + t] + xs[f] : 0.0;
+      b = t < L.n ? rsave[1][off + t] : 0.0;
+      __syncthreads();
+      for (int it = 0; it < n_post; ++it) x = reg_jacobi(R, t, x, b, xs);
+    }
+    if (t < Ls[kt].n) Ls[kt].x[pout][t] = x;
+    cur[kt] = pout;
     __syncthreads();
   } else {
     const LevelDev L = Ls[nlev - 1];

@@ -112,9 +112,10 @@ def test_gamg_gauss_seidel_bit_exact(gpu_device, n, jit, sig):
 def test_gamg_pair_hierarchy_bit_exact(gpu_device, n, jit, sig):
     """Jacobi V-cycles over a deep pair hierarchy (down to a few cells) against
     the oracle's V-cycle over the oracle's pair maps and coarse matrices,
-    bit-exact after 6 cycles.  At 16^3 one cycle crosses every execution
-    mode: grid launches (4096 .. 512 cells), the one-workgroup tail (256, 128)
-    and the one-wave levels (64 cells and fewer, registers and cross-lane
+    bit-exact after 6 cycles.  At 16^3 one cycle crosses every Jacobi
+    execution mode: grid launches (4096, 2048 cells), the one-workgroup tail's
+    one-row-per-thread levels (1024 .. 128: registers, x exchanged through
+    LDS) and its one-wave levels (64 cells and fewer: registers, cross-lane
     reads); at 6^3 the whole hierarchy is in the tail."""
     c, d, lo, up = system(n, jit, sig)
     print(f"gamg pair n={n}", flush=True)
