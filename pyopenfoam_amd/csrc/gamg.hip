@@ -233,6 +233,7 @@ struct WaveRow {
   double vl[kWaveSlots], vu[kWaveSlots];
   double d, inv;
   int n;
+  int wl, wu;  // the slice's group widths (uniform across its 64 rows)
 };
 
 // Whether level L can be held by one wave.
@@ -248,6 +249,8 @@ __device__ __forceinline__ void wave_load(const LevelDev &L, int t, WaveRow &R) 
   R.n = L.n;
   SliceInfo si{0, 0, 0};
   if ((t >> 6) < L.nslices) si = load_slice(L.slices, t >> 6);
+  R.wl = si.wl;
+  R.wu = si.wu;
   const int lane = t & 63;
   const int bl = si.ptr + lane, bu = si.ptr + si.wl * kSlice + lane;
 #pragma unroll
@@ -264,15 +267,21 @@ __device__ __forceinline__ void wave_load(const LevelDev &L, int t, WaveRow &R) 
 __device__ __forceinline__ double wave_ax(const WaveRow &R, int lane, double x) {
   const double dx = lane < R.n ? R.d * x : 0.0;
   double accL = 0.0, accU = 0.0;
+  // Slots at or beyond the slice's width are padding on every lane: skip
+  // their cross-lane reads (the width is wave-uniform).
 #pragma unroll
   for (int j = 0; j < kWaveSlots; ++j) {
-    const double xv = __shfl(x, R.cl[j] >= 0 ? R.cl[j] : 0);
-    if (R.cl[j] >= 0) accL += R.vl[j] * xv;
+    if (j < R.wl) {
+      const double xv = __shfl(x, R.cl[j] >= 0 ? R.cl[j] : 0);
+      if (R.cl[j] >= 0) accL += R.vl[j] * xv;
+    }
   }
 #pragma unroll
   for (int j = 0; j < kWaveSlots; ++j) {
-    const double xv = __shfl(x, R.cu[j] >= 0 ? R.cu[j] : 0);
-    if (R.cu[j] >= 0) accU += R.vu[j] * xv;
+    if (j < R.wu) {
+      const double xv = __shfl(x, R.cu[j] >= 0 ? R.cu[j] : 0);
+      if (R.cu[j] >= 0) accU += R.vu[j] * xv;
+    }
   }
   return (dx + accL) + accU;
 }
