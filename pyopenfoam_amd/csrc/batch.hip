@@ -303,6 +303,14 @@ __device__ __forceinline__ XcdRange xcd_range(int nact) {
     }                                                                           \
   }
 
+// Latency: in the tail of a batch only a few tiles remain active and each
+// wave's slices are the whole of its work, so the row loops below keep two
+// slices' loads in flight (the next slice's operands are loaded before the
+// current slice's stores, or two SpMV rows are formed side by side), and a
+// tile's first operands are loaded before its stop flag and partial sums are
+// read.  Per-row arithmetic and summation order are those of one slice at a
+// time.
+
 // Ap = A p; tile partial p.Ap.  Tiles of a system that stopped earlier in the
 // chunk compute values nobody reads, so this kernel skips the stop check.
 __global__ __launch_bounds__(kBlock) void k_b_spmv(const int4 *__restrict__ act,
@@ -318,14 +326,26 @@ __global__ __launch_bounds__(kBlock) void k_b_spmv(const int4 *__restrict__ act,
   __shared__ double lds[kMaxWaves];
   FOR_ACTIVE_TILES(false, {
     (void)k;
+    const int lane = threadIdx.x & 63;
+    const int nw = blockDim.x >> 6;
     double pap = 0.0;
-    TILE_ROWS({
-      const SliceInfo si = load_slice(sl, s);
-      const double pi = p[row];
-      const double v = row_spmv<double>(si, lane, diag[row] * pi, col, val, p);
-      Ap[row] = v;
-      pap += pi * v;
-    })
+    for (int s = s0 + (threadIdx.x >> 6); s < s1; s += 2 * nw) {
+      // rows of slices s and s + nw (the latter recomputed from s when past
+      // the tile's end, and then not stored), formed side by side
+      const bool two = s + nw < s1;
+      const int sb = two ? s + nw : s;
+      const SliceInfo sa_ = load_slice(sl, s), sb_ = load_slice(sl, sb);
+      const int ra = s * kSlice + lane, rb = sb * kSlice + lane;
+      const double pa = p[ra], pb = p[rb];
+      const double va = row_spmv<double>(sa_, lane, diag[ra] * pa, col, val, p);
+      const double vb = row_spmv<double>(sb_, lane, diag[rb] * pb, col, val, p);
+      Ap[ra] = va;
+      pap += pa * va;
+      if (two) {
+        Ap[rb] = vb;
+        pap += pb * vb;
+      }
+    }
     pap = wg_sum(pap, lds);
     if (threadIdx.x == 0) part[t] = pap;
   })
@@ -342,9 +362,33 @@ __global__ __launch_bounds__(kBlock) void k_b_update(const int4 *__restrict__ ac
                                                       double *__restrict__ part,
                                                       BGlobal *g, BSys *sys, int j) {
   __shared__ double lds[kMaxWaves];
+  __shared__ int s_done;
   const int it = g->base + j + 1;
-  FOR_ACTIVE_TILES(true, {
+  const int lane = threadIdx.x & 63;
+  const int nw = blockDim.x >> 6;
+  const XcdRange rg = xcd_range(g->nact);
+  for (int i = rg.begin; i < rg.end; i += rg.step) {
+    const int4 dsc = act[i];
+    const int t = dsc.x, k = dsc.y, s0 = dsc.z, s1 = dsc.w;
+    int s = s0 + (threadIdx.x >> 6);
+    int row = s * kSlice + lane;
+    double xv = 0.0, pv = 0.0, rv = 0.0, av = 0.0, dv = 1.0;
+    if (s < s1) {
+      xv = x[row];
+      pv = p[row];
+      rv = r[row];
+      av = Ap[row];
+      if (rD) dv = rD[row];
+    }
+    // The tile range and r.z of the previous iteration depend only on k and
+    // are not written by this kernel: load them with the stop flag.
     const int t0 = sys_t0[k], t1 = sys_t0[k + 1];
+    const double rz_old = sys[k].rz[(it - 1) & 1];
+    if (threadIdx.x == 0) s_done = sys[k].done;  // see FOR_ACTIVE_TILES
+    __syncthreads();
+    const int done_k = s_done;
+    __syncthreads();
+    if (done_k) continue;
     const double PAP = sys_sum(part, t0, t1, lds);
     if (fabs(PAP) < 1e-30) {
       if (t == t0 && threadIdx.x == 0) {
@@ -354,23 +398,38 @@ __global__ __launch_bounds__(kBlock) void k_b_update(const int4 *__restrict__ ac
       }
       continue;
     }
-    const double alpha = sys[k].rz[(it - 1) & 1] / PAP;
+    const double alpha = rz_old / PAP;
     double rr = 0.0, rz = 0.0;
-    TILE_ROWS({
-      x[row] = x[row] + alpha * p[row];
-      const double ri = r[row] - alpha * Ap[row];
+    for (; s < s1; s += nw) {
+      const int sn = s + nw, rn = sn * kSlice + lane;
+      double xn = 0.0, pn = 0.0, rnx = 0.0, an = 0.0, dn = 1.0;
+      if (sn < s1) {
+        xn = x[rn];
+        pn = p[rn];
+        rnx = r[rn];
+        an = Ap[rn];
+        if (rD) dn = rD[rn];
+      }
+      x[row] = xv + alpha * pv;
+      const double ri = rv - alpha * av;
       r[row] = ri;
-      const double zi = rD ? ri / rD[row] : ri;
+      const double zi = rD ? ri / dv : ri;
       rr += ri * ri;
       rz += ri * zi;
-    })
+      row = rn;
+      xv = xn;
+      pv = pn;
+      rv = rnx;
+      av = an;
+      dv = dn;
+    }
     rr = wg_sum(rr, lds);
     rz = wg_sum(rz, lds);
     if (threadIdx.x == 0) {
       part[ntiles + t] = rr;
       part[2 * ntiles + t] = rz;
     }
-  })
+  }
 }
 
 // monitor per system; beta; p = z + beta p.
@@ -382,9 +441,28 @@ __global__ __launch_bounds__(kBlock) void k_b_pupdate(const int4 *__restrict__ a
                                                        const double *__restrict__ part,
                                                        BGlobal *g, BSys *sys, int j) {
   __shared__ double lds[kMaxWaves];
+  __shared__ int s_done;
   const int it = g->base + j + 1;
-  FOR_ACTIVE_TILES(true, {
+  const int lane = threadIdx.x & 63;
+  const int nw = blockDim.x >> 6;
+  const XcdRange rg = xcd_range(g->nact);
+  for (int i = rg.begin; i < rg.end; i += rg.step) {
+    const int4 dsc = act[i];
+    const int t = dsc.x, k = dsc.y, s0 = dsc.z, s1 = dsc.w;
+    int s = s0 + (threadIdx.x >> 6);
+    int row = s * kSlice + lane;
+    double rv = 0.0, pv = 0.0, dv = 1.0;
+    if (s < s1) {
+      rv = r[row];
+      pv = p[row];
+      if (rD) dv = rD[row];
+    }
     const int t0 = sys_t0[k], t1 = sys_t0[k + 1];
+    if (threadIdx.x == 0) s_done = sys[k].done;  // see FOR_ACTIVE_TILES
+    __syncthreads();
+    const int done_k = s_done;
+    __syncthreads();
+    if (done_k) continue;
     const double RR = sys_sum(part + ntiles, t0, t1, lds);
     const double RZ = sys_sum(part + 2 * ntiles, t0, t1, lds);
     const BSys S = sys[k];
@@ -405,12 +483,22 @@ __global__ __launch_bounds__(kBlock) void k_b_pupdate(const int4 *__restrict__ a
     }
     if (stop) continue;
     const double beta = RZ / S.rz[(it - 1) & 1];
-    TILE_ROWS({
-      const double ri = r[row];
-      const double zi = rD ? ri / rD[row] : ri;
-      p[row] = zi + beta * p[row];
-    })
-  })
+    for (; s < s1; s += nw) {
+      const int sn = s + nw, rn = sn * kSlice + lane;
+      double rnx = 0.0, pn = 0.0, dn = 1.0;
+      if (sn < s1) {
+        rnx = r[rn];
+        pn = p[rn];
+        if (rD) dn = rD[rn];
+      }
+      const double zi = rD ? rv / dv : rv;
+      p[row] = zi + beta * pv;
+      row = rn;
+      rv = rnx;
+      pv = pn;
+      dv = dn;
+    }
+  }
 }
 
 inline size_t align256(size_t b) { return (b + 255) & ~size_t(255); }
