@@ -143,6 +143,33 @@ def test_gamg_pair_hierarchy_bit_exact(gpu_device, n, jit, sig):
     assert bits_equal(x.cpu().numpy(), xo)
 
 
+def test_gamg_tail_wide_row_bit_exact(gpu_device):
+    """A level too wide for the registers (a hub cell coupled to 21 others:
+    22 owner-side slots) takes the one-workgroup block path while the coarser
+    levels, where the hub's neighbours have been aggregated away, run one row
+    per thread (140, 69 cells) and in one wave (34, 17, 9 cells); bit-exact
+    against the oracle's V-cycle after 6 cycles."""
+    n = 300
+    faces = sorted(set((i, i + 1) for i in range(n - 1)) | set((0, j) for j in range(10, 31)))
+    own = np.array([f[0] for f in faces], np.int64)
+    nei = np.array([f[1] for f in faces], np.int64)
+    rng = np.random.default_rng(3)
+    lo = -(0.5 + rng.random(len(faces)))
+    up = -(0.5 + rng.random(len(faces)))
+    d = np.full(n, 0.1)
+    np.add.at(d, own, np.abs(lo))
+    np.add.at(d, nei, np.abs(up))
+    b = rng.uniform(-1.0, 1.0, n)
+    m = _matrix(n, own, nei, d, lo, up, gpu_device)
+    solver = GAMGSolver(tolerance=1e-300, rel_tol=1e-300, max_iter=6)
+    assert [s[0] for s in solver.hierarchy(m).level_sizes()] == [300, 140, 69, 34, 17, 9]
+    x, it, _ = solver(m, torch.from_numpy(b), torch.zeros(n, dtype=torch.float64))
+    xo, io, _, info = ogamg.gamg(ogamg.Level(n, d, lo, up, own, nei), b, np.zeros(n),
+                                 tolerance=1e-300, rel_tol=1e-300, max_iter=6)
+    assert it == io == 6
+    assert bits_equal(x.cpu().numpy(), xo)
+
+
 def _pair(c, d, lo, up, dev, **kw):
     m = _matrix(c.n_cells, c.owner, c.neighbour, d, lo, up, dev)
     base = dict(tolerance=1e-300, agglomeration="pair", coarse="galerkin", max_levels=12)
