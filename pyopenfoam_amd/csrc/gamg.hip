@@ -94,6 +94,23 @@ __global__ __launch_bounds__(kBlock) void k_inv_diag(int n, const double *__rest
   inv[i] = 1.0 / (a < 1e-30 ? 1e-30 : a);
 }
 
+// flag = 0 if any of a0[0..n0), a1[0..n1), a2[0..n2) is not finite (the flag
+// is set to 1 beforehand by k_flag_one).
+template <typename T>
+__global__ __launch_bounds__(kBlock) void k_finite(int64_t n0, const T *__restrict__ a0,
+                                                    int64_t n1, const T *__restrict__ a1,
+                                                    int64_t n2, const T *__restrict__ a2,
+                                                    int32_t *flag) {
+  const int64_t tot = n0 + n1 + n2;
+  for (int64_t i = blockIdx.x * (int64_t)kBlock + threadIdx.x; i < tot;
+       i += (int64_t)gridDim.x * kBlock) {
+    const T v = i < n0 ? a0[i] : i < n0 + n1 ? a1[i - n0] : a2[i - n0 - n1];
+    if (!isfinite(v)) *flag = 0;
+  }
+}
+
+__global__ void k_flag_one(int32_t *flag) { *flag = 1; }
+
 // ------------------------------------------------------------- V-cycle -----
 template <typename T>
 __device__ __forceinline__ T level_ax(const LevelDevT<T> &L, int s, int lane, int row, const T *x) {
@@ -134,18 +151,40 @@ __global__ __launch_bounds__(kBlock) void k_resid(LevelDevT<T> L, int cur, const
   if (row < L.n) L.r[row] = L.b[row] - ax;
 }
 
+// Row c of a level's first Jacobi sweep from x = 0, given b_c, written to
+// x[1] (k_jacobi's value).  With finite coefficients every product in A 0 is
+// a signed zero and the sums start from +0, so A 0 is exactly +0 and the
+// sweep is 0 + inv*(b - 0), element-wise; otherwise the row's products are
+// formed as k_jacobi forms them (NaN and infinity propagate the same way).
+// c runs over whole slices: one wave = 64 consecutive rows = one slice.
+template <typename T>
+__device__ __forceinline__ void sweep_from_zero(const LevelDevT<T> &L, int c, T bc) {
+  if (*L.finite) {
+    if (c < L.n) L.x[1][c] = T(0) + L.inv[c] * (bc - T(0));
+    return;
+  }
+  if (c >= L.nslices * kSlice) return;
+  const SliceInfo si = load_slice(L.slices, c >> 6);
+  const T dx = c < L.n ? L.diag[c] * T(0) : T(0);
+  const T ax = row_spmv_g<T>(si, c & 63, dx, ColAbs{L.col}, L.packed, [](int) { return T(0); });
+  if (c < L.n) L.x[1][c] = T(0) + L.inv[c] * (bc - ax);
+}
+
 // b_c = sum of r over each aggregate (fine-cell order, accumulated in the
-// coarse level's type); x_c[0] = 0.
-template <typename TF, typename TC>
+// coarse level's type); then x_c[0] = 0, or (kSweep) the coarse level's first
+// Jacobi sweep from zero into x_c[1].
+template <typename TF, typename TC, bool kSweep = false>
 __global__ __launch_bounds__(kBlock) void k_restrict(LevelDevT<TF> F, LevelDevT<TC> Cc,
                                                       const KState *st) {
   if (st->done) return;
   const int c = blockIdx.x * kBlock + threadIdx.x;
-  if (c >= F.nc) return;
   TC s = TC(0);
-  for (int k = F.agg_ptr[c]; k < F.agg_ptr[c + 1]; ++k) s += (TC)F.r[F.agg_cells[k]];
-  Cc.b[c] = s;
-  Cc.x[0][c] = TC(0);
+  if (c < F.nc) {
+    for (int k = F.agg_ptr[c]; k < F.agg_ptr[c + 1]; ++k) s += (TC)F.r[F.agg_cells[k]];
+    Cc.b[c] = s;
+    if constexpr (!kSweep) Cc.x[0][c] = TC(0);
+  }
+  if constexpr (kSweep) sweep_from_zero(Cc, c, s);
 }
 
 template <typename TF, typename TC>
@@ -588,6 +627,22 @@ __global__ __launch_bounds__(kBlock) void k_gm_cast(int n, const double *__restr
   x[i] = T(0);
 }
 
+// b = (T) r on level L and the level's first Jacobi sweep from x = 0 into
+// x[1] (sweep_from_zero); grid over whole slices of L.
+template <typename T>
+__global__ __launch_bounds__(kBlock) void k_gm_cast_sweep(LevelDevT<T> L,
+                                                           const double *__restrict__ r,
+                                                           const KState *st) {
+  if (st->done) return;
+  const int c = blockIdx.x * kBlock + threadIdx.x;
+  T bc = T(0);
+  if (c < L.n) {
+    bc = (T)r[c];
+    L.b[c] = bc;
+  }
+  sweep_from_zero(L, c, bc);
+}
+
 template <typename T>
 __global__ __launch_bounds__(kBlock) void k_gm_correct(int n, const T *__restrict__ e,
                                                         double *__restrict__ x, const KState *st) {
@@ -787,6 +842,7 @@ size_t gamg_layout(const pfm_gamg_s *g, int max_iter, GamgWork *w, char *base) {
     D.inv = (double *)take(sizeof(double) * n);
     D.r = (double *)take(sizeof(double) * n);
     D.x[1] = (double *)take(sizeof(double) * n);
+    D.finite = (int32_t *)take(sizeof(int32_t));
     if (k > 0) {
       t.diag[k] = (double *)take(sizeof(double) * n);
       t.lower[k] = (double *)take(sizeof(double) * std::max<size_t>(nf, 1));
@@ -887,6 +943,7 @@ size_t gamg_mixed_layout(const pfm_gamg_s *g, int max_outer, int fp32_levels, Ga
     D.packed = t.packed32[k];
     D.diag = t.diag32[k];
     D.inv = t.inv32[k];
+    D.finite = (int32_t *)take(sizeof(int32_t));
     D.x[0] = (float *)take(sizeof(float) * n);
     D.x[1] = (float *)take(sizeof(float) * n);
     D.b = (float *)take(sizeof(float) * n);
@@ -959,6 +1016,19 @@ void free_level(GamgLevelHost &H) {
 
 }  // namespace
 
+// Set *flag (device) to whether a0[0..n0), a1[0..n1) and a2[0..n2) are all
+// finite.
+template <typename T>
+void mark_finite(const int32_t *flag, int64_t n0, const T *a0, int64_t n1, const T *a1,
+                 int64_t n2, const T *a2, hipStream_t s) {
+  int32_t *f = const_cast<int32_t *>(flag);
+  k_flag_one<<<1, 1, 0, s>>>(f);
+  const int64_t tot = n0 + n1 + n2;
+  if (tot > 0)
+    k_finite<T><<<(int)std::min<int64_t>((tot + kBlock - 1) / kBlock, 1024), kBlock, 0, s>>>(
+        n0, a0, n1, a1, n2, a2, f);
+}
+
 // The levels one V-cycle runs on: levels [0, nbig) with values of type T,
 // the others in FP64: [nbig, tail_k) by per-level launches, [tail_k, L) in
 // the one-workgroup tail (host descriptors and their device copy).
@@ -975,7 +1045,7 @@ struct CycleLevels {
 // level 0's Jacobi buffer before and after the cycle.
 template <typename T>
 void enqueue_cycle(const pfm_gamg_s *g, const CycleLevels<T> &C, const KState *st, int &cur0,
-                   hipStream_t s) {
+                   hipStream_t s, bool swept0 = false) {
   const int L = (int)g->lv.size();
   const int npre = g->prm.n_pre, npost = g->prm.n_post, ncoarse = g->prm.n_coarsest;
   const bool gs = g->prm.smoother == PFM_SMOOTHER_GAUSS_SEIDEL;
@@ -1003,23 +1073,38 @@ void enqueue_cycle(const pfm_gamg_s *g, const CycleLevels<T> &C, const KState *s
     else
       smooth_level(C.small[k], k, cur);
   };
+  // A launched coarse level starts from x = 0: its first Jacobi pre-sweep is
+  // done by the restriction (sweep_from_zero), and likewise level 0's when
+  // the caller's kernel did it (swept0, with cur0 = 1).
   int cur[kMaxLevels];
+  bool swept[kMaxLevels] = {};
   cur[0] = cur0;
+  swept[0] = swept0;
+  auto restrict_to = [&](const auto &F, const auto &Cc, bool sweep) {
+    using TF = typename std::remove_const<typename std::remove_pointer<decltype(F.packed)>::type>::type;
+    using TC = typename std::remove_const<typename std::remove_pointer<decltype(Cc.packed)>::type>::type;
+    if (sweep)
+      k_restrict<TF, TC, true><<<grid1d(F.nc), kBlock, 0, s>>>(F, Cc, st);
+    else
+      k_restrict<TF, TC, false><<<grid1d(F.nc), kBlock, 0, s>>>(F, Cc, st);
+  };
   for (int k = 0; k < tail_k; ++k) {  // descend through the large levels
-    for (int it = 0; it < npre; ++it) smooth(k, cur[k]);
+    for (int it = swept[k] ? 1 : 0; it < npre; ++it) smooth(k, cur[k]);
+    const bool sweep = !gs && npre >= 1 && k + 1 < tail_k;
     if (k < nbig) {
       const LevelDevT<T> &F = C.big[k];
       k_resid<T><<<blocks_for_slices(F.nslices), kBlock, 0, s>>>(F, cur[k], st);
       if (k + 1 < nbig)
-        k_restrict<T, T><<<grid1d(F.nc), kBlock, 0, s>>>(F, C.big[k + 1], st);
+        restrict_to(F, C.big[k + 1], sweep);
       else
-        k_restrict<T, double><<<grid1d(F.nc), kBlock, 0, s>>>(F, C.small[k + 1], st);
+        restrict_to(F, C.small[k + 1], sweep);
     } else {
       const LevelDev &F = C.small[k];
       k_resid<double><<<blocks_for_slices(F.nslices), kBlock, 0, s>>>(F, cur[k], st);
-      k_restrict<double, double><<<grid1d(F.nc), kBlock, 0, s>>>(F, C.small[k + 1], st);
+      restrict_to(F, C.small[k + 1], sweep);
     }
-    cur[k + 1] = 0;
+    cur[k + 1] = sweep ? 1 : 0;
+    swept[k + 1] = sweep;
   }
   // levels tail_k .. L-1 in one workgroup
   k_tail<<<1, kTailThreads, 0, s>>>(C.d_small, tail_k, L, cur[tail_k], npre, npost, ncoarse,
@@ -1259,6 +1344,7 @@ int pfm_gamg_setup_values(pfm_gamg g, const double *diag, const double *lower, c
       if (st != PFM_OK) return st;
     }
     if (H.n) k_inv_diag<<<grid1d(H.n), kBlock, 0, s>>>(H.n, d, (double *)w.lv[k].inv);
+    mark_finite<double>(w.lv[k].finite, H.n, d, H.nf, lo, H.nf, up, s);
   }
   PFM_CHECK_HIP(hipGetLastError());
   return PFM_OK;
@@ -1389,6 +1475,8 @@ int pfm_gamg_mixed_solve(pfm_gamg g, const pfm_solve_params *prm, double inner_r
       return st;
     k_to_f32<<<grid1d(H.n), kBlock, 0, s>>>(H.n, d64, m.diag32[k]);
     k_to_f32<<<grid1d(H.n), kBlock, 0, s>>>(H.n, w.lv[k].inv, m.inv32[k]);
+    mark_finite<float>(m.lv32[k].finite, H.n, m.diag32[k], H.nf ? H.ldu->nslots : 0,
+                       m.packed32[k], 0, (const float *)nullptr, s);
   }
   // FP64 descriptors; with no FP32 level the inner cycle starts on the FP64
   // level 0, whose solution and right-hand side are the inner buffers
@@ -1531,6 +1619,8 @@ int pfm_gamg_pcg_solve(pfm_gamg g, const pfm_solve_params *prm, int32_t fp32_lev
       return st;
     k_to_f32<<<grid1d(H.n), kBlock, 0, s>>>(H.n, d64, m.diag32[k]);
     k_to_f32<<<grid1d(H.n), kBlock, 0, s>>>(H.n, w.lv[k].inv, m.inv32[k]);
+    mark_finite<float>(m.lv32[k].finite, H.n, m.diag32[k], H.nf ? H.ldu->nslots : 0,
+                       m.packed32[k], 0, (const float *)nullptr, s);
   }
   // the preconditioner's level 0 solves for e64 (FP64 path) with rhs b64
   std::vector<LevelDev> small = w.lv;
@@ -1551,13 +1641,21 @@ int pfm_gamg_pcg_solve(pfm_gamg g, const pfm_solve_params *prm, int32_t fp32_lev
   const int ns0 = g->fine->nslices;
   // z = M^-1 r: level-0 rhs = r, solution from zero, one cycle
   int zpar = 0;
+  // Level 0 starts from zero: with launched levels (tail_k > 0) and Jacobi
+  // smoothing its first sweep is done where r is cast (k_gm_cast_sweep).
+  const bool fold0 = tail_k > 0 && g->prm.smoother != PFM_SMOOTHER_GAUSS_SEIDEL &&
+                     g->prm.n_pre >= 1;
   auto precondition = [&](int it) {
-    if (nbig > 0)
+    if (fold0 && nbig > 0)
+      k_gm_cast_sweep<float><<<grid1d(n0), kBlock, 0, s>>>(m.lv32[0], W.r, w.st);
+    else if (fold0)
+      k_gm_cast_sweep<double><<<grid1d(n0), kBlock, 0, s>>>(small[0], W.r, w.st);
+    else if (nbig > 0)
       k_gm_cast<float><<<grid1d(n0), kBlock, 0, s>>>(n0, W.r, m.lv32[0].b, m.lv32[0].x[0], w.st);
     else
       k_gm_cast<double><<<grid1d(n0), kBlock, 0, s>>>(n0, W.r, m.b64, m.e64, w.st);
-    int cur0 = 0;
-    enqueue_cycle<float>(g, C, w.st, cur0, s);
+    int cur0 = fold0 ? 1 : 0;
+    enqueue_cycle<float>(g, C, w.st, cur0, s, fold0);
     zpar = cur0;
     if (nbig > 0)
       k_gp_z<float><<<G, kBlock, 0, s>>>(n0, ns0, m.lv32[0].x[zpar], W.r, W.z, W.part, w.st);

@@ -23,6 +23,7 @@ struct LevelDevT {
   const T *packed;
   const T *diag;
   const T *inv;           // 1 / max(|diag|, 1e-30)
+  const int32_t *finite;  // 1 when diag and every off-diagonal value are finite
   T *x[2];                // Jacobi double buffer
   T *b;
   T *r;

@@ -170,6 +170,31 @@ def test_gamg_tail_wide_row_bit_exact(gpu_device):
     assert bits_equal(x.cpu().numpy(), xo)
 
 
+def test_gamg_non_finite_coefficient_like_the_oracle(gpu_device):
+    """A coarse level's first sweep from x = 0 is element-wise only while its
+    coefficients are finite; with an infinite coefficient (which the coarse
+    levels inherit) the products with zero are formed and NaN/inf propagate
+    as in the oracle's V-cycle: same NaN and inf positions, finite entries
+    bit-identical."""
+    c, d, lo, up = system(16, 0.2, 1.0)
+    lo = lo.copy()
+    lo[5] = np.inf
+    m = _matrix(c.n_cells, c.owner, c.neighbour, d, lo, up, gpu_device)
+    solver = GAMGSolver(tolerance=1e-300, rel_tol=1e-300, max_iter=2)
+    assert solver.hierarchy(m).level_sizes()[1][0] > 1024  # a launched coarse level
+    x, it, _ = solver(m, torch.from_numpy(c.b), torch.zeros(c.n_cells, dtype=torch.float64))
+    with np.errstate(all="ignore"):  # inf * 0 in the oracle, as intended
+        xo, io, _, _ = ogamg.gamg(ogamg.Level(c.n_cells, d, lo, up, c.owner, c.neighbour), c.b,
+                                  np.zeros(c.n_cells), tolerance=1e-300, rel_tol=1e-300,
+                                  max_iter=2)
+    xg = x.cpu().numpy()
+    assert it == io == 2
+    assert np.isnan(xo).any()
+    assert np.array_equal(np.isnan(xg), np.isnan(xo)) and np.array_equal(np.isinf(xg), np.isinf(xo))
+    fin = np.isfinite(xo)
+    assert bits_equal(xg[fin], xo[fin])
+
+
 def _pair(c, d, lo, up, dev, **kw):
     m = _matrix(c.n_cells, c.owner, c.neighbour, d, lo, up, dev)
     base = dict(tolerance=1e-300, agglomeration="pair", coarse="galerkin", max_levels=12)
