@@ -547,6 +547,33 @@ __global__ __launch_bounds__(kBlock) void k_res_norm(LevelDevT<T> L, int cur,
   if (threadIdx.x == 0) part[blockIdx.x] = acc;
 }
 
+// k_res_norm fused with the next cycle's first Jacobi sweep of level 0, which
+// forms the same r = b - A x: partials of r.r as k_res_norm (same slices per
+// block, same order) and x[cur ^ 1] = x + inv*r as k_jacobi.
+template <typename T>
+__global__ __launch_bounds__(kBlock) void k_res_norm_jacobi(LevelDevT<T> L, int cur,
+                                                             double *__restrict__ part,
+                                                             const KState *st) {
+  __shared__ double lds[kWavesPerBlock];
+  if (st->done) return;
+  double acc = 0.0;
+  const SliceRange sr = block_slices(L.nslices);
+  const int lane = threadIdx.x & 63;
+  const T *x = L.x[cur];
+  for (int s = sr.begin + (threadIdx.x >> 6); s < sr.end; s += kWavesPerBlock) {
+    const int row = s * kSlice + lane;
+    const T ax = level_ax(L, s, lane, row, x);
+    if (row < L.n) {
+      const T rt = L.b[row] - ax;
+      const double ri = (double)rt;
+      acc += ri * ri;
+      L.x[cur ^ 1][row] = x[row] + L.inv[row] * rt;
+    }
+  }
+  acc = block_sum(acc, lds);
+  if (threadIdx.x == 0) part[blockIdx.x] = acc;
+}
+
 __global__ __launch_bounds__(kBlock) void k_gamg_fin(const double *__restrict__ part, int G, KState *st,
                                                       double *__restrict__ hist) {
   __shared__ double lds[kWavesPerBlock];
@@ -1404,12 +1431,34 @@ int pfm_gamg_solve(pfm_gamg g, const pfm_solve_params *prm, const double *diag, 
   // value after every enqueued cycle is kept
   std::vector<int> cur_after;
   cur_after.reserve(prm->max_iter + 1);
+  // With a launched level 0 and Jacobi smoothing, cycle i's residual is
+  // formed by cycle i+1's first level-0 sweep (k_res_norm_jacobi) and
+  // monitored there; the remaining kernels of cycle i+1 return once the
+  // monitor has stopped the solve, and the last cycle's residual is formed
+  // after the loop.  Otherwise each cycle ends with k_res_norm.
+  const bool fold = C.tail_k > 0 && g->prm.smoother != PFM_SMOOTHER_GAUSS_SEIDEL &&
+                    g->prm.n_pre >= 1;
+  int ncyc = 0;
   int rc = drive_cycles(g, w.st, prm->max_iter, s, cur_after, cur0, [&]() {
-    enqueue_cycle<double>(g, C, w.st, cur0, s);
-    k_res_norm<double><<<G, kBlock, 0, s>>>(w.lv[0], cur0, w.part, w.st);
-    k_gamg_fin<<<1, kBlock, 0, s>>>(w.part, G, w.st, w.hist);
+    if (fold && ncyc > 0) {
+      k_res_norm_jacobi<double><<<G, kBlock, 0, s>>>(w.lv[0], cur0, w.part, w.st);
+      k_gamg_fin<<<1, kBlock, 0, s>>>(w.part, G, w.st, w.hist);
+      cur0 ^= 1;
+      enqueue_cycle<double>(g, C, w.st, cur0, s, true);
+    } else {
+      enqueue_cycle<double>(g, C, w.st, cur0, s);
+      if (!fold) {
+        k_res_norm<double><<<G, kBlock, 0, s>>>(w.lv[0], cur0, w.part, w.st);
+        k_gamg_fin<<<1, kBlock, 0, s>>>(w.part, G, w.st, w.hist);
+      }
+    }
+    ++ncyc;
   });
   if (rc != PFM_OK) return rc;
+  if (fold && ncyc > 0) {  // the last enqueued cycle's residual (no-op once stopped)
+    k_res_norm<double><<<G, kBlock, 0, s>>>(w.lv[0], cur0, w.part, w.st);
+    k_gamg_fin<<<1, kBlock, 0, s>>>(w.part, G, w.st, w.hist);
+  }
   KState hs;
   PFM_CHECK_HIP(hipMemcpyAsync(&hs, w.st, sizeof(KState), hipMemcpyDeviceToHost, s));
   PFM_CHECK_HIP(hipStreamSynchronize(s));
