@@ -457,15 +457,20 @@ __global__ __launch_bounds__(kBlock) void k_b_pupdate(const int4 *__restrict__ a
       pv = p[row];
       if (rD) dv = rD[row];
     }
+    // The system's state is read once, before the partial sums: within this
+    // kernel only the system's first tile writes it (W below), and the
+    // fields the other tiles use (res0, r.z of the previous iteration) are
+    // not among those it changes; the stop flag is taken from thread 0's
+    // copy (see FOR_ACTIVE_TILES).
     const int t0 = sys_t0[k], t1 = sys_t0[k + 1];
-    if (threadIdx.x == 0) s_done = sys[k].done;  // see FOR_ACTIVE_TILES
+    const BSys S = sys[k];
+    if (threadIdx.x == 0) s_done = S.done;
     __syncthreads();
     const int done_k = s_done;
     __syncthreads();
     if (done_k) continue;
     const double RR = sys_sum(part + ntiles, t0, t1, lds);
     const double RZ = sys_sum(part + 2 * ntiles, t0, t1, lds);
-    const BSys S = sys[k];
     const double res = sqrt(RR);
     const bool conv = sys_decision(g, S, res, it);
     const bool stop = conv || it >= g->max_iter;

@@ -48,7 +48,9 @@ def _solve(B, dev, tile=0, chunk=16, pre="DIC", rel=1e-8, transpose=False, batch
     return x.cpu().numpy(), res, batch
 
 
-@pytest.mark.parametrize("tile", [0, 1, 3])
+# 5 and 32 slices per tile: with 4 waves per workgroup each wave walks two or
+# more slices, two at a time, the last one alone when the count is odd.
+@pytest.mark.parametrize("tile", [0, 1, 3, 5, 32])
 @pytest.mark.parametrize("pre", ["DIC", "none"])
 def test_batch_matches_oracle_per_system(gpu_device, tile, pre):
     B = _concat(SYM, zero_rhs=(3,))
