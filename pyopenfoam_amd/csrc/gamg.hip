@@ -654,8 +654,8 @@ __global__ __launch_bounds__(kBlock) void k_gm_cast(int n, const double *__restr
   x[i] = T(0);
 }
 
-// b = (T) r on level L and the level's first Jacobi sweep from x = 0 into
-// x[1] (sweep_from_zero); grid over whole slices of L.
+// b = (T) r on level L, x[0] = 0, and the level's first Jacobi sweep from
+// x = 0 into x[1] (sweep_from_zero); grid over whole slices of L.
 template <typename T>
 __global__ __launch_bounds__(kBlock) void k_gm_cast_sweep(LevelDevT<T> L,
                                                            const double *__restrict__ r,
@@ -666,6 +666,7 @@ __global__ __launch_bounds__(kBlock) void k_gm_cast_sweep(LevelDevT<T> L,
   if (c < L.n) {
     bc = (T)r[c];
     L.b[c] = bc;
+    L.x[0][c] = T(0);  // the solution when no cycle runs
   }
   sweep_from_zero(L, c, bc);
 }
@@ -1547,6 +1548,7 @@ int pfm_gamg_mixed_solve(pfm_gamg g, const pfm_solve_params *prm, double inner_r
   const int G = red_grid(g->fine);
   int inner_total = 0;
   std::vector<int> cur_after;
+  int cur0_ = 0;  // level 0's buffer parity in the inner cycles
   for (int outer = 0;; ++outer) {
     k_gm_resid<<<G, kBlock, 0, s>>>(w.lv[0], m.r64, w.part, w.st);
     const int budget = prm->max_iter - inner_total;
@@ -1556,21 +1558,45 @@ int pfm_gamg_mixed_solve(pfm_gamg g, const pfm_solve_params *prm, double inner_r
     PFM_CHECK_HIP(hipMemcpyAsync(&done, &w.st->done, sizeof(int32_t), hipMemcpyDeviceToHost, s));
     PFM_CHECK_HIP(hipStreamSynchronize(s));
     if (done) break;
-    if (nbig > 0)
+    // As in pfm_gamg_solve: with a launched level 0 and Jacobi smoothing, the
+    // first inner cycle's first level-0 sweep (from e = 0) is done where r is
+    // cast, and inner cycle i's residual is formed by cycle i+1's first
+    // level-0 sweep; the last one after the loop.
+    const bool fold = C.tail_k > 0 && g->prm.smoother != PFM_SMOOTHER_GAUSS_SEIDEL &&
+                      g->prm.n_pre >= 1;
+    auto res_norm = [&](bool sweep) {
+      if (nbig > 0 && sweep)
+        k_res_norm_jacobi<float><<<G, kBlock, 0, s>>>(m.lv32[0], cur0_, w.part, m.in);
+      else if (nbig > 0)
+        k_res_norm<float><<<G, kBlock, 0, s>>>(m.lv32[0], cur0_, w.part, m.in);
+      else if (sweep)
+        k_res_norm_jacobi<double><<<G, kBlock, 0, s>>>(small[0], cur0_, w.part, m.in);
+      else
+        k_res_norm<double><<<G, kBlock, 0, s>>>(small[0], cur0_, w.part, m.in);
+      k_gamg_fin<<<1, kBlock, 0, s>>>(w.part, G, m.in, nullptr);
+    };
+    if (fold && nbig > 0)
+      k_gm_cast_sweep<float><<<grid1d(n0), kBlock, 0, s>>>(m.lv32[0], m.r64, w.st);
+    else if (fold)
+      k_gm_cast_sweep<double><<<grid1d(n0), kBlock, 0, s>>>(small[0], m.r64, w.st);
+    else if (nbig > 0)
       k_gm_cast<float><<<grid1d(n0), kBlock, 0, s>>>(n0, m.r64, m.lv32[0].b, m.lv32[0].x[0], w.st);
     else
       k_gm_cast<double><<<grid1d(n0), kBlock, 0, s>>>(n0, m.r64, m.b64, m.e64, w.st);
-    int cur0 = 0;
+    cur0_ = fold ? 1 : 0;
+    int ncyc = 0;
     cur_after.clear();
-    int rc = drive_cycles(g, m.in, budget, s, cur_after, cur0, [&]() {
-      enqueue_cycle<float>(g, C, m.in, cur0, s);
-      if (nbig > 0)
-        k_res_norm<float><<<G, kBlock, 0, s>>>(m.lv32[0], cur0, w.part, m.in);
-      else
-        k_res_norm<double><<<G, kBlock, 0, s>>>(small[0], cur0, w.part, m.in);
-      k_gamg_fin<<<1, kBlock, 0, s>>>(w.part, G, m.in, nullptr);
+    int rc = drive_cycles(g, m.in, budget, s, cur_after, cur0_, [&]() {
+      if (fold && ncyc > 0) {
+        res_norm(true);
+        cur0_ ^= 1;
+      }
+      enqueue_cycle<float>(g, C, m.in, cur0_, s, fold);
+      if (!fold) res_norm(false);
+      ++ncyc;
     });
     if (rc != PFM_OK) return rc;
+    if (fold && ncyc > 0) res_norm(false);  // the last cycle's residual (no-op once stopped)
     int32_t executed = 0;
     PFM_CHECK_HIP(hipMemcpyAsync(&executed, &m.in->iter, sizeof(int32_t), hipMemcpyDeviceToHost, s));
     PFM_CHECK_HIP(hipStreamSynchronize(s));
