@@ -153,11 +153,15 @@ __global__ __launch_bounds__(kBlock) void k_pcg_spmv(int n, int nslices, const S
 }
 
 // alpha = rz / pAp (breakdown if |pAp| < 1e-30), kept for k_pcg_pupdate;
-// r -= alpha Ap; partials r.r and r.z (region B).
+// r -= alpha Ap; partials r.r and r.z (region B).  On a breakdown at an even
+// iteration (pprev = p_{it-1} given) the x update deferred at iteration it-1
+// is applied before the solve ends.
 __global__ __launch_bounds__(kBlock) void k_pcg_update(int n, int nslices, int it,
                                                         const double *__restrict__ rD,
                                                         const double *__restrict__ Ap,
                                                         double *__restrict__ r,
+                                                        double *__restrict__ x,
+                                                        const double *__restrict__ pprev,
                                                         double *__restrict__ part, KState *st) {
   __shared__ double lds[kWavesPerBlock];
   if (st->done) return;
@@ -177,6 +181,12 @@ __global__ __launch_bounds__(kBlock) void k_pcg_update(int n, int nslices, int i
   const double PAP = reduce_partials(part, G, lds);
   if (fabs(PAP) < 1e-30) {
     if (blockIdx.x == 0 && threadIdx.x == 0) finish(st, PFM_SOLVE_BREAKDOWN);
+    if (pprev) {  // the x update deferred at iteration it-1 (see k_pcg_pupdate)
+      const double ap_ = st->alphas[(it - 1) & 1];
+      FOR_ROWS({
+        if (row < n) x[row] = x[row] + ap_ * pprev[row];
+      })
+    }
     return;
   }
   const double alpha = st->rzs[(it - 1) & 1] / PAP;
@@ -200,30 +210,41 @@ __global__ __launch_bounds__(kBlock) void k_pcg_update(int n, int nslices, int i
   store_partials<2>(acc, part + G, lds);
 }
 
-// monitor(||r||, i); x += alpha p; beta = r.z' / r.z; p = M^-1 r + beta p
-// (not after the last iteration).
+// monitor(||r||, i); x update; beta = r.z' / r.z; p' = M^-1 r + beta p (not
+// after the last iteration).  x is updated every second iteration, which
+// saves one x read and write per pair: at even `it` as
+// (x + alpha_{it-1} p_{it-1}) + alpha_it p_it, which rounds exactly as the two
+// single updates, with p_{it-1} read from pprev; at odd `it` only when the
+// solve ends there.  p' goes to pn, a buffer other than p; at even `it` pn is
+// pprev's buffer (each row reads p_{it-1} before p_{it+1} is written over it),
+// so neither is restrict-qualified.
 __global__ __launch_bounds__(kBlock) void k_pcg_pupdate(int n, int nslices, int it,
                                                          const double *__restrict__ rD,
                                                          const double *__restrict__ r,
-                                                         double *__restrict__ p,
+                                                         const double *__restrict__ p,
+                                                         const double *pprev, double *pn,
                                                          double *__restrict__ x,
                                                          const double *__restrict__ part, KState *st,
                                                          double *__restrict__ hist) {
   __shared__ double lds[kWavesPerBlock];
   if (st->done) return;  // set by an earlier kernel only (a breakdown in k_pcg_update)
   const int G = gridDim.x;
-  // First round's r, rD, p, x loaded before the reductions (they do not
-  // depend on beta), so their latency overlaps the partial sums'.
+  const bool pair = pprev != nullptr;
+  // First round's operands loaded before the reductions (they do not depend
+  // on beta), so their latency overlaps the partial sums'.
   const int lane = threadIdx.x & 63;
   const int stride = G * kWavesPerBlock;
   int s = xcd_remap(blockIdx.x, G) * kWavesPerBlock + (threadIdx.x >> 6);
   int row = s * kSlice + lane;
-  double rp = 0.0, dp = 1.0, pp = 0.0, xp = 0.0;
+  double rp = 0.0, dp = 1.0, pp = 0.0, qp = 0.0, xp = 0.0;
   if (s < nslices && row < n) {
     rp = r[row];
     if (rD) dp = rD[row];
     pp = p[row];
-    xp = x[row];
+    if (pair) {
+      qp = pprev[row];
+      xp = x[row];
+    }
   }
   const double RR = reduce_partials(part + G, G, lds);
   const double RZ = reduce_partials(part + 2 * G, G, lds);
@@ -232,6 +253,7 @@ __global__ __launch_bounds__(kBlock) void k_pcg_pupdate(int n, int nslices, int 
   const bool last = conv || it >= st->max_iter;
   const double rz_old = st->rzs[(it - 1) & 1];
   const double alpha = st->alphas[it & 1];
+  const double alpha_prev = pair ? st->alphas[(it - 1) & 1] : 0.0;
   if (blockIdx.x == 0 && threadIdx.x == 0) {
     monitor_update(st, hist, res, it);
     st->iter = it;
@@ -244,10 +266,13 @@ __global__ __launch_bounds__(kBlock) void k_pcg_pupdate(int n, int nslices, int 
   const double beta = RZ / rz_old;
   for (; s < nslices; s += stride, row = s * kSlice + lane) {
     if (row < n) {
-      x[row] = xp + alpha * pp;
+      if (pair)
+        x[row] = (xp + alpha_prev * qp) + alpha * pp;
+      else if (last)
+        x[row] = x[row] + alpha * pp;
       if (!last) {
         const double zi = rD ? rp / dp : rp;
-        p[row] = zi + beta * pp;
+        pn[row] = zi + beta * pp;
       }
     }
     const int sn = s + stride, rn = sn * kSlice + lane;
@@ -255,7 +280,10 @@ __global__ __launch_bounds__(kBlock) void k_pcg_pupdate(int n, int nslices, int 
       rp = r[rn];
       if (rD) dp = rD[rn];
       pp = p[rn];
-      xp = x[rn];
+      if (pair) {
+        qp = pprev[rn];
+        xp = x[rn];
+      }
     }
   }
 }
@@ -784,7 +812,7 @@ extern "C" {
 
 int64_t pfm_pcg_workspace_bytes(pfm_ldu h, int32_t max_iter) {
   if (!h) return -1;
-  return (int64_t)work_layout(h, max_iter, 3, nullptr, nullptr);
+  return (int64_t)work_layout(h, max_iter, 4, nullptr, nullptr);
 }
 
 int64_t pfm_pbicgstab_workspace_bytes(pfm_ldu h, int32_t max_iter) {
@@ -805,8 +833,10 @@ int pfm_pcg_solve(pfm_ldu h, const pfm_solve_params *prm, const double *diag, co
   }
   hipStream_t s = (hipStream_t)stream;
   Work w;
-  work_layout(h, prm->max_iter, 3, &w, (char *)work);
-  double *r = w.vec[0], *p = w.vec[1], *Ap = w.vec[2];
+  work_layout(h, prm->max_iter, 4, &w, (char *)work);
+  // Two direction buffers: p_it lives in P[(it + 1) & 1] (k_pcg_init writes
+  // p_1 to P[0]); k_pcg_pupdate writes p_{it+1} to P[it & 1].
+  double *r = w.vec[0], *P[2] = {w.vec[1], w.vec[3]}, *p = P[0], *Ap = w.vec[2];
   const double *pd = prm->precond == PFM_PRECOND_DIAG ? rD : nullptr;
   const int hist_cap = prm->max_iter + 1;
   const int G = red_grid(h);
@@ -823,14 +853,16 @@ int pfm_pcg_solve(pfm_ldu h, const pfm_solve_params *prm, const double *diag, co
   k_pcg_init_fin<<<1, kBlock, 0, s>>>(w.part, G, w.st, w.hist);
   PFM_CHECK_HIP(hipGetLastError());
   rc = drive(h, w.st, prm->max_iter, s, [&](int it) {
+    double *pc = P[(it + 1) & 1], *po = P[it & 1];  // p_it; p_{it+1} (and p_{it-1})
+    const double *pprev = (it & 1) ? nullptr : po;  // pairs of x updates end at even it
     if (c16)
-      k_pcg_spmv<true><<<G, kBlock, 0, s>>>(n, ns, h->d_slices, h->d_col, c16, packed, diag, p,
+      k_pcg_spmv<true><<<G, kBlock, 0, s>>>(n, ns, h->d_slices, h->d_col, c16, packed, diag, pc,
                                             Ap, w.part, w.st);
     else
-      k_pcg_spmv<false><<<G, kBlock, 0, s>>>(n, ns, h->d_slices, h->d_col, c16, packed, diag, p,
+      k_pcg_spmv<false><<<G, kBlock, 0, s>>>(n, ns, h->d_slices, h->d_col, c16, packed, diag, pc,
                                              Ap, w.part, w.st);
-    k_pcg_update<<<G, kBlock, 0, s>>>(n, ns, it, pd, Ap, r, w.part, w.st);
-    k_pcg_pupdate<<<G, kBlock, 0, s>>>(n, ns, it, pd, r, p, x, w.part, w.st, w.hist);
+    k_pcg_update<<<G, kBlock, 0, s>>>(n, ns, it, pd, Ap, r, x, pprev, w.part, w.st);
+    k_pcg_pupdate<<<G, kBlock, 0, s>>>(n, ns, it, pd, r, pc, pprev, po, x, w.part, w.st, w.hist);
   });
   if (rc != PFM_OK) return rc;
   PFM_CHECK_HIP(hipGetLastError());

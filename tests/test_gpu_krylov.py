@@ -81,6 +81,47 @@ def test_pcg_matches_oracle(gpu_device, n, jit, sig, pre):
         assert np.linalg.norm(xg - xo) / np.linalg.norm(xo) <= 1e-9
 
 
+@pytest.mark.parametrize("case", ["one_cell", "diag3", "two_cell"])
+@pytest.mark.parametrize("pre", ["none", "DIC"])
+def test_pcg_breakdown_after_exact_solve(gpu_device, case, pre):
+    """With tolerance 0 the monitor never stops the solve: once CG has solved a
+    small system exactly, r = 0, p = 0 and p.Ap = 0 is the reference's
+    breakdown (pcg.py:150-152, not converged).  On a diagonal system with the
+    diagonal preconditioner that happens at iteration 2, an even iteration, at
+    which the device still holds iteration 1's x update (x is updated every
+    second iteration); other cases end at odd iterations or run to the cap.
+    Iterations, decision and every bit of x as the oracle's."""
+    own, nei, lo, up = [], [], [], []
+    if case == "one_cell":
+        d, b = [4.0], [2.0]
+    elif case == "diag3":
+        d, b = [2.0, 4.0, 8.0], [1.0, 3.0, 5.0]
+    else:
+        d, b, own, nei, lo, up = [4.0, 4.0], [2.0, 7.0], [0], [1], [-1.0], [-1.0]
+    n = len(d)
+    m = LduMatrix(n, torch.tensor(own, dtype=torch.int64), torch.tensor(nei, dtype=torch.int64),
+                  device=gpu_device)
+    m.diag = torch.tensor(d, dtype=torch.float64)
+    if own:
+        m.lower = torch.tensor(lo, dtype=torch.float64)
+        m.upper = torch.tensor(up, dtype=torch.float64)
+    solver = PCGSolver(tolerance=0.0, rel_tol=0.0, max_iter=10, preconditioner=pre)
+    x, it, _ = solver(m, torch.tensor(b, dtype=torch.float64), torch.zeros(n, dtype=torch.float64))
+    dd, ll, uu = np.array(d), np.array(lo), np.array(up)
+    oo, ne = np.array(own, np.int64), np.array(nei, np.int64)
+    rD = opre.dic_build(dd, uu, oo, ne) if pre == "DIC" else None
+    xo, io, _, info = okry.pcg(lambda v: oldu.ax(dd, ll, uu, oo, ne, v),
+                               (lambda r: r / rD) if rD is not None else None, np.array(b),
+                               np.zeros(n), tolerance=0.0, rel_tol=0.0, max_iter=10,
+                               dot=okry.dot_device(n))
+    assert (it, solver.last_info.converged) == (io, info.converged), (it, io)
+    assert bits_equal(x.cpu().numpy(), xo)
+    if case == "one_cell" or (case == "diag3" and pre == "DIC"):
+        # breakdown at iteration 2: the history holds r0 and iteration 1
+        assert io == 2 and not info.converged
+        np.testing.assert_array_equal(xo, np.array(b) / np.array(d))
+
+
 def test_pcg_known_answer_and_semantics(gpu_device):
     # 3-cell SPD chain, reference tests/unit/solvers/conftest.py:10-35 builder
     own, nei = torch.tensor([0, 1]), torch.tensor([1, 2])
