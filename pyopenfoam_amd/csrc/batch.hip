@@ -24,7 +24,9 @@ using namespace pfm;
 struct BSys {
   double res0, res;
   double rz[2];
-  int32_t nhist, done, status, pad;
+  double alpha[2];  // alpha by iteration parity (k_b_update -> k_b_pupdate)
+  int32_t nhist, done, status;
+  int32_t last_it;  // the last iteration the system runs (INT32_MAX until it stops)
 };
 
 struct alignas(256) BGlobal {
@@ -141,6 +143,7 @@ __global__ void k_b_state_init(BGlobal *g, BSys *sys, int ns, double tol, double
   if (k < ns) {
     BSys z;
     memset(&z, 0, sizeof(z));
+    z.last_it = 0x7fffffff;
     sys[k] = z;
   }
 }
@@ -206,9 +209,11 @@ __global__ void k_b_init_fin(int ns, const int32_t *__restrict__ sys_t0, int nti
   if (sys_decision(g, S, res, 0)) {
     S.done = 1;
     S.status = PFM_SOLVE_CONVERGED;
+    S.last_it = 0;
   } else if (g->max_iter < 1) {
     S.done = 1;
     S.status = PFM_SOLVE_MAXITER;
+    S.last_it = 0;
   }
   sys[k] = S;
   if (S.done) atomicAdd(&g->ndone, 1);
@@ -351,13 +356,12 @@ __global__ __launch_bounds__(kBlock) void k_b_spmv(const int4 *__restrict__ act,
   })
 }
 
-// alpha per system; x += alpha p; r -= alpha Ap; tile partials r.r, r.z.
+// alpha per system (kept for k_b_pupdate); r -= alpha Ap; tile partials r.r,
+// r.z.  A breakdown ends the system before this iteration's x update.
 __global__ __launch_bounds__(kBlock) void k_b_update(const int4 *__restrict__ act, int ntiles,
                                                       const int32_t *__restrict__ sys_t0,
                                                       const double *__restrict__ rD,
-                                                      const double *__restrict__ p,
                                                       const double *__restrict__ Ap,
-                                                      double *__restrict__ x,
                                                       double *__restrict__ r,
                                                       double *__restrict__ part,
                                                       BGlobal *g, BSys *sys, int j) {
@@ -372,10 +376,8 @@ __global__ __launch_bounds__(kBlock) void k_b_update(const int4 *__restrict__ ac
     const int t = dsc.x, k = dsc.y, s0 = dsc.z, s1 = dsc.w;
     int s = s0 + (threadIdx.x >> 6);
     int row = s * kSlice + lane;
-    double xv = 0.0, pv = 0.0, rv = 0.0, av = 0.0, dv = 1.0;
+    double rv = 0.0, av = 0.0, dv = 1.0;
     if (s < s1) {
-      xv = x[row];
-      pv = p[row];
       rv = r[row];
       av = Ap[row];
       if (rD) dv = rD[row];
@@ -394,31 +396,28 @@ __global__ __launch_bounds__(kBlock) void k_b_update(const int4 *__restrict__ ac
       if (t == t0 && threadIdx.x == 0) {
         sys[k].done = 1;
         sys[k].status = PFM_SOLVE_BREAKDOWN;
+        sys[k].last_it = it - 1;
         atomicAdd((int *)&g->ndone, 1);
       }
       continue;
     }
     const double alpha = rz_old / PAP;
+    if (t == t0 && threadIdx.x == 0) sys[k].alpha[it & 1] = alpha;
     double rr = 0.0, rz = 0.0;
     for (; s < s1; s += nw) {
       const int sn = s + nw, rn = sn * kSlice + lane;
-      double xn = 0.0, pn = 0.0, rnx = 0.0, an = 0.0, dn = 1.0;
+      double rnx = 0.0, an = 0.0, dn = 1.0;
       if (sn < s1) {
-        xn = x[rn];
-        pn = p[rn];
         rnx = r[rn];
         an = Ap[rn];
         if (rD) dn = rD[rn];
       }
-      x[row] = xv + alpha * pv;
       const double ri = rv - alpha * av;
       r[row] = ri;
       const double zi = rD ? ri / dv : ri;
       rr += ri * ri;
       rz += ri * zi;
       row = rn;
-      xv = xn;
-      pv = pn;
       rv = rnx;
       av = an;
       dv = dn;
@@ -432,16 +431,18 @@ __global__ __launch_bounds__(kBlock) void k_b_update(const int4 *__restrict__ ac
   }
 }
 
-// monitor per system; beta; p = z + beta p.
+// monitor per system; x += alpha p (where p is read anyway); beta; p = z +
+// beta p (not after the system's last iteration).
 __global__ __launch_bounds__(kBlock) void k_b_pupdate(const int4 *__restrict__ act, int ntiles,
                                                        const int32_t *__restrict__ sys_t0,
                                                        const double *__restrict__ rD,
                                                        const double *__restrict__ r,
                                                        double *__restrict__ p,
+                                                       double *__restrict__ x,
                                                        const double *__restrict__ part,
                                                        BGlobal *g, BSys *sys, int j) {
   __shared__ double lds[kMaxWaves];
-  __shared__ int s_done;
+  __shared__ int s_skip;
   const int it = g->base + j + 1;
   const int lane = threadIdx.x & 63;
   const int nw = blockDim.x >> 6;
@@ -451,24 +452,28 @@ __global__ __launch_bounds__(kBlock) void k_b_pupdate(const int4 *__restrict__ a
     const int t = dsc.x, k = dsc.y, s0 = dsc.z, s1 = dsc.w;
     int s = s0 + (threadIdx.x >> 6);
     int row = s * kSlice + lane;
-    double rv = 0.0, pv = 0.0, dv = 1.0;
+    double rv = 0.0, pv = 0.0, xv = 0.0, dv = 1.0;
     if (s < s1) {
       rv = r[row];
       pv = p[row];
+      xv = x[row];
       if (rD) dv = rD[row];
     }
     // The system's state is read once, before the partial sums: within this
-    // kernel only the system's first tile writes it (W below), and the
-    // fields the other tiles use (res0, r.z of the previous iteration) are
-    // not among those it changes; the stop flag is taken from thread 0's
-    // copy (see FOR_ACTIVE_TILES).
+    // kernel only the system's first tile writes it (W below), and the fields
+    // the other tiles use (res0, alpha and r.z of the previous iterations) are
+    // not among those it changes.  A tile skips the system only if it stopped
+    // in an earlier iteration (done with last_it < it): the first tile's write
+    // of done and last_it may be seen in part, and every mix of old and new
+    // values keeps a system that stops in this iteration (whose x update every
+    // tile must still make).
     const int t0 = sys_t0[k], t1 = sys_t0[k + 1];
     const BSys S = sys[k];
-    if (threadIdx.x == 0) s_done = S.done;
+    if (threadIdx.x == 0) s_skip = S.done && S.last_it < it;
     __syncthreads();
-    const int done_k = s_done;
+    const int skip = s_skip;
     __syncthreads();
-    if (done_k) continue;
+    if (skip) continue;
     const double RR = sys_sum(part + ntiles, t0, t1, lds);
     const double RZ = sys_sum(part + 2 * ntiles, t0, t1, lds);
     const double res = sqrt(RR);
@@ -482,25 +487,31 @@ __global__ __launch_bounds__(kBlock) void k_b_pupdate(const int4 *__restrict__ a
       if (stop) {
         W.done = 1;
         W.status = conv ? PFM_SOLVE_CONVERGED : PFM_SOLVE_MAXITER;
+        W.last_it = it;
       }
       sys[k] = W;
       if (stop) atomicAdd((int *)&g->ndone, 1);
     }
-    if (stop) continue;
+    const double alpha = S.alpha[it & 1];
     const double beta = RZ / S.rz[(it - 1) & 1];
     for (; s < s1; s += nw) {
       const int sn = s + nw, rn = sn * kSlice + lane;
-      double rnx = 0.0, pn = 0.0, dn = 1.0;
+      double rnx = 0.0, pn = 0.0, xn = 0.0, dn = 1.0;
       if (sn < s1) {
         rnx = r[rn];
         pn = p[rn];
+        xn = x[rn];
         if (rD) dn = rD[rn];
       }
-      const double zi = rD ? rv / dv : rv;
-      p[row] = zi + beta * pv;
+      x[row] = xv + alpha * pv;
+      if (!stop) {
+        const double zi = rD ? rv / dv : rv;
+        p[row] = zi + beta * pv;
+      }
       row = rn;
       rv = rnx;
       pv = pn;
+      xv = xn;
       dv = dn;
     }
   }
@@ -587,9 +598,9 @@ void launch_iteration(const pfm_batch_s *B, const BWork &w, const double *pd, in
   const int T = B->ntiles, G = iter_grid(B);
   k_b_spmv<<<G, kBlock, 0, q>>>(w.act, L->d_slices, L->d_col, w.packed, w.dp, w.p, w.Ap, w.part,
                                 w.g, w.sys);
-  k_b_update<<<G, kBlock, 0, q>>>(w.act, T, B->d_sys_t0, pd, w.p, w.Ap, w.xp, w.r, w.part, w.g,
-                                  w.sys, j);
-  k_b_pupdate<<<G, kBlock, 0, q>>>(w.act, T, B->d_sys_t0, pd, w.r, w.p, w.part, w.g, w.sys, j);
+  k_b_update<<<G, kBlock, 0, q>>>(w.act, T, B->d_sys_t0, pd, w.Ap, w.r, w.part, w.g, w.sys, j);
+  k_b_pupdate<<<G, kBlock, 0, q>>>(w.act, T, B->d_sys_t0, pd, w.r, w.p, w.xp, w.part, w.g, w.sys,
+                                   j);
 }
 
 constexpr int kTileCand[6] = {1, 2, 4, 8, 16, 32};
