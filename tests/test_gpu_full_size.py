@@ -1,0 +1,103 @@
+"""The hot path at the benchmark's full size: the 128^3 jittered mesh
+(2,097,152 cells, 6,242,304 internal faces, lognormal(0, 1) diffusivity),
+built as the benchmark builds it.
+
+Exact where the oracle finishes in seconds at this size: the assembled
+coefficients (fvm.laplacian + the Dirichlet term), the SpMV, the DIC factor
+and the first 60 PCG-DIC iterations (history and x, with the oracle summing
+in the device's order).  Size-independent properties for the rest: the
+converged PCG-DIC and GAMG solves reach the tolerance on the true residual,
+and a repeated solve is bit-identical."""
+
+import numpy as np
+import pytest
+import torch
+
+from oracle import krylov as okry
+from oracle import ldu as oldu
+from oracle import precond as opre
+from pyopenfoam_amd import GAMGSolver, PCGSolver
+from pyopenfoam_amd.cases import poisson_case
+from pyopenfoam_amd.discretisation.fvm import assemble
+from pyopenfoam_amd.mesh.fv_mesh import FvMesh
+from pyopenfoam_amd.solvers.preconditioners import DICPreconditioner
+from tests.helpers import bits_equal
+
+pytestmark = pytest.mark.gpu
+N_SIDE = 128
+_STATE = {}
+
+
+def _full(dev):
+    if "mat" not in _STATE:
+        print(f"building the {N_SIDE}^3 case", flush=True)
+        c = poisson_case(N_SIDE, jitter=0.2, sigma=1.0, seed=0)
+        mat = assemble(c, FvMesh.from_case(c, dev))
+        nfi = c.mesh.n_internal_faces
+        d, lo, up = oldu.laplacian(c.gamma, c.geo.mag_sf[:nfi], c.geo.delta[:nfi],
+                                   c.geo.cell_volumes, c.owner, c.neighbour)
+        d = oldu.dirichlet_diag(d, c.dir_gamma, c.dir_mag, c.dir_delta, c.geo.cell_volumes,
+                                c.dir_cells)
+        _STATE.update(case=c, mat=mat, d=d, lo=lo, up=up)
+    return _STATE
+
+
+def test_full_size_assembly_and_spmv(gpu_device):
+    """fvm.laplacian coefficients and A x at 2.1M cells, bit for bit."""
+    S = _full(gpu_device)
+    c, mat = S["case"], S["mat"]
+    assert c.n_cells == N_SIDE ** 3 and c.mesh.n_internal_faces == 3 * N_SIDE ** 2 * (N_SIDE - 1)
+    for k in ("d", "lo", "up"):
+        dev = {"d": mat.diag, "lo": mat.lower, "up": mat.upper}[k]
+        assert bits_equal(dev.cpu().numpy(), S[k]), k
+    x = np.random.default_rng(7).uniform(-1.0, 1.0, c.n_cells)
+    y = mat.Ax(torch.from_numpy(x).to(gpu_device)).cpu().numpy()
+    assert bits_equal(y, oldu.ax(S["d"], S["lo"], S["up"], c.owner, c.neighbour, x))
+
+
+def test_full_size_dic_and_pcg_prefix(gpu_device):
+    """The DIC factor (382 dependency levels) and 60 PCG-DIC iterations at
+    2.1M cells: residual history and x bit-identical to the oracle summing in
+    the device's order."""
+    S = _full(gpu_device)
+    c, mat = S["case"], S["mat"]
+    pc = DICPreconditioner(mat)
+    rd_dev = pc.diagonal() if callable(pc.diagonal) else pc.diagonal
+    rD = opre.dic_build(S["d"], S["up"], c.owner, c.neighbour)
+    assert bits_equal(rd_dev.cpu().numpy(), rD)
+    its = 60
+    s = PCGSolver(tolerance=1e-300, rel_tol=1e-300, max_iter=its)
+    x, it, _ = s(mat, mat.source, torch.zeros(c.n_cells, dtype=torch.float64, device=gpu_device))
+    mv = lambda v: oldu.ax(S["d"], S["lo"], S["up"], c.owner, c.neighbour, v)
+    xo, io, _, info = okry.pcg(mv, lambda r: r / rD, mat.source.cpu().numpy(),
+                               np.zeros(c.n_cells), tolerance=1e-300, rel_tol=1e-300,
+                               max_iter=its, dot=okry.dot_device(c.n_cells))
+    assert it == io == its + 1
+    assert bits_equal(np.array(s.last_info.residual_history), np.array(info.residual_history))
+    assert bits_equal(x.cpu().numpy(), xo)
+
+
+def test_full_size_solves_reach_tolerance(gpu_device):
+    """PCG-DIC and pair/Galerkin GAMG to rel 1e-8 at 2.1M cells: the true
+    relative residual |b - A x| / |b| meets the tolerance (PCG-DIC, whose
+    stopping rule uses the recursive residual, within 1.1e-8), and a repeated
+    PCG-DIC solve gives the same iterations and the same x bit for bit."""
+    S = _full(gpu_device)
+    c, mat = S["case"], S["mat"]
+    b = mat.source
+    z = torch.zeros(c.n_cells, dtype=torch.float64, device=gpu_device)
+    runs = []
+    for _ in range(2):
+        s = PCGSolver(tolerance=1e-300, rel_tol=1e-8, max_iter=200000)
+        x, it, _ = s(mat, b, z)
+        assert s.last_info.converged
+        runs.append((it, x.cpu().numpy()))
+    assert runs[0][0] == runs[1][0] and bits_equal(runs[0][1], runs[1][1])
+    rel = (b - mat.Ax(torch.from_numpy(runs[0][1]).to(gpu_device))).norm() / b.norm()
+    assert rel.item() < 1.1e-8, rel.item()
+    g = GAMGSolver(tolerance=1e-300, rel_tol=1e-8, max_iter=2000, agglomeration="pair",
+                   coarse="galerkin", max_levels=20)
+    xg, _, _ = g(mat, b, z)
+    assert g.last_info.converged
+    relg = (b - mat.Ax(xg)).norm() / b.norm()
+    assert relg.item() < 1e-8, relg.item()
