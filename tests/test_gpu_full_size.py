@@ -3,16 +3,18 @@
 built as the benchmark builds it.
 
 Exact where the oracle finishes in seconds at this size: the assembled
-coefficients (fvm.laplacian + the Dirichlet term), the SpMV, the DIC factor
-and the first 60 PCG-DIC iterations (history and x, with the oracle summing
-in the device's order).  Size-independent properties for the rest: the
-converged PCG-DIC and GAMG solves reach the tolerance on the true residual,
-and a repeated solve is bit-identical."""
+coefficients (fvm.laplacian + the Dirichlet term), the SpMV, the DIC factor,
+the first 60 PCG-DIC iterations (history and x, with the oracle summing in
+the device's order) and two V-cycles over the 20-level pair hierarchy.
+Size-independent properties for the rest: the converged PCG-DIC and GAMG
+solves reach the tolerance on the true residual, and a repeated solve is
+bit-identical."""
 
 import numpy as np
 import pytest
 import torch
 
+from oracle import gamg as ogamg
 from oracle import krylov as okry
 from oracle import ldu as oldu
 from oracle import precond as opre
@@ -101,3 +103,35 @@ def test_full_size_solves_reach_tolerance(gpu_device):
     assert g.last_info.converged
     relg = (b - mat.Ax(xg)).norm() / b.norm()
     assert relg.item() < 1e-8, relg.item()
+
+
+def test_full_size_gamg_cycles_bit_exact(gpu_device):
+    """The pair hierarchy at 2.1M cells (20 levels down to a few cells: grid
+    launches, the one-workgroup tail, the one-wave levels): level sizes as the
+    oracle's pair maps, and x after two Jacobi V-cycles bit for bit as the
+    oracle's V-cycle over the oracle's maps and coarse matrices."""
+    S = _full(gpu_device)
+    c, mat = S["case"], S["mat"]
+    ncyc = 2
+    solver = GAMGSolver(tolerance=1e-300, rel_tol=1e-300, max_iter=ncyc, agglomeration="pair",
+                        coarse="reference", max_levels=20)
+    sizes = solver.hierarchy(mat).level_sizes()
+    x, it, _ = solver(mat, mat.source,
+                      torch.zeros(c.n_cells, dtype=torch.float64, device=gpu_device))
+    lv = ogamg.pair_levels(c.n_cells, c.owner, c.neighbour, n_levels=len(sizes) - 1)
+    levels = [ogamg.Level(c.n_cells, S["d"], S["lo"], S["up"], c.owner, c.neighbour)]
+    aggs, prev = [], np.arange(c.n_cells)
+    for L in lv:
+        f2c = np.empty(levels[-1].n, np.int64)
+        f2c[prev] = L["cell_map"]
+        aggs.append((L["n_cells_coarse"], f2c))
+        levels.append(ogamg.coarse_matrix(levels[-1], L["n_cells_coarse"], f2c)[0])
+        prev = L["cell_map"]
+    assert [(v.n, len(v.owner)) for v in levels] == [tuple(z) for z in sizes]
+    assert len(sizes) >= 15 and sizes[-1][0] <= 8
+    xo = np.zeros(c.n_cells)
+    b = mat.source.cpu().numpy()
+    for _ in range(ncyc):
+        xo = ogamg.v_cycle(levels, aggs, 0, xo, b, 2, 2)
+    assert it == ncyc
+    assert bits_equal(x.cpu().numpy(), xo)
