@@ -3,9 +3,10 @@
 built as the benchmark builds it.
 
 Exact where the oracle finishes in seconds at this size: the assembled
-coefficients (fvm.laplacian + the Dirichlet term), the SpMV, the DIC factor,
-the first 60 PCG-DIC iterations (history and x, with the oracle summing in
-the device's order) and two V-cycles over the 20-level pair hierarchy.
+coefficients (fvm.laplacian + the Dirichlet term), the SpMV, the DIC and
+DILU factors, the first 60 PCG-DIC iterations and the PBiCGStab-DILU solve
+(history and x, with the oracle summing in the device's order) and two
+V-cycles over the 20-level pair hierarchy.
 Size-independent properties for the rest: the converged PCG-DIC and GAMG
 solves reach the tolerance on the true residual, and a repeated solve is
 bit-identical."""
@@ -18,11 +19,11 @@ from oracle import gamg as ogamg
 from oracle import krylov as okry
 from oracle import ldu as oldu
 from oracle import precond as opre
-from pyopenfoam_amd import GAMGSolver, PCGSolver
+from pyopenfoam_amd import GAMGSolver, PBiCGSTABSolver, PCGSolver
 from pyopenfoam_amd.cases import poisson_case
 from pyopenfoam_amd.discretisation.fvm import assemble
 from pyopenfoam_amd.mesh.fv_mesh import FvMesh
-from pyopenfoam_amd.solvers.preconditioners import DICPreconditioner
+from pyopenfoam_amd.solvers.preconditioners import DICPreconditioner, DILUPreconditioner
 from tests.helpers import bits_equal
 
 pytestmark = pytest.mark.gpu
@@ -134,4 +135,26 @@ def test_full_size_gamg_cycles_bit_exact(gpu_device):
     for _ in range(ncyc):
         xo = ogamg.v_cycle(levels, aggs, 0, xo, b, 2, 2)
     assert it == ncyc
+    assert bits_equal(x.cpu().numpy(), xo)
+
+
+def test_full_size_dilu_and_pbicgstab(gpu_device):
+    """The DILU factor (the reference's two-sided face recurrence, 33,147
+    dependent face levels at this size) and PBiCGStab-DILU at 2.1M cells,
+    which ends at the reference's omega breakdown: decision, iterations,
+    history and x bit-identical to the oracle summing in the device's order."""
+    S = _full(gpu_device)
+    c, mat = S["case"], S["mat"]
+    pc = DILUPreconditioner(mat)
+    rd_dev = pc.diagonal() if callable(pc.diagonal) else pc.diagonal
+    rD = opre.dilu_build(S["d"], S["lo"], S["up"], c.owner, c.neighbour)
+    assert bits_equal(rd_dev.cpu().numpy(), rD)
+    s = PBiCGSTABSolver(tolerance=1e-300, rel_tol=1e-8, max_iter=200, preconditioner="DILU")
+    x, it, _ = s(mat, mat.source, torch.zeros(c.n_cells, dtype=torch.float64, device=gpu_device))
+    mv = lambda v: oldu.ax(S["d"], S["lo"], S["up"], c.owner, c.neighbour, v)
+    xo, io, _, info = okry.pbicgstab(mv, lambda r: r / rD, mat.source.cpu().numpy(),
+                                     np.zeros(c.n_cells), tolerance=1e-300, rel_tol=1e-8,
+                                     max_iter=200, dot=okry.dot_device(c.n_cells))
+    assert (it, s.last_info.converged) == (io, info.converged), (it, io)
+    assert bits_equal(np.array(s.last_info.residual_history), np.array(info.residual_history))
     assert bits_equal(x.cpu().numpy(), xo)
