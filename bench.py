@@ -408,7 +408,8 @@ def main() -> int:
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py measures one GPU only; --gpus {args.gpus} is not supported (the path "
-              "shards by domain decomposition, see DESIGN.md)", file=sys.stderr)
+              "would shard by domain decomposition: one rank per GPU, halo exchange of the "
+              "interface x values, scalar all-reductions)", file=sys.stderr)
         return 2
 
     import numpy as np
