@@ -2,8 +2,9 @@
 
 TEST INFRASTRUCTURE ONLY.  Only ``tests/``, ``__graft_entry__.smoke()`` and the
 ``cpu_baseline`` leg of ``bench.py`` may import this package, and only as the
-checker (or the timed CPU baseline); the product package ``pyopenfoam_amd``
-never imports it and has no CPU fallback through it.
+reference results are compared against (or the timed CPU baseline); the
+product package ``pyopenfoam_amd`` never imports it and has no CPU fallback
+through it.
 
 Each function restates one reference function (paths relative to
 ``src/pyfoam/`` of pyOpenFOAM) in float64 with the reference's operation order:

@@ -1,6 +1,7 @@
 /*
  * CPU oracle, C part.  TEST INFRASTRUCTURE ONLY: used by tests/, the smoke
- * check and bench.py's cpu_baseline leg as the checker; never by the product.
+ * test and bench.py's cpu_baseline leg as the results compared against; never
+ * by the product.
  *
  * Plain sequential restatements of the reference's per-element Python loops,
  * which are too slow to run at the benchmark sizes.  Each function cites the
