@@ -11,8 +11,8 @@
   same number of faces, in the reference's operation order: sums from 0 in
   vertex / list order, cells face by face and edge by edge, the tet centroid
   ((ref + fc) + v_j + v_{j+1}) / 4 weighted by |t| (mesh_geometry.py:129-136,
-  213-247), the distinct vertices summed in ascending vertex order.  It is the
-  checker of the device geometry; both loop and vectorised forms are pinned to
+  213-247), the distinct vertices summed in ascending vertex order.  The device
+  geometry is compared against it; both loop and vectorised forms are pinned to
   the reference's own output (golden) at 1e-13.
 """
 
