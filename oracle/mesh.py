@@ -204,12 +204,14 @@ def geometry_vec(points, faces, owner, neighbour, n_cells):
 
 
 def dirichlet_delta(face_centres, face_areas, cell_centres, owner, face_ids):
-    """laplacian_foam.py:342-358: 1 / clamp(|(x_f - C_P).n_hat|, 1e-30)."""
-    out = np.zeros(len(face_ids))
-    for k, f in enumerate(face_ids):
-        d = face_centres[f] - cell_centres[int(owner[f])]
-        s = face_areas[f]
-        m = np.linalg.norm(s)
-        nh = s / m if m > 1e-30 else np.ones(3) / np.sqrt(3.0)
-        out[k] = 1.0 / max(abs(np.dot(d, nh)), 1e-30)
-    return out
+    """src/applications/laplacian_foam.py:341-353: 1 / clamp(|(x_f - C_P).n_hat|, 1e-30)
+    with n_hat = safe / |safe|, safe = S_f where |S_f| > 1e-30 and (1, 1, 1)
+    otherwise; both norms in torch's FMA order, the dot summed x, y, z in turn."""
+    from .ldu import face_mag
+    f = np.asarray(face_ids, dtype=np.int64)
+    d = face_centres[f] - cell_centres[np.asarray(owner)[f]]
+    s = np.asarray(face_areas, dtype=np.float64)[f]
+    safe = np.where((face_mag(s) > 1e-30)[:, None], s, 1.0)
+    nh = safe / face_mag(safe)[:, None]
+    dn = np.abs((d[:, 0] * nh[:, 0] + d[:, 1] * nh[:, 1]) + d[:, 2] * nh[:, 2])
+    return 1.0 / np.maximum(dn, 1e-30)

@@ -133,11 +133,14 @@ def compute_geometry(mesh: HexMesh, progress=None, device=None,
 
 def dirichlet_delta(mesh: HexMesh, geo: MeshGeometry, face_ids: np.ndarray) -> np.ndarray:
     """Boundary delta 1/max(|(x_f - C_P).n_hat|, 1e-30) on the given boundary
-    faces (a few per cent of the faces; host)."""
+    faces (a few per cent of the faces; host), as
+    src/applications/laplacian_foam.py:341-353 forms it: n_hat = S_f/|S_f|, and
+    (1, 1, 1)/sqrt(3) for a face with |S_f| <= 1e-30."""
     face_ids = np.asarray(face_ids, dtype=np.int64)
     d = geo.face_centres[face_ids] - geo.cell_centres[mesh.owner[face_ids]]
     s = geo.face_areas[face_ids]
     m = geo.mag_sf[face_ids]
-    nh = s / np.where(m > 1e-30, m, 1.0)[:, None]
+    ok = (m > 1e-30)[:, None]
+    nh = np.where(ok, s / np.where(ok, m[:, None], 1.0), 1.0 / np.sqrt(3.0))
     dn = np.abs((d[:, 0] * nh[:, 0] + d[:, 1] * nh[:, 1]) + d[:, 2] * nh[:, 2])
     return 1.0 / np.maximum(dn, 1e-30)

@@ -66,6 +66,36 @@ def test_geometry_golden(sysg):
     assert np.all(g["delta"][g["nfi"]:] == 0.0)  # boundary delta is 0 in the reference
 
 
+def test_dirichlet_delta_golden(sysg):
+    """The Dirichlet patch's boundary delta (laplacian_foam.py:341-353): the
+    oracle and the host helper the case builder uses both give the reference's
+    own values on its geometry; a face of zero area takes the normal
+    (1, 1, 1)/sqrt(3) and a face through its cell centre the 1e-30 clamp."""
+    from pyopenfoam_amd.mesh.geometry import MeshGeometry, dirichlet_delta
+    g = sysg
+    fb = g["dir_faces"]
+    od = omesh.dirichlet_delta(g["face_centres"], g["face_areas"], g["cell_centres"],
+                               g["owner"], fb)
+    np.testing.assert_allclose(od, g["dir_delta"], rtol=1e-15)
+    mag = oldu.face_mag(g["face_areas"])
+    assert bits_equal(mag[fb], g["dir_mag"])
+    geo = MeshGeometry(face_centres=g["face_centres"], face_areas=g["face_areas"], mag_sf=mag,
+                       cell_volumes=g["cell_volumes"], cell_centres=g["cell_centres"],
+                       delta=g["delta"])
+    pd = dirichlet_delta(SimpleNamespace(owner=g["owner"]), geo, fb)
+    assert bits_equal(pd, od)
+    # degenerate faces, closed forms: d = (0.3, -0.2, 0.5) from the cell centre
+    fc = np.array([[0.3, -0.2, 0.5], [0.0, 0.0, 0.0], [0.3, -0.2, 0.5]])
+    fa = np.array([[0.0, 0.0, 0.0], [0.0, 0.0, 2.0], [0.0, 0.0, 4.0]])
+    cc, own, ids = np.zeros((1, 3)), np.zeros(3, np.int64), np.arange(3)
+    want = [math.sqrt(3.0) / 0.6, 1e30, 2.0]  # |d.(1,1,1)|/sqrt(3) = 0.6/sqrt(3); clamp; 1/0.5
+    od = omesh.dirichlet_delta(fc, fa, cc, own, ids)
+    np.testing.assert_allclose(od, want, rtol=1e-15)
+    geo = MeshGeometry(face_centres=fc, face_areas=fa, mag_sf=oldu.face_mag(fa),
+                       cell_volumes=np.ones(1), cell_centres=cc, delta=np.zeros(3))
+    assert bits_equal(dirichlet_delta(SimpleNamespace(owner=own), geo, ids), od)
+
+
 def test_laplacian_golden_bit_exact(sysg):
     g = sysg
     nfi = g["nfi"]
