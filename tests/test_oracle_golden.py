@@ -298,6 +298,39 @@ def test_jacobi_oracle_known_answers():
     np.testing.assert_allclose(x, xs, rtol=1e-14)
 
 
+def test_gauss_seidel_v_cycle_oracle_pins():
+    """The V-cycle with Gauss-Seidel level smoothing (the device's
+    level_smoother="GaussSeidel" is compared with it), on the symmetric 8^3
+    system (a hierarchy of 512, 256, 61, 15 and 3 cells): the one-level cycle
+    is the coarsest solve (exact), the exact solution is a fixed point (a
+    dropped smoother term or a correction from b instead of r moves it), and
+    the coarse correction contracts the A-norm error below what the same
+    smoothing alone reaches (a dropped or sign-flipped correction does not:
+    measured 0.82x, 1.00x and 1.22x of smoothing alone after six cycles)."""
+    from tests.helpers import system
+    c, d, lo, up = system(8, 0.0, 0.5, 3)
+    A = oldu.dense(d, lo, up, c.owner, c.neighbour)
+    assert np.array_equal(A, A.T)
+    L0 = ogamg.Level(c.n_cells, d, lo, up, c.owner, c.neighbour)
+    levels, aggs = ogamg.build_hierarchy(L0, 10, 10)
+    assert [lv.n for lv in levels] == [512, 256, 61, 15, 3]
+    lc = levels[-1]
+    rhs = np.array([1.0, -2.0, 0.5])
+    one = ogamg.v_cycle([lc], [], 0, np.zeros(lc.n), rhs, smoother=ogamg.smooth_gs)
+    np.testing.assert_allclose(one, np.linalg.solve(
+        oldu.dense(lc.diag, lc.lower, lc.upper, lc.owner, lc.neighbour), rhs), rtol=1e-12)
+    xs = np.linalg.solve(A, c.b)
+    xf = ogamg.v_cycle(levels, aggs, 0, xs, c.b, 2, 2, ogamg.smooth_gs)
+    assert np.abs(xf - xs).max() <= 1e-13 * np.abs(xs).max()
+    anorm = lambda e: math.sqrt(e @ (A @ e))
+    x = np.zeros(c.n_cells)
+    y = np.zeros(c.n_cells)
+    for _ in range(6):
+        x = ogamg.v_cycle(levels, aggs, 0, x, c.b, 2, 2, ogamg.smooth_gs)
+        y = ogamg.smooth_gs(L0, y, c.b, 4)
+    assert anorm(x - xs) < 0.9 * anorm(y - xs)
+
+
 def test_smooth_solver_oracle_known_answers():
     """SmoothSolver's Gauss-Seidel sweep (smooth_solver.py:150-180): a 2-cell
     system swept twice by hand (d = (4, 4), l = u = -1, b = (2, 7), x0 = 0:
