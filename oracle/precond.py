@@ -3,7 +3,7 @@
 Restates ``solvers/preconditioners.py``:
   * ``dic_build``  -- DICPreconditioner._build (:88-133)
   * ``dilu_build`` -- DILUPreconditioner._build (:228-272)
-  * ``apply``      -- diagonal apply r / d (:135-151, :274-288)
+  * the diagonal apply (:135-151, :274-288) is r / rD, which callers write inline
   * ``dic_apply_full`` / ``dilu_apply_full`` -- triangular sweeps (:153-201, :290-337)
 The ``*_py`` variants are literal Python loops for small systems; the default
 variants call the C restatement in ``c/oracle.c`` (same loops, same rounding).
@@ -60,10 +60,6 @@ def dilu_build(diag, lower, upper, owner, neighbour) -> np.ndarray:
     out = np.empty_like(d)
     clib().orc_dilu_build(d.shape[0], o.shape[0], p(o), p(q), p(d), p(lo), p(u), p(out))
     return out
-
-
-def apply(rD, r) -> np.ndarray:
-    return r / rD
 
 
 def dic_apply_full(rD, upper, owner, neighbour, r) -> np.ndarray:
