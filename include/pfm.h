@@ -45,6 +45,7 @@ extern "C" {
 #define PFM_PRECOND_DIAG 1 /* z = r / rD, rD from pfm_dic_factor / pfm_dilu_factor */
 
 const char *pfm_last_error(void);
+/* ABI version of this header (no reference counterpart: checked by the loader). */
 int pfm_abi_version(void);
 
 /* ------------------------------------------------------------------ */
@@ -232,7 +233,10 @@ typedef struct {
   double final_residual;
 } pfm_solve_result;
 
-/* Device workspace (bytes) for one solve with at most max_iter iterations. */
+/* Device workspace (bytes) for one solve with at most max_iter iterations.
+ * No reference counterpart: PCGSolver._solve / PBiCGSTABSolver._solve
+ * (pcg.py:97-187, pbicgstab.py:103-239) allocate their vectors per call; here
+ * the caller allocates them once and passes them in. */
 int64_t pfm_pcg_workspace_bytes(pfm_ldu h, int32_t max_iter);
 int64_t pfm_pbicgstab_workspace_bytes(pfm_ldu h, int32_t max_iter);
 
@@ -400,13 +404,16 @@ int pfm_gamg_level_info(pfm_gamg g, int32_t level, int32_t *n_cells, int32_t *n_
 int pfm_gamg_level_maps(pfm_gamg g, int32_t level, int32_t *f2c_h, int32_t *face_map_h,
                         int32_t *c_owner_h, int32_t *c_neighbour_h);
 
+/* Workspace of pfm_gamg_solve (no reference counterpart, as for PCG above). */
 int64_t pfm_gamg_workspace_bytes(pfm_gamg g, int32_t max_iter);
-/* Coarsen the coefficients into the workspace (diag by aggregate sums in fine
+/* GAMGSolver._build_coarse_matrix (solvers/gamg.py:242-330) for every level:
+ * coarsen the coefficients into the workspace (diag by aggregate sums in fine
  * cell order, off-diagonals by coarse-face sums in fine face order), pack every
  * level and compute the Jacobi inverse diagonals.  pfm_gamg_solve calls it. */
 int pfm_gamg_setup_values(pfm_gamg g, const double *diag, const double *lower,
                           const double *upper, void *work, void *stream);
-/* Copy the coarsened coefficients of `level` (>= 1) out of the workspace. */
+/* Copy the coarsened coefficients of `level` (>= 1) out of the workspace (the
+ * reference's GAMGLevel diag / lower / upper, gamg.py:326-333). */
 int pfm_gamg_level_values(pfm_gamg g, void *work, int32_t level, double *diag,
                           double *lower, double *upper, void *stream);
 /* GAMGSolver._solve (gamg.py:499-585): V-cycles with undamped Jacobi
@@ -557,8 +564,10 @@ int pfm_adjoint_ldu_grads(pfm_ldu h, const double *lam, const double *x,
                           double *g_diag, double *g_lower, double *g_upper,
                           void *stream);
 
-/* Chain rule of the assembly above: gradient w.r.t. gamma_f given gradients
- * w.r.t. (diag, lower, upper):
+/* Chain rule of the assembly above (what torch autograd does through
+ * fvm.laplacian, discretisation/operators.py:432-518, and the Dirichlet term,
+ * applications/laplacian_foam.py:341-366): gradient w.r.t. gamma_f given
+ * gradients w.r.t. (diag, lower, upper):
  *   g_gamma_f = |S_f| delta_f ((g_diag[P]-g_lower_f)/V_P + (g_diag[N]-g_upper_f)/V_N)
  * and for Dirichlet faces g_gamma_b = |S_b| delta_b g_diag[P] / V_P. */
 int pfm_laplacian_backward(pfm_ldu h, const double *g_diag,
@@ -569,7 +578,9 @@ int pfm_laplacian_backward(pfm_ldu h, const double *g_diag,
                            const double *dir_delta, double *g_gamma,
                            double *g_dir_gamma, void *stream);
 
-/* Fused adjoint face-gradient scatter straight to dJ/dgamma:
+/* Fused adjoint face-gradient scatter straight to dJ/dgamma: the two entry
+ * points above composed (differentiable/linear_solver.py:141-149 followed by
+ * autograd through the assembly) in one pass over the faces:
  *   dJ/dgamma_f = -|S_f| delta_f (x_P - x_N) (lam_P/V_P - lam_N/V_N)
  *   dJ/dgamma_b = -|S_b| delta_b lam_P x_P / V_P      (Dirichlet faces). */
 int pfm_adjoint_gamma_grad(pfm_ldu h, const double *lam, const double *x,
