@@ -214,6 +214,37 @@ def test_gauss_seidel_and_adjoint_golden():
         np.testing.assert_allclose(a, g[b], rtol=1e-8, atol=1e-14 * np.abs(g[b]).max())
 
 
+def test_laplacian_two_cell_known_answers():
+    """The oracle's geometry + fvm.laplacian on the reference tests' two-cell hex
+    mesh (two unit cubes stacked in z, tests/unit/discretisation/test_laplacian.py:18-65)
+    against that test's assertions (:82-170) with this mesh's values: |S| = delta
+    = V = 1 on the shared face, so lower = upper = -1 and diag = [1, 1] (to the
+    rounding of the geometry); A 1 = 0 exactly; D per cell [1, 3] interpolates to
+    2 on the face (weights 1/2)."""
+    pts = [[0, 0, 0], [1, 0, 0], [1, 1, 0], [0, 1, 0], [0, 0, 1], [1, 0, 1], [1, 1, 1],
+           [0, 1, 1], [0, 0, 2], [1, 0, 2], [1, 1, 2], [0, 1, 2]]
+    faces = np.array([[4, 5, 6, 7], [0, 3, 2, 1], [0, 1, 5, 4], [3, 7, 6, 2], [0, 4, 7, 3],
+                      [1, 2, 6, 5], [8, 9, 10, 11], [4, 5, 9, 8], [7, 11, 10, 6],
+                      [4, 8, 11, 7], [5, 6, 10, 9]])
+    own = np.array([0, 0, 0, 0, 0, 0, 1, 1, 1, 1, 1])
+    nei = np.array([1])
+    geo = omesh.geometry_vec(np.array(pts, np.float64), faces, own, nei, 2)
+    np.testing.assert_allclose(geo["cell_volumes"], [1.0, 1.0], rtol=0, atol=1e-14)
+    np.testing.assert_allclose(geo["cell_centres"], [[0.5, 0.5, 0.5], [0.5, 0.5, 1.5]],
+                               rtol=0, atol=1e-14)
+    args = (geo["mag_sf"][:1], geo["delta"][:1], geo["cell_volumes"], own[:1], nei)
+    d, lo, up = oldu.laplacian(np.ones(1), *args)
+    np.testing.assert_allclose(lo, [-1.0], rtol=0, atol=1e-12)
+    np.testing.assert_allclose(up, [-1.0], rtol=0, atol=1e-12)
+    np.testing.assert_allclose(d, [1.0, 1.0], rtol=0, atol=1e-12)
+    assert oldu.ax(d, lo, up, own[:1], nei, np.ones(2)).tolist() == [0.0, 0.0]
+    d2, lo2, up2 = oldu.laplacian(np.full(1, 2.0), *args)
+    assert bits_equal(lo2, 2.0 * lo) and bits_equal(up2, 2.0 * up) and bits_equal(d2, 2.0 * d)
+    dc, loc, upc = oldu.laplacian(np.array([0.5 * 1.0 + 0.5 * 3.0]), *args)
+    np.testing.assert_allclose(loc, [-2.0], rtol=0, atol=1e-12)
+    np.testing.assert_allclose(dc, [2.0, 2.0], rtol=0, atol=1e-12)
+
+
 def test_ax_known_answers():
     # reference tests/unit/core/test_ldu_matrix.py:151-202
     o, q = np.array([0]), np.array([1])
