@@ -540,18 +540,39 @@ int pfm_adjust_phi(int32_t n_faces, int32_t n_internal, double *phi,
                    int closed, double *work, double *net_out_h,
                    int32_t *adjusted_out_h, void *stream);
 
+/* Dirichlet (fixedValue) patch faces of a mesh for pfm_laplacian_assemble,
+ * the term of applications/laplacian_foam.py:318-366, which loops over the
+ * patches and adds each one's scatter_add to the diagonal in turn.
+ * dir_cell_h: the owner cell of each of n_dir faces (host array) in that
+ * order -- patch after patch, patch p holding entries patch_ptr_h[p] ..
+ * patch_ptr_h[p+1]-1, its faces in face order.  The library groups the
+ * entries by cell once, on the host, so that the assembly kernel gives every
+ * cell to one thread whatever the caller's order, and that thread adds the
+ * cell's terms as the reference does: diag + s_1, then + s_2, ..., where s_p
+ * is patch p's scatter_add sum over the cell's faces of that patch.  Cells
+ * must lie in [0, n_cells of h).  Device state is allocated on the current
+ * device; destroy with pfm_dirichlet_destroy. */
+typedef struct pfm_dirichlet_s *pfm_dirichlet;
+int pfm_dirichlet_create(pfm_ldu h, int32_t n_dir, const int32_t *dir_cell_h,
+                         int32_t n_patches, const int32_t *patch_ptr_h,
+                         pfm_dirichlet *out);
+int pfm_dirichlet_destroy(pfm_dirichlet d);
+
 /* fvm.laplacian (discretisation/operators.py:490-517) plus the
  * laplacianFoam Dirichlet diagonal term (applications/laplacian_foam.py:342-366):
  *   c_f = gamma_f*|S_f|*delta_f, lower_f = -c_f/V_P, upper_f = -c_f/V_N,
  *   diag_i = sum c/V_P over owner faces + sum c/V_N over neighbour faces
- *            + sum over Dirichlet faces b of cell i of gamma_b*|S_b|*delta_b/V_i.
- * Dirichlet faces are given by owner cell (dir_cell, n_dir entries, sorted by
- * cell), gamma_b, |S_b| and delta_b = 1/max(|(x_f - C_P).n_hat|, 1e-30).
- * Every product and quotient is rounded in the reference's order, so the
- * result is bit-identical to the reference's torch ops. */
+ *            + per patch, the sum over its faces b of cell i of
+ *              gamma_b*|S_b|*delta_b/V_i.
+ * Dirichlet faces (dir = NULL for none) are described by a plan from
+ * pfm_dirichlet_create on the same handle; dir_gamma, dir_mag_sf (|S_b|) and
+ * dir_delta (1/max(|(x_f - C_P).n_hat|, 1e-30)) hold one value per entry in
+ * the plan's (the reference's) order.  Every product, quotient and sum is
+ * rounded in the reference's order, so the result is bit-identical to the
+ * reference's torch ops. */
 int pfm_laplacian_assemble(pfm_ldu h, const double *gamma, const double *mag_sf,
-                           const double *delta, const double *vol, int32_t n_dir,
-                           const int32_t *dir_cell, const double *dir_gamma,
+                           const double *delta, const double *vol,
+                           pfm_dirichlet dir, const double *dir_gamma,
                            const double *dir_mag_sf, const double *dir_delta,
                            double *diag, double *lower, double *upper,
                            void *stream);

@@ -85,3 +85,12 @@ def dirichlet_diag(diag, gamma_b, mag_b, delta_b, vol, cells):
     n = diag.shape[0]
     coeff = gamma_b * mag_b * delta_b
     return diag + _scatter(coeff / vol[cells], cells, n)
+
+
+def dirichlet_diag_patches(diag, patches, vol):
+    """The patch loop of ``applications/laplacian_foam.py:318-366``: each patch
+    (gamma_b, |S_b|, delta_b, cells), faces in face order, adds its own
+    scatter_add to the diagonal in turn (:357)."""
+    for gamma_b, mag_b, delta_b, cells in patches:
+        diag = dirichlet_diag(diag, gamma_b, mag_b, delta_b, vol, cells)
+    return diag

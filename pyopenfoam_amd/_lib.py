@@ -136,7 +136,9 @@ _SIGS = {
                                  C.POINTER(SolveResult), P]),
     "pfm_gamg_mixed_solve": (INT, [P, C.POINTER(SolveParams), C.c_double, I32, I32, P, P, P, P, P,
                                    P, P, C.POINTER(SolveResult), P, P]),
-    "pfm_laplacian_assemble": (INT, [P, P, P, P, P, I32, P, P, P, P, P, P, P, P]),
+    "pfm_dirichlet_create": (INT, [P, I32, P, I32, P, C.POINTER(P)]),
+    "pfm_dirichlet_destroy": (INT, [P]),
+    "pfm_laplacian_assemble": (INT, [P, P, P, P, P, P, P, P, P, P, P, P, P]),
     "pfm_adjoint_ldu_grads": (INT, [P, P, P, P, P, P, P]),
     "pfm_laplacian_backward": (INT, [P, P, P, P, P, P, P, I32, P, P, P, P, P, P]),
     "pfm_adjoint_gamma_grad": (INT, [P, P, P, P, P, P, I32, P, P, P, P, P, P]),

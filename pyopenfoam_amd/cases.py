@@ -33,7 +33,7 @@ class PoissonCase:
     mesh: HexMesh
     geo: MeshGeometry
     gamma: np.ndarray          # (n_internal_faces,)
-    dir_faces: np.ndarray      # boundary face ids of the Dirichlet patch, sorted by owner
+    dir_faces: np.ndarray      # boundary face ids of the Dirichlet patch, in face order
     dir_cells: np.ndarray      # owner of each Dirichlet face
     dir_gamma: np.ndarray
     dir_mag: np.ndarray
@@ -96,8 +96,6 @@ def _finish_case(n, mesh, rng, sigma, seed, dirichlet_patch, geo) -> PoissonCase
     patch = mesh.patch(dirichlet_patch)
     faces = np.arange(patch["start"], patch["start"] + patch["size"])
     cells = mesh.owner[faces]
-    order = np.argsort(cells, kind="stable")
-    faces, cells = faces[order], cells[order]
     return PoissonCase(
         n=n, mesh=mesh, geo=geo, gamma=gamma_all[:nfi].copy(), dir_faces=faces, dir_cells=cells,
         dir_gamma=gamma_all[faces].copy(), dir_mag=geo.mag_sf[faces].copy(),
@@ -130,7 +128,7 @@ class BatchCase:
     mag: np.ndarray            # (F,)
     delta: np.ndarray
     gamma: np.ndarray
-    dir_cells: np.ndarray      # global, sorted
+    dir_cells: np.ndarray      # global; each system's patch in face order
     dir_gamma: np.ndarray
     dir_mag: np.ndarray
     dir_delta: np.ndarray

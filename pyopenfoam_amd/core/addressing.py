@@ -18,7 +18,7 @@ import torch
 
 from .. import _lib
 
-__all__ = ["LduAddressing", "addressing_for"]
+__all__ = ["LduAddressing", "DirichletPlan", "addressing_for"]
 
 
 class LduAddressing:
@@ -61,11 +61,67 @@ class LduAddressing:
         _lib.check(_lib.lib().pfm_ldu_set_progress(self._h, self._progress, None,
                                                    float(interval_s)))
 
+    def dirichlet_plan(self, cells, patch_sizes=None) -> "DirichletPlan":
+        """Plan of the Dirichlet faces for ``pfm_laplacian_assemble``.
+
+        ``cells``: the owner cell of each Dirichlet face (numpy or torch), in
+        the reference's order -- patch after patch, faces in face order;
+        ``patch_sizes``: faces per patch (default: one patch).  Plans are
+        cached on this handle by the cells' values and the patch sizes.
+        """
+        c = _lib.host_i32(cells)
+        sizes = (int(c.shape[0]),) if patch_sizes is None else tuple(int(k) for k in patch_sizes)
+        if sum(sizes) != c.shape[0]:
+            raise ValueError("patch sizes do not add up to the number of Dirichlet faces")
+        key = (sizes, c.tobytes())
+        plans = self.__dict__.setdefault("_dir_plans", OrderedDict())
+        with _cache_lock:
+            hit = plans.get(key)
+            if hit is not None:
+                plans.move_to_end(key)
+                return hit
+        plan = DirichletPlan(self, c, sizes)
+        with _cache_lock:
+            plans[key] = plan
+            while len(plans) > _CACHE_SIZE:
+                plans.popitem(last=False)
+        return plan
+
     def __del__(self):
         h = getattr(self, "_h", None)
         if h is not None and h.value:
             try:
                 _lib.lib().pfm_ldu_destroy(h)
+            except Exception:
+                pass
+            self._h = None
+
+
+class DirichletPlan:
+    """``pfm_dirichlet`` handle: Dirichlet faces grouped by cell in the library."""
+
+    def __init__(self, addr: LduAddressing, cells: np.ndarray, patch_sizes: tuple):
+        self.addr = addr  # the topology handle outlives the plan
+        self.n_dir = int(cells.shape[0])
+        self.patch_sizes = patch_sizes
+        ptr = np.zeros(len(patch_sizes) + 1, dtype=np.int32)
+        ptr[1:] = np.cumsum(patch_sizes)
+        h = C.c_void_p()
+        with torch.cuda.device(addr.device):
+            _lib.check(_lib.lib().pfm_dirichlet_create(addr.handle, self.n_dir, _lib.ptr(cells),
+                                                       len(patch_sizes), _lib.ptr(ptr),
+                                                       C.byref(h)), "pfm_dirichlet_create")
+        self._h = h
+
+    @property
+    def handle(self) -> C.c_void_p:
+        return self._h
+
+    def __del__(self):
+        h = getattr(self, "_h", None)
+        if h is not None and h.value:
+            try:
+                _lib.lib().pfm_dirichlet_destroy(h)
             except Exception:
                 pass
             self._h = None

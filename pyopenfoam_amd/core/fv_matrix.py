@@ -50,10 +50,11 @@ class FvMatrix(LduMatrix):
         self._check_compute()
         r = torch.empty_like(x)
         if self._n_cells:
-            _lib.check(_lib.lib().pfm_residual(self.addressing.handle,
-                                               _lib.ptr(self._diag.detach().contiguous()),
+            d = self._diag.detach().contiguous()
+            src = self._source.detach().contiguous()
+            _lib.check(_lib.lib().pfm_residual(self.addressing.handle, _lib.ptr(d),
                                                _lib.ptr(self.packed(False)), _lib.ptr(x.detach()),
-                                               _lib.ptr(self._source.detach().contiguous()),
-                                               _lib.ptr(r), current_stream(self._device)),
+                                               _lib.ptr(src), _lib.ptr(r),
+                                               current_stream(self._device)),
                        "pfm_residual")
         return r

@@ -116,6 +116,17 @@ class LduMatrix:
             raise TypeError("libpfm LDU kernels compute in float64; use mixed-precision solvers "
                             "for float32 inner work")
 
+    def coefficients(self) -> tuple[torch.Tensor, torch.Tensor, torch.Tensor]:
+        """(diag, lower, upper), detached and contiguous.
+
+        The caller binds the tuple for as long as the device pointers are in
+        use: a contiguous copy of a strided coefficient is a new tensor, and
+        one freed before its kernel runs can be handed by the caching
+        allocator to the next copy, so two arguments would share one block.
+        """
+        return (self._diag.detach().contiguous(), self._lower.detach().contiguous(),
+                self._upper.detach().contiguous())
+
     def packed(self, transpose: bool = False) -> torch.Tensor:
         """Row-slice packed off-diagonal coefficients (cached)."""
         self._check_compute()
@@ -126,8 +137,9 @@ class LduMatrix:
             return hit[1]
         addr = self.addressing
         buf = torch.empty(max(addr.n_slots, 1), device=self._device, dtype=torch.float64)
-        _lib.check(_lib.lib().pfm_ldu_pack(addr.handle, int(transpose), _lib.ptr(lo.contiguous()),
-                                           _lib.ptr(up.contiguous()), _lib.ptr(buf),
+        loc, upc = lo.contiguous(), up.contiguous()
+        _lib.check(_lib.lib().pfm_ldu_pack(addr.handle, int(transpose), _lib.ptr(loc),
+                                           _lib.ptr(upc), _lib.ptr(buf),
                                            current_stream(self._device)), "pfm_ldu_pack")
         self._packed[bool(transpose)] = (key, buf)
         return buf
@@ -142,8 +154,9 @@ class LduMatrix:
             return hit[1]
         addr = self.addressing
         buf = torch.empty(max(addr.n_slots, 1), device=self._device, dtype=torch.float32)
+        loc, upc = lo.contiguous(), up.contiguous()
         _lib.check(_lib.lib().pfm_ldu_pack_f32(addr.handle, int(transpose),
-                                               _lib.ptr(lo.contiguous()), _lib.ptr(up.contiguous()),
+                                               _lib.ptr(loc), _lib.ptr(upc),
                                                _lib.ptr(buf), current_stream(self._device)),
                    "pfm_ldu_pack_f32")
         self._packed[("f32", bool(transpose))] = (key, buf)
@@ -185,8 +198,8 @@ class LduMatrix:
         pk = self.packed(transpose)
         y = torch.empty(self._n_cells, device=self._device, dtype=self._dtype)
         if self._n_cells:
-            _lib.check(_lib.lib().pfm_spmv(self.addressing.handle,
-                                           _lib.ptr(self._diag.detach().contiguous()),
+            d = self._diag.detach().contiguous()
+            _lib.check(_lib.lib().pfm_spmv(self.addressing.handle, _lib.ptr(d),
                                            _lib.ptr(pk), _lib.ptr(x.detach()), _lib.ptr(y),
                                            current_stream(self._device)), "pfm_spmv")
         return y

@@ -45,7 +45,7 @@ struct pfm_batch_s {
   int32_t ntiles = 0, tile_slices = 0;
   int32_t *d_tile_sys = nullptr, *d_tile_s0 = nullptr, *d_tile_s1 = nullptr, *d_sys_t0 = nullptr;
   int32_t *d_cell_map = nullptr;  // unpadded cell -> padded row
-  int32_t *h_flag = nullptr;
+  pfm::PollPool poll;  // pinned stop-flag slots of the solves
   float tuned_us[8] = {0};
 };
 
@@ -670,7 +670,7 @@ int pfm_batch_destroy(pfm_batch B) {
   (void)hipFree(B->d_tile_s1);
   (void)hipFree(B->d_sys_t0);
   (void)hipFree(B->d_cell_map);
-  if (B->h_flag) (void)hipHostFree(B->h_flag);
+  poll_pool_free(B->poll);
   if (B->ldu) pfm_ldu_destroy(B->ldu);
   delete B;
   return PFM_OK;
@@ -716,11 +716,7 @@ int pfm_batch_create(int32_t ns, const int32_t *cell_offsets, const int32_t *fac
   }
   int st = pfm_ldu_create(B->np, B->nf, own.data(), nei.data(), &B->ldu);
   if (st == PFM_OK) st = upload(&B->d_cell_map, cmap);
-  if (st == PFM_OK &&
-      hipHostMalloc((void **)&B->h_flag, 4 * sizeof(int32_t), hipHostMallocDefault) != hipSuccess) {
-    set_error("hipHostMalloc failed");
-    st = PFM_ERR_HIP;
-  }
+  if (st == PFM_OK) st = poll_pool_init(B->poll);
   if (st != PFM_OK) {
     pfm_batch_destroy(B);
     return st;
@@ -802,11 +798,16 @@ int pfm_batch_pcg_solve(pfm_batch B, const pfm_solve_params *prm, int transpose,
     }
   }
   // replay chunks until every system has stopped (one chunk queued ahead)
-  int32_t *flag = B->h_flag;
-  hipEvent_t ev[2];
-  PFM_CHECK_HIP(hipEventCreateWithFlags(&ev[0], hipEventDisableTiming));
-  PFM_CHECK_HIP(hipEventCreateWithFlags(&ev[1], hipEventDisableTiming));
-  int enq = 0, slot = 0, rc = PFM_OK;
+  PollLease lease;
+  int rc = lease.acquire(&B->poll, true);
+  if (rc != PFM_OK) {
+    if (exec) (void)hipGraphExecDestroy(exec);
+    if (cap) (void)hipStreamDestroy(cap);
+    return rc;
+  }
+  int32_t *flag = lease.flag;
+  hipEvent_t *ev = lease.ev;
+  int enq = 0, slot = 0;
   bool pending = false;
   while (true) {
     bool queued = false;
@@ -842,8 +843,6 @@ int pfm_batch_pcg_solve(pfm_batch B, const pfm_solve_params *prm, int transpose,
     pending = true;
     slot ^= 1;
   }
-  (void)hipEventDestroy(ev[0]);
-  (void)hipEventDestroy(ev[1]);
   if (exec) (void)hipGraphExecDestroy(exec);
   if (cap) (void)hipStreamDestroy(cap);
   if (rc != PFM_OK) {

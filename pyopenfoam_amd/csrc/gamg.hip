@@ -1172,10 +1172,11 @@ void enqueue_cycle(const pfm_gamg_s *g, const CycleLevels<T> &C, const KState *s
 template <typename Cycle>
 int drive_cycles(const pfm_gamg_s *g, const KState *st, int max_cycles, hipStream_t s,
                  std::vector<int> &cur_after, int &cur0, Cycle cycle) {
-  int32_t *flag = g->h_flag;
-  hipEvent_t ev[2];
-  PFM_CHECK_HIP(hipEventCreateWithFlags(&ev[0], hipEventDisableTiming));
-  PFM_CHECK_HIP(hipEventCreateWithFlags(&ev[1], hipEventDisableTiming));
+  PollLease lease;
+  int rc0 = lease.acquire(const_cast<PollPool *>(&g->poll), true);
+  if (rc0 != PFM_OK) return rc0;
+  int32_t *flag = lease.flag;
+  hipEvent_t *ev = lease.ev;
   int enq = 0, chunk = 2, slot = 0, rc = PFM_OK;
   bool pending = false;
   double last_report = now_seconds();
@@ -1209,8 +1210,6 @@ int drive_cycles(const pfm_gamg_s *g, const KState *st, int max_cycles, hipStrea
     pending = true;
     slot ^= 1;
   }
-  (void)hipEventDestroy(ev[0]);
-  (void)hipEventDestroy(ev[1]);
   if (rc != PFM_OK) set_error("HIP error while driving the GAMG cycles");
   return rc;
 }
@@ -1220,7 +1219,7 @@ extern "C" {
 int pfm_gamg_destroy(pfm_gamg g) {
   if (!g) return PFM_OK;
   for (auto &H : g->lv) free_level(H);
-  if (g->h_flag) (void)hipHostFree(g->h_flag);
+  poll_pool_free(g->poll);
   delete g;
   return PFM_OK;
 }
@@ -1305,11 +1304,7 @@ int pfm_gamg_create(pfm_ldu fine, const pfm_gamg_params *prm, pfm_gamg *out) {
     set_error("too many GAMG levels");
     st = PFM_ERR_ARG;
   }
-  if (st == PFM_OK &&
-      hipHostMalloc((void **)&g->h_flag, 4 * sizeof(int32_t), hipHostMallocDefault) != hipSuccess) {
-    set_error("hipHostMalloc failed");
-    st = PFM_ERR_HIP;
-  }
+  if (st == PFM_OK) st = poll_pool_init(g->poll);
   if (st != PFM_OK) {
     pfm_gamg_destroy(g);
     return st;

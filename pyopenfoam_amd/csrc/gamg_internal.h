@@ -60,5 +60,5 @@ struct pfm_gamg_s {
   pfm_gamg_params prm{};
   std::vector<GamgLevelHost> lv;
   int device = 0;
-  int32_t *h_flag = nullptr;
+  pfm::PollPool poll;  // pinned stop-flag slots of the solves
 };

@@ -154,8 +154,11 @@ __global__ __launch_bounds__(kBlock) void k_pcg_spmv(int n, int nslices, const S
 
 // alpha = rz / pAp (breakdown if |pAp| < 1e-30), kept for k_pcg_pupdate;
 // r -= alpha Ap; partials r.r and r.z (region B).  On a breakdown at an even
-// iteration (pprev = p_{it-1} given) the x update deferred at iteration it-1
-// is applied before the solve ends.
+// iteration (pprev = p_{it-1} given) every workgroup applies the x update
+// deferred at iteration it-1 to its rows.  The breakdown is recorded in
+// `pending`, which this kernel never tests, and becomes `done` in the next
+// k_pcg_spmv: a `done` stored here would make a workgroup dispatched after
+// the store return at the entry test and skip its rows' update.
 __global__ __launch_bounds__(kBlock) void k_pcg_update(int n, int nslices, int it,
                                                         const double *__restrict__ rD,
                                                         const double *__restrict__ Ap,
@@ -180,7 +183,7 @@ __global__ __launch_bounds__(kBlock) void k_pcg_update(int n, int nslices, int i
   }
   const double PAP = reduce_partials(part, G, lds);
   if (fabs(PAP) < 1e-30) {
-    if (blockIdx.x == 0 && threadIdx.x == 0) finish(st, PFM_SOLVE_BREAKDOWN);
+    if (blockIdx.x == 0 && threadIdx.x == 0) st->pending = PFM_SOLVE_BREAKDOWN;
     if (pprev) {  // the x update deferred at iteration it-1 (see k_pcg_pupdate)
       const double ap_ = st->alphas[(it - 1) & 1];
       FOR_ROWS({
@@ -227,7 +230,10 @@ __global__ __launch_bounds__(kBlock) void k_pcg_pupdate(int n, int nslices, int 
                                                          const double *__restrict__ part, KState *st,
                                                          double *__restrict__ hist) {
   __shared__ double lds[kWavesPerBlock];
-  if (st->done) return;  // set by an earlier kernel only (a breakdown in k_pcg_update)
+  // A breakdown in this iteration's k_pcg_update ended the solve.  This
+  // kernel's workgroup 0 writes `pending` only as CONVERGED or MAXITER, so a
+  // workgroup that reads it after that store still proceeds, as it must.
+  if (st->done || st->pending == PFM_SOLVE_BREAKDOWN) return;
   const int G = gridDim.x;
   const bool pair = pprev != nullptr;
   // First round's operands loaded before the reductions (they do not depend
@@ -747,11 +753,13 @@ int check_solve_args(pfm_ldu h, const pfm_solve_params *prm, const double *diag,
 // Enqueue `body()` (one iteration) until the device stop flag is seen.
 // One chunk stays queued ahead of the poll so the GPU never idles on it.
 template <typename Body>
-int drive(pfm_ldu h, const KState *st_dev, int max_iter, hipStream_t s, Body body) {
-  int32_t *flag = h->h_flag;
-  hipEvent_t ev[2];
-  PFM_CHECK_HIP(hipEventCreateWithFlags(&ev[0], hipEventDisableTiming));
-  PFM_CHECK_HIP(hipEventCreateWithFlags(&ev[1], hipEventDisableTiming));
+int drive(pfm_ldu h, const KState *st_dev, int max_iter, hipStream_t s, Body body,
+          bool may_allocate = true) {
+  PollLease lease;
+  int rc0 = lease.acquire(&h->poll, may_allocate);
+  if (rc0 != PFM_OK) return rc0;
+  int32_t *flag = lease.flag;
+  hipEvent_t *ev = lease.ev;
   int enq = 0, chunk = 4, slot = 0, rc = PFM_OK;
   bool pending = false;
   double last_report = now_seconds();
@@ -783,10 +791,11 @@ int drive(pfm_ldu h, const KState *st_dev, int max_iter, hipStream_t s, Body bod
     slot ^= 1;
   }
   if (rc != PFM_OK) set_error("HIP error while driving the solver loop");
-  (void)hipEventDestroy(ev[0]);
-  (void)hipEventDestroy(ev[1]);
   return rc;
 }
+
+int collect(const KState *st_dev, const double *hist_dev, double *history_h, int hist_cap,
+            pfm_solve_result *res, hipStream_t s);
 
 int collect(const KState *st_dev, const double *hist_dev, double *history_h, int hist_cap,
             pfm_solve_result *res, hipStream_t s) {
@@ -823,6 +832,16 @@ int64_t pfm_pbicgstab_workspace_bytes(pfm_ldu h, int32_t max_iter) {
 int pfm_pcg_solve(pfm_ldu h, const pfm_solve_params *prm, const double *diag, const double *packed,
                   const double *rD, const double *b, double *x, void *work, double *history_h,
                   pfm_solve_result *res, void *stream) {
+  return pfm::pcg_solve(h, prm, diag, packed, rD, b, x, work, history_h, res, stream, true);
+}
+
+}  // extern "C"
+
+namespace pfm {
+
+int pcg_solve(pfm_ldu h, const pfm_solve_params *prm, const double *diag, const double *packed,
+              const double *rD, const double *b, double *x, void *work, double *history_h,
+              pfm_solve_result *res, void *stream, bool may_allocate) {
   int rc = check_solve_args(h, prm, diag, packed, rD, b, x, work, res);
   if (rc != PFM_OK) return rc;
   memset(res, 0, sizeof(*res));
@@ -863,11 +882,15 @@ int pfm_pcg_solve(pfm_ldu h, const pfm_solve_params *prm, const double *diag, co
                                              Ap, w.part, w.st);
     k_pcg_update<<<G, kBlock, 0, s>>>(n, ns, it, pd, Ap, r, x, pprev, w.part, w.st);
     k_pcg_pupdate<<<G, kBlock, 0, s>>>(n, ns, it, pd, r, pc, pprev, po, x, w.part, w.st, w.hist);
-  });
+  }, may_allocate);
   if (rc != PFM_OK) return rc;
   PFM_CHECK_HIP(hipGetLastError());
   return collect(w.st, w.hist, history_h, hist_cap, res, s);
 }
+
+}  // namespace
+
+extern "C" {
 
 int pfm_pbicgstab_solve(pfm_ldu h, const pfm_solve_params *prm, const double *diag,
                         const double *packed, const double *rD, const double *b, double *x,

@@ -7,6 +7,7 @@
 // the DIC/DILU factors and the SpMV are bit-identical to the sequential
 // reference recurrences.
 
+#include <algorithm>
 #include <cmath>
 
 #include <cstdlib>
@@ -255,20 +256,37 @@ __global__ __launch_bounds__(kBlock) void k_lap_diag(int n, int nslices,
 
 // Dirichlet faces (sorted by owner cell): the first face of each cell's run
 // sums the run's gamma_b*|S_b|*delta_b/V in face order and adds it.
-__global__ __launch_bounds__(kBlock) void k_lap_dirichlet(int nd, const int32_t *__restrict__ cell,
+// One thread per distinct Dirichlet cell (group g of the plan).  The group's
+// entries are in the caller's order (patch, then face); each patch's terms
+// are summed from zero as its scatter_add does, and each patch's sum is added
+// to the diagonal in patch order.
+__global__ __launch_bounds__(kBlock) void k_lap_dirichlet(int ngroups,
+                                                           const int32_t *__restrict__ gptr,
+                                                           const int32_t *__restrict__ gcell,
+                                                           const int32_t *__restrict__ order,
+                                                           const int32_t *__restrict__ patch,
                                                            const double *__restrict__ g,
                                                            const double *__restrict__ mag,
                                                            const double *__restrict__ dlt,
                                                            const double *__restrict__ vol,
                                                            double *__restrict__ diag) {
   const int k = blockIdx.x * kBlock + threadIdx.x;
-  if (k >= nd) return;
-  const int c = cell[k];
-  if (k > 0 && cell[k - 1] == c) return;
+  if (k >= ngroups) return;
+  const int c = gcell[k];
   const double v = vol[c];
-  double s = 0.0;
-  for (int m = k; m < nd && cell[m] == c; ++m) s += ((g[m] * mag[m]) * dlt[m]) / v;
-  diag[c] = diag[c] + s;
+  double d = diag[c], s = 0.0;
+  const int j0 = gptr[k], j1 = gptr[k + 1];
+  int cur = patch[j0];
+  for (int j = j0; j < j1; ++j) {
+    if (patch[j] != cur) {
+      d = d + s;
+      s = 0.0;
+      cur = patch[j];
+    }
+    const int e = order[j];
+    s += ((g[e] * mag[e]) * dlt[e]) / v;
+  }
+  diag[c] = d + s;
 }
 
 // ---------------------------------------------------------------------------
@@ -378,7 +396,7 @@ int pfm_ldu_destroy(pfm_ldu h) {
   (void)hipFree(h->d_gs_fwd_cells);
   (void)hipFree(h->d_gs_bwd_ptr);
   (void)hipFree(h->d_gs_bwd_cells);
-  if (h->h_flag) (void)hipHostFree(h->h_flag);
+  poll_pool_free(h->poll);
   (void)hipSetDevice(dev_prev);
   delete h;
   return PFM_OK;
@@ -451,11 +469,9 @@ int pfm_ldu_create(int32_t n, int32_t nf, const int32_t *own, const int32_t *nei
       return st;
     }
   }
-  e = hipHostMalloc((void **)&h->h_flag, 4 * sizeof(int32_t), hipHostMallocDefault);
-  if (e != hipSuccess) {
+  if ((st = poll_pool_init(h->poll)) != PFM_OK) {
     pfm_ldu_destroy(h);
-    set_error("hipHostMalloc failed: %s", hipGetErrorString(e));
-    return PFM_ERR_HIP;
+    return st;
   }
   *out = h;
   return PFM_OK;
@@ -642,13 +658,73 @@ int pfm_dilu_factor(pfm_ldu h, int transpose, const double *diag, const double *
   return PFM_OK;
 }
 
-int pfm_laplacian_assemble(pfm_ldu h, const double *gamma, const double *mag_sf,
-                           const double *delta, const double *vol, int32_t n_dir,
-                           const int32_t *dir_cell, const double *dir_gamma,
-                           const double *dir_mag_sf, const double *dir_delta, double *diag,
-                           double *lower, double *upper, void *stream) {
+int pfm_dirichlet_create(pfm_ldu h, int32_t n_dir, const int32_t *dir_cell_h, int32_t n_patches,
+                         const int32_t *patch_ptr_h, pfm_dirichlet *out) {
+  PFM_REQUIRE(out, "null output handle");
+  *out = nullptr;
   PFM_REQUIRE(h, "null handle");
-  PFM_REQUIRE(n_dir >= 0, "negative n_dir");
+  PFM_REQUIRE(n_dir >= 0 && n_patches >= 0, "negative size");
+  PFM_REQUIRE(n_dir == 0 || (dir_cell_h && patch_ptr_h && n_patches >= 1),
+              "Dirichlet faces need cells and patch offsets");
+  std::vector<int32_t> patch(n_dir);
+  if (n_dir) {
+    PFM_REQUIRE(patch_ptr_h[0] == 0 && patch_ptr_h[n_patches] == n_dir,
+                "patch offsets must run from 0 to n_dir");
+    for (int32_t p = 0; p < n_patches; ++p) {
+      PFM_REQUIRE(patch_ptr_h[p] <= patch_ptr_h[p + 1], "patch offsets must not decrease");
+      for (int32_t j = patch_ptr_h[p]; j < patch_ptr_h[p + 1]; ++j) patch[j] = p;
+    }
+    for (int32_t j = 0; j < n_dir; ++j)
+      PFM_REQUIRE(dir_cell_h[j] >= 0 && dir_cell_h[j] < h->n,
+                  "Dirichlet cell %d of entry %d outside [0, %d)", dir_cell_h[j], j, h->n);
+  }
+  // entries grouped by cell; within a cell in the caller's (patch, face) order
+  std::vector<int32_t> order(n_dir);
+  for (int32_t j = 0; j < n_dir; ++j) order[j] = j;
+  std::stable_sort(order.begin(), order.end(),
+                   [&](int32_t a, int32_t b) { return dir_cell_h[a] < dir_cell_h[b]; });
+  std::vector<int32_t> gptr, gcell, spatch(n_dir);
+  for (int32_t j = 0; j < n_dir; ++j) {
+    const int32_t c = dir_cell_h[order[j]];
+    if (j == 0 || c != gcell.back()) {
+      gptr.push_back(j);
+      gcell.push_back(c);
+    }
+    spatch[j] = patch[order[j]];
+  }
+  gptr.push_back(n_dir);
+  pfm_dirichlet d = new pfm_dirichlet_s();
+  d->n_dir = n_dir;
+  d->n_groups = (int32_t)gcell.size();
+  d->ldu = h;
+  int st;
+  if ((st = upload(&d->d_gptr, gptr)) || (st = upload(&d->d_gcell, gcell)) ||
+      (st = upload(&d->d_order, order)) || (st = upload(&d->d_patch, spatch))) {
+    pfm_dirichlet_destroy(d);
+    return st;
+  }
+  *out = d;
+  return PFM_OK;
+}
+
+int pfm_dirichlet_destroy(pfm_dirichlet d) {
+  if (!d) return PFM_OK;
+  (void)hipFree(d->d_gptr);
+  (void)hipFree(d->d_gcell);
+  (void)hipFree(d->d_order);
+  (void)hipFree(d->d_patch);
+  delete d;
+  return PFM_OK;
+}
+
+int pfm_laplacian_assemble(pfm_ldu h, const double *gamma, const double *mag_sf,
+                           const double *delta, const double *vol, pfm_dirichlet dir,
+                           const double *dir_gamma, const double *dir_mag_sf,
+                           const double *dir_delta, double *diag, double *lower, double *upper,
+                           void *stream) {
+  PFM_REQUIRE(h, "null handle");
+  PFM_REQUIRE(!dir || dir->ldu == h, "Dirichlet plan made for another handle");
+  const int32_t n_dir = dir ? dir->n_dir : 0;
   hipStream_t s = (hipStream_t)stream;
   if (h->n == 0) return PFM_OK;
   PFM_NONNULL(vol);
@@ -665,12 +741,12 @@ int pfm_laplacian_assemble(pfm_ldu h, const double *gamma, const double *mag_sf,
   k_lap_diag<<<blocks_for_slices(h->nslices), kBlock, 0, s>>>(h->n, h->nslices, h->d_slices,
                                                               h->d_map, lower, upper, diag);
   if (n_dir > 0) {
-    PFM_NONNULL(dir_cell);
     PFM_NONNULL(dir_gamma);
     PFM_NONNULL(dir_mag_sf);
     PFM_NONNULL(dir_delta);
-    k_lap_dirichlet<<<grid1d(n_dir), kBlock, 0, s>>>(n_dir, dir_cell, dir_gamma, dir_mag_sf,
-                                                     dir_delta, vol, diag);
+    k_lap_dirichlet<<<grid1d(dir->n_groups), kBlock, 0, s>>>(
+        dir->n_groups, dir->d_gptr, dir->d_gcell, dir->d_order, dir->d_patch, dir_gamma,
+        dir_mag_sf, dir_delta, vol, diag);
   }
   PFM_LAUNCH_CHECK();
   return PFM_OK;

@@ -362,6 +362,8 @@ int pfm_pcg_mixed_solve(pfm_ldu h, const pfm_solve_params *prm, double inner_rel
   int inner_total = 0, rc = PFM_OK;
   KState hs;
   IState is;
+  PollLease lease;
+  if ((rc = lease.acquire(&h->poll, true)) != PFM_OK) return rc;
   for (int outer = 0;; ++outer) {
     k_ir_resid<<<G, kBlock, 0, s>>>(n, ns, h->d_slices, h->d_col, packed, diag, b, x, w.r64,
                                     w.part);
@@ -374,10 +376,8 @@ int pfm_pcg_mixed_solve(pfm_ldu h, const pfm_solve_params *prm, double inner_rel
     // inner FP32 PCG
     k_ir_inner_init<<<G, kBlock, 0, s>>>(n, ns, w.r64, pd, w.r, w.p, w.e, w.part, w.st);
     k_ir_inner_init_fin<<<1, kBlock, 0, s>>>(w.part, G, w.st, w.in);
-    int32_t *flag = h->h_flag;
-    hipEvent_t ev[2];
-    PFM_CHECK_HIP(hipEventCreateWithFlags(&ev[0], hipEventDisableTiming));
-    PFM_CHECK_HIP(hipEventCreateWithFlags(&ev[1], hipEventDisableTiming));
+    int32_t *flag = lease.flag;
+    hipEvent_t *ev = lease.ev;
     int enq = 0, chunk = 4, slot = 0;
     bool pending = false;
     double last_report = now_seconds();
@@ -418,8 +418,6 @@ int pfm_pcg_mixed_solve(pfm_ldu h, const pfm_solve_params *prm, double inner_rel
       pending = true;
       slot ^= 1;
     }
-    (void)hipEventDestroy(ev[0]);
-    (void)hipEventDestroy(ev[1]);
     if (rc != PFM_OK) {
       set_error("HIP error while driving the inner loop");
       return rc;

@@ -3,6 +3,7 @@
 
 #include <hip/hip_runtime.h>
 
+#include <atomic>
 #include <cstdint>
 #include <vector>
 
@@ -78,9 +79,51 @@ int build_host_layout(int32_t n, int32_t nf, const int32_t *own,
 int validate_topology(int32_t n, int32_t nf, const int32_t *own,
                       const int32_t *nei);
 
+// Pinned host words through which a solve polls its device stop flag, with
+// the two events that order each read.  A handle keeps kPollSlots pairs and
+// every solve in flight on it leases a pair of its own, so solves that run
+// concurrently on one handle (threads sharing a topology) never read each
+// other's flags.
+constexpr int kPollSlots = 8;
+struct PollPool {
+  int32_t *flags = nullptr;  // pinned, two words per slot
+  hipEvent_t ev[2 * kPollSlots] = {};
+  std::atomic<uint32_t> busy{0};
+};
+int poll_pool_init(PollPool &p);  // allocates; called at handle creation
+void poll_pool_free(PollPool &p);
+
+class PollLease {
+ public:
+  PollLease() = default;
+  PollLease(const PollLease &) = delete;
+  PollLease &operator=(const PollLease &) = delete;
+  // Waits for the last flag copy to land, then returns the pair.
+  ~PollLease();
+  // A free pair of `pool`.  With every pair leased: a pair allocated for this
+  // solve when may_allocate, else PFM_ERR_STATE.
+  int acquire(PollPool *pool, bool may_allocate);
+  int32_t *flag = nullptr;
+  hipEvent_t ev[2] = {};
+
+ private:
+  PollPool *pool_ = nullptr;
+  int slot_ = -1;
+};
+
 }  // namespace pfm
 
 struct pfm_batch_s;
+
+// Dirichlet faces grouped by cell (pfm_dirichlet_create).
+struct pfm_dirichlet_s {
+  int32_t n_dir = 0, n_groups = 0;
+  const struct pfm_ldu_s *ldu = nullptr;
+  int32_t *d_gptr = nullptr;   // group g: sorted entries gptr[g] .. gptr[g+1]-1
+  int32_t *d_gcell = nullptr;  // cell of group g
+  int32_t *d_order = nullptr;  // caller's entry index of each sorted entry
+  int32_t *d_patch = nullptr;  // patch of each sorted entry
+};
 
 // Device-resident addressing of one LDU topology.
 struct pfm_ldu_s {
@@ -104,7 +147,7 @@ struct pfm_ldu_s {
   std::vector<int32_t> gs_fwd_ptr, gs_bwd_ptr;  // host copies of GS level offsets
   int32_t *d_gs_fwd_ptr = nullptr, *d_gs_fwd_cells = nullptr;
   int32_t *d_gs_bwd_ptr = nullptr, *d_gs_bwd_cells = nullptr;
-  int32_t *h_flag = nullptr;            // pinned: solver progress polling
+  pfm::PollPool poll;                   // pinned stop-flag slots of the solves
   bool oriented_sorted = false;         // every owner < neighbour, owners non-decreasing
   pfm_progress_fn progress = nullptr;   // optional host progress report
   void *progress_user = nullptr;
@@ -112,6 +155,13 @@ struct pfm_ldu_s {
 };
 
 namespace pfm {
+// The body of pfm_pcg_solve: `work` holds pfm_pcg_workspace_bytes(h,
+// prm->max_iter) bytes of device memory; may_allocate = false restricts the
+// host loop to the handle's pooled poll slots (no allocation at all).
+int pcg_solve(pfm_ldu h, const pfm_solve_params *prm, const double *diag, const double *packed,
+              const double *rD, const double *b, double *x, void *work, double *history_h,
+              pfm_solve_result *res, void *stream, bool may_allocate);
+
 // Calls the handle's progress report when its interval has passed since `last`.
 void report_progress(const pfm_ldu_s *h, int32_t iterations, double &last);
 double now_seconds();

@@ -169,7 +169,7 @@ extern "C" {
 
 const char *pfm_last_error(void) { return g_err; }
 
-int pfm_abi_version(void) { return 1; }
+int pfm_abi_version(void) { return 2; }
 
 int pfm_rcm(int32_t n, int32_t nf, const int32_t *own, const int32_t *nei,
             int32_t *perm_out) {

@@ -463,11 +463,11 @@ int pfm_smooth_solve(pfm_ldu h, const pfm_solve_params *prm, int smoother, int32
   PFM_CHECK_HIP(hipGetLastError());
   const int nfl = (int)h->gs_fwd_ptr.size() - 1;
   const bool wg = h->n < 2048 * std::max(nfl, 1);
-  int rc = PFM_OK;
-  int32_t *flag = h->h_flag;
-  hipEvent_t ev[2];
-  PFM_CHECK_HIP(hipEventCreateWithFlags(&ev[0], hipEventDisableTiming));
-  PFM_CHECK_HIP(hipEventCreateWithFlags(&ev[1], hipEventDisableTiming));
+  PollLease lease;
+  int rc = lease.acquire(&h->poll, true);
+  if (rc != PFM_OK) return rc;
+  int32_t *flag = lease.flag;
+  hipEvent_t *ev = lease.ev;
   int enq = 0, chunk = 2, slot = 0;
   bool pending = false;
   double last_report = now_seconds();
@@ -528,8 +528,6 @@ int pfm_smooth_solve(pfm_ldu h, const pfm_solve_params *prm, int smoother, int32
     pending = true;
     slot ^= 1;
   }
-  (void)hipEventDestroy(ev[0]);
-  (void)hipEventDestroy(ev[1]);
   if (rc != PFM_OK) return rc;
   KState hs;
   PFM_CHECK_HIP(hipMemcpyAsync(&hs, w.st, sizeof(KState), hipMemcpyDeviceToHost, s));

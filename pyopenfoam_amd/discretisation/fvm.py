@@ -37,10 +37,17 @@ def _ptr(t):
 
 
 class LaplacianAssemble(torch.autograd.Function):
-    """(gamma_internal, gamma_dirichlet) -> (diag, lower, upper)."""
+    """(gamma_internal, gamma_dirichlet) -> (diag, lower, upper).
+
+    The Dirichlet arrays (``dir_cells`` int32 on the device, ``dir_gamma``,
+    ``dir_mag``, ``dir_delta``) hold one entry per face in the reference's
+    order: patch after patch (``patch_sizes`` faces each; default one patch),
+    faces in face order.
+    """
 
     @staticmethod
-    def forward(ctx, gamma, dir_gamma, addr, mag, delta, vol, dir_cells, dir_mag, dir_delta):
+    def forward(ctx, gamma, dir_gamma, addr, mag, delta, vol, dir_cells, dir_mag, dir_delta,
+                patch_sizes=None):
         dev = gamma.device
         n, nf = addr.n_cells, addr.n_faces
         diag = torch.empty(n, device=dev, dtype=torch.float64)
@@ -49,9 +56,10 @@ class LaplacianAssemble(torch.autograd.Function):
         nd = 0 if dir_cells is None else int(dir_cells.numel())
         g = gamma.detach().contiguous()
         dg = dir_gamma.detach().contiguous() if nd else None
+        plan = addr.dirichlet_plan(dir_cells, patch_sizes) if nd else None
         _lib.check(_lib.lib().pfm_laplacian_assemble(
-            addr.handle, _ptr(g), _ptr(mag), _ptr(delta), _ptr(vol), nd,
-            _ptr(dir_cells) if nd else None, _ptr(dg) if nd else None,
+            addr.handle, _ptr(g), _ptr(mag), _ptr(delta), _ptr(vol),
+            plan.handle if nd else None, _ptr(dg) if nd else None,
             _ptr(dir_mag) if nd else None, _ptr(dir_delta) if nd else None,
             _ptr(diag), _ptr(lower), _ptr(upper), current_stream(dev)), "pfm_laplacian_assemble")
         ctx.addr = addr
@@ -81,7 +89,7 @@ class LaplacianAssemble(torch.autograd.Function):
             _ptr(saved[3]) if nd else None, _ptr(saved[4]) if nd else None,
             _ptr(saved[5]) if nd else None, _ptr(gg), _ptr(gdir), current_stream(dev)),
             "pfm_laplacian_backward")
-        return gg, gdir, None, None, None, None, None, None, None
+        return gg, gdir, None, None, None, None, None, None, None, None
 
 
 def _face_D(D, mesh) -> torch.Tensor:
@@ -129,7 +137,7 @@ def laplacian(D, phi=None, scheme: str = "Gauss linear corrected", *, mesh=None,
     delta = mesh.delta_coefficients
     vol = mesh.cell_volumes
     g_int = Df[:nfi] if Df.shape[0] != nfi else Df
-    dir_cells = dir_mag = dir_delta = None
+    dir_cells = dir_mag = dir_delta = sizes = None
     g_dir = torch.zeros(0, dtype=torch.float64, device=dev)
     src_terms = []
     if dirichlet:
@@ -143,13 +151,14 @@ def laplacian(D, phi=None, scheme: str = "Gauss linear corrected", *, mesh=None,
                 torch.ones(len(faces), dtype=torch.float64, device=dev) * (Df[0] if Df.numel() else 1.0)
             g_l.append(gf)
             src_terms.append((cells, m_b, d_b, gf, float(value)))
-        dir_cells = torch.cat(cells_l)
-        order = torch.argsort(dir_cells, stable=True)
-        dir_cells, dir_mag, dir_delta = dir_cells[order].contiguous(), torch.cat(mag_l)[order].contiguous(), torch.cat(dlt_l)[order].contiguous()
-        g_dir = torch.cat(g_l)[order]
+        # the reference's order: patch after patch (laplacian_foam.py:318-366)
+        sizes = tuple(int(c.numel()) for c in cells_l)
+        dir_cells = torch.cat(cells_l).to(torch.int32).contiguous()
+        dir_mag, dir_delta = torch.cat(mag_l).contiguous(), torch.cat(dlt_l).contiguous()
+        g_dir = torch.cat(g_l)
     diag, lower, upper = LaplacianAssemble.apply(g_int.contiguous(), g_dir.contiguous(), addr,
                                                  mag[:nfi].contiguous(), delta[:nfi].contiguous(),
-                                                 vol, dir_cells, dir_mag, dir_delta)
+                                                 vol, dir_cells, dir_mag, dir_delta, sizes)
     mat._diag, mat._lower, mat._upper = diag, lower, upper
     for cells, m_b, d_b, gf, value in src_terms:
         if value != 0.0:

@@ -114,9 +114,9 @@ class FvMesh:
             nf, nfi = self.n_faces, self.n_internal_faces
             w = torch.empty(nf, dtype=CFD_DTYPE, device=self._device)
             own, nei = self.owner_i32, self.neighbour_i32
+            fc, cc = self.face_centres.contiguous(), self.cell_centres.contiguous()
             _lib.check(_lib.lib().pfm_face_weights(
-                nf, nfi, _lib.ptr(own), _lib.ptr(nei), _lib.ptr(self.face_centres.contiguous()),
-                _lib.ptr(self.cell_centres.contiguous()), _lib.ptr(w),
+                nf, nfi, _lib.ptr(own), _lib.ptr(nei), _lib.ptr(fc), _lib.ptr(cc), _lib.ptr(w),
                 current_stream(self._device)), "pfm_face_weights")
             self._t["w"] = w
         return self._t["w"]
@@ -158,13 +158,12 @@ class FvMesh:
         return np.arange(p["start"], p["start"] + p["size"])
 
     def dirichlet_data(self, patch: str):
-        """(cells, |S_b|, delta_b) of a patch, sorted by owner cell, on device."""
+        """(faces, cells, |S_b|, delta_b) of a patch in face order; cells, |S_b|
+        and delta_b on the device."""
         key = ("dir", patch)
         if key not in self._t:
             faces = self.patch_faces(patch)
             cells = self._mesh.owner[faces]
-            order = np.argsort(cells, kind="stable")
-            faces, cells = faces[order], cells[order]
             geo = self.geometry
             self._t[key] = (
                 faces,

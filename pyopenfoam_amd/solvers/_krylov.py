@@ -57,7 +57,8 @@ def device_solve(kind: str, matrix: LduMatrix, source: torch.Tensor, x0: torch.T
     res = _lib.SolveResult()
     cap = 2 * max_iter + 2
     hist = np.zeros(cap, dtype=np.float64)
-    _lib.check(getattr(L, fn_name)(addr.handle, C.byref(prm), _lib.ptr(matrix.diag.detach().contiguous()),
+    d = matrix.diag.detach().contiguous()
+    _lib.check(getattr(L, fn_name)(addr.handle, C.byref(prm), _lib.ptr(d),
                                    _lib.ptr(pk), _lib.ptr(rD), _lib.ptr(b), _lib.ptr(x),
                                    _lib.ptr(work), hist.ctypes.data, C.byref(res),
                                    current_stream(dev)), fn_name)
@@ -89,8 +90,9 @@ def device_solve_mixed(matrix: LduMatrix, source: torch.Tensor, x0: torch.Tensor
     res = _lib.SolveResult()
     hist = np.zeros(int(max_outer) + 2)
     inner = C.c_int32()
+    d = matrix.diag.detach().contiguous()
     _lib.check(L.pfm_pcg_mixed_solve(addr.handle, C.byref(prm), float(inner_rel_tol),
-                                     int(max_outer), _lib.ptr(matrix.diag.detach().contiguous()),
+                                     int(max_outer), _lib.ptr(d),
                                      _lib.ptr(pk), _lib.ptr(pk32), _lib.ptr(rD), _lib.ptr(b),
                                      _lib.ptr(x), _lib.ptr(work), hist.ctypes.data, C.byref(res),
                                      C.byref(inner), current_stream(dev)), "pfm_pcg_mixed_solve")

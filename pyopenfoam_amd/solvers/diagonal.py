@@ -33,8 +33,8 @@ class DiagonalSolver(LinearSolverBase):
         work = torch.empty(matrix.n_cells // 256 + 3, device=dev, dtype=torch.float64)
         res = C.c_double()
         if matrix.n_cells:
-            _lib.check(_lib.lib().pfm_diagonal_solve(matrix.addressing.handle,
-                                                     _lib.ptr(matrix.diag.detach().contiguous()),
+            d = matrix.diag.detach().contiguous()
+            _lib.check(_lib.lib().pfm_diagonal_solve(matrix.addressing.handle, _lib.ptr(d),
                                                      _lib.ptr(b), _lib.ptr(x), _lib.ptr(work),
                                                      C.byref(res), current_stream(dev)),
                        "pfm_diagonal_solve")

@@ -173,10 +173,9 @@ class GAMGSolver(LinearSolverBase):
             return self._solve_mixed(matrix, hier, prm, b, x, monitor)
         work = self._ws.get(int(L.pfm_gamg_workspace_bytes(hier.handle, max_iter)), dev)
         hist = np.zeros(max_iter + 1)
+        d, lo, up = matrix.coefficients()
         _lib.check(L.pfm_gamg_solve(hier.handle, C.byref(prm),
-                                    _lib.ptr(matrix.diag.detach().contiguous()),
-                                    _lib.ptr(matrix.lower.detach().contiguous()),
-                                    _lib.ptr(matrix.upper.detach().contiguous()), _lib.ptr(b),
+                                    _lib.ptr(d), _lib.ptr(lo), _lib.ptr(up), _lib.ptr(b),
                                     _lib.ptr(x), _lib.ptr(work), hist.ctypes.data, C.byref(res),
                                     current_stream(dev)), "pfm_gamg_solve")
         monitor.load_history(hist[: min(res.iterations, max_iter + 1)].tolist())
@@ -193,11 +192,10 @@ class GAMGSolver(LinearSolverBase):
         res = _lib.SolveResult()
         hist = np.zeros(mo + 1)
         inner = C.c_int32()
+        d, lo, up = matrix.coefficients()
         _lib.check(L.pfm_gamg_mixed_solve(hier.handle, C.byref(prm), self._inner_rel_tol, mo,
                                           self._fp32_levels,
-                                          _lib.ptr(matrix.diag.detach().contiguous()),
-                                          _lib.ptr(matrix.lower.detach().contiguous()),
-                                          _lib.ptr(matrix.upper.detach().contiguous()),
+                                          _lib.ptr(d), _lib.ptr(lo), _lib.ptr(up),
                                           _lib.ptr(b), _lib.ptr(x), _lib.ptr(work),
                                           hist.ctypes.data, C.byref(res), C.byref(inner),
                                           current_stream(dev)), "pfm_gamg_mixed_solve")
@@ -220,10 +218,9 @@ class GAMGSolver(LinearSolverBase):
                                int(monitor.min_iter), 0, 0)
         res = _lib.SolveResult()
         hist = np.zeros(max_iter + 1)
+        d, lo, up = matrix.coefficients()
         _lib.check(L.pfm_gamg_pcg_solve(hier.handle, C.byref(prm), int(fp32_levels),
-                                        _lib.ptr(matrix.diag.detach().contiguous()),
-                                        _lib.ptr(matrix.lower.detach().contiguous()),
-                                        _lib.ptr(matrix.upper.detach().contiguous()), _lib.ptr(b),
+                                        _lib.ptr(d), _lib.ptr(lo), _lib.ptr(up), _lib.ptr(b),
                                         _lib.ptr(x), _lib.ptr(work), hist.ctypes.data,
                                         C.byref(res), current_stream(dev)), "pfm_gamg_pcg_solve")
         monitor.load_history(hist[: min(res.iterations, max_iter + 1)].tolist())
@@ -238,9 +235,8 @@ class GAMGSolver(LinearSolverBase):
         L = _lib.lib()
         work = self._ws.get(int(L.pfm_gamg_workspace_bytes(hier.handle, 0)), dev)
         s = current_stream(dev)
-        _lib.check(L.pfm_gamg_setup_values(hier.handle, _lib.ptr(matrix.diag.detach().contiguous()),
-                                           _lib.ptr(matrix.lower.detach().contiguous()),
-                                           _lib.ptr(matrix.upper.detach().contiguous()),
+        dd, ll, uu = matrix.coefficients()
+        _lib.check(L.pfm_gamg_setup_values(hier.handle, _lib.ptr(dd), _lib.ptr(ll), _lib.ptr(uu),
                                            _lib.ptr(work), s), "pfm_gamg_setup_values")
         n, nf = hier.level_sizes()[level]
         d = torch.empty(n, dtype=torch.float64, device=dev)

@@ -62,9 +62,9 @@ class PBiCGSolver(LinearSolverBase):
         res = _lib.SolveResult()
         cap = 2 * max_iter + 2
         hist = np.zeros(cap)
-        _lib.check(L.pfm_pbicg_solve(addr.handle, C.byref(prm),
-                                     _lib.ptr(matrix.diag.detach().contiguous()),
-                                     _lib.ptr(matrix.packed(False)), _lib.ptr(matrix.packed(True)),
+        d, pk, pkt = matrix.diag.detach().contiguous(), matrix.packed(False), matrix.packed(True)
+        _lib.check(L.pfm_pbicg_solve(addr.handle, C.byref(prm), _lib.ptr(d),
+                                     _lib.ptr(pk), _lib.ptr(pkt),
                                      _lib.ptr(rD), _lib.ptr(b), _lib.ptr(x), _lib.ptr(work),
                                      hist.ctypes.data, C.byref(res), current_stream(dev)),
                    "pfm_pbicg_solve")

@@ -82,8 +82,8 @@ class DICPreconditioner(_DiagonalFactor):
         if m.n_cells == 0:
             return rd
         pk = m.packed(self._transpose)
-        _lib.check(_lib.lib().pfm_dic_factor(m.addressing.handle,
-                                             _lib.ptr(m.diag.detach().contiguous()), _lib.ptr(pk),
+        d = m.diag.detach().contiguous()
+        _lib.check(_lib.lib().pfm_dic_factor(m.addressing.handle, _lib.ptr(d), _lib.ptr(pk),
                                              _lib.ptr(rd), current_stream(self._device)),
                    "pfm_dic_factor")
         return rd
@@ -97,10 +97,9 @@ class DILUPreconditioner(_DiagonalFactor):
         rd = torch.empty(m.n_cells, device=self._device, dtype=torch.float64)
         if m.n_cells == 0:
             return rd
+        d, lo, up = m.coefficients()
         _lib.check(_lib.lib().pfm_dilu_factor(m.addressing.handle, int(self._transpose),
-                                              _lib.ptr(m.diag.detach().contiguous()),
-                                              _lib.ptr(m.lower.detach().contiguous()),
-                                              _lib.ptr(m.upper.detach().contiguous()),
+                                              _lib.ptr(d), _lib.ptr(lo), _lib.ptr(up),
                                               _lib.ptr(rd), current_stream(self._device)),
                    "pfm_dilu_factor")
         return rd

@@ -69,11 +69,11 @@ class SmoothSolver(LinearSolverBase):
                                int(monitor.min_iter), 0, 0)
         res = _lib.SolveResult()
         hist = np.zeros(max_iter + 1)
+        d, lo, up = matrix.coefficients()
+        pk = matrix.packed(False)
         _lib.check(L.pfm_smooth_solve(addr.handle, C.byref(prm), kind, self._n_sweeps, self._omega,
-                                      _lib.ptr(matrix.diag.detach().contiguous()),
-                                      _lib.ptr(matrix.lower.detach().contiguous()),
-                                      _lib.ptr(matrix.upper.detach().contiguous()),
-                                      _lib.ptr(matrix.packed(False)), _lib.ptr(rD), _lib.ptr(b),
+                                      _lib.ptr(d), _lib.ptr(lo), _lib.ptr(up),
+                                      _lib.ptr(pk), _lib.ptr(rD), _lib.ptr(b),
                                       _lib.ptr(x), _lib.ptr(work), hist.ctypes.data, C.byref(res),
                                       current_stream(dev)), "pfm_smooth_solve")
         monitor.load_history(hist[: min(res.iterations, max_iter + 1)].tolist())

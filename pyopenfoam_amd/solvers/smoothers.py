@@ -60,9 +60,9 @@ class GaussSeidelSmoother(_DeviceSmoother):
         m = self._matrix
         if n_iterations <= 0 or m.n_cells == 0:
             return xx
-        _lib.check(_lib.lib().pfm_gs_smooth(m.addressing.handle,
-                                            _lib.ptr(m.diag.detach().contiguous()),
-                                            _lib.ptr(m.packed(False)), _lib.ptr(bb), _lib.ptr(xx),
+        d, pk = m.diag.detach().contiguous(), m.packed(False)
+        _lib.check(_lib.lib().pfm_gs_smooth(m.addressing.handle, _lib.ptr(d),
+                                            _lib.ptr(pk), _lib.ptr(bb), _lib.ptr(xx),
                                             int(n_iterations), self._omega,
                                             current_stream(self._device)), "pfm_gs_smooth")
         return xx
@@ -78,9 +78,9 @@ class JacobiSmoother(_DeviceSmoother):
         if n_iterations <= 0 or m.n_cells == 0:
             return xx
         work = torch.empty_like(xx)
-        _lib.check(_lib.lib().pfm_jacobi_smooth(m.addressing.handle,
-                                                _lib.ptr(m.diag.detach().contiguous()),
-                                                _lib.ptr(m.packed(False)), _lib.ptr(bb),
+        d, pk = m.diag.detach().contiguous(), m.packed(False)
+        _lib.check(_lib.lib().pfm_jacobi_smooth(m.addressing.handle, _lib.ptr(d),
+                                                _lib.ptr(pk), _lib.ptr(bb),
                                                 _lib.ptr(xx), int(n_iterations), self._omega,
                                                 _lib.ptr(work), current_stream(self._device)),
                    "pfm_jacobi_smooth")

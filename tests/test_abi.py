@@ -31,7 +31,7 @@ def test_library_exports_every_declared_symbol():
     for name in names:
         assert hasattr(L, name), name
     assert set(names) == set(_lib.EXPORTED_SYMBOLS)
-    assert L.pfm_abi_version() == 1
+    assert L.pfm_abi_version() == 2
 
 
 def test_error_reporting():

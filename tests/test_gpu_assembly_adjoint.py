@@ -144,3 +144,49 @@ def test_laplacian_backward_chain(gpu_device):
                                 c.dir_delta)
     np.testing.assert_allclose(gam.grad.cpu().numpy(), ref, rtol=1e-13, atol=1e-300)
     np.testing.assert_allclose(gamb.grad.cpu().numpy(), refb, rtol=1e-13)
+
+
+def test_dirichlet_patches_in_reference_order(gpu_device):
+    """Several fixedValue patches add to the diagonal one after another,
+    ((diag + s_x0) + s_y0) + s_z0 (laplacian_foam.py:318-366), where s_p is the
+    patch's own scatter_add; the cells on the cube's edges and corner carry
+    faces of two and three patches.  The same faces given as one patch in a
+    shuffled order are summed in that order first and then added once.  Both
+    bit-exact against the oracle; the two conventions differ on some cells,
+    so a kernel that merged patches (or ignored the order) fails one of them."""
+    from pyopenfoam_amd.mesh.geometry import dirichlet_delta
+    c = case(12, 0.2, 1.0, seed=2)
+    dev = gpu_device
+    nfi = len(c.neighbour)
+    rng = np.random.default_rng(11)
+    patches = []
+    for name in ("x0", "y0", "z0"):
+        p = c.mesh.patch(name)
+        faces = np.arange(p["start"], p["start"] + p["size"])
+        patches.append((rng.lognormal(0.0, 1.0, faces.shape[0]), c.geo.mag_sf[faces].copy(),
+                        dirichlet_delta(c.mesh, c.geo, faces), c.mesh.owner[faces].copy()))
+    m = LduMatrix(c.n_cells, torch.from_numpy(c.owner), torch.from_numpy(c.neighbour), device=dev)
+    addr = m.addressing
+    t = lambda a: torch.from_numpy(np.ascontiguousarray(a)).to(dev)
+    d0, lo0, up0 = oldu.laplacian(c.gamma, c.geo.mag_sf[:nfi], c.geo.delta[:nfi],
+                                  c.geo.cell_volumes, c.owner, c.neighbour)
+    cat = lambda k, order=None: np.concatenate([p[k] for p in patches])[
+        slice(None) if order is None else order]
+
+    def gpu(order, sizes):
+        cells = t(cat(3, order).astype(np.int32))
+        d, lo, up = LaplacianAssemble.apply(t(c.gamma), t(cat(0, order)), addr, t(c.geo.mag_sf[:nfi]),
+                                            t(c.geo.delta[:nfi]), t(c.geo.cell_volumes), cells,
+                                            t(cat(1, order)), t(cat(2, order)), sizes)
+        assert bits_equal(lo.cpu().numpy(), lo0) and bits_equal(up.cpu().numpy(), up0)
+        return d.cpu().numpy()
+
+    ref3 = oldu.dirichlet_diag_patches(d0, patches, c.geo.cell_volumes)
+    got3 = gpu(None, tuple(len(p[3]) for p in patches))
+    assert bits_equal(got3, ref3)
+    shuf = rng.permutation(sum(len(p[3]) for p in patches))
+    ref1 = oldu.dirichlet_diag(d0, cat(0, shuf), cat(1, shuf), cat(2, shuf), c.geo.cell_volumes,
+                               cat(3, shuf))
+    got1 = gpu(shuf, None)
+    assert bits_equal(got1, ref1)
+    assert not bits_equal(ref1, ref3)  # the conventions are distinguishable here

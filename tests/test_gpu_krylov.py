@@ -122,6 +122,32 @@ def test_pcg_breakdown_after_exact_solve(gpu_device, case, pre):
         np.testing.assert_array_equal(xo, np.array(b) / np.array(d))
 
 
+def test_pcg_breakdown_spans_every_workgroup(gpu_device):
+    """The diag3 breakdown above at 2^21 decoupled cells (d in {2, 4, 8}, so
+    z = r / d, A z = r and alpha = 1 are exact): r = 0 after iteration 1 and
+    p.Ap = 0 at iteration 2, an even iteration, whose kernel applies the x
+    update deferred at iteration 1 in every one of its 2048 workgroups before
+    the solve ends.  The breakdown must not stop a workgroup that starts after
+    another has seen it (pcg.py:150-156 returns the fully updated x): x and
+    the iteration count bit-identical to the oracle."""
+    n = 1 << 21
+    rng = np.random.default_rng(21)
+    d = np.array([2.0, 4.0, 8.0])[rng.integers(0, 3, n)]
+    b = rng.uniform(-1.0, 1.0, n)
+    e = np.zeros(0, np.int64)
+    m = LduMatrix(n, torch.from_numpy(e), torch.from_numpy(e), device=gpu_device)
+    m.diag = torch.from_numpy(d)
+    solver = PCGSolver(tolerance=0.0, rel_tol=0.0, max_iter=10, preconditioner="DIC")
+    x, it, _ = solver(m, torch.from_numpy(b), torch.zeros(n, dtype=torch.float64))
+    xo, io, _, info = okry.pcg(lambda v: d * v, lambda r: r / d, b, np.zeros(n), tolerance=0.0,
+                               rel_tol=0.0, max_iter=10, dot=okry.dot_device(n))
+    assert io == 2 and not info.converged
+    assert (it, solver.last_info.converged) == (io, info.converged), (it, io)
+    assert solver.last_info.status == 3  # PFM_SOLVE_BREAKDOWN
+    assert bits_equal(x.cpu().numpy(), xo)
+    assert bits_equal(xo, b / d)
+
+
 def test_pcg_known_answer_and_semantics(gpu_device):
     # 3-cell SPD chain, reference tests/unit/solvers/conftest.py:10-35 builder
     own, nei = torch.tensor([0, 1]), torch.tensor([1, 2])
