@@ -247,6 +247,25 @@ int pfm_pcg_solve(pfm_ldu h, const pfm_solve_params *prm, const double *diag,
                   const double *packed, const double *rD, const double *b,
                   double *x, void *work, double *history_h,
                   pfm_solve_result *res, void *stream);
+/* pfm_pcg_solve without a caller workspace and without any allocation.  No
+ * counterpart in the reference: for callers that cannot allocate device
+ * memory (code under HIP graph capture, tight serving loops, harnesses that
+ * forbid hipMalloc).  The workspace lives in a file-scope device array of
+ * pfm_pcg_noalloc_max_bytes() bytes (68 MiB: 2^21 cells with max_iter up to
+ * 250,000) that each GPU the library runs on holds until the process ends;
+ * a solve needing more (pfm_pcg_workspace_bytes) is rejected with
+ * PFM_ERR_ARG before anything is enqueued.  The handle's device must be the
+ * current device.  Calls on one GPU share that array: calls that overlap in
+ * time on one GPU race, and each call starts by writing every word it later
+ * reads (nothing is cleared).  The host loop polls through the handle's
+ * pooled pinned slots; with all of them leased by concurrent solves it
+ * returns PFM_ERR_STATE instead of allocating. */
+int64_t pfm_pcg_noalloc_max_bytes(void);
+int pfm_pcg_solve_noalloc(pfm_ldu h, const pfm_solve_params *prm,
+                          const double *diag, const double *packed,
+                          const double *rD, const double *b, double *x,
+                          double *history_h, pfm_solve_result *res,
+                          void *stream);
 /* PBiCGStab (solvers/pbicgstab.py:103-239), same conventions. */
 int pfm_pbicgstab_solve(pfm_ldu h, const pfm_solve_params *prm,
                         const double *diag, const double *packed,

@@ -109,6 +109,9 @@ _SIGS = {
     "pfm_pcg_solve": (INT, [P, C.POINTER(SolveParams), P, P, P, P, P, P, P, C.POINTER(SolveResult), P]),
     "pfm_pbicgstab_solve": (INT, [P, C.POINTER(SolveParams), P, P, P, P, P, P, P,
                                   C.POINTER(SolveResult), P]),
+    "pfm_pcg_noalloc_max_bytes": (I64, []),
+    "pfm_pcg_solve_noalloc": (INT, [P, C.POINTER(SolveParams), P, P, P, P, P, P,
+                                    C.POINTER(SolveResult), P]),
     "pfm_pbicg_workspace_bytes": (I64, [P, I32]),
     "pfm_pbicg_solve": (INT, [P, C.POINTER(SolveParams), P, P, P, P, P, P, P, P,
                               C.POINTER(SolveResult), P]),
