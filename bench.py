@@ -11,13 +11,25 @@ A step packs the LDU coefficients, factors DIC and runs the device PCG loop;
 the assembled system is resident in HBM before timing starts.
 
 Also measured in the same run:
-  * the SpMV kernel (pfm_spmv) alone, HIP events on the launch stream:
-    algorithmic bytes 24N + 24F per launch -> GB/s vs the 8 TB/s HBM peak;
-  * the adjoint: d(sum w*x)/d(gamma) through the fused adjoint solve +
-    face-gradient kernel, as a ratio to the forward solve time;
-  * the CPU baseline: the oracle's numpy restatement of the reference PCG on
-    the same system, a bounded number of iterations, scaled to the solve's
-    iteration count (plus the oracle's DIC factor).
+  * the SpMV kernel (pfm_spmv) alone, HIP events on the launch stream, over
+    four independent copies of the 128^3 system launched in turn (800 MB,
+    beyond the 256 MB MALL, so every launch streams from HBM): algorithmic
+    bytes 24N + 24F per launch -> GB/s vs the 8 TB/s HBM peak; beside it the
+    same kernel on one MALL-resident copy, a plain streaming read of the same
+    byte count (the achievable ceiling), and the HBM bytes per launch from two
+    rocprofv3 counter passes (FETCH_SIZE, WRITE_SIZE) over a short run of this
+    script (--roofline-probe), scaled by what the counters read for a copy of
+    known size in the same run;
+  * the adjoint: d(sum w*x)/d(gamma) with forward and adjoint solves to rel
+    1e-10, as a ratio to the forward solve time;
+  * the precision section: the 256^3 system (lognormal(0, 1.5) diffusivity)
+    solved by FP64 pair/Galerkin GAMG and by its FP32-in-FP64-refinement form
+    to rel 1e-8 (speed-up, true residuals, error against the FP64 solution),
+    and a bounded PCG-DIC run on it (the reference algorithm stagnates there);
+  * the CPU baseline: the reference's PCG-DIC op sequence in torch on the
+    host cores (torch's default threads; oracle.torch_ref), a bounded number
+    of iterations scaled to the device solve's iteration count, plus the DIC
+    factor of the oracle's C restatement.
 
 Prints one JSON line on stdout; progress goes to stderr.
 """
@@ -35,52 +47,236 @@ sys.path.insert(0, ROOT)
 
 METRIC = "ms per 1e-8 Poisson solve @2.1M cells, SpMV GB/s vs HBM peak, adjoint/fwd"
 HBM_PEAK_GBS = 8000.0
-# Bytes per k_spmv launch on the 128^3 system leaving L2, from rocprofv3 FETCH_SIZE and
-# WRITE_SIZE runs of this script (FETCH_SIZE scaled by 2.0 and WRITE_SIZE by 1.0,
-# the factors a 512 MiB device copy reads): 2 x 91,550.2 KiB + 16,384 KiB.
-SPMV_TRAFFIC_128 = 2 * 91550.2 * 1024 + 16384 * 1024
+ROOFLINE_COPIES = 4        # independent 128^3 systems the roofline launches rotate over
+CALIB_BYTES = 256 << 20    # the counter-calibration copy of the roofline probe
 
 
 def log(msg: str) -> None:
     print(f"[bench {time.strftime('%H:%M:%S')}] {msg}", file=sys.stderr, flush=True)
 
 
-def cpu_baseline(case, d, lo, up, gpu_iters: int, budget_s: float):
-    """Oracle PCG-DIC on the host: bounded sample, scaled to gpu_iters."""
-    import numpy as np
-
-    from oracle import krylov as okry
-    from oracle import ldu as oldu
-    from oracle import precond as opre
-
+def _cpu_model() -> str:
     try:
-        from threadpoolctl import threadpool_limits
-        limiter = threadpool_limits(limits=1)
-    except Exception:  # pragma: no cover
-        limiter = None
+        with open("/proc/cpuinfo") as f:
+            for line in f:
+                if line.startswith("model name"):
+                    return line.split(":", 1)[1].strip()
+    except OSError:
+        pass
+    return "unknown"
+
+
+def cpu_baseline(case, d, lo, up, gpu_iters: int, budget_s: float):
+    """The reference's PCG-DIC op sequence in torch on the host cores
+    (oracle.torch_ref: gather + scatter_add products, torch.dot, torch.norm,
+    r / d), torch's default intra-op threads; a bounded sample of iterations,
+    scaled to gpu_iters, plus the DIC factor (the oracle's C restatement of
+    the reference's sequential recurrence)."""
+    import numpy as np
+    import torch
+
+    from oracle import precond as opre
+    from oracle import torch_ref
+
     t0 = time.perf_counter()
     rD = opre.dic_build(d, up, case.owner, case.neighbour)
     t_fac = time.perf_counter() - t0
-    mv = lambda v: oldu.ax(d, lo, up, case.owner, case.neighbour, v)
-    # time one iteration first, then size the sample to the budget
+    A = torch_ref.LduCpu(d, lo, up, case.owner, case.neighbour)
+    x0 = np.zeros(case.n_cells)
+    # time a few iterations first, then size the sample to the budget
     t0 = time.perf_counter()
-    okry.pcg(mv, lambda r: r / rD, case.b, np.zeros(case.n_cells), tolerance=1e-300,
-             rel_tol=1e-300, max_iter=1)
-    t1 = time.perf_counter() - t0
-    k = int(max(2, min(gpu_iters, budget_s / max(t1, 1e-6))))
+    torch_ref.pcg(A, case.b, x0, rD, tolerance=1e-300, rel_tol=1e-300, max_iter=3)
+    t3 = (time.perf_counter() - t0) / 3
+    k = int(max(3, min(gpu_iters, budget_s / max(t3, 1e-6))))
     t0 = time.perf_counter()
-    okry.pcg(mv, lambda r: r / rD, case.b, np.zeros(case.n_cells), tolerance=1e-300,
-             rel_tol=1e-300, max_iter=k)
+    torch_ref.pcg(A, case.b, x0, rD, tolerance=1e-300, rel_tol=1e-300, max_iter=k)
     t_k = time.perf_counter() - t0
-    if limiter is not None:
-        limiter.unregister()
     per_it = t_k / k
     ms = (t_fac + per_it * gpu_iters) * 1e3
-    return {"value": round(ms, 1), "unit": "ms per solve", "cores": 1, "kind": "port",
-            "sample": f"oracle numpy PCG-DIC on the same 128^3 system: DIC factor (C "
-                      f"restatement, {t_fac * 1e3:.0f} ms) + {k} iterations ({t_k:.1f} s, "
-                      f"{per_it * 1e3:.1f} ms/iter) scaled to the GPU solve's {gpu_iters} "
-                      f"iterations; BLAS limited to 1 thread"}
+    threads = torch.get_num_threads()
+    return {"value": round(ms, 1), "unit": "ms per solve", "cores": threads, "kind": "port",
+            "cpu_count": os.cpu_count(), "torch_threads": threads, "cpu_model": _cpu_model(),
+            "sample": f"the reference's PCG-DIC op sequence in torch on the host (oracle.torch_ref, "
+                      f"{threads} torch threads, os.cpu_count() = {os.cpu_count()}) on the same "
+                      f"128^3 system: {k} iterations ({t_k:.1f} s, {per_it * 1e3:.1f} ms/iter) "
+                      f"scaled to the GPU solve's {gpu_iters} iterations, plus the DIC factor "
+                      f"(oracle C restatement, {t_fac * 1e3:.0f} ms)"}
+
+
+def _roofline_copies(case, mat, dev, k: int):
+    """k independent copies of the system (own topology handle, coefficients,
+    packed values, x, y): launched in turn, no copy stays MALL-resident."""
+    import numpy as np
+    import torch
+
+    from pyopenfoam_amd import LduMatrix
+
+    out = []
+    for i in range(k):
+        m = LduMatrix(case.n_cells, torch.from_numpy(case.owner.copy()),
+                      torch.from_numpy(case.neighbour.copy()), device=dev)
+        m.diag, m.lower, m.upper = mat.diag.clone(), mat.lower.clone(), mat.upper.clone()
+        x = torch.from_numpy(np.random.default_rng(100 + i).standard_normal(case.n_cells)).to(dev)
+        out.append((m, m.packed(False), x, torch.empty_like(x)))
+    return out
+
+
+def _spmv_launcher(cps, dev):
+    from pyopenfoam_amd import _lib
+    from pyopenfoam_amd.core.device import current_stream
+
+    L, s = _lib.lib(), current_stream(dev)
+    args = [(m.addressing.handle, _lib.ptr(m.diag), _lib.ptr(pk), _lib.ptr(x), _lib.ptr(y))
+            for (m, pk, x, y) in cps]
+
+    def launch(i):
+        h, d, p, x, y = args[i % len(args)]
+        L.pfm_spmv(h, d, p, x, y, s)
+    return launch
+
+
+def roofline_probe(args) -> int:
+    """--roofline-probe: the short run the counter passes profile: rotating
+    k_spmv launches over the roofline copies, then a device copy of
+    CALIB_BYTES (pfm_bandwidth_probe mode 1) whose bytes are known."""
+    import torch
+
+    from pyopenfoam_amd import _lib
+    from pyopenfoam_amd.cases import poisson_case
+    from pyopenfoam_amd.core.device import current_stream
+    from pyopenfoam_amd.discretisation.fvm import assemble
+    from pyopenfoam_amd.mesh.fv_mesh import FvMesh
+
+    dev = torch.device("cuda", 0)
+    torch.cuda.set_device(dev)
+    case = poisson_case(args.n, jitter=args.jitter, sigma=args.sigma, seed=0)
+    mat = assemble(case, FvMesh.from_case(case, dev))
+    cps = _roofline_copies(case, mat, dev, ROOFLINE_COPIES)
+    launch = _spmv_launcher(cps, dev)
+    for i in range(8 * ROOFLINE_COPIES):
+        launch(i)
+    a = torch.ones(CALIB_BYTES // 8, dtype=torch.float64, device=dev)
+    b = torch.empty_like(a)
+    L = _lib.lib()
+    for _ in range(4):
+        _lib.check(L.pfm_bandwidth_probe(1, _lib.ptr(a), _lib.ptr(b), CALIB_BYTES,
+                                         current_stream(dev)), "copy probe")
+    torch.cuda.synchronize()
+    print("roofline probe done", file=sys.stderr, flush=True)
+    return 0
+
+
+def measure_traffic(args, timeout_s: float = 300.0):
+    """HBM bytes per k_spmv launch: two rocprofv3 counter passes (FETCH_SIZE,
+    then WRITE_SIZE, KiB per dispatch) over --roofline-probe, each scaled by
+    what the counter reads for the calibration copy of CALIB_BYTES in the same
+    run.  Returns a dict, or None with the reason logged."""
+    import csv
+    import glob
+    import shutil
+    import subprocess
+    import tempfile
+
+    prof = shutil.which("rocprofv3")
+    if prof is None:
+        log("traffic: rocprofv3 not found")
+        return None
+    res = {"command": []}
+    for counter in ("FETCH_SIZE", "WRITE_SIZE"):
+        out = tempfile.mkdtemp(prefix=f"pfm_{counter.lower()}_", dir="/tmp")
+        cmd = [prof, "--pmc", counter, "-d", out, "-o", "pmc", "--output-format", "csv", "--",
+               sys.executable, os.path.abspath(__file__), "--roofline-probe", "--n", str(args.n),
+               "--jitter", str(args.jitter), "--sigma", str(args.sigma)]
+        res["command"].append(" ".join(cmd[:7] + ["--", "python", "bench.py"] + cmd[10:]))
+        env = dict(os.environ, TMPDIR="/tmp")
+        try:
+            r = subprocess.run(cmd, capture_output=True, text=True, timeout=timeout_s, env=env,
+                               cwd="/tmp")
+        except subprocess.TimeoutExpired:
+            log(f"traffic: rocprofv3 {counter} pass timed out")
+            return None
+        files = glob.glob(os.path.join(out, "**", "*counter_collection.csv"), recursive=True)
+        if not files:
+            log(f"traffic: rocprofv3 {counter} pass wrote no counters (rc {r.returncode}): "
+                f"{r.stderr[-400:]}")
+            return None
+        spmv, calib = [], []
+        for fn in files:
+            with open(fn) as f:
+                for row in csv.DictReader(f):
+                    if row.get("Counter_Name") != counter:
+                        continue
+                    name, v = row.get("Kernel_Name", ""), float(row.get("Counter_Value", "nan"))
+                    if "k_spmv<double, true>" in name:
+                        spmv.append(v)
+                    elif "k_bw_copy" in name:
+                        calib.append(v)
+        shutil.rmtree(out, ignore_errors=True)
+        if not spmv or not calib:
+            log(f"traffic: {counter} pass found {len(spmv)} k_spmv and {len(calib)} copy dispatches")
+            return None
+        kib = sum(spmv) / len(spmv)
+        scale = (CALIB_BYTES / 1024) / (sum(calib) / len(calib))
+        res[counter] = {"kib_per_dispatch_raw": round(kib, 1), "dispatches": len(spmv),
+                        "scale_from_calibration_copy": round(scale, 4),
+                        "bytes_per_launch": round(kib * 1024 * scale)}
+        log(f"traffic {counter}: {kib:.1f} KiB raw x {scale:.3f} per k_spmv launch "
+            f"({len(spmv)} dispatches)")
+    res["bytes_per_launch"] = res["FETCH_SIZE"]["bytes_per_launch"] + \
+        res["WRITE_SIZE"]["bytes_per_launch"]
+    return res
+
+
+def bench_roofline(case, mat, dev, args):
+    """k_spmv from HBM: HIP events on the launch stream around rotating
+    launches over ROOFLINE_COPIES independent copies; the same kernel on one
+    MALL-resident copy; a plain streaming read of the same bytes."""
+    import numpy as np
+    import torch
+
+    from pyopenfoam_amd import _lib
+    from pyopenfoam_amd.core.device import current_stream
+
+    N, F = case.n_cells, case.mesh.n_internal_faces
+    nbytes = 24 * N + 24 * F
+    cps = _roofline_copies(case, mat, dev, ROOFLINE_COPIES)
+    launch = _spmv_launcher(cps, dev)
+    ref = mat.Ax_packed(cps[0][2])
+    launch(0)
+    torch.cuda.synchronize()
+    assert torch.equal(cps[0][3], ref), "roofline copy differs from the solver's matrix"
+
+    def timed(fn, reps):
+        for i in range(2 * ROOFLINE_COPIES):
+            fn(i)
+        e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+        e0.record()
+        for i in range(reps):
+            fn(i)
+        e1.record()
+        torch.cuda.synchronize()
+        return e0.elapsed_time(e1) / reps * 1e3
+
+    us_hbm = timed(launch, 50 * ROOFLINE_COPIES)
+    us_res = timed(lambda i: launch(0), 200)
+    L, s = _lib.lib(), current_stream(dev)
+    nb16 = nbytes // 16 * 16
+    bufs = [torch.ones(nb16 // 8, dtype=torch.float64, device=dev) for _ in range(ROOFLINE_COPIES)]
+    part = torch.empty(int(L.pfm_bandwidth_probe_out_bytes()) // 8, dtype=torch.float64, device=dev)
+    us_read = timed(lambda i: L.pfm_bandwidth_probe(0, _lib.ptr(bufs[i % len(bufs)]),
+                                                    _lib.ptr(part), nb16, s),
+                    20 * ROOFLINE_COPIES)
+    del bufs, cps
+    torch.cuda.empty_cache()
+    gbs = lambda us, b=nbytes: b / (us * 1e-6) / 1e9
+    log(f"SpMV from HBM ({ROOFLINE_COPIES} copies rotated): {us_hbm:.2f} us/launch, "
+        f"{gbs(us_hbm):.0f} GB/s ({gbs(us_hbm) / HBM_PEAK_GBS:.3f} of 8 TB/s); MALL-resident "
+        f"{us_res:.2f} us ({gbs(us_res):.0f} GB/s); streaming read of the same bytes "
+        f"{gbs(us_read, nb16):.0f} GB/s")
+    traffic = None if args.no_traffic else measure_traffic(args)
+    return {"us_hbm": us_hbm, "us_resident": us_res, "gbs_hbm": gbs(us_hbm),
+            "gbs_resident": gbs(us_res), "read_ceiling_gbs": gbs(us_read, nb16),
+            "bytes": nbytes, "traffic": traffic}
 
 
 def bench_batch(args, dev):
@@ -375,6 +571,108 @@ def bench_family(mat, dev):
     return out
 
 
+def bench_precision(args, dev):
+    """the 256^3 jittered mesh with lognormal(0, 1.5) diffusivity.
+    FP64 pair/Galerkin GAMG and its mixed-precision form (FP32 V-cycles on the
+    levels above 4096 cells inside FP64 refinement on the true residual, FP64
+    coarse levels and coarsest solve) to rel 1e-8; the SpMV at this size (1.6 GB
+    per product, HBM-streaming); and PCG-DIC (the reference algorithm) for a
+    bounded number of iterations, where it stagnates far above 1e-8."""
+    import torch
+
+    from pyopenfoam_amd import GAMGSolver, PCGSolver, _lib
+    from pyopenfoam_amd.cases import poisson_case
+    from pyopenfoam_amd.core.device import current_stream
+    from pyopenfoam_amd.discretisation.fvm import assemble
+    from pyopenfoam_amd.mesh.fv_mesh import FvMesh
+
+    n, sigma = args.precision_n, args.precision_sigma
+    t0 = time.perf_counter()
+    case = poisson_case(n, jitter=args.jitter, sigma=sigma, seed=0)
+    mesh = FvMesh.from_case(case, dev)
+    mat = assemble(case, mesh)
+    torch.cuda.synchronize()
+    N, F = case.n_cells, case.mesh.n_internal_faces
+    log(f"precision: {n}^3, {N} cells, {F} faces, lognormal(0,{sigma}), built in "
+        f"{time.perf_counter() - t0:.1f} s")
+    x0 = torch.zeros(N, dtype=torch.float64, device=dev)
+    out = {"workload": f"{n}^3 hex mesh jitter {args.jitter}, permuted+RCM, lognormal(0,{sigma}) "
+                       f"diffusivity, Dirichlet x0 patch, rel 1e-8 (tolerance 1e-300)",
+           "n_cells": N, "n_internal_faces": F}
+    # SpMV at this size
+    pk = mat.packed(False)
+    y = torch.empty_like(x0)
+    xr = torch.ones_like(x0)
+    L, s = _lib.lib(), current_stream(dev)
+    h = mat.addressing.handle
+    for _ in range(3):
+        L.pfm_spmv(h, _lib.ptr(mat.diag), _lib.ptr(pk), _lib.ptr(xr), _lib.ptr(y), s)
+    e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+    e0.record()
+    for _ in range(20):
+        L.pfm_spmv(h, _lib.ptr(mat.diag), _lib.ptr(pk), _lib.ptr(xr), _lib.ptr(y), s)
+    e1.record()
+    torch.cuda.synchronize()
+    us = e0.elapsed_time(e1) / 20 * 1e3
+    sb = 24 * N + 24 * F
+    out["spmv"] = {"us": round(us, 2), "algorithmic_bytes": sb,
+                   "gbs": round(sb / (us * 1e-6) / 1e9, 1),
+                   "frac": round(sb / (us * 1e-6) / 1e9 / HBM_PEAK_GBS, 4)}
+    log(f"precision: SpMV {us:.1f} us, {sb / (us * 1e-6) / 1e9:.0f} GB/s")
+    bn = mat.source.norm().item()
+    kw = dict(agglomeration="pair", coarse="galerkin", max_levels=20)
+    x64 = None
+    for name, extra in (("gamg_fp64", {}),
+                        ("gamg_mixed", dict(precision="mixed", inner_rel_tol=0.1,
+                                            max_outer=100))):
+        gs = GAMGSolver(tolerance=1e-300, rel_tol=1e-8, max_iter=args.precision_max_cycles,
+                        **kw, **extra)
+        mat.solve(gs, x0, 1e-300, 2)  # warm: hierarchy, coarse handles, workspace
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        xg, it, _ = mat.solve(gs, x0, 1e-300, args.precision_max_cycles)
+        torch.cuda.synchronize()
+        ms = (time.perf_counter() - t0) * 1e3
+        info = gs.last_info
+        rt = (mat.source - mat.Ax(xg)).norm().item() / bn
+        cyc = info.inner_iterations if extra else int(it)
+        rec = {"ms": round(ms, 1), "cycles": int(cyc), "converged": bool(info.converged),
+               "true_rel_residual": rt, "levels": gs.hierarchy(mat).n_levels}
+        if extra:
+            rec["refinement_steps"] = int(it) - 1
+            rec["inner_rel_tol"] = 0.1
+            if x64 is not None:
+                rec["speedup_vs_fp64"] = round(out["gamg_fp64"]["ms"] / ms, 3)
+                rec["rel_err_vs_fp64_solution"] = ((xg - x64).norm() / x64.norm()).item()
+        else:
+            x64 = xg
+        out[name] = rec
+        log(f"precision {name}: {ms:.0f} ms, {cyc} cycles, converged {info.converged}, "
+            f"true rel residual {rt:.2e}" + (f", {rec.get('speedup_vs_fp64')}x FP64, error "
+                                              f"{rec.get('rel_err_vs_fp64_solution', 0):.2e}"
+                                              if extra else ""))
+    # the reference algorithm on this system: bounded PCG-DIC, FP64 and mixed
+    for name, extra in (("pcg_dic_fp64", {}), ("pcg_dic_mixed", dict(precision="mixed"))):
+        k = args.precision_pcg_iters
+        ps = PCGSolver(tolerance=1e-300, rel_tol=1e-8, max_iter=k, **extra)
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        xp, it, _ = mat.solve(ps, x0, 1e-300, k)
+        torch.cuda.synchronize()
+        ms = (time.perf_counter() - t0) * 1e3
+        rt = (mat.source - mat.Ax(xp)).norm().item() / bn
+        inner = getattr(ps.last_info, "inner_iterations", None)
+        iters = int(inner) if extra else int(it) - 1
+        out[name] = {"iterations": iters, "ms": round(ms, 1),
+                     "us_per_iteration": round(ms * 1e3 / max(iters, 1), 2),
+                     "converged": bool(ps.last_info.converged), "true_rel_residual": rt}
+        log(f"precision {name}: {iters} iterations in {ms:.0f} ms, converged "
+            f"{ps.last_info.converged}, true rel residual {rt:.2e}")
+    del mat, mesh, case
+    torch.cuda.empty_cache()
+    return out
+
+
 def main() -> int:
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -403,9 +701,21 @@ def main() -> int:
     ap.add_argument("--batch-max-iter", type=int, default=30000)
     ap.add_argument("--batch-seq-sample", type=int, default=32,
                     help="systems solved one at a time to estimate the unbatched loop")
+    ap.add_argument("--roofline-probe", action="store_true",
+                    help="only the rotating SpMV launches and calibration copy the counter "
+                         "passes profile")
+    ap.add_argument("--no-traffic", action="store_true", help="skip the rocprofv3 counter passes")
+    ap.add_argument("--no-precision", action="store_true")
+    ap.add_argument("--precision-n", type=int, default=256)
+    ap.add_argument("--precision-sigma", type=float, default=1.5)
+    ap.add_argument("--precision-max-cycles", type=int, default=4000)
+    ap.add_argument("--precision-pcg-iters", type=int, default=5000)
+    ap.add_argument("--adjoint-rel-tol", type=float, default=1e-10)
     ap.add_argument("--copy-probe-mib", type=int, default=0,
                     help="also run a plain device copy of this size (HBM counter calibration)")
     args = ap.parse_args()
+    if args.roofline_probe:
+        return roofline_probe(args)
     if args.gpus != 1:
         print(f"bench.py measures one GPU only; --gpus {args.gpus} is not supported (the path "
               "would shard by domain decomposition: one rank per GPU, halo exchange of the "
@@ -473,27 +783,10 @@ def main() -> int:
     if args.history:
         np.save(args.history, np.array(solver.last_info.residual_history))
 
-    # --- SpMV kernel alone (HIP events on the launch stream) ---------------
-    pk = mat.packed(False)
-    y = torch.empty_like(x)
-    stream = current_stream(dev)
-    L = _lib.lib()
-    h = mat.addressing.handle
-    dptr, pptr, xptr, yptr = (_lib.ptr(t) for t in (mat.diag, pk, x, y))
-    for _ in range(20):
-        L.pfm_spmv(h, dptr, pptr, xptr, yptr, stream)
-    reps = 200
+    # --- SpMV kernel alone: from HBM (rotating copies), resident, ceiling ---
+    roof = bench_roofline(case, mat, dev, args)
+    spmv_us, spmv_gbs, spmv_bytes = roof["us_hbm"], roof["gbs_hbm"], roof["bytes"]
     e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
-    e0.record()
-    for _ in range(reps):
-        L.pfm_spmv(h, dptr, pptr, xptr, yptr, stream)
-    e1.record()
-    torch.cuda.synchronize()
-    spmv_us = e0.elapsed_time(e1) / reps * 1e3
-    spmv_bytes = 24 * N + 24 * F
-    spmv_gbs = spmv_bytes / (spmv_us * 1e-6) / 1e9
-    log(f"SpMV: {spmv_us:.2f} us/launch, {spmv_gbs:.0f} GB/s algorithmic "
-        f"({spmv_gbs / HBM_PEAK_GBS * 100:.1f}% of 8 TB/s)")
 
     if args.copy_probe_mib:
         nb = args.copy_probe_mib * (1 << 20)
@@ -527,7 +820,7 @@ def main() -> int:
         dir_data = mesh.dirichlet_data("x0")[1:]
         cells = torch.from_numpy(case.dir_cells.astype(np.int32)).to(dev)
         dd = (cells, torch.from_numpy(case.dir_mag).to(dev), torch.from_numpy(case.dir_delta).to(dev))
-        asolver = PCGSolver(tolerance=1e-300, rel_tol=1e-8, max_iter=max_iter)
+        asolver = PCGSolver(tolerance=1e-300, rel_tol=args.adjoint_rel_tol, max_iter=max_iter)
         tf, tb = [], []
         for rep in range(2):
             torch.cuda.synchronize()
@@ -544,7 +837,8 @@ def main() -> int:
             g.grad = None
             gb.grad = None
         adjoint = {"forward_ms": round(tf[-1] * 1e3, 2), "backward_ms": round(tb[-1] * 1e3, 2),
-                   "ratio": round(tb[-1] / tf[-1], 3)}
+                   "ratio": round(tb[-1] / tf[-1], 3), "rel_tol": args.adjoint_rel_tol,
+                   "forward_iterations": int(asolver.last_info.iterations)}
         log(f"adjoint: forward {tf[-1] * 1e3:.1f} ms, backward {tb[-1] * 1e3:.1f} ms, "
             f"ratio {tb[-1] / tf[-1]:.3f}")
 
@@ -626,6 +920,11 @@ def main() -> int:
             log(f"gamg {name}: {gms:.1f} ms, {cyc} cycles ({gms / cyc:.2f} ms/cycle), converged "
                 f"{gs.last_info.converged}, true rel residual {rt:.2e}, {hier.n_levels} levels")
 
+    # --- precision: 256^3, lognormal(0, 1.5) --------------------
+    precision = None
+    if not args.no_precision:
+        precision = bench_precision(args, dev)
+
     # --- batching: many uneven systems in one graph-replayed PCG-DIC ---------
     batch = None
     if not args.no_batch:
@@ -671,14 +970,23 @@ def main() -> int:
         "spmv_gbs": round(spmv_gbs, 1),
         "adjoint_over_forward": adjoint["ratio"] if adjoint else None,
         "mixed_precision": mixed,
+        "precision_256": precision,
         "adjoint": adjoint,
         "batch": batch,
         "gamg": gamg,
-        "roofline": {"bound": "hbm", "kernel": "k_spmv<double> (pfm_spmv)",
+        "roofline": {"bound": "hbm",
+                     "kernel": f"k_spmv<double, true> (pfm_spmv), {n}^3, launches rotating over "
+                               f"{ROOFLINE_COPIES} independent copies (HBM-streaming)",
                      "achieved": round(spmv_gbs, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": round(spmv_gbs / HBM_PEAK_GBS, 4),
-                     "traffic": round(SPMV_TRAFFIC_128) if n == 128 else None,
-                     "algorithmic_bytes": spmv_bytes},
+                     "traffic": roof["traffic"]["bytes_per_launch"] if roof["traffic"] else None,
+                     "algorithmic_bytes": spmv_bytes,
+                     "us_per_launch": round(spmv_us, 3),
+                     "mall_resident": {"us_per_launch": round(roof["us_resident"], 3),
+                                       "gbs": round(roof["gbs_resident"], 1)},
+                     "read_ceiling_gbs": round(roof["read_ceiling_gbs"], 1),
+                     "frac_of_read_ceiling": round(spmv_gbs / roof["read_ceiling_gbs"], 4),
+                     "traffic_detail": roof["traffic"]},
         "cpu_baseline": cpu,
         "setup_s": {"case_host": round(t_case, 2), "handle_and_assembly": round(t_asm, 2)},
         "geometry": geometry,

@@ -185,11 +185,37 @@ int pfm_ldu_pack_f32(pfm_ldu h, int transpose, const double *lower,
 
 /* y = A x on a packed matrix.  Row sums follow the reference association
  * y = (diag*x + sum_owner-faces lower*x[nei]) + sum_neighbour-faces upper*x[own]
- * with each sum in face order, i.e. LduMatrix.Ax ldu_matrix.py:200-217. */
+ * with each sum in face order, i.e. LduMatrix.Ax ldu_matrix.py:200-217.
+ * A product that streams at least 128 MiB (half the MALL) loads its
+ * coefficients, columns and diagonal and stores y nontemporally. */
 int pfm_spmv(pfm_ldu h, const double *diag, const double *packed,
              const double *x, double *y, void *stream);
 int pfm_spmv_f32(pfm_ldu h, const float *diag, const float *packed,
                  const float *x, float *y, void *stream);
+
+/* Y = A X for k right-hand sides: LduMatrix.Ax_batched / Ax_sparse on an
+ * (n_cells, k) input (ldu_matrix.py:345-381), x and y row-major
+ * (x[i*k + j]).  Each column is summed in the association of pfm_spmv
+ * (LduMatrix.Ax), so column j equals the single product of x[:, j] bit for
+ * bit; the reference's CSR product rounds in its own order. */
+int pfm_spmv_multi(pfm_ldu h, const double *diag, const double *packed, int32_t k,
+                   const double *x, double *y, void *stream);
+
+/* FvMatrix.relax (core/fv_matrix.py:192-235): diag_out = diag / alpha,
+ * source_out = source + ((1 - alpha) * diag_out) * field_old; alpha in
+ * (0, 1] (PFM_ERR_ARG otherwise).  Outputs may alias the inputs. */
+int pfm_relax(int32_t n, double alpha, const double *diag, const double *source,
+              const double *field_old, double *diag_out, double *source_out,
+              void *stream);
+
+/* Achievable-bandwidth probes (no reference counterpart: measurement of the
+ * ceiling the bandwidth-bound kernels are judged against).  mode 0 reads
+ * nbytes of src with nontemporal 16-byte loads and writes one partial sum per
+ * wave to dst (pfm_bandwidth_probe_out_bytes() bytes); mode 1 copies nbytes
+ * from src to dst.  nbytes: a multiple of 16; buffers 16-byte aligned. */
+int pfm_bandwidth_probe(int mode, const void *src, void *dst, int64_t nbytes,
+                        void *stream);
+int64_t pfm_bandwidth_probe_out_bytes(void);
 /* y = A x (transpose = 0) or A^T x (transpose = 1) straight from the face
  * arrays, no packing: LduMatrix.Ax / Ax_T (ldu_matrix.py:172-258). */
 int pfm_spmv_ldu(pfm_ldu h, int transpose, const double *diag,

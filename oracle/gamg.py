@@ -2,7 +2,13 @@
 
 Restates:
   * ``aggregate``      -- GAMGSolver._build_aggregates (solvers/gamg.py:164-239)
-  * ``coarse_matrix``  -- GAMGSolver._build_coarse_matrix (:242-333)
+  * ``coarse_matrix``  -- GAMGSolver._build_coarse_matrix (:242-333); with
+                          ``galerkin=True`` the build's additive extension
+                          that keeps the couplings the reference drops
+                          inside an aggregate (:297-299) on the coarse
+                          diagonal (parity pinned by the dense P^T A P of
+                          the piecewise-constant prolongation, see
+                          tests/test_oracle_golden.py)
   * ``gamg``           -- GAMGSolver._solve with _v_cycle/_smooth/_restrict/
                           _prolongate (:340-585)
   * ``pair_levels``    -- PairGamgAgglomeration.build_levels/_pair_cells
@@ -72,8 +78,16 @@ def aggregate_py(n, owner, neighbour, target):
     return nagg, f2c
 
 
-def coarse_matrix(level: Level, nagg: int, f2c: np.ndarray):
-    """Returns (coarse Level, face_map) with face_map[f] = coarse face or -1."""
+def coarse_matrix(level: Level, nagg: int, f2c: np.ndarray, galerkin: bool = False):
+    """Returns (coarse Level, face_map) with face_map[f] = coarse face or -1.
+
+    The reference (gamg.py:242-333): coarse diag = scatter-sum of the fine
+    diagonal; a face between two aggregates adds its lower / upper to the
+    coarse face keyed by the aggregate pair (first-seen orientation); a face
+    inside one aggregate is dropped (:297-299).  ``galerkin``: such a face
+    instead adds lower + upper to its aggregate's diagonal, one face after
+    another in face order after the diagonal sum, so the coarse diagonal is
+    the diagonal of P^T A P for the piecewise-constant P of ``f2c``."""
     diag = _scatter(level.diag, f2c, nagg)
     nf = level.owner.shape[0]
     o, q, m = i64(level.owner), i64(level.neighbour), i64(f2c)
@@ -85,6 +99,9 @@ def coarse_matrix(level: Level, nagg: int, f2c: np.ndarray):
     fmap = np.empty(max(nf, 1), np.int64)
     ncf = clib().orc_gamg_coarse_faces(nf, p(o), p(q), p(m), p(lo), p(up), p(c_own), p(c_nei),
                                        p(c_lo), p(c_up), p(fmap))
+    if galerkin and nf:
+        inside = np.nonzero(fmap[:nf] < 0)[0]
+        np.add.at(diag, m[o[inside]], lo[inside] + up[inside])  # sequential, face order
     return (Level(nagg, diag, c_lo[:ncf].copy(), c_up[:ncf].copy(), c_own[:ncf].copy(),
                   c_nei[:ncf].copy()), fmap[:nf].copy())
 

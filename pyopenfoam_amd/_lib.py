@@ -92,6 +92,10 @@ _SIGS = {
     "pfm_ldu_pack_f32": (INT, [P, INT, P, P, P, P]),
     "pfm_spmv": (INT, [P, P, P, P, P, P]),
     "pfm_spmv_f32": (INT, [P, P, P, P, P, P]),
+    "pfm_bandwidth_probe": (INT, [INT, P, P, I64, P]),
+    "pfm_spmv_multi": (INT, [P, P, P, I32, P, P, P]),
+    "pfm_relax": (INT, [I32, C.c_double, P, P, P, P, P, P]),
+    "pfm_bandwidth_probe_out_bytes": (I64, []),
     "pfm_spmv_ldu": (INT, [P, INT, P, P, P, P, P, P]),
     "pfm_residual": (INT, [P, P, P, P, P, P, P]),
     "pfm_ldu_set_progress": (INT, [P, P, P, C.c_double]),

@@ -134,6 +134,7 @@ class BatchCase:
     dir_delta: np.ndarray
     b: np.ndarray
     seed: int
+    systems: list | None = None  # the per-system PoissonCases (keep_systems=True)
 
     @property
     def n_systems(self) -> int:
@@ -152,12 +153,13 @@ def _batch_mesh(args):
 
 def poisson_batch(n_systems: int = 512, n_min: int = 8, n_max: int = 48, *, seed: int = 0,
                   jitter: float = 0.2, sigma: float = 1.0, workers: int = 8,
-                  geometry=None) -> BatchCase:
+                  geometry=None, keep_systems: bool = False) -> BatchCase:
     """Batch of independent cube systems: side n_k ~ U{n_min..n_max} from
     ``default_rng(seed)``; system k is ``poisson_case(n_k, seed=seed*100003 + k)``
     (jittered, permuted, RCM-renumbered, lognormal diffusivity, zero-mean RHS).
     The meshes are built in ``workers`` host processes; the geometry
-    (``geometry``, default the device kernels) runs in this process."""
+    (``geometry``, default the device kernels) runs in this process.
+    ``keep_systems`` keeps the per-system cases in ``BatchCase.systems``."""
     rng = np.random.default_rng(seed)
     sizes = rng.integers(n_min, n_max + 1, size=n_systems)
     jobs = [(int(n), seed * 100003 + k, jitter, sigma) for k, n in enumerate(sizes)]
@@ -188,4 +190,5 @@ def poisson_batch(n_systems: int = 512, n_min: int = 8, n_max: int = 48, *, seed
         gamma=cat([c.gamma for c in cases]),
         dir_cells=cat([c.dir_cells + cell_off[k] for k, c in enumerate(cases)]),
         dir_gamma=cat([c.dir_gamma for c in cases]), dir_mag=cat([c.dir_mag for c in cases]),
-        dir_delta=cat([c.dir_delta for c in cases]), b=cat([c.b for c in cases]), seed=seed)
+        dir_delta=cat([c.dir_delta for c in cases]), b=cat([c.b for c in cases]), seed=seed,
+        systems=cases if keep_systems else None)

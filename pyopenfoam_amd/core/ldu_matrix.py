@@ -12,6 +12,13 @@ The packed (row-slice) copy of the coefficients used by the solvers is cached
 per transpose flag and rebuilt when a coefficient tensor is replaced or
 modified in place (tracked through the tensor version counters), the analogue
 of the reference's CSR cache and ``invalidate_cache`` (``:309-343``).
+
+``to_sparse_coo`` / ``to_sparse_csr`` / ``to_sparse_csr_cached`` (``:278-343``)
+build torch sparse tensors of the same entries (format conversion, not
+arithmetic); ``Ax_sparse`` / ``Ax_batched`` (``:345-381``) take (n,) or (n, k)
+and run the multi-right-hand-side row-slice kernel ``pfm_spmv_multi``, whose
+every column equals ``Ax`` of that column bit for bit (the reference's CSR
+product rounds each row in column order instead).
 """
 
 from __future__ import annotations
@@ -44,6 +51,7 @@ class LduMatrix:
         self._upper = torch.zeros(self._n_internal_faces, device=self._device, dtype=self._dtype)
         self._addr: LduAddressing | None = None
         self._packed: dict = {}
+        self._csr_cache: dict = {}
 
     # ------------------------------------------------------------ properties
     @property
@@ -109,6 +117,7 @@ class LduMatrix:
 
     def invalidate_cache(self) -> None:
         self._packed.clear()
+        self._csr_cache.clear()
 
     def _check_compute(self) -> None:
         require_gpu(self._diag)
@@ -203,6 +212,64 @@ class LduMatrix:
                                            _lib.ptr(pk), _lib.ptr(x.detach()), _lib.ptr(y),
                                            current_stream(self._device)), "pfm_spmv")
         return y
+
+    # ------------------------------------------------------------ sparse forms
+    def to_sparse_coo(self) -> torch.Tensor:
+        """Coalesced COO tensor of A (``ldu_matrix.py:278-305``): entries
+        (i, i) = diag, (owner, neighbour) = lower, (neighbour, owner) = upper."""
+        n = self._n_cells
+        ar = torch.arange(n, device=self._device, dtype=torch.int64)
+        own, nei = self._owner.long(), self._neighbour.long()
+        if self._n_internal_faces:
+            idx = torch.cat([torch.stack([ar, ar]), torch.stack([own, nei]),
+                             torch.stack([nei, own])], dim=1)
+            val = torch.cat([self._diag, self._lower, self._upper])
+        else:
+            idx, val = torch.stack([ar, ar]), self._diag
+        return torch.sparse_coo_tensor(idx, val, (n, n), device=self._device).coalesce()
+
+    def to_sparse_csr(self) -> torch.Tensor:
+        """CSR tensor of A (``ldu_matrix.py:307-313``)."""
+        return self.to_sparse_coo().to_sparse_csr()
+
+    def to_sparse_csr_cached(self) -> torch.Tensor:
+        """CSR tensor of A, cached until a coefficient setter or
+        ``invalidate_cache`` (``ldu_matrix.py:315-333``)."""
+        hit = self._csr_cache.get("csr")
+        if hit is not None and hit.device == self._device:
+            return hit
+        csr = self.to_sparse_csr()
+        self._csr_cache["csr"] = csr
+        return csr
+
+    def Ax_sparse(self, x: torch.Tensor) -> torch.Tensor:
+        """y = A x for x of shape (n,) or (n, k) (``ldu_matrix.py:345-364``)."""
+        assert_floating(x, "x")
+        if x.dim() not in (1, 2) or x.shape[0] != self._n_cells:
+            raise ValueError(f"x has shape {tuple(x.shape)}, expected ({self._n_cells},) or "
+                             f"({self._n_cells}, k)")
+        x = x.to(device=self._device, dtype=self._dtype)
+        if x.dim() == 1:
+            return self.Ax_packed(x)
+        self._check_compute()
+        k = int(x.shape[1])
+        xc = x.detach().contiguous()
+        y = torch.empty(self._n_cells, k, device=self._device, dtype=self._dtype)
+        if self._n_cells and k:
+            d = self._diag.detach().contiguous()
+            pk = self.packed(False)
+            _lib.check(_lib.lib().pfm_spmv_multi(self.addressing.handle, _lib.ptr(d), _lib.ptr(pk),
+                                                 k, _lib.ptr(xc), _lib.ptr(y),
+                                                 current_stream(self._device)), "pfm_spmv_multi")
+        return y
+
+    def Ax_batched(self, x: torch.Tensor) -> torch.Tensor:
+        """Y = A X for an (n, k) block of right-hand sides (``ldu_matrix.py:366-381``)."""
+        return self.Ax_sparse(x)
+
+    def __repr__(self) -> str:
+        return (f"LduMatrix(n_cells={self._n_cells}, n_internal_faces={self._n_internal_faces}, "
+                f"device={self._device}, dtype={self._dtype})")
 
     def add_to_diag(self, values: torch.Tensor) -> None:
         if values.dim() == 0:

@@ -72,22 +72,36 @@ __device__ __forceinline__ SliceRange block_slices(int nslices) {
   return r;
 }
 
+// Load of an element that one pass reads once (kNT: a nontemporal load, so the
+// stream does not displace data that is read again from L2 and the MALL).
+template <bool kNT, typename T>
+__device__ __forceinline__ T ld(const T *p) {
+  if constexpr (kNT)
+    return __builtin_nontemporal_load(p);
+  else
+    return *p;
+}
+
 // Column of slot k of a row: the int32 column (-1 = padding slot), or an
 // int16 offset from the row (kColPad16 = padding slot), the compact form the
 // handle keeps when every offset fits (half the index bytes per slot).
 constexpr int kColPad16 = -32768;
-struct ColAbs {
+template <bool kNT = false>
+struct ColAbsT {
   const int32_t *__restrict__ c;
-  __device__ __forceinline__ int operator()(int k) const { return c[k]; }
+  __device__ __forceinline__ int operator()(int k) const { return ld<kNT>(c + k); }
 };
-struct ColRel {
+template <bool kNT = false>
+struct ColRelT {
   const int16_t *__restrict__ d;
   int row;
   __device__ __forceinline__ int operator()(int k) const {
-    const int v = d[k];
+    const int v = ld<kNT>(d + k);
     return v == kColPad16 ? -1 : row + v;
   }
 };
+using ColAbs = ColAbsT<false>;
+using ColRel = ColRelT<false>;
 
 // One row of y = A x on the packed row-slice layout.  Returns
 // (diag*x + sum_lower) + sum_upper, each sum in face order from 0, which is
@@ -97,8 +111,10 @@ struct ColRel {
 // where it is read).  kUpperFirst returns (diag*x + sum_upper) + sum_lower,
 // the association of the reference's Ax_T (ldu_matrix.py:248-257: the
 // neighbour-side scatter first) when the packed values are the transposed
-// ones.
-template <typename T, typename Gather, bool kUpperFirst = false, typename ColF = ColAbs>
+// ones.  kNT loads the coefficients nontemporally (pass a ColF of the same
+// kind for the columns).
+template <typename T, typename Gather, bool kUpperFirst = false, typename ColF = ColAbs,
+          bool kNT = false>
 __device__ __forceinline__ T row_spmv_g(const SliceInfo si, int lane, T dx, ColF col,
                                         const T *__restrict__ val, Gather gx) {
   constexpr int U = 4;
@@ -114,9 +130,9 @@ __device__ __forceinline__ T row_spmv_g(const SliceInfo si, int lane, T dx, ColF
 #pragma unroll
     for (int j = 0; j < U; ++j) {
       c[j] = j < wl ? col(bl + j * kSlice) : -1;
-      v[j] = j < wl ? val[bl + j * kSlice] : T(0);
+      v[j] = j < wl ? ld<kNT>(val + bl + j * kSlice) : T(0);
       c[U + j] = j < wu ? col(bu + j * kSlice) : -1;
-      v[U + j] = j < wu ? val[bu + j * kSlice] : T(0);
+      v[U + j] = j < wu ? ld<kNT>(val + bu + j * kSlice) : T(0);
     }
     T xv[2 * U];
 #pragma unroll
@@ -136,7 +152,7 @@ __device__ __forceinline__ T row_spmv_g(const SliceInfo si, int lane, T dx, ColF
 #pragma unroll
     for (int j = 0; j < U; ++j) {
       c[j] = (j0 + j < wl) ? col(base + j * kSlice) : -1;
-      v[j] = (j0 + j < wl) ? val[base + j * kSlice] : T(0);
+      v[j] = (j0 + j < wl) ? ld<kNT>(val + base + j * kSlice) : T(0);
     }
 #pragma unroll
     for (int j = 0; j < U; ++j)
@@ -150,7 +166,7 @@ __device__ __forceinline__ T row_spmv_g(const SliceInfo si, int lane, T dx, ColF
 #pragma unroll
     for (int j = 0; j < U; ++j) {
       c[j] = (j0 + j < wu) ? col(base + j * kSlice) : -1;
-      v[j] = (j0 + j < wu) ? val[base + j * kSlice] : T(0);
+      v[j] = (j0 + j < wu) ? ld<kNT>(val + base + j * kSlice) : T(0);
     }
 #pragma unroll
     for (int j = 0; j < U; ++j)

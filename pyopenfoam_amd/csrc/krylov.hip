@@ -49,8 +49,17 @@ __device__ __forceinline__ void store_partials(double (&v)[K], double *part, dou
     }                                                                           \
   }
 
+// Which PCG streams are loaded nontemporally.  Across iterations the 256 MB
+// MALL keeps much of the matrix (about 142 MB at 128^3) and the vectors, and
+// an iteration moves more than the MALL holds: nontemporal matrix loads
+// (59.6 -> 68.8 us per iteration at 128^3) or x / p_{it-1} loads (-> 60.2 us)
+// cost more than they save; Ap, read once after the SpMV writes it, gains
+// 0.7 % (59.6 -> 59.2 us).
+constexpr bool kNtMat = false;
+constexpr bool kNtAp = true;
+
 // One row of A v with the handle's int16 row offsets (kRel) or int32 columns.
-template <bool kRel>
+template <bool kRel, bool kNT = false>
 __device__ __forceinline__ double row_av(const SliceInfo si, int lane, int row, double dx,
                                          const int32_t *__restrict__ col,
                                          const int16_t *__restrict__ col16,
@@ -58,10 +67,11 @@ __device__ __forceinline__ double row_av(const SliceInfo si, int lane, int row, 
                                          const double *__restrict__ v) {
   auto gx = [v](int c) { return v[c]; };
   if constexpr (kRel)
-    return row_spmv_g<double, decltype(gx), false, ColRel>(si, lane, dx, ColRel{col16, row},
-                                                           val, gx);
+    return row_spmv_g<double, decltype(gx), false, ColRelT<kNT>, kNT>(
+        si, lane, dx, ColRelT<kNT>{col16, row}, val, gx);
   else
-    return row_spmv_g<double>(si, lane, dx, ColAbs{col}, val, gx);
+    return row_spmv_g<double, decltype(gx), false, ColAbsT<kNT>, kNT>(si, lane, dx,
+                                                                      ColAbsT<kNT>{col}, val, gx);
 }
 
 // ---------------------------------------------------------------- PCG ------
@@ -142,8 +152,8 @@ __global__ __launch_bounds__(kBlock) void k_pcg_spmv(int n, int nslices, const S
   FOR_ROWS({
     const SliceInfo si = load_slice(sl, s);
     const double pi = row < n ? p[row] : 0.0;
-    const double dx = row < n ? diag[row] * pi : 0.0;
-    const double v = row_av<kRel>(si, lane, row, dx, col, col16, val, p);
+    const double dx = row < n ? ld<kNtMat>(diag + row) * pi : 0.0;
+    const double v = row_av<kRel, kNtMat>(si, lane, row, dx, col, col16, val, p);
     if (row < n) {
       Ap[row] = v;
       acc[0] += pi * v;
@@ -178,7 +188,7 @@ __global__ __launch_bounds__(kBlock) void k_pcg_update(int n, int nslices, int i
   double rp = 0.0, ap = 0.0, dp = 1.0;
   if (s < nslices && row < n) {
     rp = r[row];
-    ap = Ap[row];
+    ap = ld<kNtAp>(Ap + row);
     if (rD) dp = rD[row];
   }
   const double PAP = reduce_partials(part, G, lds);
@@ -206,7 +216,7 @@ __global__ __launch_bounds__(kBlock) void k_pcg_update(int n, int nslices, int i
     const int sn = s + stride, rn = sn * kSlice + lane;
     if (sn < nslices && rn < n) {
       rp = r[rn];
-      ap = Ap[rn];
+      ap = ld<kNtAp>(Ap + rn);
       if (rD) dp = rD[rn];
     }
   }

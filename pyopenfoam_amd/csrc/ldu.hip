@@ -43,7 +43,10 @@ __global__ __launch_bounds__(kBlock) void k_pack(int64_t nslots, int nslices,
   }
 }
 
-template <typename T>
+// One wave per slice.  kNT: the streams read or written once per product
+// (coefficients, columns, diagonal, y) are nontemporal, so they pass through
+// L2 and the MALL without displacing x, whose entries every row gathers.
+template <typename T, bool kNT>
 __global__ __launch_bounds__(kBlock) void k_spmv(int n, int nslices,
                                                   const SliceInfo *__restrict__ sl,
                                                   const int32_t *__restrict__ col,
@@ -56,9 +59,124 @@ __global__ __launch_bounds__(kBlock) void k_spmv(int n, int nslices,
   const SliceInfo si = load_slice(sl, s);
   const int lane = threadIdx.x & 63;
   const int row = s * kSlice + lane;
-  const T dx = row < n ? diag[row] * x[row] : T(0);
-  const T v = row_spmv<T>(si, lane, dx, col, val, x);
-  if (row < n) y[row] = v;
+  const T dx = row < n ? ld<kNT>(diag + row) * x[row] : T(0);
+  auto gx = [x](int c) { return x[c]; };
+  const T v = row_spmv_g<T, decltype(gx), false, ColAbsT<kNT>, kNT>(si, lane, dx,
+                                                                      ColAbsT<kNT>{col}, val, gx);
+  if (row < n) {
+    if constexpr (kNT)
+      __builtin_nontemporal_store(v, y + row);
+    else
+      y[row] = v;
+  }
+}
+
+// Y = A X for k right-hand sides stored row-major (X[i*k + j]): one wave per
+// slice, the row's coefficients and columns loaded once and reused for every
+// column j, each column summed in the association of the single product.
+__global__ __launch_bounds__(kBlock) void k_spmv_multi(int n, int nslices, int k,
+                                                        const SliceInfo *__restrict__ sl,
+                                                        const int32_t *__restrict__ col,
+                                                        const double *__restrict__ val,
+                                                        const double *__restrict__ diag,
+                                                        const double *__restrict__ X,
+                                                        double *__restrict__ Y) {
+  const int s = xcd_remap(blockIdx.x, gridDim.x) * kWavesPerBlock + (threadIdx.x >> 6);
+  if (s >= nslices) return;
+  const SliceInfo si = load_slice(sl, s);
+  const int lane = threadIdx.x & 63;
+  const int row = s * kSlice + lane;
+  const double dg = row < n ? diag[row] : 0.0;
+  constexpr int U = 4;
+  const int wl = si.wl, wu = si.wu;
+  if (wl <= U && wu <= U) {
+    const int bl = si.ptr + lane, bu = si.ptr + wl * kSlice + lane;
+    int c[2 * U];
+    double v[2 * U];
+#pragma unroll
+    for (int j = 0; j < U; ++j) {
+      c[j] = j < wl ? col[bl + j * kSlice] : -1;
+      v[j] = j < wl ? val[bl + j * kSlice] : 0.0;
+      c[U + j] = j < wu ? col[bu + j * kSlice] : -1;
+      v[U + j] = j < wu ? val[bu + j * kSlice] : 0.0;
+    }
+    for (int q = 0; q < k; ++q) {
+      double accL = 0.0, accU = 0.0;
+#pragma unroll
+      for (int j = 0; j < U; ++j)
+        if (c[j] >= 0) accL += v[j] * X[(int64_t)c[j] * k + q];
+#pragma unroll
+      for (int j = 0; j < U; ++j)
+        if (c[U + j] >= 0) accU += v[U + j] * X[(int64_t)c[U + j] * k + q];
+      if (row < n) Y[(int64_t)row * k + q] = (dg * X[(int64_t)row * k + q] + accL) + accU;
+    }
+    return;
+  }
+  for (int q = 0; q < k; ++q) {  // wide rows: the single-product row sum per column
+    auto gx = [X, k, q](int cc) { return X[(int64_t)cc * k + q]; };
+    const double dx = row < n ? dg * X[(int64_t)row * k + q] : 0.0;
+    const double y = row_spmv_g<double>(si, lane, dx, ColAbs{col}, val, gx);
+    if (row < n) Y[(int64_t)row * k + q] = y;
+  }
+}
+
+// FvMatrix.relax (core/fv_matrix.py:192-235): d' = d / alpha,
+// s' = s + ((1 - alpha) * d') * phi_old, rounded as the reference's ops.
+// (element-wise: outputs may alias inputs, so no restrict qualifiers)
+__global__ __launch_bounds__(kBlock) void k_relax(int n, double alpha, const double *d,
+                                                   const double *src, const double *phi,
+                                                   double *d_out, double *s_out) {
+  const int i = blockIdx.x * kBlock + threadIdx.x;
+  if (i >= n) return;
+  const double dn = d[i] / alpha;
+  d_out[i] = dn;
+  s_out[i] = src[i] + ((1.0 - alpha) * dn) * phi[i];
+}
+
+// Achievable-bandwidth probes: a nontemporal 16-byte-per-lane read of a
+// buffer (one partial per wave, so the loads cannot be dropped) and a copy.
+typedef double dvec2 __attribute__((ext_vector_type(2)));
+__global__ __launch_bounds__(kBlock) void k_bw_read(const dvec2 *__restrict__ a, int64_t n2,
+                                                     double *__restrict__ out) {
+  double s = 0.0;
+  for (int64_t i = blockIdx.x * (int64_t)kBlock + threadIdx.x; i < n2;
+       i += (int64_t)gridDim.x * kBlock) {
+    const dvec2 v = __builtin_nontemporal_load(a + i);
+    s += v.x + v.y;
+  }
+  s = wave_sum(s);
+  if ((threadIdx.x & 63) == 0) out[blockIdx.x * kWavesPerBlock + (threadIdx.x >> 6)] = s;
+}
+
+__global__ __launch_bounds__(kBlock) void k_bw_copy(const dvec2 *__restrict__ a,
+                                                     dvec2 *__restrict__ b, int64_t n2) {
+  for (int64_t i = blockIdx.x * (int64_t)kBlock + threadIdx.x; i < n2;
+       i += (int64_t)gridDim.x * kBlock)
+    b[i] = __builtin_nontemporal_load(a + i);
+}
+
+constexpr int kBwGrid = 2048;
+
+// Bytes one product streams (coefficients and columns per slot; diagonal, x
+// and y per row) from which pfm_spmv streams nontemporally: half the 256 MB
+// MALL.  A product that large is evicted between calls by any other work of
+// its size; a smaller one repeated alone stays MALL-resident, where plain
+// loads are faster (128^3, 200 MB: 28.4 us resident with plain loads, 33.7 us
+// with nontemporal ones; from HBM 39.5 us plain, 34.5 us nontemporal).
+constexpr int64_t kStreamBytes = int64_t(128) << 20;
+
+template <typename T>
+int launch_spmv(const pfm_ldu_s *h, const T *diag, const T *packed, const T *x, T *y,
+                hipStream_t s) {
+  const int64_t bytes = h->nslots * (int64_t)(sizeof(T) + 4) + 3 * (int64_t)h->n * sizeof(T);
+  if (bytes >= kStreamBytes)
+    k_spmv<T, true><<<blocks_for_slices(h->nslices), kBlock, 0, s>>>(
+        h->n, h->nslices, h->d_slices, h->d_col, packed, diag, x, y);
+  else
+    k_spmv<T, false><<<blocks_for_slices(h->nslices), kBlock, 0, s>>>(
+        h->n, h->nslices, h->d_slices, h->d_col, packed, diag, x, y);
+  PFM_LAUNCH_CHECK();
+  return PFM_OK;
 }
 
 // r = b - A x
@@ -542,10 +660,59 @@ int pfm_spmv(pfm_ldu h, const double *diag, const double *packed, const double *
   if (h->nslots) PFM_NONNULL(packed);
   // One slice per wave with the int32 columns: the fastest form standalone
   // (the int16 offsets pay off in the solvers' grid-stride kernels only).
-  k_spmv<double><<<blocks_for_slices(h->nslices), kBlock, 0, (hipStream_t)stream>>>(
-      h->n, h->nslices, h->d_slices, h->d_col, packed, diag, x, y);
+  return launch_spmv<double>(h, diag, packed, x, y, (hipStream_t)stream);
+}
+
+int pfm_spmv_multi(pfm_ldu h, const double *diag, const double *packed, int32_t k,
+                   const double *x, double *y, void *stream) {
+  PFM_REQUIRE(h, "null handle");
+  PFM_REQUIRE(k >= 1, "k must be >= 1");
+  if (h->n == 0) return PFM_OK;
+  PFM_NONNULL(diag);
+  PFM_NONNULL(x);
+  PFM_NONNULL(y);
+  if (h->nslots) PFM_NONNULL(packed);
+  k_spmv_multi<<<blocks_for_slices(h->nslices), kBlock, 0, (hipStream_t)stream>>>(
+      h->n, h->nslices, k, h->d_slices, h->d_col, packed, diag, x, y);
   PFM_LAUNCH_CHECK();
   return PFM_OK;
+}
+
+int pfm_relax(int32_t n, double alpha, const double *diag, const double *source,
+              const double *field_old, double *diag_out, double *source_out, void *stream) {
+  PFM_REQUIRE(n >= 0, "negative n");
+  PFM_REQUIRE(alpha > 0.0 && alpha <= 1.0, "under-relaxation factor must be in (0, 1], got %g",
+              alpha);
+  if (n == 0) return PFM_OK;
+  PFM_NONNULL(diag);
+  PFM_NONNULL(source);
+  PFM_NONNULL(field_old);
+  PFM_NONNULL(diag_out);
+  PFM_NONNULL(source_out);
+  k_relax<<<grid1d(n), kBlock, 0, (hipStream_t)stream>>>(n, alpha, diag, source, field_old,
+                                                         diag_out, source_out);
+  PFM_LAUNCH_CHECK();
+  return PFM_OK;
+}
+
+int pfm_bandwidth_probe(int mode, const void *src, void *dst, int64_t nbytes, void *stream) {
+  PFM_REQUIRE(mode == 0 || mode == 1, "unknown probe mode %d", mode);
+  PFM_REQUIRE(nbytes >= 0 && nbytes % 16 == 0, "nbytes must be a non-negative multiple of 16");
+  PFM_REQUIRE(((uintptr_t)src & 15) == 0 && ((uintptr_t)dst & 15) == 0,
+              "buffers must be 16-byte aligned");
+  PFM_NONNULL(src);
+  PFM_NONNULL(dst);
+  hipStream_t s = (hipStream_t)stream;
+  if (mode == 0)
+    k_bw_read<<<kBwGrid, kBlock, 0, s>>>((const dvec2 *)src, nbytes / 16, (double *)dst);
+  else
+    k_bw_copy<<<kBwGrid, kBlock, 0, s>>>((const dvec2 *)src, (dvec2 *)dst, nbytes / 16);
+  PFM_LAUNCH_CHECK();
+  return PFM_OK;
+}
+
+int64_t pfm_bandwidth_probe_out_bytes(void) {
+  return (int64_t)kBwGrid * kWavesPerBlock * sizeof(double);
 }
 
 int pfm_spmv_f32(pfm_ldu h, const float *diag, const float *packed, const float *x, float *y,
@@ -556,10 +723,7 @@ int pfm_spmv_f32(pfm_ldu h, const float *diag, const float *packed, const float 
   PFM_NONNULL(x);
   PFM_NONNULL(y);
   if (h->nslots) PFM_NONNULL(packed);
-  k_spmv<float><<<blocks_for_slices(h->nslices), kBlock, 0, (hipStream_t)stream>>>(
-      h->n, h->nslices, h->d_slices, h->d_col, packed, diag, x, y);
-  PFM_LAUNCH_CHECK();
-  return PFM_OK;
+  return launch_spmv<float>(h, diag, packed, x, y, (hipStream_t)stream);
 }
 
 int pfm_spmv_ldu(pfm_ldu h, int transpose, const double *diag, const double *lower,

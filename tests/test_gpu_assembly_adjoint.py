@@ -190,3 +190,83 @@ def test_dirichlet_patches_in_reference_order(gpu_device):
     got1 = gpu(shuf, None)
     assert bits_equal(got1, ref1)
     assert not bits_equal(ref1, ref3)  # the conventions are distinguishable here
+
+
+def test_adjoint_gradient_64_faces_fd_and_reference_semantics(gpu_device):
+    """The differentiation workload at a size the oracle solves in minutes:
+    48^3 jittered mesh (110,592 cells), lognormal(0, 1) face diffusivity as the
+    leaf, J = sum(w * x) with w ~ N(0, 1), forward PCG-DIC to rel 1e-10 and the
+    hand-written adjoint (A^T lambda = w with the same solver, fused dJ/dgamma
+    kernel).  dJ/dgamma on 64 seeded faces against
+    (a) central differences with eps = 1e-6 (absolute perturbation of gamma_f),
+        each perturbed system solved on the device from the unperturbed
+        solution to rel 1e-12 of its own initial residual, and
+    (b) the reference DifferentiableLinearSolve semantics evaluated by the
+        oracle (tests/golden/make_adjoint48.py: x and lambda from the oracle
+        PCG-DIC to rel 1e-10, lambda on the swapped LDU from zero,
+        differentiable/linear_solver.py:99-151, chained through the assembly),
+        plus the gradient's max |.| and 2-norm over every face.
+    Tolerances: (a) 1e-4 of |dJ/dgamma_f| plus 1e-6 of the largest gradient,
+    (b) 1e-6 of the largest gradient (two solves to 1e-10 in different
+    summation orders); a wrong sign, a swapped owner/neighbour or a dropped
+    1/V term is off by O(1)."""
+    import hashlib
+    dev = gpu_device
+    g48 = dict(np.load(G / "adjoint48.npz"))
+    c, d, lo, up = system(48, 0.2, 1.0, 4)
+    h = hashlib.sha1()
+    for a in (d, lo, up, c.b):
+        h.update(np.ascontiguousarray(a, dtype=np.float64).tobytes())
+    assert np.frombuffer(h.digest(), np.uint8).tolist() == g48["digest"].tolist()
+    mesh = FvMesh.from_case(c, dev)
+    gam = torch.from_numpy(c.gamma).to(dev).requires_grad_(True)
+    gamb = torch.from_numpy(c.dir_gamma).to(dev).requires_grad_(True)
+    b = torch.from_numpy(c.b).to(dev)
+    wn = np.random.default_rng(48).standard_normal(c.n_cells)
+    w = torch.from_numpy(wn).to(dev)
+    dd = (torch.from_numpy(c.dir_cells.astype(np.int32)).to(dev),
+          torch.from_numpy(c.dir_mag).to(dev), torch.from_numpy(c.dir_delta).to(dev))
+    solver = PCGSolver(tolerance=1e-300, rel_tol=1e-10, max_iter=50000)
+    x = poisson_solve(gam, gamb, b, solver, mesh, dd)
+    (w * x).sum().backward()
+    assert solver.last_info.converged
+    gg = gam.grad.cpu().numpy()
+    xs = x.detach()
+    scale = float(g48["grad_max"])
+    faces = g48["faces"]
+    err_b = np.abs(gg[faces] - g48["grad"]).max() / scale
+    print(f"adjoint 48^3: forward {solver.last_info.iterations} iterations (oracle "
+          f"{g48['iterations'][0]}); vs the reference-semantics gradient: max error "
+          f"{err_b:.2e} of max |g|", flush=True)
+    assert err_b <= 1e-6, err_b
+    np.testing.assert_allclose(np.abs(gg).max(), scale, rtol=1e-6)
+    np.testing.assert_allclose(np.linalg.norm(gg), float(g48["grad_norm"]), rtol=1e-6)
+    np.testing.assert_allclose(float((w * x).sum().item()), float(g48["J"]), rtol=1e-8)
+    # (a) central differences on the same 64 faces
+    eps = 1e-6
+    m = LduMatrix(c.n_cells, torch.from_numpy(c.owner), torch.from_numpy(c.neighbour), device=dev)
+    fd_solver = PCGSolver(tolerance=1e-300, rel_tol=1e-12, max_iter=50000)
+    worst = 0.0
+    for f in faces:
+        xs_pm = []
+        for sgn in (+1.0, -1.0):
+            gp = c.gamma.copy()
+            gp[f] += sgn * eps
+            _, dp, lp, upp = _dense_system_coeffs(c, gp)
+            m.diag, m.lower, m.upper = (torch.from_numpy(v).to(dev) for v in (dp, lp, upp))
+            xp, _, _ = fd_solver(m, b, xs)
+            assert fd_solver.last_info.converged
+            xs_pm.append(xp)
+        fd = float((w * (xs_pm[0] - xs_pm[1])).sum().item()) / (2 * eps)
+        err = abs(fd - gg[f])
+        worst = max(worst, err / (abs(gg[f]) + 1e-2 * scale))
+        assert err <= 1e-4 * abs(gg[f]) + 1e-6 * scale, (int(f), fd, gg[f])
+    print(f"  vs central differences on 64 faces: worst error {worst:.2e}", flush=True)
+
+
+def _dense_system_coeffs(c, gamma):
+    nfi = len(c.neighbour)
+    d, lo, up = oldu.laplacian(gamma, c.geo.mag_sf[:nfi], c.geo.delta[:nfi], c.geo.cell_volumes,
+                               c.owner, c.neighbour)
+    d = oldu.dirichlet_diag(d, c.dir_gamma, c.dir_mag, c.dir_delta, c.geo.cell_volumes, c.dir_cells)
+    return None, d, lo, up

@@ -108,16 +108,19 @@ def test_full_size_solves_reach_tolerance(gpu_device):
     assert relg.item() < 1e-8, relg.item()
 
 
-def test_full_size_gamg_cycles_bit_exact(gpu_device):
+@pytest.mark.parametrize("coarse", ["reference", "galerkin"])
+def test_full_size_gamg_cycles_bit_exact(gpu_device, coarse):
     """The pair hierarchy at 2.1M cells (20 levels down to a few cells: grid
     launches, the one-workgroup tail, the one-wave levels): level sizes as the
     oracle's pair maps, and x after two Jacobi V-cycles bit for bit as the
-    oracle's V-cycle over the oracle's maps and coarse matrices."""
+    oracle's V-cycle over the oracle's maps and coarse matrices (the
+    reference's, or with the Galerkin coarse diagonal the bench's GAMG uses)."""
     S = _full(gpu_device)
     c, mat = S["case"], S["mat"]
     ncyc = 2
+    print(f"full-size gamg cycles coarse={coarse}", flush=True)
     solver = GAMGSolver(tolerance=1e-300, rel_tol=1e-300, max_iter=ncyc, agglomeration="pair",
-                        coarse="reference", max_levels=20)
+                        coarse=coarse, max_levels=20)
     sizes = solver.hierarchy(mat).level_sizes()
     x, it, _ = solver(mat, mat.source,
                       torch.zeros(c.n_cells, dtype=torch.float64, device=gpu_device))
@@ -128,7 +131,8 @@ def test_full_size_gamg_cycles_bit_exact(gpu_device):
         f2c = np.empty(levels[-1].n, np.int64)
         f2c[prev] = L["cell_map"]
         aggs.append((L["n_cells_coarse"], f2c))
-        levels.append(ogamg.coarse_matrix(levels[-1], L["n_cells_coarse"], f2c)[0])
+        levels.append(ogamg.coarse_matrix(levels[-1], L["n_cells_coarse"], f2c,
+                                          galerkin=coarse == "galerkin")[0])
         prev = L["cell_map"]
     assert [(v.n, len(v.owner)) for v in levels] == [tuple(z) for z in sizes]
     assert len(sizes) >= 15 and sizes[-1][0] <= 8

@@ -108,21 +108,23 @@ def test_gamg_gauss_seidel_bit_exact(gpu_device, n, jit, sig):
     np.testing.assert_allclose(solver.last_info.residual_history, info.residual_history, rtol=1e-12)
 
 
+@pytest.mark.parametrize("coarse", ["reference", "galerkin"])
 @pytest.mark.parametrize("n,jit,sig", [(16, 0.2, 1.0), (6, 0.2, 1.0)])
-def test_gamg_pair_hierarchy_bit_exact(gpu_device, n, jit, sig):
+def test_gamg_pair_hierarchy_bit_exact(gpu_device, n, jit, sig, coarse):
     """Jacobi V-cycles over a deep pair hierarchy (down to a few cells) against
-    the oracle's V-cycle over the oracle's pair maps and coarse matrices,
-    bit-exact after 6 cycles.  At 16^3 one cycle crosses every Jacobi
+    the oracle's V-cycle over the oracle's pair maps and coarse matrices (the
+    reference's, or with the Galerkin coarse diagonal), bit-exact after 6
+    cycles.  At 16^3 one cycle crosses every Jacobi
     execution mode: grid launches (4096, 2048 cells), the one-workgroup tail's
     one-row-per-thread levels (1024 .. 128: registers, x exchanged through
     LDS) and its one-wave levels (64 cells and fewer: registers, cross-lane
     reads); at 6^3 the whole hierarchy is in the tail."""
     c, d, lo, up = system(n, jit, sig)
-    print(f"gamg pair n={n}", flush=True)
+    print(f"gamg pair n={n} coarse={coarse}", flush=True)
     m = _matrix(c.n_cells, c.owner, c.neighbour, d, lo, up, gpu_device)
     ncyc = 6
     solver = GAMGSolver(tolerance=1e-300, rel_tol=1e-300, max_iter=ncyc, agglomeration="pair",
-                        coarse="reference", max_levels=20)
+                        coarse=coarse, max_levels=20)
     sizes = solver.hierarchy(m).level_sizes()
     x, it, _ = solver(m, torch.from_numpy(c.b), torch.zeros(c.n_cells, dtype=torch.float64))
     lv = ogamg.pair_levels(c.n_cells, c.owner, c.neighbour, n_levels=len(sizes) - 1)
@@ -132,7 +134,8 @@ def test_gamg_pair_hierarchy_bit_exact(gpu_device, n, jit, sig):
         f2c = np.empty(levels[-1].n, np.int64)
         f2c[prev] = L["cell_map"]
         aggs.append((L["n_cells_coarse"], f2c))
-        levels.append(ogamg.coarse_matrix(levels[-1], L["n_cells_coarse"], f2c)[0])
+        levels.append(ogamg.coarse_matrix(levels[-1], L["n_cells_coarse"], f2c,
+                                          galerkin=coarse == "galerkin")[0])
         prev = L["cell_map"]
     assert [(v.n, len(v.owner)) for v in levels] == [tuple(s) for s in sizes]
     assert min(sizes)[0] <= 8 and len(sizes) >= 4
@@ -285,6 +288,51 @@ def test_pcg_gamg_preconditioner_matches_oracle(gpu_device, n, jit, sig):
     nx = np.linalg.norm(xo)
     spread = max(np.linalg.norm(r[0] - xo) for r in runs.values()) / nx
     assert np.linalg.norm(xg - xo) / nx <= max(10 * spread, 1e-9), spread
+
+
+def _oracle_pair_hierarchy(c, d, lo, up, n_coarse_levels, galerkin):
+    lv = ogamg.pair_levels(c.n_cells, c.owner, c.neighbour, n_levels=n_coarse_levels)
+    levels = [ogamg.Level(c.n_cells, d, lo, up, c.owner, c.neighbour)]
+    aggs, prev = [], np.arange(c.n_cells)
+    for L in lv:
+        f2c = np.empty(levels[-1].n, np.int64)
+        f2c[prev] = L["cell_map"]
+        aggs.append((L["n_cells_coarse"], f2c))
+        levels.append(ogamg.coarse_matrix(levels[-1], L["n_cells_coarse"], f2c,
+                                          galerkin=galerkin)[0])
+        prev = L["cell_map"]
+    return levels, aggs
+
+
+@pytest.mark.parametrize("n", [16, 24])
+def test_pcg_gamg_pair_galerkin_matches_oracle(gpu_device, n):
+    """The bench's PCG-GAMG: PCG preconditioned by one V-cycle over the pair
+    hierarchy with the Galerkin coarse diagonal, against the oracle PCG over
+    the oracle's V-cycle on the oracle's pair maps and Galerkin coarse
+    matrices, summing in the device's order: iterations, history and x bit
+    for bit."""
+    from oracle import krylov as okry
+    from oracle import ldu as oldu
+    from pyopenfoam_amd import PCGSolver
+    c, d, lo, up = system(n, 0.2, 1.0)
+    print(f"pcg-gamg pair/galerkin n={n}", flush=True)
+    m = _matrix(c.n_cells, c.owner, c.neighbour, d, lo, up, gpu_device)
+    opts = dict(agglomeration="pair", coarse="galerkin", max_levels=20)
+    solver = PCGSolver(tolerance=1e-300, rel_tol=1e-8, max_iter=500, preconditioner="GAMG",
+                       gamg_options=opts)
+    x, it, _ = solver(m, torch.from_numpy(c.b), torch.zeros(c.n_cells, dtype=torch.float64))
+    sizes = GAMGSolver(**opts).hierarchy(m).level_sizes()
+    levels, aggs = _oracle_pair_hierarchy(c, d, lo, up, len(sizes) - 1, True)
+    assert [(v.n, len(v.owner)) for v in levels] == [tuple(z) for z in sizes]
+    pre = lambda r: ogamg.v_cycle(levels, aggs, 0, np.zeros_like(r), r, 2, 2)
+    mv = lambda v: oldu.ax(d, lo, up, c.owner, c.neighbour, v)
+    xd, idev, _, infod = okry.pcg(mv, pre, c.b, np.zeros(c.n_cells), tolerance=1e-300,
+                                  rel_tol=1e-8, max_iter=500,
+                                  dot=okry.dot_device(c.n_cells, layout="contiguous"))
+    assert infod.converged and solver.last_info.converged
+    assert it == idev, (it, idev)
+    assert bits_equal(np.array(solver.last_info.residual_history), infod.residual_history)
+    assert bits_equal(x.cpu().numpy(), xd)
 
 
 def test_pcg_gamg_mixed_converges(gpu_device):

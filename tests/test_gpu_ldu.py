@@ -166,3 +166,97 @@ def test_strided_coefficients_match_contiguous(gpu_device):
         xs, it, _ = solver(mat, b, z)
         out.append((xs.cpu().numpy(), it))
     assert out[0][1] == out[1][1] and bits_equal(out[0][0], out[1][0])
+
+
+def test_bandwidth_probe_reads_and_copies(gpu_device):
+    """pfm_bandwidth_probe: the read mode's per-wave partials add up to the
+    buffer's sum (every element read once), the copy mode copies; sizes that
+    are not multiples of 16 bytes are rejected before any launch."""
+    from pyopenfoam_amd import _lib
+    from pyopenfoam_amd.core.device import current_stream
+    L = _lib.lib()
+    n = (1 << 20) + 6  # doubles: 16-byte multiple, not a multiple of the grid
+    a = torch.arange(n, dtype=torch.float64, device=gpu_device)
+    out = torch.zeros(int(L.pfm_bandwidth_probe_out_bytes()) // 8, dtype=torch.float64,
+                      device=gpu_device)
+    s = current_stream(gpu_device)
+    _lib.check(L.pfm_bandwidth_probe(0, _lib.ptr(a), _lib.ptr(out), 8 * n, s), "read")
+    assert out.sum().item() == n * (n - 1) / 2  # integers below 2^53: exact
+    b = torch.empty_like(a)
+    _lib.check(L.pfm_bandwidth_probe(1, _lib.ptr(a), _lib.ptr(b), 8 * n, s), "copy")
+    assert torch.equal(a, b)
+    assert L.pfm_bandwidth_probe(0, _lib.ptr(a), _lib.ptr(out), 8 * n - 8, s) == _lib.PFM_ERR_ARG
+
+
+@pytest.mark.parametrize("n,jit,sig", [(8, 0.2, 1.0), (16, 0.2, 1.0)])
+def test_batched_and_sparse_products(gpu_device, n, jit, sig):
+    """Ax_batched / Ax_sparse (ldu_matrix.py:345-381): every column of A X for
+    k right-hand sides bit-identical to Ax of that column (and so to the
+    oracle's scatter-order product); a 1-D input is Ax itself.  The sparse
+    forms (to_sparse_coo / _csr / _csr_cached, :278-333) hold exactly the
+    dense matrix's entries; the CSR cache is reused until a coefficient
+    setter invalidates it."""
+    c, d, lo, up = system(n, jit, sig)
+    m = _matrix(c, d, lo, up, gpu_device)
+    X = np.random.default_rng(7).standard_normal((c.n_cells, 5))
+    Y = m.Ax_batched(torch.from_numpy(X)).cpu().numpy()
+    assert Y.shape == X.shape
+    for j in range(5):
+        ref = oldu.ax(d, lo, up, c.owner, c.neighbour, X[:, j])
+        assert bits_equal(Y[:, j], ref), j
+        assert bits_equal(Y[:, j], m.Ax(torch.from_numpy(X[:, j].copy())).cpu().numpy())
+    x1 = torch.from_numpy(X[:, 2].copy())
+    assert bits_equal(m.Ax_sparse(x1).cpu().numpy(), Y[:, 2])
+    with pytest.raises(ValueError):
+        m.Ax_batched(torch.zeros(c.n_cells + 1, 2, dtype=torch.float64))
+    with pytest.raises(TypeError):
+        m.Ax_batched(torch.zeros(c.n_cells, 2, dtype=torch.int64))
+    dense = oldu.dense(d, lo, up, c.owner, c.neighbour)
+    coo = m.to_sparse_coo()
+    assert coo.is_coalesced() and coo.shape == (c.n_cells, c.n_cells)
+    assert np.array_equal(coo.to_dense().cpu().numpy(), dense)
+    csr = m.to_sparse_csr_cached()
+    assert m.to_sparse_csr_cached() is csr
+    assert np.array_equal(csr.to_dense().cpu().numpy(), dense)
+    m.diag = torch.from_numpy(d * 2.0)
+    assert m.to_sparse_csr_cached() is not csr
+
+
+def test_fv_matrix_relax_sources_and_boundary_hooks(gpu_device):
+    """FvMatrix.relax (fv_matrix.py:192-235): diag / alpha and
+    source + (1 - alpha) * diag' * field_old, bit for bit as those ops; alpha
+    = 1 changes nothing; alpha outside (0, 1] raises ValueError.
+    add_explicit_source adds a tensor or a scalar; add_boundary_contribution
+    takes the new diagonal and source from bc.matrix_contributions."""
+    from pyopenfoam_amd.core.fv_matrix import FvMatrix
+    c, d, lo, up = system(8, 0.2, 1.0)
+    dev = gpu_device
+    m = FvMatrix(c.n_cells, torch.from_numpy(c.owner), torch.from_numpy(c.neighbour), device=dev)
+    m.diag, m.lower, m.upper = (torch.from_numpy(v) for v in (d, lo, up))
+    m.source = torch.from_numpy(c.b)
+    phi = np.random.default_rng(3).standard_normal(c.n_cells)
+    m.relax(torch.from_numpy(phi), 1.0)
+    assert m.relaxation_factor == 1.0 and bits_equal(m.diag.cpu().numpy(), d)
+    a = 0.7
+    m.relax(torch.from_numpy(phi), a)
+    dn = d / a
+    assert m.relaxation_factor == a
+    assert bits_equal(m.diag.cpu().numpy(), dn)
+    assert bits_equal(m.source.cpu().numpy(), c.b + ((1.0 - a) * dn) * phi)
+    for bad in (0.0, -0.5, 1.5):
+        with pytest.raises(ValueError):
+            m.relax(torch.from_numpy(phi), bad)
+    s0 = m.source.cpu().numpy()
+    m.add_explicit_source(torch.from_numpy(phi))
+    m.add_explicit_source(torch.tensor(2.0, dtype=torch.float64))
+    assert bits_equal(m.source.cpu().numpy(), (s0 + phi) + 2.0)
+
+    class PenaltyBC:  # a boundary condition object as the reference's BCs expose it
+        def matrix_contributions(self, field, n_cells, diag, source):
+            assert field.shape[0] == n_cells and not field.any()
+            return diag + 1.0, source - 1.0
+
+    d0, s1 = m.diag.cpu().numpy(), m.source.cpu().numpy()
+    m.add_boundary_contribution(PenaltyBC())
+    assert bits_equal(m.diag.cpu().numpy(), d0 + 1.0)
+    assert bits_equal(m.source.cpu().numpy(), s1 - 1.0)

@@ -593,3 +593,75 @@ def test_pressure_oracle_adjust_phi_pins():
     assert opr.adjust_phi(phi2, 30)[0] is False
     assert opr.adjust_phi(phi, 30, closed=False)[0] is False
     assert opr.adjust_phi(phi[:30], 30)[0] is False
+
+
+@pytest.mark.parametrize("n,jit,sig,agglom", [(6, 0.0, 0.0, "pair"), (6, 0.2, 1.0, "pair"),
+                                               (7, 0.2, 1.0, "greedy"), (5, 0.0, 1.0, "greedy")])
+def test_galerkin_coarse_matrix_is_ptap(n, jit, sig, agglom):
+    """``coarse_matrix(galerkin=True)`` against the closed form P^T A P of the
+    piecewise-constant prolongation P[i, f2c[i]] = 1: its diagonal on every
+    system, and the whole matrix on systems with lower = upper (unjittered
+    cubes, where the reference's first-seen face orientation cannot matter).
+    The reference coarse diagonal (intra-aggregate couplings dropped) misses
+    P^T A P wherever an aggregate holds a face, so a dropped term, a sign or
+    lower-only sum fails here."""
+    from tests.helpers import system
+    c, d, lo, up = system(n, jit, sig)
+    lv0 = ogamg.Level(c.n_cells, d, lo, up, c.owner, c.neighbour)
+    if agglom == "pair":
+        L = ogamg.pair_levels(c.n_cells, c.owner, c.neighbour, n_levels=1)[0]
+        nagg, f2c = L["n_cells_coarse"], L["cell_map"].astype(np.int64)
+    else:
+        nagg, f2c = ogamg.aggregate(c.n_cells, c.owner, c.neighbour, max(c.n_cells // 2, 10))
+    A = oldu.dense(d, lo, up, c.owner, c.neighbour)
+    P = np.zeros((c.n_cells, nagg))
+    P[np.arange(c.n_cells), f2c] = 1.0
+    Ac = P.T @ A @ P
+    gal, fmap = ogamg.coarse_matrix(lv0, nagg, f2c, galerkin=True)
+    ref, _ = ogamg.coarse_matrix(lv0, nagg, f2c)
+    scale = np.abs(Ac).max()
+    np.testing.assert_allclose(gal.diag, np.diag(Ac), rtol=1e-12, atol=1e-13 * scale)
+    assert (fmap < 0).any()
+    assert np.abs(ref.diag - np.diag(Ac)).max() > 1e-6 * scale
+    if jit == 0.0:
+        np.testing.assert_allclose(lo, up, rtol=1e-14)  # volumes equal up to rounding
+        np.testing.assert_allclose(oldu.dense(gal.diag, gal.lower, gal.upper, gal.owner,
+                                              gal.neighbour), Ac, rtol=1e-12, atol=1e-13 * scale)
+    # off-diagonal couplings are the reference's in both forms
+    assert bits_equal(gal.lower, ref.lower) and bits_equal(gal.upper, ref.upper)
+
+
+def test_torch_cpu_pcg_matches_numpy_oracle():
+    """The torch-CPU restatement (the benchmark's CPU baseline) against the
+    golden-pinned numpy oracle: A x equal to rounding (scatter order); on a
+    symmetric system (unjittered: every summation order takes the same
+    iterations) the same iteration count within one and the solution within
+    1e-9; on a jittered (non-symmetric) one, where CG's iteration count
+    depends on the summation order itself, convergence and a true residual at
+    the tolerance; a planted sign error in the restatement's product fails."""
+    import torch
+    from oracle import torch_ref
+    from tests.helpers import system
+    for jit in (0.0, 0.2):
+        c, d, lo, up = system(10, jit, 1.0)
+        A = torch_ref.LduCpu(d, lo, up, c.owner, c.neighbour)
+        x = np.random.default_rng(2).standard_normal(c.n_cells)
+        mv = lambda v: oldu.ax(d, lo, up, c.owner, c.neighbour, v)
+        np.testing.assert_allclose(A.ax(torch.from_numpy(x)).numpy(), mv(x), rtol=1e-13,
+                                   atol=1e-12)
+        rD = opre.dic_build(d, up, c.owner, c.neighbour)
+        xt, it, _, conv, hist = torch_ref.pcg(A, c.b, np.zeros(c.n_cells), rD, tolerance=1e-300,
+                                              rel_tol=1e-8, max_iter=5000)
+        runs = [okry.pcg(mv, lambda r: r / rD, c.b, np.zeros(c.n_cells), tolerance=1e-300,
+                         rel_tol=1e-8, max_iter=5000, dot=dot) for dot in okry.DOTS.values()]
+        its = [r[1] for r in runs]
+        assert conv and all(r[3].converged for r in runs)
+        np.testing.assert_allclose(hist[:5], runs[0][3].residual_history[:5], rtol=1e-10)
+        xo = runs[0][0]
+        if jit == 0.0:
+            assert max(its) == min(its) and abs(it - its[0]) <= 1, (it, its)
+            assert np.linalg.norm(xt.numpy() - xo) <= 1e-9 * np.linalg.norm(xo)
+        else:
+            assert np.linalg.norm(c.b - mv(xt.numpy())) <= 1.1e-8 * np.linalg.norm(c.b)
+        bad = torch_ref.LduCpu(d, -lo, up, c.owner, c.neighbour)
+        assert not np.allclose(bad.ax(torch.from_numpy(x)).numpy(), mv(x))
