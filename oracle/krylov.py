@@ -52,7 +52,8 @@ def dot_pairwise(a, b):
 DOTS = {"blas": dot_blas, "exact": dot_exact, "pairwise": dot_pairwise}
 
 
-def dot_device(n, *, layout="strided", max_red_blocks=2048, block=256, slice_rows=64, xcds=8):
+def dot_device(n, *, layout="strided", max_red_blocks=2048, block=256, slice_rows=64, xcds=8,
+               tile_slices=None):
     """The summation order of the device solvers' fused dot products, so that
     an oracle solve can be compared with a device solve bit for bit.  This is
     one more summation order (the reference leaves the order to torch), set
@@ -70,14 +71,32 @@ def dot_device(n, *, layout="strided", max_red_blocks=2048, block=256, slice_row
       lane 0's value taken; the 4 waves are added in order (partial of b);
     * the G partials: thread t sums part[t + 256 k], k = 0..7, from 0, then
       the same wave butterfly and in-order wave sum over 256 threads.
+
+    ``tile_slices=T`` gives the order of the batched solver instead (one
+    system of a batch): tiles of T consecutive slices in their natural order,
+    ceil(n_slices / T) of them (no XCD remap), each tile's four waves walking
+    slices tile*T + w + 4k as ``"contiguous"`` does, then the same sum of the
+    tile partials.
     """
     waves = block // slice_rows
     nsl = -(-n // slice_rows)
-    G = max(1, min(-(-nsl // waves), max_red_blocks))
-    per = G // xcds
-    b = np.arange(G)
-    lb = np.where(b < per * xcds, (b % xcds) * per + b // xcds, b)
-    if layout == "strided":
+    if tile_slices is not None:
+        G = max(1, -(-nsl // tile_slices))
+        b = np.arange(G)
+        lb = b
+        layout = "tiles"
+    else:
+        G = max(1, min(-(-nsl // waves), max_red_blocks))
+        per = G // xcds
+        b = np.arange(G)
+        lb = np.where(b < per * xcds, (b % xcds) * per + b // xcds, b)
+    if layout == "tiles":
+        per = tile_slices
+        steps = max(1, -(-per // waves))
+        s = (lb[:, None, None] * per + np.arange(waves)[None, :, None]
+             + np.arange(steps)[None, None, :] * waves)
+        end = np.minimum((lb[:, None, None] + 1) * per, nsl) + 0 * s
+    elif layout == "strided":
         steps = max(1, -(-nsl // (G * waves)))
         s = (lb[:, None, None] * waves + np.arange(waves)[None, :, None]
              + np.arange(steps)[None, None, :] * (G * waves))    # (G, waves, steps)
@@ -94,7 +113,7 @@ def dot_device(n, *, layout="strided", max_red_blocks=2048, block=256, slice_row
     valid = (s[:, :, None, :] < end[:, :, None, :]) & (rows < n)
     idx = np.where(valid, rows, n)                                 # n -> a 0.0 pad
     lanes = np.arange(slice_rows)
-    kthreads = max_red_blocks // block
+    kthreads = max(max_red_blocks // block, -(-G // block))  # tiles: any count
     tl = np.arange(block)
 
     def tree(v):  # (..., 64) -> (...,): the wave butterfly, lane 0

@@ -193,13 +193,19 @@ __global__ __launch_bounds__(kBlock) void k_b_init(const int32_t *__restrict__ t
 }
 
 // One thread per system: initial residual, monitor(0), rz.
-__global__ void k_b_init_fin(int ns, const int32_t *__restrict__ sys_t0, int ntiles,
-                             const double *__restrict__ part, BGlobal *g, BSys *sys) {
-  const int k = blockIdx.x * blockDim.x + threadIdx.x;
+// One workgroup per system: the tile partials are added in the order of the
+// iteration kernels (sys_sum), so every dot product of a system's solve has
+// one summation order.
+__global__ __launch_bounds__(kBlock) void k_b_init_fin(int ns, const int32_t *__restrict__ sys_t0,
+                                                        int ntiles,
+                                                        const double *__restrict__ part,
+                                                        BGlobal *g, BSys *sys) {
+  __shared__ double lds[kMaxWaves];
+  const int k = blockIdx.x;
   if (k >= ns) return;
-  double RR = 0.0, RZ = 0.0;
-  for (int t = sys_t0[k]; t < sys_t0[k + 1]; ++t) RR += part[t];
-  for (int t = sys_t0[k]; t < sys_t0[k + 1]; ++t) RZ += part[ntiles + t];
+  const double RR = sys_sum(part, sys_t0[k], sys_t0[k + 1], lds);
+  const double RZ = sys_sum(part + ntiles, sys_t0[k], sys_t0[k + 1], lds);
+  if (threadIdx.x != 0) return;
   BSys S = sys[k];
   const double res = sqrt(RR);
   S.res0 = res;
@@ -776,7 +782,7 @@ int pfm_batch_pcg_solve(pfm_batch B, const pfm_solve_params *prm, int transpose,
                                                    prm->max_iter, prm->min_iter);
   k_b_init<<<T, kBlock, 0, s>>>(B->d_tile_sys, B->d_tile_s0, B->d_tile_s1, T, L->d_slices, L->d_col,
                                 w.packed, w.dp, pd, w.bp, w.xp, w.r, w.p, w.part);
-  k_b_init_fin<<<(ns + 255) / 256, 256, 0, s>>>(ns, B->d_sys_t0, T, w.part, w.g, w.sys);
+  k_b_init_fin<<<ns, kBlock, 0, s>>>(ns, B->d_sys_t0, T, w.part, w.g, w.sys);
   PFM_CHECK_HIP(hipGetLastError());
   auto iteration = [&](int j, hipStream_t q) { launch_iteration(B, w, pd, j, q); };
   const int chunk = std::max(1, graph_chunk);

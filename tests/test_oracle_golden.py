@@ -413,7 +413,7 @@ def test_diagonal_solver_oracle_known_answers():
     assert np.isfinite(x).all() and x[1] == 1.0 and x[0] == 1.0 / 1e-30
 
 
-@pytest.mark.parametrize("layout", ["strided", "contiguous"])
+@pytest.mark.parametrize("layout", ["strided", "contiguous", "tiles1", "tiles5", "tiles32"])
 @pytest.mark.parametrize("n", [1, 63, 64, 65, 257, 6161, 131_073, 600_001])
 def test_dot_device_sums_every_product_once(n, layout):
     """The device-order dot (the device solvers' reduction tree: grid, XCD
@@ -424,7 +424,10 @@ def test_dot_device_sums_every_product_once(n, layout):
     grid, ragged last slices, a grid that is not a multiple of 8 (remap tail)
     and the 2048-workgroup cap with several slices per wave."""
     rng = np.random.default_rng(n)
-    dot = okry.dot_device(n, layout=layout)
+    if layout.startswith("tiles"):  # the batched solver's order, tiles of T slices
+        dot = okry.dot_device(n, tile_slices=int(layout[5:]))
+    else:
+        dot = okry.dot_device(n, layout=layout)
     a = rng.integers(-1000, 1000, n)
     c = rng.integers(-1000, 1000, n)
     assert dot(a.astype(float), c.astype(float)) == float(np.dot(a, c))
