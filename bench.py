@@ -571,6 +571,33 @@ def bench_family(mat, dev):
     return out
 
 
+def gs_launches_per_cycle(hier, solver) -> int:
+    """Gauss-Seidel level launches of one V-cycle: levels above 1024 cells
+    run one launch per dependency level of a symmetric sweep (forward and
+    backward), n_pre + n_post sweeps per level (the smaller levels run inside
+    the one-workgroup tail)."""
+    import ctypes as C
+
+    import numpy as np
+
+    from pyopenfoam_amd import _lib
+
+    L = _lib.lib()
+    sweeps = solver._n_pre_smooth + solver._n_post_smooth
+    total = 0
+    for k, (n, nf) in enumerate(hier.level_sizes()):
+        if n <= 1024:
+            continue
+        own, nei = hier.level_topology(k)
+        o, q = _lib.host_i32(own), _lib.host_i32(nei)
+        fl, bl = np.empty(n, np.int32), np.empty(n, np.int32)
+        nfl, nbl = C.c_int32(), C.c_int32()
+        _lib.check(L.pfm_gs_levels(n, int(o.shape[0]), _lib.ptr(o), _lib.ptr(q), _lib.ptr(fl),
+                                   _lib.ptr(bl), C.byref(nfl), C.byref(nbl)), "pfm_gs_levels")
+        total += sweeps * (nfl.value + nbl.value)
+    return total
+
+
 def bench_precision(args, dev):
     """the 256^3 jittered mesh with lognormal(0, 1.5) diffusivity.
     FP64 pair/Galerkin GAMG and its mixed-precision form (FP32 V-cycles on the
@@ -876,6 +903,9 @@ def main() -> int:
                          ("pair_galerkin_jacobi_mixed", dict(agglomeration="pair", coarse="galerkin",
                                                               max_levels=20, precision="mixed",
                                                               inner_rel_tol=0.1)),
+                         ("pair_galerkin_gauss_seidel", dict(agglomeration="pair",
+                                                             coarse="galerkin", max_levels=20,
+                                                             level_smoother="GaussSeidel")),
                          ("reference", dict()),
                          ("pcg_gamg_precond", dict(pcg=True)),
                          ("pcg_gamg_precond_mixed", dict(pcg=True, precision="mixed"))):
@@ -890,6 +920,8 @@ def main() -> int:
                 gs = GAMGSolver(tolerance=1e-300, rel_tol=1e-8, max_iter=args.gamg_max_iter, **kw)
                 hier = gs.hierarchy(mat)
             mi = args.gamg_max_iter if name != "reference" else 100
+            if kw.get("level_smoother"):
+                mi = min(mi, 500)  # ~110 ms per cycle: one launch per Gauss-Seidel level
             mat.solve(gs, x0, 1e-300, 2)  # warm (setup of coarse handles and workspace)
             torch.cuda.synchronize()
             t0 = time.perf_counter()
@@ -914,6 +946,10 @@ def main() -> int:
                     extra["rel_err_vs_fp64_solution"] = ((xg - x_pcg64).norm() / x_pcg64.norm()).item()
             elif kw.get("pcg"):
                 x_pcg64 = xg
+            if kw.get("level_smoother"):
+                extra["gs_level_launches_per_cycle"] = gs_launches_per_cycle(hier, gs)
+                extra["us_per_gs_launch"] = round(gms / cyc * 1e3 / extra["gs_level_launches_per_cycle"],
+                                                  2)
             gamg[name] = {"ms": round(gms, 2), "cycles": int(cyc), "ms_per_cycle": round(gms / cyc, 3),
                           "converged": bool(gs.last_info.converged), "true_rel_residual": rt,
                           "levels": hier.n_levels, **extra, **{k: v for k, v in kw.items()}}

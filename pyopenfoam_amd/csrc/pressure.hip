@@ -114,12 +114,15 @@ __global__ __launch_bounds__(kBlock) void k_cff(int nf_all, int nf,
 
 // Gauss gradient with linearly interpolated face pressure (boundary faces
 // take the owner value), one running sum per cell in index_add_ order; then
-// U = HbyA - (alpha/A_p) grad p, limited to |U| <= 1e4 (:228-265).
+// U = HbyA - (alpha/A_p) grad p, limited to |U| <= 1e4 (:228-265).  A slot of
+// row r holds a face of r and the face's other cell in its column: on the
+// owner side p[own] = p[r] and p[nei] = p[col], on the neighbour side the
+// reverse, so the face's two pressures come from the row and the slot column
+// (no face-indexed owner/neighbour gathers).
 __global__ __launch_bounds__(kBlock) void k_cvel(int n, int nf, int nslices,
                                                   const SliceInfo *__restrict__ sl,
                                                   const int32_t *__restrict__ map,
-                                                  const int32_t *__restrict__ own,
-                                                  const int32_t *__restrict__ nei,
+                                                  const int32_t *__restrict__ col,
                                                   const double *__restrict__ w,
                                                   const double *__restrict__ p,
                                                   const double *__restrict__ sf,
@@ -136,11 +139,14 @@ __global__ __launch_bounds__(kBlock) void k_cvel(int n, int nf, int nslices,
   const int row = s * kSlice + lane;
   double gx = 0.0, gy = 0.0, gz = 0.0;
   int base = si.ptr + lane;
+  const double prow = row < n ? p[row] : 0.0;
   for (int j = 0; j < si.wl + si.wu; ++j, base += kSlice) {
     const int f = map[base];
     if (f < 0) continue;
     const double wf = w[f];
-    const double pf = wf * p[own[f]] + (1.0 - wf) * p[nei[f]];
+    const double pc = p[col[base]];
+    const double p_own = j < si.wl ? prow : pc, p_nei = j < si.wl ? pc : prow;
+    const double pf = wf * p_own + (1.0 - wf) * p_nei;
     const double cx = pf * sf[3 * f], cy = pf * sf[3 * f + 1], cz = pf * sf[3 * f + 2];
     if (j < si.wl) {  // owner side: + p_f S_f
       gx += cx;
@@ -153,7 +159,7 @@ __global__ __launch_bounds__(kBlock) void k_cvel(int n, int nf, int nslices,
     }
   }
   if (row >= n) return;
-  const double pp = p[row];
+  const double pp = prow;
   for (int k = bptr[row]; k < bptr[row + 1]; ++k) {
     const int64_t b = (int64_t)nf + bface[k];
     gx += pp * sf[3 * b];
@@ -291,7 +297,7 @@ int pfm_correct_velocity(pfm_ldu h, int32_t n_faces, const int32_t *bnd_ptr,
   if (h->nf > 0) PFM_NONNULL(weights);
   hipStream_t s = (hipStream_t)stream;
   k_cvel<<<blocks_for_slices(h->nslices), kBlock, 0, s>>>(
-      h->n, h->nf, h->nslices, h->d_slices, h->d_map, h->d_own, h->d_nei, weights, p,
+      h->n, h->nf, h->nslices, h->d_slices, h->d_map, h->d_col, weights, p,
       face_areas, bnd_ptr, bnd_face, vol, a_p, hbya, alpha, u_out);
   PFM_CHECK_HIP(hipGetLastError());
   return PFM_OK;

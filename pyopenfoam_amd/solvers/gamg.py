@@ -93,6 +93,14 @@ class GamgHierarchy:
                                                   co.ctypes.data, cn.ctypes.data))
         return f2c, fmap, co, cn
 
+    def level_topology(self, level: int) -> tuple[np.ndarray, np.ndarray]:
+        """(owner, neighbour) of a level: the fine handle's on level 0, the
+        coarse faces built from the previous level's maps above it."""
+        if level == 0:
+            return self._addr.owner_host, self._addr.neighbour_host
+        _, _, co, cn = self.level_maps(level - 1)
+        return co, cn
+
     def __del__(self):
         h = getattr(self, "_h", None)
         if h is not None and h.value:
