@@ -1,0 +1,148 @@
+"""The batching workload at its full size: 512 independent Poisson systems,
+cube side n ~ U{8..48} (seed 0; 512 to 110,592 cells, 18.7M cells in all),
+each jittered, permuted, RCM-renumbered, lognormal(0, 1) diffusivity,
+zero-mean uniform RHS, solved in one batched PCG-DIC (32-slice tiles,
+graph-replayed chunks) to rel 1e-8 with max_iter 30,000.
+
+Pinned system by system against fixtures the oracle wrote in the build
+container (tests/golden/make_batch_outcomes.py,
+tests/golden/make_batch_device_order.py):
+* the system is the one the fixtures solved (SHA-1 of its diag, lower, upper
+  and b as the device assembled them);
+* bit for bit against the oracle PCG (solvers/pcg.py:97-187) summing in the
+  batched solver's order: iterations, decision and every bit of x;
+* against the reference's own summation orders (BLAS, pairwise, and the
+  exactly rounded one where either fails): the same decision wherever they
+  agree, and iterations within one of their spread on every system they all
+  converge on -- including the systems that do not converge in 30,000
+  iterations, which fail under every order: CG on the per-volume Laplacian of
+  a jittered mesh is CG on a non-symmetric matrix, the reference's own
+  algorithm failing, not the batch;
+* the batched adjoint of J = sum(w * x) against dense solves and the chain
+  rule on the systems of at most 1,000 cells (rel 1e-6)."""
+
+import hashlib
+from pathlib import Path
+
+import numpy as np
+import pytest
+import torch
+
+from oracle import adjoint as oadj
+from oracle import ldu as oldu
+from pyopenfoam_amd.cases import poisson_batch
+from pyopenfoam_amd.discretisation.fvm import LaplacianAssemble
+from pyopenfoam_amd.differentiable.linear_solver import _Ctx
+from pyopenfoam_amd.solvers.batch import BatchedLdu, BatchedPCGSolver, batch_poisson_solve
+
+pytestmark = pytest.mark.gpu
+G = Path(__file__).resolve().parent / "golden"
+
+
+def _sha1(*arrays):
+    h = hashlib.sha1()
+    for a in arrays:
+        h.update(np.ascontiguousarray(a, dtype=np.float64).tobytes())
+    return np.frombuffer(h.digest(), np.uint8)
+
+
+@pytest.fixture(scope="module")
+def batch512(gpu_device):
+    dev = gpu_device
+    print("building the 512-system batch", flush=True)
+    bc = poisson_batch(512, 8, 48, seed=0, jitter=0.2, sigma=1.0, workers=16)
+    t = lambda a: torch.from_numpy(np.ascontiguousarray(a)).to(dev)
+    B = BatchedLdu(bc.cell_offsets, bc.face_offsets, bc.owner_local, bc.neighbour_local, dev,
+                   tile_slices=32, global_owner=bc.owner, global_neighbour=bc.neighbour)
+    geo = (t(bc.mag), t(bc.delta), t(bc.vol), t(bc.dir_cells.astype(np.int32)), t(bc.dir_mag),
+           t(bc.dir_delta))
+    gam, gamb, b = t(bc.gamma), t(bc.dir_gamma), t(bc.b)
+    d, lo, up = LaplacianAssemble.forward(_Ctx(), gam, gamb, B.global_addressing, *geo)
+    torch.cuda.synchronize()
+    return dict(bc=bc, B=B, geo=geo, gam=gam, gamb=gamb, b=b, d=d, lo=lo, up=up)
+
+
+@pytest.fixture(scope="module")
+def solved512(batch512):
+    S = batch512
+    bc, B = S["bc"], S["B"]
+    co, fo = bc.cell_offsets, bc.face_offsets
+    d, lo, up = (S[k].cpu().numpy() for k in ("d", "lo", "up"))
+    digests = np.stack([_sha1(d[co[k]:co[k + 1]], lo[fo[k]:fo[k + 1]], up[fo[k]:fo[k + 1]],
+                              bc.b[co[k]:co[k + 1]]) for k in range(bc.n_systems)])
+    solver = BatchedPCGSolver(tolerance=1e-300, rel_tol=1e-8, max_iter=30000)
+    x, res = solver.solve(B, S["d"], S["lo"], S["up"], S["b"])
+    x = x.cpu().numpy()
+    its, conv = np.asarray(res.iterations), np.asarray(res.converged)
+    print(f"batch512: converged {int(conv.sum())}/{bc.n_systems}, iterations max {its.max()}",
+          flush=True)
+    return dict(digests=digests, x=x, its=its, conv=conv)
+
+
+def test_batch512_bit_exact_in_device_order(gpu_device, batch512, solved512):
+    bc, B, R = batch512["bc"], batch512["B"], solved512
+    fd_ = dict(np.load(G / "batch512_device_order.npz"))
+    assert np.array_equal(fd_["sizes"], bc.sizes) and int(fd_["tile_slices"]) == B.tile_slices
+    assert np.array_equal(R["digests"], fd_["digest"])  # the systems the oracle solved
+    co, its, conv = bc.cell_offsets, R["its"], R["conv"]
+    for k in range(bc.n_systems):
+        assert its[k] == fd_["iterations"][k] and conv[k] == fd_["converged"][k], \
+            (k, its[k], fd_["iterations"][k])
+        assert np.array_equal(_sha1(R["x"][co[k]:co[k + 1]]), fd_["x_digest"][k]), k
+    print(f"  bit-exact on all {bc.n_systems} systems; not converged: "
+          f"{np.nonzero(~conv)[0].tolist()}", flush=True)
+
+
+def test_batch512_within_reference_orders(gpu_device, batch512, solved512):
+    bc, R = batch512["bc"], solved512
+    if not (G / "batch512_outcomes.npz").exists():
+        pytest.skip("tests/golden/batch512_outcomes.npz not written yet "
+                    "(python tests/golden/make_batch_outcomes.py)")
+    fo_ = dict(np.load(G / "batch512_outcomes.npz"))
+    assert np.array_equal(fo_["sizes"], bc.sizes)
+    assert np.array_equal(R["digests"], fo_["digest"])
+    its, conv = R["its"], R["conv"]
+    run = fo_["iterations"] >= 0
+    all_conv = np.where(run, fo_["converged"], True).all(axis=1)
+    none_conv = ~np.where(run, fo_["converged"], False).any(axis=1)
+    assert np.array_equal(conv[all_conv], np.ones(all_conv.sum(), bool))
+    assert not conv[none_conv].any()
+    failing = np.nonzero(~conv)[0]
+    print(f"  not converged in the batch: {failing.tolist()}; under every oracle order: "
+          f"{np.nonzero(none_conv)[0].tolist()}", flush=True)
+    assert set(failing) <= set(np.nonzero(~all_conv)[0])
+    lo_it = np.where(run, fo_["iterations"], np.iinfo(np.int32).max).min(axis=1)
+    hi_it = np.where(run, fo_["iterations"], -1).max(axis=1)
+    inside = (lo_it - 1 <= its) & (its <= hi_it + 1)
+    outside = np.nonzero(all_conv & ~inside)[0]
+    print(f"  iterations within one of the reference orders' spread on "
+          f"{int((all_conv & inside).sum())}/{int(all_conv.sum())} converged systems; "
+          f"outside: {[(int(k), int(its[k]), int(lo_it[k]), int(hi_it[k])) for k in outside]}",
+          flush=True)
+    assert outside.size == 0
+
+
+def test_batch512_adjoint_small_systems_dense(gpu_device, batch512):
+    S = batch512
+    bc, B = S["bc"], S["B"]
+    g = S["gam"].clone().requires_grad_(True)
+    gb = S["gamb"].clone().requires_grad_(True)
+    wn = np.random.default_rng(5).standard_normal(bc.n_cells)
+    solver = BatchedPCGSolver(tolerance=1e-300, rel_tol=1e-12, max_iter=30000)
+    xs = batch_poisson_solve(g, gb, S["b"], solver, B, S["geo"])
+    (torch.from_numpy(wn).to(gpu_device) * xs).sum().backward()
+    gg = g.grad.cpu().numpy()
+    d, lo, up = (S[k].cpu().numpy() for k in ("d", "lo", "up"))
+    co, fo = bc.cell_offsets, bc.face_offsets
+    small = [k for k in range(bc.n_systems) if co[k + 1] - co[k] <= 1000]
+    assert len(small) >= 3
+    for k in small:
+        c0, c1, f0, f1 = co[k], co[k + 1], fo[k], fo[k + 1]
+        own, nei = bc.owner_local[f0:f1], bc.neighbour_local[f0:f1]
+        A = oldu.dense(d[c0:c1], lo[f0:f1], up[f0:f1], own, nei)
+        xk = np.linalg.solve(A, bc.b[c0:c1])
+        lam = np.linalg.solve(A.T, wn[c0:c1])
+        gd, gl, gu = oadj.ldu_grads(lam, xk, own, nei)
+        ref, _ = oadj.gamma_grad(gd, gl, gu, bc.mag[f0:f1], bc.delta[f0:f1], bc.vol[c0:c1], own,
+                                 nei)
+        np.testing.assert_allclose(gg[f0:f1], ref, rtol=1e-6, atol=1e-9 * np.abs(ref).max())
