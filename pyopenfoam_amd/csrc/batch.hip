@@ -148,10 +148,28 @@ __global__ void k_b_state_init(BGlobal *g, BSys *sys, int ns, double tol, double
   }
 }
 
-__device__ __forceinline__ bool sys_decision(const BGlobal *g, const BSys &S, double res, int it) {
+// The fields of a system's state that iteration `it`'s direction update
+// reads, loaded one by one (a copy of the whole struct, indexed by parity,
+// would live in private memory).
+struct SysView {
+  double res0, alpha, rz_prev;
+  int32_t nhist, done, last_it;
+};
+__device__ __forceinline__ SysView sys_view(const BSys &s, int it) {
+  SysView v;
+  v.res0 = s.res0;
+  v.alpha = (it & 1) ? s.alpha[1] : s.alpha[0];
+  v.rz_prev = (it & 1) ? s.rz[0] : s.rz[1];
+  v.nhist = s.nhist;
+  v.done = s.done;
+  v.last_it = s.last_it;
+  return v;
+}
+
+__device__ __forceinline__ bool sys_decision(const BGlobal *g, double res0, double res, int it) {
   if (it >= g->min_iter) {
     if (res < g->tol) return true;
-    if (S.res0 > 0.0 && res / S.res0 < g->rel_tol) return true;
+    if (res0 > 0.0 && res / res0 < g->rel_tol) return true;
   }
   return false;
 }
@@ -212,7 +230,7 @@ __global__ __launch_bounds__(kBlock) void k_b_init_fin(int ns, const int32_t *__
   S.res = res;
   S.nhist = 1;
   S.rz[0] = RZ;
-  if (sys_decision(g, S, res, 0)) {
+  if (sys_decision(g, S.res0, res, 0)) {
     S.done = 1;
     S.status = PFM_SOLVE_CONVERGED;
     S.last_it = 0;
@@ -474,7 +492,7 @@ __global__ __launch_bounds__(kBlock) void k_b_pupdate(const int4 *__restrict__ a
     // values keeps a system that stops in this iteration (whose x update every
     // tile must still make).
     const int t0 = sys_t0[k], t1 = sys_t0[k + 1];
-    const BSys S = sys[k];
+    const SysView S = sys_view(sys[k], it);
     if (threadIdx.x == 0) s_skip = S.done && S.last_it < it;
     __syncthreads();
     const int skip = s_skip;
@@ -483,23 +501,25 @@ __global__ __launch_bounds__(kBlock) void k_b_pupdate(const int4 *__restrict__ a
     const double RR = sys_sum(part + ntiles, t0, t1, lds);
     const double RZ = sys_sum(part + 2 * ntiles, t0, t1, lds);
     const double res = sqrt(RR);
-    const bool conv = sys_decision(g, S, res, it);
+    const bool conv = sys_decision(g, S.res0, res, it);
     const bool stop = conv || it >= g->max_iter;
-    if (t == t0 && threadIdx.x == 0) {
-      BSys W = S;
-      W.nhist += 1;
+    if (t == t0 && threadIdx.x == 0) {  // field by field: no private copy of the state
+      BSys &W = sys[k];
+      W.nhist = S.nhist + 1;
       W.res = res;
-      W.rz[it & 1] = RZ;
+      if (it & 1)
+        W.rz[1] = RZ;
+      else
+        W.rz[0] = RZ;
       if (stop) {
         W.done = 1;
         W.status = conv ? PFM_SOLVE_CONVERGED : PFM_SOLVE_MAXITER;
         W.last_it = it;
       }
-      sys[k] = W;
       if (stop) atomicAdd((int *)&g->ndone, 1);
     }
-    const double alpha = S.alpha[it & 1];
-    const double beta = RZ / S.rz[(it - 1) & 1];
+    const double alpha = S.alpha;
+    const double beta = RZ / S.rz_prev;
     for (; s < s1; s += nw) {
       const int sn = s + nw, rn = sn * kSlice + lane;
       double rnx = 0.0, pn = 0.0, xn = 0.0, dn = 1.0;
@@ -519,6 +539,247 @@ __global__ __launch_bounds__(kBlock) void k_b_pupdate(const int4 *__restrict__ a
       pv = pn;
       xv = xn;
       dv = dn;
+    }
+  }
+}
+
+// ---------------------------------------------------------- tail mode -----
+// Once few systems remain, an iteration is a chain of three launches whose
+// cost is each tile's walk over its slices: four waves take a 32-slice tile
+// in eight steps.  In tail mode a tile is run by a 1024-thread workgroup
+// whose 16 waves hold every slice of the tile at once (at most two per wave).
+// On the 512-system batch (n ~ U{8..48}) switching at 32 systems left takes
+// the solve from 3.87 s to 3.73 s; at 8 left, 3.75 s; from the start, 3.71 s
+// (the autotuned 256-thread kernels are kept while the batch is full).
+// The tile partials keep the summation order of the 256-thread kernels: each
+// row's product goes to LDS, then waves 0-3 add, lane by lane, the products
+// of slices v, v + 4, v + 8, ... in that order (the running sums of the
+// 256-thread kernels' four waves) and the four wave sums are added as wg_sum
+// adds them; a system's sum over its tiles is formed by threads 0-255 as
+// sys_sum forms it with 256 threads.  Results are bit-identical to the
+// 256-thread kernels'.
+constexpr int kTailThreads = 1024;
+constexpr int kTailWaves = kTailThreads / kSlice;  // 16
+constexpr int kTailTileSlices = 2 * kTailWaves;    // tiles of at most this many slices
+constexpr int kTailGrid = 512;                     // a multiple of the XCD count
+constexpr int kTailSystems = 32;                   // tail mode once at most this many remain
+
+// wg_sum of a 256-thread workgroup, on a 1024-thread one: v of waves 0-3.
+__device__ __forceinline__ double wg_sum_first4(double v, double *lds /*[kMaxWaves]*/) {
+  const int w = threadIdx.x >> 6;
+  if (w < kMaxWaves) {
+    v = wave_sum(v);
+    if ((threadIdx.x & 63) == 0) lds[w] = v;
+  }
+  __syncthreads();
+  double s = lds[0];
+  for (int i = 1; i < kMaxWaves; ++i) s += lds[i];
+  __syncthreads();
+  return s;
+}
+
+// sys_sum as 256 threads form it.
+__device__ __forceinline__ double sys_sum_first4(const double *__restrict__ part, int t0, int t1,
+                                                 double *lds) {
+  double s = 0.0;
+  if (threadIdx.x < kBlock)
+    for (int t = t0 + (int)threadIdx.x; t < t1; t += kBlock) s += part[t];
+  return wg_sum_first4(s, lds);
+}
+
+// The tile partial of the 256-thread kernels from the products prod[q * 64 +
+// lane] of the tile's slices q = 0..nsl-1.  Called by every thread after the
+// products are in LDS and a barrier.
+__device__ __forceinline__ double ordered_tile_sum(const double *prod, int nsl, double *lds) {
+  const int w = threadIdx.x >> 6, lane = threadIdx.x & 63;
+  double acc = 0.0;
+  if (w < kMaxWaves)
+    for (int q = w; q < nsl; q += kMaxWaves) acc += prod[q * kSlice + lane];
+  return wg_sum_first4(acc, lds);
+}
+
+// k_b_spmv in tail mode.
+__global__ __launch_bounds__(kTailThreads) void k_bt_spmv(const int4 *__restrict__ act,
+                                                           const SliceInfo *__restrict__ sl,
+                                                           const int32_t *__restrict__ col,
+                                                           const double *__restrict__ val,
+                                                           const double *__restrict__ diag,
+                                                           const double *__restrict__ p,
+                                                           double *__restrict__ Ap,
+                                                           double *__restrict__ part,
+                                                           const BGlobal *__restrict__ g) {
+  __shared__ double prod[kTailTileSlices * kSlice];
+  __shared__ double lds[kMaxWaves];
+  const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+  const XcdRange rg = xcd_range(g->nact);
+  for (int i = rg.begin; i < rg.end; i += rg.step) {
+    const int4 dsc = act[i];
+    const int t = dsc.x, s0 = dsc.z, nsl = dsc.w - dsc.z;
+    if (w < nsl) {
+      const bool two = w + kTailWaves < nsl;
+      const int sa = s0 + w, sb = two ? sa + kTailWaves : sa;
+      const SliceInfo ia = load_slice(sl, sa), ib = load_slice(sl, sb);
+      const int ra = sa * kSlice + lane, rb = sb * kSlice + lane;
+      const double pa = p[ra], pb = p[rb];
+      const double va = row_spmv<double>(ia, lane, diag[ra] * pa, col, val, p);
+      const double vb = row_spmv<double>(ib, lane, diag[rb] * pb, col, val, p);
+      Ap[ra] = va;
+      prod[w * kSlice + lane] = pa * va;
+      if (two) {
+        Ap[rb] = vb;
+        prod[(w + kTailWaves) * kSlice + lane] = pb * vb;
+      }
+    }
+    __syncthreads();
+    const double pap = ordered_tile_sum(prod, nsl, lds);
+    if (threadIdx.x == 0) part[t] = pap;
+  }
+}
+
+// k_b_update in tail mode.
+__global__ __launch_bounds__(kTailThreads) void k_bt_update(const int4 *__restrict__ act,
+                                                             int ntiles,
+                                                             const int32_t *__restrict__ sys_t0,
+                                                             const double *__restrict__ rD,
+                                                             const double *__restrict__ Ap,
+                                                             double *__restrict__ r,
+                                                             double *__restrict__ part,
+                                                             BGlobal *g, BSys *sys, int j) {
+  __shared__ double prr[kTailTileSlices * kSlice], prz[kTailTileSlices * kSlice];
+  __shared__ double lds[kMaxWaves];
+  __shared__ int s_done;
+  const int it = g->base + j + 1;
+  const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+  const XcdRange rg = xcd_range(g->nact);
+  for (int i = rg.begin; i < rg.end; i += rg.step) {
+    const int4 dsc = act[i];
+    const int t = dsc.x, k = dsc.y, s0 = dsc.z, nsl = dsc.w - dsc.z;
+    const bool has = w < nsl, two = w + kTailWaves < nsl;
+    const int ra = (s0 + w) * kSlice + lane, rb = ra + kTailWaves * kSlice;
+    double rva = 0.0, ava = 0.0, dva = 1.0, rvb = 0.0, avb = 0.0, dvb = 1.0;
+    if (has) {
+      rva = r[ra];
+      ava = Ap[ra];
+      if (rD) dva = rD[ra];
+    }
+    if (two) {
+      rvb = r[rb];
+      avb = Ap[rb];
+      if (rD) dvb = rD[rb];
+    }
+    const int t0 = sys_t0[k], t1 = sys_t0[k + 1];
+    const double rz_old = sys[k].rz[(it - 1) & 1];
+    if (threadIdx.x == 0) s_done = sys[k].done;  // see FOR_ACTIVE_TILES
+    __syncthreads();
+    const int done_k = s_done;
+    __syncthreads();
+    if (done_k) continue;
+    const double PAP = sys_sum_first4(part, t0, t1, lds);
+    if (fabs(PAP) < 1e-30) {
+      if (t == t0 && threadIdx.x == 0) {
+        sys[k].done = 1;
+        sys[k].status = PFM_SOLVE_BREAKDOWN;
+        sys[k].last_it = it - 1;
+        atomicAdd((int *)&g->ndone, 1);
+      }
+      continue;
+    }
+    const double alpha = rz_old / PAP;
+    if (t == t0 && threadIdx.x == 0) sys[k].alpha[it & 1] = alpha;
+    if (has) {
+      const double ri = rva - alpha * ava;
+      r[ra] = ri;
+      const double zi = rD ? ri / dva : ri;
+      prr[w * kSlice + lane] = ri * ri;
+      prz[w * kSlice + lane] = ri * zi;
+    }
+    if (two) {
+      const double ri = rvb - alpha * avb;
+      r[rb] = ri;
+      const double zi = rD ? ri / dvb : ri;
+      prr[(w + kTailWaves) * kSlice + lane] = ri * ri;
+      prz[(w + kTailWaves) * kSlice + lane] = ri * zi;
+    }
+    __syncthreads();
+    const double rr = ordered_tile_sum(prr, nsl, lds);
+    const double rz = ordered_tile_sum(prz, nsl, lds);
+    if (threadIdx.x == 0) {
+      part[ntiles + t] = rr;
+      part[2 * ntiles + t] = rz;
+    }
+  }
+}
+
+// k_b_pupdate in tail mode.
+__global__ __launch_bounds__(kTailThreads) void k_bt_pupdate(const int4 *__restrict__ act,
+                                                              int ntiles,
+                                                              const int32_t *__restrict__ sys_t0,
+                                                              const double *__restrict__ rD,
+                                                              const double *__restrict__ r,
+                                                              double *__restrict__ p,
+                                                              double *__restrict__ x,
+                                                              const double *__restrict__ part,
+                                                              BGlobal *g, BSys *sys, int j) {
+  __shared__ double lds[kMaxWaves];
+  __shared__ int s_skip;
+  const int it = g->base + j + 1;
+  const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+  const XcdRange rg = xcd_range(g->nact);
+  for (int i = rg.begin; i < rg.end; i += rg.step) {
+    const int4 dsc = act[i];
+    const int t = dsc.x, k = dsc.y, s0 = dsc.z, nsl = dsc.w - dsc.z;
+    const bool has = w < nsl, two = w + kTailWaves < nsl;
+    const int ra = (s0 + w) * kSlice + lane, rb = ra + kTailWaves * kSlice;
+    double rva = 0.0, pva = 0.0, xva = 0.0, dva = 1.0, rvb = 0.0, pvb = 0.0, xvb = 0.0, dvb = 1.0;
+    if (has) {
+      rva = r[ra];
+      pva = p[ra];
+      xva = x[ra];
+      if (rD) dva = rD[ra];
+    }
+    if (two) {
+      rvb = r[rb];
+      pvb = p[rb];
+      xvb = x[rb];
+      if (rD) dvb = rD[rb];
+    }
+    // as in k_b_pupdate: the state is read once, before the partial sums
+    const int t0 = sys_t0[k], t1 = sys_t0[k + 1];
+    const SysView S = sys_view(sys[k], it);
+    if (threadIdx.x == 0) s_skip = S.done && S.last_it < it;
+    __syncthreads();
+    const int skip = s_skip;
+    __syncthreads();
+    if (skip) continue;
+    const double RR = sys_sum_first4(part + ntiles, t0, t1, lds);
+    const double RZ = sys_sum_first4(part + 2 * ntiles, t0, t1, lds);
+    const double res = sqrt(RR);
+    const bool conv = sys_decision(g, S.res0, res, it);
+    const bool stop = conv || it >= g->max_iter;
+    if (t == t0 && threadIdx.x == 0) {  // field by field: no private copy of the state
+      BSys &W = sys[k];
+      W.nhist = S.nhist + 1;
+      W.res = res;
+      if (it & 1)
+        W.rz[1] = RZ;
+      else
+        W.rz[0] = RZ;
+      if (stop) {
+        W.done = 1;
+        W.status = conv ? PFM_SOLVE_CONVERGED : PFM_SOLVE_MAXITER;
+        W.last_it = it;
+      }
+      if (stop) atomicAdd((int *)&g->ndone, 1);
+    }
+    const double alpha = S.alpha;
+    const double beta = RZ / S.rz_prev;
+    if (has) {
+      x[ra] = xva + alpha * pva;
+      if (!stop) p[ra] = (rD ? rva / dva : rva) + beta * pva;
+    }
+    if (two) {
+      x[rb] = xvb + alpha * pvb;
+      if (!stop) p[rb] = (rD ? rvb / dvb : rvb) + beta * pvb;
     }
   }
 }
@@ -607,6 +868,19 @@ void launch_iteration(const pfm_batch_s *B, const BWork &w, const double *pd, in
   k_b_update<<<G, kBlock, 0, q>>>(w.act, T, B->d_sys_t0, pd, w.Ap, w.r, w.part, w.g, w.sys, j);
   k_b_pupdate<<<G, kBlock, 0, q>>>(w.act, T, B->d_sys_t0, pd, w.r, w.p, w.xp, w.part, w.g, w.sys,
                                    j);
+}
+
+// Iteration j of a chunk in tail mode (tiles of at most kTailTileSlices slices).
+void launch_iteration_tail(const pfm_batch_s *B, const BWork &w, const double *pd, int j,
+                           hipStream_t q) {
+  const pfm_ldu L = B->ldu;
+  const int T = B->ntiles;
+  k_bt_spmv<<<kTailGrid, kTailThreads, 0, q>>>(w.act, L->d_slices, L->d_col, w.packed, w.dp, w.p,
+                                               w.Ap, w.part, w.g);
+  k_bt_update<<<kTailGrid, kTailThreads, 0, q>>>(w.act, T, B->d_sys_t0, pd, w.Ap, w.r, w.part, w.g,
+                                                 w.sys, j);
+  k_bt_pupdate<<<kTailGrid, kTailThreads, 0, q>>>(w.act, T, B->d_sys_t0, pd, w.r, w.p, w.xp,
+                                                  w.part, w.g, w.sys, j);
 }
 
 constexpr int kTileCand[6] = {1, 2, 4, 8, 16, 32};
@@ -784,42 +1058,58 @@ int pfm_batch_pcg_solve(pfm_batch B, const pfm_solve_params *prm, int transpose,
                                 w.packed, w.dp, pd, w.bp, w.xp, w.r, w.p, w.part);
   k_b_init_fin<<<ns, kBlock, 0, s>>>(ns, B->d_sys_t0, T, w.part, w.g, w.sys);
   PFM_CHECK_HIP(hipGetLastError());
-  auto iteration = [&](int j, hipStream_t q) { launch_iteration(B, w, pd, j, q); };
+  // Tail mode (1024-thread tiles, same results) once at most kTailSystems
+  // systems are left, as seen at the last poll.
+  const bool tail_ok = B->tile_slices <= kTailTileSlices;
+  auto iteration = [&](int j, hipStream_t q, bool tail) {
+    if (tail)
+      launch_iteration_tail(B, w, pd, j, q);
+    else
+      launch_iteration(B, w, pd, j, q);
+  };
   const int chunk = std::max(1, graph_chunk);
-  hipGraphExec_t exec = nullptr;
+  hipGraphExec_t exec[2] = {nullptr, nullptr};  // chunk graphs: normal, tail mode
   hipStream_t cap = nullptr;
-  if (graph_chunk > 0) {
-    PFM_CHECK_HIP(hipStreamCreateWithFlags(&cap, hipStreamNonBlocking));
+  auto release = [&] {
+    for (hipGraphExec_t &e : exec)
+      if (e) (void)hipGraphExecDestroy(e);
+    if (cap) (void)hipStreamDestroy(cap);
+  };
+  auto capture = [&](bool tail) -> int {
+    if (!cap && hipStreamCreateWithFlags(&cap, hipStreamNonBlocking) != hipSuccess) {
+      set_error("stream creation failed");
+      return PFM_ERR_HIP;
+    }
     hipGraph_t graph = nullptr;
-    PFM_CHECK_HIP(hipStreamBeginCapture(cap, hipStreamCaptureModeThreadLocal));
-    launch_compact(B, w, chunk, cap);
-    for (int k = 0; k < chunk; ++k) iteration(k, cap);
-    hipError_t e = hipStreamEndCapture(cap, &graph);
-    if (e == hipSuccess) e = hipGraphInstantiate(&exec, graph, nullptr, nullptr, 0);
+    hipError_t e = hipStreamBeginCapture(cap, hipStreamCaptureModeThreadLocal);
+    if (e == hipSuccess) {
+      launch_compact(B, w, chunk, cap);
+      for (int k = 0; k < chunk; ++k) iteration(k, cap, tail);
+      e = hipStreamEndCapture(cap, &graph);
+    }
+    if (e == hipSuccess) e = hipGraphInstantiate(&exec[tail], graph, nullptr, nullptr, 0);
     if (graph) (void)hipGraphDestroy(graph);
     if (e != hipSuccess) {
-      (void)hipStreamDestroy(cap);
       set_error("graph capture failed: %s", hipGetErrorString(e));
       return PFM_ERR_HIP;
     }
-  }
+    return PFM_OK;
+  };
   // replay chunks until every system has stopped (one chunk queued ahead)
   PollLease lease;
   int rc = lease.acquire(&B->poll, true);
-  if (rc != PFM_OK) {
-    if (exec) (void)hipGraphExecDestroy(exec);
-    if (cap) (void)hipStreamDestroy(cap);
-    return rc;
-  }
+  if (rc != PFM_OK) return rc;
   int32_t *flag = lease.flag;
   hipEvent_t *ev = lease.ev;
-  int enq = 0, slot = 0;
+  int enq = 0, slot = 0, seen = 0;
   bool pending = false;
   while (true) {
     bool queued = false;
     if (enq < prm->max_iter) {
-      if (exec) {
-        if (hipGraphLaunch(exec, s) != hipSuccess) {
+      const bool tail = tail_ok && ns - seen <= kTailSystems;
+      if (graph_chunk > 0) {
+        if (!exec[tail] && (rc = capture(tail)) != PFM_OK) break;
+        if (hipGraphLaunch(exec[tail], s) != hipSuccess) {
           rc = PFM_ERR_HIP;
           break;
         }
@@ -827,7 +1117,7 @@ int pfm_batch_pcg_solve(pfm_batch B, const pfm_solve_params *prm, int transpose,
       } else {
         const int c = std::min(8, prm->max_iter - enq);
         launch_compact(B, w, c, s);
-        for (int k = 0; k < c; ++k) iteration(k, s);
+        for (int k = 0; k < c; ++k) iteration(k, s, tail);
         enq += c;
       }
       if (hipMemcpyAsync(&flag[slot], &w.g->ndone, sizeof(int32_t), hipMemcpyDeviceToHost, s) !=
@@ -843,16 +1133,17 @@ int pfm_batch_pcg_solve(pfm_batch B, const pfm_solve_params *prm, int transpose,
         rc = PFM_ERR_HIP;
         break;
       }
-      if (flag[slot ^ 1] >= ns) break;
+      seen = flag[slot ^ 1];
+      if (seen >= ns) break;
     }
     if (!queued) break;
     pending = true;
     slot ^= 1;
   }
-  if (exec) (void)hipGraphExecDestroy(exec);
-  if (cap) (void)hipStreamDestroy(cap);
+  if (rc != PFM_OK) (void)hipStreamSynchronize(s);  // queued chunks may still use the graphs
+  release();
   if (rc != PFM_OK) {
-    set_error("HIP error while driving the batched solve");
+    if (rc == PFM_ERR_HIP) set_error("HIP error while driving the batched solve");
     return rc;
   }
   if (B->n) k_b_gather<<<grid1d(B->n), kBlock, 0, s>>>(B->n, B->d_cell_map, w.xp, x);
