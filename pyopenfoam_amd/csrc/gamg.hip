@@ -217,6 +217,9 @@ __global__ __launch_bounds__(kBlock) void k_prolong_jacobi(LevelDevT<TF> F, int 
   if (row < F.n) F.x[cur ^ 1][row] = xi + F.inv[row] * (F.b[row] - ax);
 }
 
+// x[c] of a level by selection (no dynamic index into the descriptor).
+__device__ __forceinline__ double *xbuf(const LevelDev &L, int c) { return c ? L.x[1] : L.x[0]; }
+
 // Single-workgroup V-cycle of levels k0..nlev-1 (small levels, always FP64).
 // Waves map onto whole slices (rows t, t + 1024, ... with 64-aligned wave
 // offsets).
@@ -428,8 +431,16 @@ __global__ __launch_bounds__(kTailThreads) void k_tail(const LevelDev *__restric
   __shared__ double rsave[2][kRegCells];
   __shared__ double xs[kTailThreads], rs[kTailThreads];
   if (st->done) return;
-  int cur[kMaxLevels];
-  cur[k0] = cur0;
+  // Jacobi buffer parity of every level, one bit per level (an array indexed
+  // by level would live in private memory).
+  uint64_t par = (uint64_t)(cur0 & 1) << k0;
+  auto parity = [&](int k) { return (int)((par >> k) & 1); };
+  auto set_parity = [&](int k, int c) { par = (par & ~(1ull << k)) | ((uint64_t)(c & 1) << k); };
+  auto smooth = [&](const LevelDev &L, int k) {
+    int c = parity(k);
+    tail_smooth(L, c, gs);
+    set_parity(k, c);
+  };
   // kw: first level of the one-wave part (every level from kw on fits one
   // wave); nlev when there is none.  kt: first level of the one-row-per-
   // thread part [kt, kw) just above it (kt = kw when there is none).  Both
@@ -447,21 +458,22 @@ __global__ __launch_bounds__(kTailThreads) void k_tail(const LevelDev *__restric
   }
   const int kb = kw < nlev ? kt : nlev - 1;
   for (int k = k0; k < kb; ++k) {
-    const LevelDev L = Ls[k];
-    for (int it = 0; it < n_pre; ++it) tail_smooth(L, cur[k], gs);
+    const LevelDev &L = Ls[k];
+    for (int it = 0; it < n_pre; ++it) smooth(L, k);
+    const double *xk = xbuf(L, parity(k));
     for (int row0 = threadIdx.x; row0 < L.nslices * kSlice; row0 += kTailThreads) {
-      const double ax = level_ax(L, row0 >> 6, row0 & 63, row0, L.x[cur[k]]);
+      const double ax = level_ax(L, row0 >> 6, row0 & 63, row0, xk);
       if (row0 < L.n) L.r[row0] = L.b[row0] - ax;
     }
     __syncthreads();
-    const LevelDev Cc = Ls[k + 1];
+    const LevelDev &Cc = Ls[k + 1];
     for (int c = threadIdx.x; c < L.nc; c += kTailThreads) {
       double s = 0.0;
       for (int q = L.agg_ptr[c]; q < L.agg_ptr[c + 1]; ++q) s += L.r[L.agg_cells[q]];
       Cc.b[c] = s;
       Cc.x[0][c] = 0.0;
     }
-    cur[k + 1] = 0;
+    set_parity(k + 1, 0);
     __syncthreads();
   }
   if (kw < nlev) {
@@ -470,8 +482,8 @@ __global__ __launch_bounds__(kTailThreads) void k_tail(const LevelDev *__restric
     // (every Jacobi step flips the parity).
     const int t = threadIdx.x;
     const int flips = kt == nlev - 1 ? n_coarsest : n_pre + n_post;
-    const int pout = (cur[kt] + flips) & 1;
-    double x = t < Ls[kt].n ? Ls[kt].x[cur[kt]][t] : 0.0;
+    const int pout = (parity(kt) + flips) & 1;
+    double x = t < Ls[kt].n ? xbuf(Ls[kt], parity(kt))[t] : 0.0;
     double b = t < Ls[kt].n ? Ls[kt].b[t] : 0.0;
     WaveRow R;
     int off = 0;
@@ -509,20 +521,20 @@ __global__ __launch_bounds__(kTailThreads) void k_tail(const LevelDev *__restric
       __syncthreads();
       for (int it = 0; it < n_post; ++it) x = reg_jacobi(R, t, x, b, xs);
     }
-    if (t < Ls[kt].n) Ls[kt].x[pout][t] = x;
-    cur[kt] = pout;
+    if (t < Ls[kt].n) xbuf(Ls[kt], pout)[t] = x;
+    set_parity(kt, pout);
     __syncthreads();
   } else {
-    const LevelDev L = Ls[nlev - 1];
-    for (int it = 0; it < n_coarsest; ++it) tail_smooth(L, cur[nlev - 1], gs);
+    const LevelDev &L = Ls[nlev - 1];
+    for (int it = 0; it < n_coarsest; ++it) smooth(L, nlev - 1);
   }
   for (int k = kb - 1; k >= k0; --k) {
-    const LevelDev L = Ls[k];
-    const double *xc = Ls[k + 1].x[cur[k + 1]];
-    for (int i = threadIdx.x; i < L.n; i += kTailThreads)
-      L.x[cur[k]][i] = L.x[cur[k]][i] + xc[L.f2c[i]];
+    const LevelDev &L = Ls[k];
+    const double *xc = xbuf(Ls[k + 1], parity(k + 1));
+    double *xk = xbuf(L, parity(k));
+    for (int i = threadIdx.x; i < L.n; i += kTailThreads) xk[i] = xk[i] + xc[L.f2c[i]];
     __syncthreads();
-    for (int it = 0; it < n_post; ++it) tail_smooth(L, cur[k], gs);
+    for (int it = 0; it < n_post; ++it) smooth(L, k);
   }
 }
 
