@@ -60,7 +60,10 @@ def test_batch_matches_oracle_per_system(gpu_device, tile, pre):
         assert batch.tile_slices == tile
     else:
         assert len(batch.autotune_us) == 6 and batch.tile_slices in batch.autotune_us
-        assert batch.autotune_us[batch.tile_slices] == min(batch.autotune_us.values())
+        # the largest tile within 5 % of the fastest
+        fastest = min(batch.autotune_us.values())
+        near = [t for t, us in batch.autotune_us.items() if us <= 1.05 * fastest * (1 + 1e-6)]
+        assert batch.tile_slices == max(near)
     for k, (c, d, lo, up) in enumerate(B["parts"]):
         xs = x[B["co"][k]:B["co"][k + 1]]
         b = B["bs"][k]
