@@ -353,6 +353,9 @@ __global__ __launch_bounds__(kBlock) void k_b_spmv(const int4 *__restrict__ act,
                                                     const BGlobal *__restrict__ g,
                                                     const BSys *sys) {
   __shared__ double lds[kMaxWaves];
+  // The batch's coefficients (1.3 GB for the 512-system batch) are read once
+  // per iteration: nontemporal loads keep the vectors cached (3.73 -> 3.64 s).
+  auto gx = [p](int c) { return p[c]; };
   FOR_ACTIVE_TILES(false, {
     (void)k;
     const int lane = threadIdx.x & 63;
@@ -366,8 +369,10 @@ __global__ __launch_bounds__(kBlock) void k_b_spmv(const int4 *__restrict__ act,
       const SliceInfo sa_ = load_slice(sl, s), sb_ = load_slice(sl, sb);
       const int ra = s * kSlice + lane, rb = sb * kSlice + lane;
       const double pa = p[ra], pb = p[rb];
-      const double va = row_spmv<double>(sa_, lane, diag[ra] * pa, col, val, p);
-      const double vb = row_spmv<double>(sb_, lane, diag[rb] * pb, col, val, p);
+      const double va = row_spmv_g<double, decltype(gx), false, ColAbsT<true>, true>(
+          sa_, lane, diag[ra] * pa, ColAbsT<true>{col}, val, gx);
+      const double vb = row_spmv_g<double, decltype(gx), false, ColAbsT<true>, true>(
+          sb_, lane, diag[rb] * pb, ColAbsT<true>{col}, val, gx);
       Ap[ra] = va;
       pap += pa * va;
       if (two) {
@@ -379,6 +384,11 @@ __global__ __launch_bounds__(kBlock) void k_b_spmv(const int4 *__restrict__ act,
     if (threadIdx.x == 0) part[t] = pap;
   })
 }
+
+// The vector streams of the 256-thread update kernels (r, Ap, rD, p, x: 150 MB
+// each for the 512-system batch) are read nontemporally, which keeps the
+// gathered p and the small per-system data cached (3.58 -> 3.41 s).
+__device__ __forceinline__ double ldnt(const double *a) { return __builtin_nontemporal_load(a); }
 
 // alpha per system (kept for k_b_pupdate); r -= alpha Ap; tile partials r.r,
 // r.z.  A breakdown ends the system before this iteration's x update.
@@ -402,9 +412,9 @@ __global__ __launch_bounds__(kBlock) void k_b_update(const int4 *__restrict__ ac
     int row = s * kSlice + lane;
     double rv = 0.0, av = 0.0, dv = 1.0;
     if (s < s1) {
-      rv = r[row];
-      av = Ap[row];
-      if (rD) dv = rD[row];
+      rv = ldnt(r + row);
+      av = ldnt(Ap + row);
+      if (rD) dv = ldnt(rD + row);
     }
     // The tile range and r.z of the previous iteration depend only on k and
     // are not written by this kernel: load them with the stop flag.
@@ -432,9 +442,9 @@ __global__ __launch_bounds__(kBlock) void k_b_update(const int4 *__restrict__ ac
       const int sn = s + nw, rn = sn * kSlice + lane;
       double rnx = 0.0, an = 0.0, dn = 1.0;
       if (sn < s1) {
-        rnx = r[rn];
-        an = Ap[rn];
-        if (rD) dn = rD[rn];
+        rnx = ldnt(r + rn);
+        an = ldnt(Ap + rn);
+        if (rD) dn = ldnt(rD + rn);
       }
       const double ri = rv - alpha * av;
       r[row] = ri;
@@ -478,10 +488,10 @@ __global__ __launch_bounds__(kBlock) void k_b_pupdate(const int4 *__restrict__ a
     int row = s * kSlice + lane;
     double rv = 0.0, pv = 0.0, xv = 0.0, dv = 1.0;
     if (s < s1) {
-      rv = r[row];
-      pv = p[row];
-      xv = x[row];
-      if (rD) dv = rD[row];
+      rv = ldnt(r + row);
+      pv = ldnt(p + row);
+      xv = ldnt(x + row);
+      if (rD) dv = ldnt(rD + row);
     }
     // The system's state is read once, before the partial sums: within this
     // kernel only the system's first tile writes it (W below), and the fields
@@ -524,10 +534,10 @@ __global__ __launch_bounds__(kBlock) void k_b_pupdate(const int4 *__restrict__ a
       const int sn = s + nw, rn = sn * kSlice + lane;
       double rnx = 0.0, pn = 0.0, xn = 0.0, dn = 1.0;
       if (sn < s1) {
-        rnx = r[rn];
-        pn = p[rn];
-        xn = x[rn];
-        if (rD) dn = rD[rn];
+        rnx = ldnt(r + rn);
+        pn = ldnt(p + rn);
+        xn = ldnt(x + rn);
+        if (rD) dn = ldnt(rD + rn);
       }
       x[row] = xv + alpha * pv;
       if (!stop) {
