@@ -48,9 +48,11 @@ def _solve(B, dev, tile=0, chunk=16, pre="DIC", rel=1e-8, transpose=False, batch
     return x.cpu().numpy(), res, batch
 
 
-# 5 and 32 slices per tile: with 4 waves per workgroup each wave walks two or
-# more slices, two at a time, the last one alone when the count is odd.
-@pytest.mark.parametrize("tile", [0, 1, 3, 5, 32])
+# A batch of at most 32 systems runs the 16-wave tail-mode kernels from the
+# first iteration when its tiles hold at most 32 slices; 64-slice tiles take
+# the 4-wave kernels, whose waves walk two slices at a time (the last one
+# alone when the count is odd).  The 512-system test covers the switch.
+@pytest.mark.parametrize("tile", [0, 1, 3, 5, 32, 64])
 @pytest.mark.parametrize("pre", ["DIC", "none"])
 def test_batch_matches_oracle_per_system(gpu_device, tile, pre):
     B = _concat(SYM, zero_rhs=(3,))
