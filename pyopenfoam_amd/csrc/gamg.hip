@@ -43,6 +43,7 @@ constexpr int kFp32MinCells = 4096;
 constexpr int kFuseCells = 65536;  // levels whose launches cost more than their traffic
 constexpr int kTailThreads = 1024;
 constexpr int kMaxLevels = 64;
+constexpr int64_t kMallBytes = int64_t(256) << 20;
 
 inline int grid1d(int64_t n) { return (int)((n + kBlock - 1) / kBlock); }
 
@@ -112,9 +113,24 @@ __global__ __launch_bounds__(kBlock) void k_finite(int64_t n0, const T *__restri
 __global__ void k_flag_one(int32_t *flag) { *flag = 1; }
 
 // ------------------------------------------------------------- V-cycle -----
+// A level whose coefficients do not fit the 256 MB MALL (the finest levels
+// at 256^3) streams them, with its per-row vectors, nontemporally, so that
+// the gathered x stays cached: 7.46 -> 7.22 s for the 256^3 FP64 GAMG solve,
+// 4.33 -> 4.12 s mixed.  Values and arithmetic are the same either way.
+template <typename T>
+__device__ __forceinline__ T lvld(const LevelDevT<T> &L, const T *a, int i) {
+  return L.nt ? __builtin_nontemporal_load(a + i) : a[i];
+}
+
 template <typename T>
 __device__ __forceinline__ T level_ax(const LevelDevT<T> &L, int s, int lane, int row, const T *x) {
   const SliceInfo si = load_slice(L.slices, s);
+  auto gx = [x](int c) { return x[c]; };
+  if (L.nt) {
+    const T dx = row < L.n ? __builtin_nontemporal_load(L.diag + row) * x[row] : T(0);
+    return row_spmv_g<T, decltype(gx), false, ColAbsT<true>, true>(si, lane, dx,
+                                                                   ColAbsT<true>{L.col}, L.packed, gx);
+  }
   const T dx = row < L.n ? L.diag[row] * x[row] : T(0);
   return row_spmv<T>(si, lane, dx, L.col, L.packed, x);
 }
@@ -127,7 +143,7 @@ __global__ __launch_bounds__(kBlock) void k_jacobi(LevelDevT<T> L, int cur, cons
   const int lane = threadIdx.x & 63, row = s * kSlice + lane;
   const T *x = L.x[cur];
   const T ax = level_ax(L, s, lane, row, x);
-  if (row < L.n) L.x[cur ^ 1][row] = x[row] + L.inv[row] * (L.b[row] - ax);
+  if (row < L.n) L.x[cur ^ 1][row] = x[row] + lvld(L, L.inv, row) * (lvld<T>(L, L.b, row) - ax);
 }
 
 // One Gauss-Seidel sweep level of a level's x[cur], in place.
@@ -148,7 +164,7 @@ __global__ __launch_bounds__(kBlock) void k_resid(LevelDevT<T> L, int cur, const
   if (s >= L.nslices) return;
   const int lane = threadIdx.x & 63, row = s * kSlice + lane;
   const T ax = level_ax(L, s, lane, row, L.x[cur]);
-  if (row < L.n) L.r[row] = L.b[row] - ax;
+  if (row < L.n) L.r[row] = lvld<T>(L, L.b, row) - ax;
 }
 
 // Row c of a level's first Jacobi sweep from x = 0, given b_c, written to
@@ -576,10 +592,10 @@ __global__ __launch_bounds__(kBlock) void k_res_norm_jacobi(LevelDevT<T> L, int 
     const int row = s * kSlice + lane;
     const T ax = level_ax(L, s, lane, row, x);
     if (row < L.n) {
-      const T rt = L.b[row] - ax;
+      const T rt = lvld<T>(L, L.b, row) - ax;
       const double ri = (double)rt;
       acc += ri * ri;
-      L.x[cur ^ 1][row] = x[row] + L.inv[row] * rt;
+      L.x[cur ^ 1][row] = x[row] + lvld(L, L.inv, row) * rt;
     }
   }
   acc = block_sum(acc, lds);
@@ -868,6 +884,7 @@ size_t gamg_layout(const pfm_gamg_s *g, int max_iter, GamgWork *w, char *base) {
     memset(&D, 0, sizeof(D));
     D.n = H.n;
     D.nslices = H.ldu->nslices;
+    D.nt = (int64_t)H.ldu->nslots * 12 >= kMallBytes;  // 8-byte value + 4-byte column
     D.slices = H.ldu->d_slices;
     D.col = H.ldu->d_col;
     D.map = H.ldu->d_map;
@@ -964,6 +981,7 @@ size_t gamg_mixed_layout(const pfm_gamg_s *g, int max_outer, int fp32_levels, Ga
     memset(&D, 0, sizeof(D));
     D.n = D64.n;
     D.nslices = D64.nslices;
+    D.nt = (int64_t)H.ldu->nslots * 8 >= kMallBytes;  // 4-byte value + 4-byte column
     D.slices = D64.slices;
     D.col = D64.col;
     D.map = D64.map;

@@ -32,6 +32,7 @@ struct LevelDevT {
   const int32_t *agg_ptr;
   const int32_t *agg_cells;
   int32_t nc;
+  int32_t nt;             // 1: coefficients beyond the MALL, streamed nontemporally
   // Gauss-Seidel smoothing: slot -> face map and the sweep level schedules
   const int32_t *map;
   const int32_t *gs_fptr, *gs_fcells, *gs_bptr, *gs_bcells;
