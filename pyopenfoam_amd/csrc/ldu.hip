@@ -292,21 +292,40 @@ __global__ __launch_bounds__(kBlock) void k_dilu_gather(int nf, const int32_t *_
 
 constexpr int kDiluThreads = 1024;
 
+// Only the d values depend on the previous level: each thread loads its
+// first face of level l + 1 (and that level's end) while level l runs, so a
+// level costs one round trip for d, the division and the barrier.
+__device__ __forceinline__ void dilu_face(int2 f, double a, double *__restrict__ d) {
+  const double dp = d[f.x], dq = d[f.y];
+  if (fabs(dq) > 1e-30) d[f.x] = dp - a / dq;
+  if (fabs(dp) > 1e-30) d[f.y] = dq - a / dp;
+}
+
 __global__ __launch_bounds__(kDiluThreads) void k_dilu_sweep(const int32_t *__restrict__ level_ptr,
                                                               int nlevels,
                                                               const int2 *__restrict__ pq,
                                                               const double *__restrict__ lu,
                                                               double *__restrict__ d) {
+  if (nlevels <= 0) return;
+  const int t = (int)threadIdx.x;
+  int b = level_ptr[0], e = level_ptr[1];
+  bool has = b + t < e;
+  int2 f = has ? pq[b + t] : make_int2(0, 0);
+  double a = has ? lu[b + t] : 0.0;
   for (int l = 0; l < nlevels; ++l) {
-    const int b = level_ptr[l], e = level_ptr[l + 1];
-    for (int k = b + (int)threadIdx.x; k < e; k += kDiluThreads) {
-      const int2 f = pq[k];
-      const double a = lu[k];
-      const double dp = d[f.x], dq = d[f.y];
-      if (fabs(dq) > 1e-30) d[f.x] = dp - a / dq;
-      if (fabs(dp) > 1e-30) d[f.y] = dq - a / dp;
-    }
+    // level l + 1: [e, en), this thread's first face of it
+    const int en = l + 1 < nlevels ? level_ptr[l + 2] : e;
+    const bool hn = e + t < en;
+    const int2 fn = hn ? pq[e + t] : make_int2(0, 0);
+    const double an = hn ? lu[e + t] : 0.0;
+    if (has) dilu_face(f, a, d);
+    for (int k = b + t + kDiluThreads; k < e; k += kDiluThreads) dilu_face(pq[k], lu[k], d);
     __syncthreads();
+    b = e;
+    e = en;
+    has = hn;
+    f = fn;
+    a = an;
   }
 }
 
