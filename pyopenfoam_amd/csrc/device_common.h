@@ -1,6 +1,9 @@
 // Device-side helpers shared by the libpfm kernels.
 #pragma once
 
+#include <cstdlib>
+#include <cstring>
+
 #include "pfm_internal.h"
 
 namespace pfm {
@@ -181,6 +184,109 @@ __device__ __forceinline__ T row_spmv(const SliceInfo si, int lane, T dx,
                                       const T *__restrict__ val,
                                       const T *__restrict__ x) {
   return row_spmv_g<T>(si, lane, dx, ColAbs{col}, val, [x](int c) { return x[c]; });
+}
+
+// Column forms of a handle: int32 columns, int16 offsets from the row, the
+// int16 offsets paired two slots per 4-byte word (row_spmv_pair).
+enum ColForm { kCols32 = 0, kCols16 = 1, kColsPair = 2 };
+
+// Column form of the Krylov solvers' SpMV kernels: the paired int16 offsets
+// where the handle has them, else int32 columns.  PFM_PCG_COLS=int16 / int32
+// selects the unpaired offsets or the int32 columns (for comparing layouts).
+inline int krylov_col_form(const pfm_ldu_s *h) {
+  static const int forced = [] {
+    const char *e = std::getenv("PFM_PCG_COLS");
+    if (!e) return -1;
+    if (std::strcmp(e, "int32") == 0) return (int)kCols32;
+    if (std::strcmp(e, "int16") == 0) return (int)kCols16;
+    return -1;
+  }();
+  if (forced == kCols32 || !h->d_col16) return kCols32;
+  if (forced == kCols16 || !h->d_colp2) return kCols16;
+  return kColsPair;
+}
+
+// Paired int16 column offsets.  Word j of lane l in slice s, at
+// pair_base(s) + 64 j + l, holds the offsets from the row of the slice's slots
+// 2j (low half) and 2j+1 (high half), slots counted over the lower group then
+// the upper group; a missing or padding slot holds kColPad16.  A 4-byte load
+// per lane then fetches two slots' columns: half the index bytes of int32
+// columns with as many bytes per load instruction (a 2-byte load per slot
+// moves half as much per instruction and was slower than int32 columns).
+// Slice s owns 32 (wl + wu + 1) words from pair_base(s) = ptr/2 + 32 s, enough
+// for 64 ceil((wl + wu) / 2); the words past them are never read.
+__device__ __forceinline__ int pair_base(const SliceInfo si, int s) { return si.ptr / 2 + 32 * s; }
+
+__device__ __forceinline__ int pair_col(uint32_t w, int hi, int row) {
+  const int v = hi ? (int)(int16_t)(w >> 16) : (int)(int16_t)(w & 0xffffu);
+  return v == kColPad16 ? -1 : row + v;
+}
+
+// The common case of row_spmv_pair for a slice with WL lower slots (known at
+// compile time, so every slot's word and half are too) and at most 4 upper
+// slots: every load of the row in flight at once, then the gathers, then the
+// two sums in face order.
+template <int WL, typename T, typename Gather, bool kUpperFirst, bool kNT>
+__device__ __forceinline__ T row_pair_fixed(int wu, int ptr, int pb, int lane, int row, T dx,
+                                            const uint32_t *__restrict__ cp,
+                                            const T *__restrict__ val, Gather gx) {
+  constexpr int U = 4;
+  constexpr int NS = WL + U;
+  constexpr int NW = (NS + 1) / 2;
+  const int nsl = WL + wu;
+  uint32_t w[NW];
+#pragma unroll
+  for (int j = 0; j < NW; ++j)
+    w[j] = 2 * j < nsl ? ld<kNT>(cp + pb + j * kSlice + lane) : 0x80008000u;
+  T v[NS];
+#pragma unroll
+  for (int j = 0; j < NS; ++j) v[j] = j < nsl ? ld<kNT>(val + ptr + j * kSlice + lane) : T(0);
+  int c[NS];
+#pragma unroll
+  for (int j = 0; j < NS; ++j) c[j] = j < nsl ? pair_col(w[j >> 1], j & 1, row) : -1;
+  T xv[NS];
+#pragma unroll
+  for (int j = 0; j < NS; ++j) xv[j] = c[j] >= 0 ? gx(c[j]) : T(0);
+  T accL = T(0), accU = T(0);
+#pragma unroll
+  for (int j = 0; j < WL; ++j)
+    if (c[j] >= 0) accL += v[j] * xv[j];
+#pragma unroll
+  for (int j = WL; j < NS; ++j)
+    if (c[j] >= 0) accU += v[j] * xv[j];
+  return kUpperFirst ? (dx + accU) + accL : (dx + accL) + accU;
+}
+
+// row_spmv_g over the paired int16 columns (same association and order).
+template <typename T, typename Gather, bool kUpperFirst = false, bool kNT = false>
+__device__ __forceinline__ T row_spmv_pair(const SliceInfo si, int s, int lane, int row, T dx,
+                                           const uint32_t *__restrict__ cp,
+                                           const T *__restrict__ val, Gather gx) {
+  const int wl = si.wl, wu = si.wu;
+  const int pb = pair_base(si, s);
+  if (wu <= 4) {
+    switch (wl) {  // wave-uniform
+      case 0: return row_pair_fixed<0, T, Gather, kUpperFirst, kNT>(wu, si.ptr, pb, lane, row, dx, cp, val, gx);
+      case 1: return row_pair_fixed<1, T, Gather, kUpperFirst, kNT>(wu, si.ptr, pb, lane, row, dx, cp, val, gx);
+      case 2: return row_pair_fixed<2, T, Gather, kUpperFirst, kNT>(wu, si.ptr, pb, lane, row, dx, cp, val, gx);
+      case 3: return row_pair_fixed<3, T, Gather, kUpperFirst, kNT>(wu, si.ptr, pb, lane, row, dx, cp, val, gx);
+      case 4: return row_pair_fixed<4, T, Gather, kUpperFirst, kNT>(wu, si.ptr, pb, lane, row, dx, cp, val, gx);
+      default: break;
+    }
+  }
+  // wide rows: one slot at a time, each sum in face order
+  T accL = T(0), accU = T(0);
+  for (int j = 0; j < wl + wu; ++j) {
+    const uint32_t wd = ld<kNT>(cp + pb + (j >> 1) * kSlice + lane);
+    const int c = pair_col(wd, j & 1, row);
+    if (c < 0) continue;
+    const T p = ld<kNT>(val + si.ptr + j * kSlice + lane) * gx(c);
+    if (j < wl)
+      accL += p;
+    else
+      accU += p;
+  }
+  return kUpperFirst ? (dx + accU) + accL : (dx + accL) + accU;
 }
 
 // Slice descriptor for a slice index that is the same on every lane of the

@@ -47,6 +47,16 @@ constexpr int64_t kMallBytes = int64_t(256) << 20;
 
 inline int grid1d(int64_t n) { return (int)((n + kBlock - 1) / kBlock); }
 
+// The paired int16 columns of a level's handle for the level kernels' SpMV,
+// unless PFM_GAMG_COLS=int32 keeps them on the int32 columns (for comparing).
+const uint32_t *level_pairs(const pfm_ldu_s *h) {
+  static const bool int32_only = [] {
+    const char *e = std::getenv("PFM_GAMG_COLS");
+    return e && std::strcmp(e, "int32") == 0;
+  }();
+  return int32_only ? nullptr : h->d_colp2;
+}
+
 // ------------------------------------------------------------ coarsening ---
 __global__ __launch_bounds__(kBlock) void k_coarse_diag(int nc, const int32_t *__restrict__ agg_ptr,
                                                          const int32_t *__restrict__ agg_cells,
@@ -126,6 +136,14 @@ template <typename T>
 __device__ __forceinline__ T level_ax(const LevelDevT<T> &L, int s, int lane, int row, const T *x) {
   const SliceInfo si = load_slice(L.slices, s);
   auto gx = [x](int c) { return x[c]; };
+  if (L.cp) {  // paired int16 columns: half the index bytes
+    if (L.nt) {
+      const T dx = row < L.n ? __builtin_nontemporal_load(L.diag + row) * x[row] : T(0);
+      return row_spmv_pair<T, decltype(gx), false, true>(si, s, lane, row, dx, L.cp, L.packed, gx);
+    }
+    const T dx = row < L.n ? L.diag[row] * x[row] : T(0);
+    return row_spmv_pair<T, decltype(gx)>(si, s, lane, row, dx, L.cp, L.packed, gx);
+  }
   if (L.nt) {
     const T dx = row < L.n ? __builtin_nontemporal_load(L.diag + row) * x[row] : T(0);
     return row_spmv_g<T, decltype(gx), false, ColAbsT<true>, true>(si, lane, dx,
@@ -229,7 +247,8 @@ __global__ __launch_bounds__(kBlock) void k_prolong_jacobi(LevelDevT<TF> F, int 
   const SliceInfo si = load_slice(F.slices, s);
   const TF xi = row < F.n ? xp(row) : TF(0);
   const TF dx = row < F.n ? F.diag[row] * xi : TF(0);
-  const TF ax = row_spmv_g<TF>(si, lane, dx, ColAbs{F.col}, F.packed, xp);
+  const TF ax = F.cp ? row_spmv_pair<TF, decltype(xp)>(si, s, lane, row, dx, F.cp, F.packed, xp)
+                     : row_spmv_g<TF>(si, lane, dx, ColAbs{F.col}, F.packed, xp);
   if (row < F.n) F.x[cur ^ 1][row] = xi + F.inv[row] * (F.b[row] - ax);
 }
 
@@ -887,6 +906,7 @@ size_t gamg_layout(const pfm_gamg_s *g, int max_iter, GamgWork *w, char *base) {
     D.nt = (int64_t)H.ldu->nslots * 12 >= kMallBytes;  // 8-byte value + 4-byte column
     D.slices = H.ldu->d_slices;
     D.col = H.ldu->d_col;
+    D.cp = level_pairs(H.ldu);
     D.map = H.ldu->d_map;
     D.gs_fptr = H.ldu->d_gs_fwd_ptr;
     D.gs_fcells = H.ldu->d_gs_fwd_cells;
@@ -984,6 +1004,7 @@ size_t gamg_mixed_layout(const pfm_gamg_s *g, int max_outer, int fp32_levels, Ga
     D.nt = (int64_t)H.ldu->nslots * 8 >= kMallBytes;  // 4-byte value + 4-byte column
     D.slices = D64.slices;
     D.col = D64.col;
+    D.cp = D64.cp;
     D.map = D64.map;
     D.gs_fptr = D64.gs_fptr;
     D.gs_fcells = D64.gs_fcells;

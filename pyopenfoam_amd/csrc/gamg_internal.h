@@ -20,6 +20,7 @@ struct LevelDevT {
   int32_t n, nslices;
   const SliceInfo *slices;
   const int32_t *col;
+  const uint32_t *cp;     // paired int16 column offsets (row_spmv_pair), or null
   const T *packed;
   const T *diag;
   const T *inv;           // 1 / max(|diag|, 1e-30)

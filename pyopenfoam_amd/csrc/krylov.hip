@@ -58,15 +58,18 @@ __device__ __forceinline__ void store_partials(double (&v)[K], double *part, dou
 constexpr bool kNtMat = false;
 constexpr bool kNtAp = true;
 
-// One row of A v with the handle's int16 row offsets (kRel) or int32 columns.
-template <bool kRel, bool kNT = false>
-__device__ __forceinline__ double row_av(const SliceInfo si, int lane, int row, double dx,
+// One row of A v in column form kCF (s: the row's slice).
+template <int kCF, bool kNT = false>
+__device__ __forceinline__ double row_av(const SliceInfo si, int s, int lane, int row, double dx,
                                          const int32_t *__restrict__ col,
                                          const int16_t *__restrict__ col16,
+                                         const uint32_t *__restrict__ cp,
                                          const double *__restrict__ val,
                                          const double *__restrict__ v) {
   auto gx = [v](int c) { return v[c]; };
-  if constexpr (kRel)
+  if constexpr (kCF == kColsPair)
+    return row_spmv_pair<double, decltype(gx), false, kNT>(si, s, lane, row, dx, cp, val, gx);
+  else if constexpr (kCF == kCols16)
     return row_spmv_g<double, decltype(gx), false, ColRelT<kNT>, kNT>(
         si, lane, dx, ColRelT<kNT>{col16, row}, val, gx);
   else
@@ -88,10 +91,11 @@ __device__ __forceinline__ double row_av(const SliceInfo si, int lane, int row, 
 // (r.r, r.z).
 
 // r = b - A x; z = M^-1 r; p = z; partials r.r and r.z (region B).
-template <bool kRel>
+template <int kCF>
 __global__ __launch_bounds__(kBlock) void k_pcg_init(int n, int nslices, const SliceInfo *__restrict__ sl,
                                                       const int32_t *__restrict__ col,
                                                       const int16_t *__restrict__ col16,
+                                                      const uint32_t *__restrict__ cp,
                                                       const double *__restrict__ val,
                                                       const double *__restrict__ diag,
                                                       const double *__restrict__ rD,
@@ -104,7 +108,7 @@ __global__ __launch_bounds__(kBlock) void k_pcg_init(int n, int nslices, const S
   FOR_ROWS({
     const SliceInfo si = load_slice(sl, s);
     const double dx = row < n ? diag[row] * x[row] : 0.0;
-    const double ax = row_av<kRel>(si, lane, row, dx, col, col16, val, x);
+    const double ax = row_av<kCF>(si, s, lane, row, dx, col, col16, cp, val, x);
     if (row < n) {
       const double ri = b[row] - ax;
       const double zi = rD ? ri / rD[row] : ri;
@@ -133,10 +137,11 @@ __global__ __launch_bounds__(kBlock) void k_pcg_init_fin(const double *__restric
 
 // Ap = A p; partial p.Ap (region A).  Ends the solve if the previous
 // iteration decided so.
-template <bool kRel>
+template <int kCF>
 __global__ __launch_bounds__(kBlock) void k_pcg_spmv(int n, int nslices, const SliceInfo *__restrict__ sl,
                                                       const int32_t *__restrict__ col,
                                                       const int16_t *__restrict__ col16,
+                                                      const uint32_t *__restrict__ cp,
                                                       const double *__restrict__ val,
                                                       const double *__restrict__ diag,
                                                       const double *__restrict__ p,
@@ -153,7 +158,7 @@ __global__ __launch_bounds__(kBlock) void k_pcg_spmv(int n, int nslices, const S
     const SliceInfo si = load_slice(sl, s);
     const double pi = row < n ? p[row] : 0.0;
     const double dx = row < n ? ld<kNtMat>(diag + row) * pi : 0.0;
-    const double v = row_av<kRel, kNtMat>(si, lane, row, dx, col, col16, val, p);
+    const double v = row_av<kCF, kNtMat>(si, s, lane, row, dx, col, col16, cp, val, p);
     if (row < n) {
       Ap[row] = v;
       acc[0] += pi * v;
@@ -730,6 +735,8 @@ inline int red_grid(const pfm_ldu_s *h) {
   return std::max(1, std::min(blocks_for_slices(h->nslices), red_cap()));
 }
 
+inline int pcg_col_form(const pfm_ldu_s *h) { return krylov_col_form(h); }
+
 size_t work_layout(const pfm_ldu_s *h, int max_iter, int nvec, Work *w, char *base) {
   const int G = red_grid(h);
   size_t off = 0;
@@ -873,23 +880,31 @@ int pcg_solve(pfm_ldu h, const pfm_solve_params *prm, const double *diag, const 
   k_state_init<<<1, 1, 0, s>>>(w.st, prm->tolerance, prm->rel_tol, prm->max_iter, prm->min_iter,
                                hist_cap);
   const int16_t *c16 = h->d_col16;
-  if (c16)
-    k_pcg_init<true><<<G, kBlock, 0, s>>>(n, ns, h->d_slices, h->d_col, c16, packed, diag, pd, b,
-                                          x, r, p, w.part);
+  const uint32_t *cp = h->d_colp2;
+  const int cf = pcg_col_form(h);
+  if (cf == kColsPair)
+    k_pcg_init<kColsPair><<<G, kBlock, 0, s>>>(n, ns, h->d_slices, h->d_col, c16, cp, packed, diag,
+                                               pd, b, x, r, p, w.part);
+  else if (cf == kCols16)
+    k_pcg_init<kCols16><<<G, kBlock, 0, s>>>(n, ns, h->d_slices, h->d_col, c16, cp, packed, diag,
+                                             pd, b, x, r, p, w.part);
   else
-    k_pcg_init<false><<<G, kBlock, 0, s>>>(n, ns, h->d_slices, h->d_col, c16, packed, diag, pd, b,
-                                           x, r, p, w.part);
+    k_pcg_init<kCols32><<<G, kBlock, 0, s>>>(n, ns, h->d_slices, h->d_col, c16, cp, packed, diag,
+                                             pd, b, x, r, p, w.part);
   k_pcg_init_fin<<<1, kBlock, 0, s>>>(w.part, G, w.st, w.hist);
   PFM_CHECK_HIP(hipGetLastError());
   rc = drive(h, w.st, prm->max_iter, s, [&](int it) {
     double *pc = P[(it + 1) & 1], *po = P[it & 1];  // p_it; p_{it+1} (and p_{it-1})
     const double *pprev = (it & 1) ? nullptr : po;  // pairs of x updates end at even it
-    if (c16)
-      k_pcg_spmv<true><<<G, kBlock, 0, s>>>(n, ns, h->d_slices, h->d_col, c16, packed, diag, pc,
-                                            Ap, w.part, w.st);
+    if (cf == kColsPair)
+      k_pcg_spmv<kColsPair><<<G, kBlock, 0, s>>>(n, ns, h->d_slices, h->d_col, c16, cp, packed,
+                                                 diag, pc, Ap, w.part, w.st);
+    else if (cf == kCols16)
+      k_pcg_spmv<kCols16><<<G, kBlock, 0, s>>>(n, ns, h->d_slices, h->d_col, c16, cp, packed,
+                                               diag, pc, Ap, w.part, w.st);
     else
-      k_pcg_spmv<false><<<G, kBlock, 0, s>>>(n, ns, h->d_slices, h->d_col, c16, packed, diag, pc,
-                                             Ap, w.part, w.st);
+      k_pcg_spmv<kCols32><<<G, kBlock, 0, s>>>(n, ns, h->d_slices, h->d_col, c16, cp, packed,
+                                               diag, pc, Ap, w.part, w.st);
     k_pcg_update<<<G, kBlock, 0, s>>>(n, ns, it, pd, Ap, r, x, pprev, w.part, w.st);
     k_pcg_pupdate<<<G, kBlock, 0, s>>>(n, ns, it, pd, r, pc, pprev, po, x, w.part, w.st, w.hist);
   }, may_allocate);

@@ -11,6 +11,7 @@
 #include <cmath>
 
 #include <cstdlib>
+#include <cstring>
 
 #include "device_common.h"
 
@@ -63,6 +64,32 @@ __global__ __launch_bounds__(kBlock) void k_spmv(int n, int nslices,
   auto gx = [x](int c) { return x[c]; };
   const T v = row_spmv_g<T, decltype(gx), false, ColAbsT<kNT>, kNT>(si, lane, dx,
                                                                       ColAbsT<kNT>{col}, val, gx);
+  if (row < n) {
+    if constexpr (kNT)
+      __builtin_nontemporal_store(v, y + row);
+    else
+      y[row] = v;
+  }
+}
+
+// k_spmv over the handle's paired int16 column offsets (row_spmv_pair): the
+// same products and sums, 2 index bytes per slot instead of 4.
+template <typename T, bool kNT>
+__global__ __launch_bounds__(kBlock) void k_spmv_p2(int n, int nslices,
+                                                     const SliceInfo *__restrict__ sl,
+                                                     const uint32_t *__restrict__ cp,
+                                                     const T *__restrict__ val,
+                                                     const T *__restrict__ diag,
+                                                     const T *__restrict__ x,
+                                                     T *__restrict__ y) {
+  const int s = xcd_remap(blockIdx.x, gridDim.x) * kWavesPerBlock + (threadIdx.x >> 6);
+  if (s >= nslices) return;
+  const SliceInfo si = load_slice(sl, s);
+  const int lane = threadIdx.x & 63;
+  const int row = s * kSlice + lane;
+  const T dx = row < n ? ld<kNT>(diag + row) * x[row] : T(0);
+  auto gx = [x](int c) { return x[c]; };
+  const T v = row_spmv_pair<T, decltype(gx), false, kNT>(si, s, lane, row, dx, cp, val, gx);
   if (row < n) {
     if constexpr (kNT)
       __builtin_nontemporal_store(v, y + row);
@@ -165,11 +192,28 @@ constexpr int kBwGrid = 2048;
 // with nontemporal ones; from HBM 39.5 us plain, 34.5 us nontemporal).
 constexpr int64_t kStreamBytes = int64_t(128) << 20;
 
+// PFM_SPMV_COLS=int32 makes pfm_spmv read the int32 columns even where the
+// paired int16 offsets exist (for comparing the two layouts).
+bool spmv_int32_columns() {
+  static const bool v = [] {
+    const char *e = std::getenv("PFM_SPMV_COLS");
+    return e && std::strcmp(e, "int32") == 0;
+  }();
+  return v;
+}
+
 template <typename T>
 int launch_spmv(const pfm_ldu_s *h, const T *diag, const T *packed, const T *x, T *y,
                 hipStream_t s) {
   const int64_t bytes = h->nslots * (int64_t)(sizeof(T) + 4) + 3 * (int64_t)h->n * sizeof(T);
-  if (bytes >= kStreamBytes)
+  if (h->d_colp2 && !spmv_int32_columns()) {
+    if (bytes >= kStreamBytes)
+      k_spmv_p2<T, true><<<blocks_for_slices(h->nslices), kBlock, 0, s>>>(
+          h->n, h->nslices, h->d_slices, h->d_colp2, packed, diag, x, y);
+    else
+      k_spmv_p2<T, false><<<blocks_for_slices(h->nslices), kBlock, 0, s>>>(
+          h->n, h->nslices, h->d_slices, h->d_colp2, packed, diag, x, y);
+  } else if (bytes >= kStreamBytes)
     k_spmv<T, true><<<blocks_for_slices(h->nslices), kBlock, 0, s>>>(
         h->n, h->nslices, h->d_slices, h->d_col, packed, diag, x, y);
   else
@@ -521,6 +565,7 @@ int pfm_ldu_destroy(pfm_ldu h) {
   (void)hipFree(h->d_slices);
   (void)hipFree(h->d_col);
   (void)hipFree(h->d_col16);
+  (void)hipFree(h->d_colp2);
   (void)hipFree(h->d_map);
   (void)hipFree(h->d_own);
   (void)hipFree(h->d_nei);
@@ -604,6 +649,26 @@ int pfm_ldu_create(int32_t n, int32_t nf, const int32_t *own, const int32_t *nei
     if (fits && (st = upload(&h->d_col16, c16))) {
       pfm_ldu_destroy(h);
       return st;
+    }
+    if (fits) {
+      // The same offsets two slots per 4-byte word (pair_base layout): slice s
+      // owns 32 (wl + wu + 1) words from ptr/2 + 32 s.
+      std::vector<uint32_t> p2((size_t)(L.nslots / 2 + (int64_t)32 * L.nslices), 0x80008000u);
+      for (size_t s = 0; s < L.slices.size(); ++s) {
+        const SliceInfo si = L.slices[s];
+        const int64_t pb = si.ptr / 2 + (int64_t)32 * s;
+        const int width = si.wl + si.wu;
+        for (int j = 0; j < width; ++j)
+          for (int lane = 0; lane < kSlice; ++lane) {
+            const uint32_t half = (uint16_t)c16[(int64_t)si.ptr + (int64_t)j * kSlice + lane];
+            uint32_t &w = p2[pb + (int64_t)(j >> 1) * kSlice + lane];
+            w = (j & 1) ? ((w & 0xffffu) | (half << 16)) : ((w & 0xffff0000u) | half);
+          }
+      }
+      if ((st = upload(&h->d_colp2, p2))) {
+        pfm_ldu_destroy(h);
+        return st;
+      }
     }
   }
   if ((st = poll_pool_init(h->poll)) != PFM_OK) {

@@ -142,10 +142,11 @@ __global__ __launch_bounds__(kBlock) void k_ir_inner_init_fin(const double *__re
 // Ap = A32 p (FP32); partial p.Ap in FP64 (region A).  kRel: columns from the
 // handle's int16 row offsets.  Ends the inner loop if the previous iteration
 // decided so.
-template <bool kRel>
+template <int kCF>
 __global__ __launch_bounds__(kBlock) void k_pcg32_spmv(int n, int nslices, const SliceInfo *__restrict__ sl,
                                                         const int32_t *__restrict__ col,
                                                         const int16_t *__restrict__ col16,
+                                                        const uint32_t *__restrict__ cp,
                                                         const float *__restrict__ val,
                                                         const float *__restrict__ diag,
                                                         const float *__restrict__ p,
@@ -164,7 +165,9 @@ __global__ __launch_bounds__(kBlock) void k_pcg32_spmv(int n, int nslices, const
     const float pi = row < n ? p[row] : 0.0f;
     const float dx = row < n ? diag[row] * pi : 0.0f;
     float v;
-    if constexpr (kRel)
+    if constexpr (kCF == kColsPair)
+      v = row_spmv_pair<float, decltype(gx)>(si, s, lane, row, dx, cp, val, gx);
+    else if constexpr (kCF == kCols16)
       v = row_spmv_g<float, decltype(gx), false, ColRel>(si, lane, dx, ColRel{col16, row}, val,
                                                          gx);
     else
@@ -352,6 +355,7 @@ int pfm_pcg_mixed_solve(pfm_ldu h, const pfm_solve_params *prm, double inner_rel
   const int n = h->n, ns = h->nslices;
   const int G = std::max(1, std::min(blocks_for_slices(ns), kMaxRedBlocks));
   const int g1 = (n + kBlock - 1) / kBlock;
+  const int cf = krylov_col_form(h);
   const bool pre = prm->precond == PFM_PRECOND_DIAG;
   k_cast<<<g1, kBlock, 0, s>>>(n, diag, w.diag32);
   if (pre) k_cast<<<g1, kBlock, 0, s>>>(n, rD, w.rD32);
@@ -388,12 +392,18 @@ int pfm_pcg_mixed_solve(pfm_ldu h, const pfm_solve_params *prm, double inner_rel
         const int c = std::min(chunk, budget - enq);
         for (int k = 0; k < c; ++k) {
           const int it = enq + k + 1;
-          if (h->d_col16)
-            k_pcg32_spmv<true><<<G, kBlock, 0, s>>>(n, ns, h->d_slices, h->d_col, h->d_col16,
-                                                    packed32, w.diag32, w.p, w.Ap, w.part, w.in);
+          if (cf == kColsPair)
+            k_pcg32_spmv<kColsPair><<<G, kBlock, 0, s>>>(n, ns, h->d_slices, h->d_col, h->d_col16,
+                                                         h->d_colp2, packed32, w.diag32, w.p, w.Ap,
+                                                         w.part, w.in);
+          else if (cf == kCols16)
+            k_pcg32_spmv<kCols16><<<G, kBlock, 0, s>>>(n, ns, h->d_slices, h->d_col, h->d_col16,
+                                                       nullptr, packed32, w.diag32, w.p, w.Ap,
+                                                       w.part, w.in);
           else
-            k_pcg32_spmv<false><<<G, kBlock, 0, s>>>(n, ns, h->d_slices, h->d_col, nullptr,
-                                                     packed32, w.diag32, w.p, w.Ap, w.part, w.in);
+            k_pcg32_spmv<kCols32><<<G, kBlock, 0, s>>>(n, ns, h->d_slices, h->d_col, nullptr,
+                                                       nullptr, packed32, w.diag32, w.p, w.Ap,
+                                                       w.part, w.in);
           k_pcg32_update<<<G, kBlock, 0, s>>>(n, ns, it, pd, w.Ap, w.r, w.part, w.in);
           k_pcg32_pupdate<<<G, kBlock, 0, s>>>(n, ns, it, pd, w.r, w.p, w.e, w.part, w.in);
         }

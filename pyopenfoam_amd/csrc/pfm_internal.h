@@ -136,6 +136,7 @@ struct pfm_ldu_s {
   pfm::SliceInfo *d_slices = nullptr;
   int32_t *d_col = nullptr;
   int16_t *d_col16 = nullptr;           // column - row per slot, when all fit (else null)
+  uint32_t *d_colp2 = nullptr;          // the same offsets paired per lane (pair_base), or null
   int32_t *d_map = nullptr;
   int32_t *d_own = nullptr;
   int32_t *d_nei = nullptr;

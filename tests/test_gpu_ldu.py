@@ -106,6 +106,20 @@ def test_factor_unordered_faces(gpu_device):
     x = rng.standard_normal(n)
     assert bits_equal(m.Ax(torch.from_numpy(x).to(gpu_device)).cpu().numpy(),
                       oldu.ax(d, lo, up, own, nei, x))
+    # rows with more than four couplings per group take the row-slice kernels'
+    # one-slot-at-a-time path (packed SpMV and the PCG's SpMV)
+    assert m.addressing.info.max_lower > 4 and m.addressing.info.max_upper > 4
+    assert bits_equal(m.Ax_packed(torch.from_numpy(x).to(gpu_device)).cpu().numpy(),
+                      oldu.ax(d, lo, up, own, nei, x))
+    from oracle import krylov as okry
+    from pyopenfoam_amd import PCGSolver
+    b = rng.uniform(-1.0, 1.0, n)
+    s = PCGSolver(tolerance=1e-300, rel_tol=1e-12, max_iter=40, preconditioner="none")
+    xs, it, _ = s(m, torch.from_numpy(b), torch.zeros(n, dtype=torch.float64))
+    xo, io, _, _ = okry.pcg(lambda v: oldu.ax(d, lo, up, own, nei, v), None, b, np.zeros(n),
+                            tolerance=1e-300, rel_tol=1e-12, max_iter=40,
+                            dot=okry.dot_device(n))
+    assert it == io and bits_equal(xs.cpu().numpy(), xo)
 
 
 def test_spmv_and_pcg_wide_bandwidth(gpu_device):
