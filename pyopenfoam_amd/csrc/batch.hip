@@ -15,6 +15,7 @@
 
 #include <algorithm>
 #include <cmath>
+#include <cstdlib>
 #include <cstring>
 
 #include "solver_state.h"
@@ -342,9 +343,11 @@ __device__ __forceinline__ XcdRange xcd_range(int nact) {
 
 // Ap = A p; tile partial p.Ap.  Tiles of a system that stopped earlier in the
 // chunk compute values nobody reads, so this kernel skips the stop check.
+template <bool kPair>
 __global__ __launch_bounds__(kBlock) void k_b_spmv(const int4 *__restrict__ act,
                                                     const SliceInfo *__restrict__ sl,
                                                     const int32_t *__restrict__ col,
+                                                    const uint32_t *__restrict__ cp,
                                                     const double *__restrict__ val,
                                                     const double *__restrict__ diag,
                                                     const double *__restrict__ p,
@@ -369,10 +372,18 @@ __global__ __launch_bounds__(kBlock) void k_b_spmv(const int4 *__restrict__ act,
       const SliceInfo sa_ = load_slice(sl, s), sb_ = load_slice(sl, sb);
       const int ra = s * kSlice + lane, rb = sb * kSlice + lane;
       const double pa = p[ra], pb = p[rb];
-      const double va = row_spmv_g<double, decltype(gx), false, ColAbsT<true>, true>(
-          sa_, lane, diag[ra] * pa, ColAbsT<true>{col}, val, gx);
-      const double vb = row_spmv_g<double, decltype(gx), false, ColAbsT<true>, true>(
-          sb_, lane, diag[rb] * pb, ColAbsT<true>{col}, val, gx);
+      double va, vb;
+      if constexpr (kPair) {  // paired int16 column offsets
+        va = row_spmv_pair<double, decltype(gx), false, true>(sa_, s, lane, ra, diag[ra] * pa, cp,
+                                                              val, gx);
+        vb = row_spmv_pair<double, decltype(gx), false, true>(sb_, sb, lane, rb, diag[rb] * pb, cp,
+                                                              val, gx);
+      } else {
+        va = row_spmv_g<double, decltype(gx), false, ColAbsT<true>, true>(
+            sa_, lane, diag[ra] * pa, ColAbsT<true>{col}, val, gx);
+        vb = row_spmv_g<double, decltype(gx), false, ColAbsT<true>, true>(
+            sb_, lane, diag[rb] * pb, ColAbsT<true>{col}, val, gx);
+      }
       Ap[ra] = va;
       pap += pa * va;
       if (two) {
@@ -609,9 +620,11 @@ __device__ __forceinline__ double ordered_tile_sum(const double *prod, int nsl, 
 }
 
 // k_b_spmv in tail mode.
+template <bool kPair>
 __global__ __launch_bounds__(kTailThreads) void k_bt_spmv(const int4 *__restrict__ act,
                                                            const SliceInfo *__restrict__ sl,
                                                            const int32_t *__restrict__ col,
+                                                           const uint32_t *__restrict__ cp,
                                                            const double *__restrict__ val,
                                                            const double *__restrict__ diag,
                                                            const double *__restrict__ p,
@@ -631,8 +644,15 @@ __global__ __launch_bounds__(kTailThreads) void k_bt_spmv(const int4 *__restrict
       const SliceInfo ia = load_slice(sl, sa), ib = load_slice(sl, sb);
       const int ra = sa * kSlice + lane, rb = sb * kSlice + lane;
       const double pa = p[ra], pb = p[rb];
-      const double va = row_spmv<double>(ia, lane, diag[ra] * pa, col, val, p);
-      const double vb = row_spmv<double>(ib, lane, diag[rb] * pb, col, val, p);
+      double va, vb;
+      if constexpr (kPair) {
+        auto gx = [p](int c) { return p[c]; };
+        va = row_spmv_pair<double, decltype(gx)>(ia, sa, lane, ra, diag[ra] * pa, cp, val, gx);
+        vb = row_spmv_pair<double, decltype(gx)>(ib, sb, lane, rb, diag[rb] * pb, cp, val, gx);
+      } else {
+        va = row_spmv<double>(ia, lane, diag[ra] * pa, col, val, p);
+        vb = row_spmv<double>(ib, lane, diag[rb] * pb, col, val, p);
+      }
       Ap[ra] = va;
       prod[w * kSlice + lane] = pa * va;
       if (two) {
@@ -868,13 +888,27 @@ void launch_compact(const pfm_batch_s *B, const BWork &w, int chunk, hipStream_t
                                  w.g, chunk);
 }
 
+// The iteration SpMVs read the paired int16 column offsets where the batch's
+// handle has them, unless PFM_BATCH_COLS=int32 (for comparing the layouts).
+bool batch_pairs(const pfm_ldu_s *L) {
+  static const bool int32_only = [] {
+    const char *e = std::getenv("PFM_BATCH_COLS");
+    return e && std::strcmp(e, "int32") == 0;
+  }();
+  return L->d_colp2 && !int32_only;
+}
+
 // Iteration j of a chunk: three kernels.
 void launch_iteration(const pfm_batch_s *B, const BWork &w, const double *pd, int j,
                       hipStream_t q) {
   const pfm_ldu L = B->ldu;
   const int T = B->ntiles, G = iter_grid(B);
-  k_b_spmv<<<G, kBlock, 0, q>>>(w.act, L->d_slices, L->d_col, w.packed, w.dp, w.p, w.Ap, w.part,
-                                w.g, w.sys);
+  if (batch_pairs(L))
+    k_b_spmv<true><<<G, kBlock, 0, q>>>(w.act, L->d_slices, L->d_col, L->d_colp2, w.packed, w.dp,
+                                        w.p, w.Ap, w.part, w.g, w.sys);
+  else
+    k_b_spmv<false><<<G, kBlock, 0, q>>>(w.act, L->d_slices, L->d_col, nullptr, w.packed, w.dp,
+                                         w.p, w.Ap, w.part, w.g, w.sys);
   k_b_update<<<G, kBlock, 0, q>>>(w.act, T, B->d_sys_t0, pd, w.Ap, w.r, w.part, w.g, w.sys, j);
   k_b_pupdate<<<G, kBlock, 0, q>>>(w.act, T, B->d_sys_t0, pd, w.r, w.p, w.xp, w.part, w.g, w.sys,
                                    j);
@@ -885,8 +919,12 @@ void launch_iteration_tail(const pfm_batch_s *B, const BWork &w, const double *p
                            hipStream_t q) {
   const pfm_ldu L = B->ldu;
   const int T = B->ntiles;
-  k_bt_spmv<<<kTailGrid, kTailThreads, 0, q>>>(w.act, L->d_slices, L->d_col, w.packed, w.dp, w.p,
-                                               w.Ap, w.part, w.g);
+  if (batch_pairs(L))
+    k_bt_spmv<true><<<kTailGrid, kTailThreads, 0, q>>>(w.act, L->d_slices, L->d_col, L->d_colp2,
+                                                       w.packed, w.dp, w.p, w.Ap, w.part, w.g);
+  else
+    k_bt_spmv<false><<<kTailGrid, kTailThreads, 0, q>>>(w.act, L->d_slices, L->d_col, nullptr,
+                                                        w.packed, w.dp, w.p, w.Ap, w.part, w.g);
   k_bt_update<<<kTailGrid, kTailThreads, 0, q>>>(w.act, T, B->d_sys_t0, pd, w.Ap, w.r, w.part, w.g,
                                                  w.sys, j);
   k_bt_pupdate<<<kTailGrid, kTailThreads, 0, q>>>(w.act, T, B->d_sys_t0, pd, w.r, w.p, w.xp,
