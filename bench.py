@@ -199,7 +199,7 @@ def measure_traffic(args, timeout_s: float = 300.0):
         cmd = [prof, "--pmc", counter, "-d", out, "-o", "pmc", "--output-format", "csv", "--",
                sys.executable, os.path.abspath(__file__), "--roofline-probe", "--n", str(args.n),
                "--jitter", str(args.jitter), "--sigma", str(args.sigma)]
-        res["command"].append(" ".join(cmd[:7] + ["--", "python", "bench.py"] + cmd[10:]))
+        res["command"].append(" ".join(cmd[:9] + ["--", "python", "bench.py"] + cmd[12:]))
         env = dict(os.environ, TMPDIR="/tmp")
         try:
             r = subprocess.run(cmd, capture_output=True, text=True, timeout=timeout_s, env=env,

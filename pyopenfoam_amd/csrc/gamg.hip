@@ -132,11 +132,13 @@ __device__ __forceinline__ T lvld(const LevelDevT<T> &L, const T *a, int i) {
   return L.nt ? __builtin_nontemporal_load(a + i) : a[i];
 }
 
-template <typename T>
+// kPairs = false keeps the int32 columns (the one-workgroup tail, whose tiny
+// levels gain nothing from the pairs and whose registers are all in use).
+template <typename T, bool kPairs = true>
 __device__ __forceinline__ T level_ax(const LevelDevT<T> &L, int s, int lane, int row, const T *x) {
   const SliceInfo si = load_slice(L.slices, s);
   auto gx = [x](int c) { return x[c]; };
-  if (L.cp) {  // paired int16 columns: half the index bytes
+  if (kPairs && L.cp) {  // paired int16 columns: half the index bytes
     if (L.nt) {
       const T dx = row < L.n ? __builtin_nontemporal_load(L.diag + row) * x[row] : T(0);
       return row_spmv_pair<T, decltype(gx), false, true>(si, s, lane, row, dx, L.cp, L.packed, gx);
@@ -262,7 +264,7 @@ __device__ void tail_jacobi(const LevelDev &L, int cur) {
   for (int row0 = threadIdx.x; row0 < L.nslices * kSlice; row0 += kTailThreads) {
     const int s = row0 >> 6, lane = row0 & 63;
     const double *x = L.x[cur];
-    const double ax = level_ax(L, s, lane, row0, x);
+    const double ax = level_ax<double, false>(L, s, lane, row0, x);
     if (row0 < L.n) L.x[cur ^ 1][row0] = x[row0] + L.inv[row0] * (L.b[row0] - ax);
   }
   __syncthreads();
@@ -497,7 +499,7 @@ __global__ __launch_bounds__(kTailThreads) void k_tail(const LevelDev *__restric
     for (int it = 0; it < n_pre; ++it) smooth(L, k);
     const double *xk = xbuf(L, parity(k));
     for (int row0 = threadIdx.x; row0 < L.nslices * kSlice; row0 += kTailThreads) {
-      const double ax = level_ax(L, row0 >> 6, row0 & 63, row0, xk);
+      const double ax = level_ax<double, false>(L, row0 >> 6, row0 & 63, row0, xk);
       if (row0 < L.n) L.r[row0] = L.b[row0] - ax;
     }
     __syncthreads();
