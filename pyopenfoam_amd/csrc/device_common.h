@@ -318,6 +318,102 @@ __device__ __forceinline__ SliceInfo load_slice_lane(const SliceInfo *sl, int s)
 // face index visits the faces in the reference's face order.
 // kAssign: x[c] = new value (SmoothSolver's sweep, smooth_solver.py:170-180)
 // instead of x[c] + omega (new - x[c]) (GaussSeidelSmoother).
+// The update of cell c whose lower slots start at bl (slot ptr + lane) in a
+// slice of group widths wl, wu; `live` false computes without storing (a
+// sweep enqueued after its solve stopped).
+template <typename T, bool kAssign = false>
+__device__ __forceinline__ void gs_core(int c, int bl, int wl, int wu,
+                                        const int32_t *__restrict__ col,
+                                        const int32_t *__restrict__ map,
+                                        const T *__restrict__ val,
+                                        const T *__restrict__ diag,
+                                        const T *__restrict__ b, T *x, T omega, bool live) {
+  const int bu = bl + wl * kSlice;
+  const T d = diag[c];
+  const T xc = x[c];
+  T row = d * xc;
+  if (wl <= 4 && wu <= 4) {
+    // All of the row's couplings loaded at once (no chain of dependent loads
+    // through the merge), the products formed, then added in face order: an
+    // entry's position in the merged order is its index in its group plus
+    // the number of the other group's faces before it (a face is in only one
+    // group of a row, so no two faces tie).  Padding slots are a suffix of
+    // each group and are skipped.
+    constexpr int U = 4;
+    int cl[U], cu[U], ml[U], mu[U];
+    T vl[U], vu[U];
+#pragma unroll
+    for (int j = 0; j < U; ++j) {
+      const bool hl = j < wl, hu = j < wu;
+      cl[j] = hl ? col[bl + j * kSlice] : -1;
+      ml[j] = hl ? map[bl + j * kSlice] : 0;
+      vl[j] = hl ? val[bl + j * kSlice] : T(0);
+      cu[j] = hu ? col[bu + j * kSlice] : -1;
+      mu[j] = hu ? map[bu + j * kSlice] : 0;
+      vu[j] = hu ? val[bu + j * kSlice] : T(0);
+    }
+    T pl[U], pu[U];
+#pragma unroll
+    for (int j = 0; j < U; ++j) {
+      pl[j] = cl[j] >= 0 ? vl[j] * x[cl[j]] : T(0);
+      pu[j] = cu[j] >= 0 ? vu[j] * x[cu[j]] : T(0);
+    }
+    int rl[U], ru[U];
+#pragma unroll
+    for (int i = 0; i < U; ++i) {
+      rl[i] = i;
+      ru[i] = i;
+#pragma unroll
+      for (int j = 0; j < U; ++j) {
+        rl[i] += (cu[j] >= 0 && mu[j] < ml[i]) ? 1 : 0;
+        ru[i] += (cl[j] >= 0 && ml[j] < mu[i]) ? 1 : 0;
+      }
+    }
+#pragma unroll
+    for (int k = 0; k < 2 * U; ++k) {
+      bool has = false;
+      T pick = T(0);
+#pragma unroll
+      for (int i = 0; i < U; ++i) {
+        if (cl[i] >= 0 && rl[i] == k) {
+          pick = pl[i];
+          has = true;
+        }
+        if (cu[i] >= 0 && ru[i] == k) {
+          pick = pu[i];
+          has = true;
+        }
+      }
+      if (has) row += pick;
+    }
+  } else {
+  int jl = 0, ju = 0;
+  int cl = jl < wl ? col[bl] : -1;
+  int cu = ju < wu ? col[bu] : -1;
+  while (cl >= 0 || cu >= 0) {
+    bool take_l = cl >= 0;
+    if (cl >= 0 && cu >= 0) take_l = map[bl + jl * kSlice] < map[bu + ju * kSlice];
+    if (take_l) {
+      row += val[bl + jl * kSlice] * x[cl];
+      ++jl;
+      cl = jl < wl ? col[bl + jl * kSlice] : -1;
+    } else {
+      row += val[bu + ju * kSlice] * x[cu];
+      ++ju;
+      cu = ju < wu ? col[bu + ju * kSlice] : -1;
+    }
+  }
+  }
+  const T off = row - d * xc;
+  if (live && fabs(d) > T(1e-30)) {
+    const T nv = (b[c] - off) / d;
+    if (kAssign)
+      x[c] = nv;
+    else
+      x[c] = xc + omega * (nv - xc);
+  }
+}
+
 template <typename T, bool kAssign = false>
 __device__ __forceinline__ void gs_cell(int c, const SliceInfo *__restrict__ sl,
                                         const int32_t *__restrict__ col,
@@ -326,35 +422,19 @@ __device__ __forceinline__ void gs_cell(int c, const SliceInfo *__restrict__ sl,
                                         const T *__restrict__ diag,
                                         const T *__restrict__ b, T *x, T omega) {
   const SliceInfo si = load_slice_lane(sl, c / kSlice);  // lanes hold cells of any slice
-  const int lane = c % kSlice;
-  const int bl = si.ptr + lane, bu = si.ptr + si.wl * kSlice + lane;
-  const T d = diag[c];
-  const T xc = x[c];
-  T row = d * xc;
-  int jl = 0, ju = 0;
-  int cl = jl < si.wl ? col[bl] : -1;
-  int cu = ju < si.wu ? col[bu] : -1;
-  while (cl >= 0 || cu >= 0) {
-    bool take_l = cl >= 0;
-    if (cl >= 0 && cu >= 0) take_l = map[bl + jl * kSlice] < map[bu + ju * kSlice];
-    if (take_l) {
-      row += val[bl + jl * kSlice] * x[cl];
-      ++jl;
-      cl = jl < si.wl ? col[bl + jl * kSlice] : -1;
-    } else {
-      row += val[bu + ju * kSlice] * x[cu];
-      ++ju;
-      cu = ju < si.wu ? col[bu + ju * kSlice] : -1;
-    }
-  }
-  const T off = row - d * xc;
-  if (fabs(d) > T(1e-30)) {
-    const T nv = (b[c] - off) / d;
-    if (kAssign)
-      x[c] = nv;
-    else
-      x[c] = xc + omega * (nv - xc);
-  }
+  gs_core<T, kAssign>(c, si.ptr + c % kSlice, si.wl, si.wu, col, map, val, diag, b, x, omega,
+                      true);
+}
+
+// gs_core from a schedule entry (cell, lower-slot base, wl | wu << 16).
+template <typename T, bool kAssign = false>
+__device__ __forceinline__ void gs_entry(const int4 e, const int32_t *__restrict__ col,
+                                         const int32_t *__restrict__ map,
+                                         const T *__restrict__ val,
+                                         const T *__restrict__ diag,
+                                         const T *__restrict__ b, T *x, T omega, bool live) {
+  gs_core<T, kAssign>(e.x, e.y, e.z & 0xffff, (int)((unsigned)e.z >> 16), col, map, val, diag, b,
+                      x, omega, live);
 }
 
 }  // namespace pfm

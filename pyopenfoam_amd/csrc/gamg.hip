@@ -167,14 +167,16 @@ __global__ __launch_bounds__(kBlock) void k_jacobi(LevelDevT<T> L, int cur, cons
 }
 
 // One Gauss-Seidel sweep level of a level's x[cur], in place.
+// The stop flag is read beside the cell's operands and checked before the
+// store, so its load does not head the chain of dependent loads.
 template <typename T>
 __global__ __launch_bounds__(kBlock) void k_gs_level(LevelDevT<T> L, int cur,
-                                                      const int32_t *__restrict__ cells, int count,
+                                                      const int4 *__restrict__ ents, int count,
                                                       const KState *st) {
-  if (st->done) return;
+  const bool live = !st->done;
   const int t = blockIdx.x * kBlock + threadIdx.x;
   if (t < count)
-    gs_cell<T>(cells[t], L.slices, L.col, L.map, L.packed, L.diag, L.b, L.x[cur], T(1));
+    gs_entry<T>(ents[t], L.col, L.map, L.packed, L.diag, L.b, L.x[cur], T(1), live);
 }
 
 template <typename T>
@@ -274,11 +276,11 @@ __device__ void tail_jacobi(const LevelDev &L, int cur) {
 __device__ void tail_gs(const LevelDev &L, int cur) {
   for (int pass = 0; pass < 2; ++pass) {
     const int32_t *ptr = pass ? L.gs_bptr : L.gs_fptr;
-    const int32_t *cells = pass ? L.gs_bcells : L.gs_fcells;
+    const int4 *ents = pass ? L.gs_bent : L.gs_fent;
     const int nl = pass ? L.gs_nbl : L.gs_nfl;
     for (int l = 0; l < nl; ++l) {
       for (int k = ptr[l] + (int)threadIdx.x; k < ptr[l + 1]; k += kTailThreads)
-        gs_cell<double>(cells[k], L.slices, L.col, L.map, L.packed, L.diag, L.b, L.x[cur], 1.0);
+        gs_entry<double>(ents[k], L.col, L.map, L.packed, L.diag, L.b, L.x[cur], 1.0, true);
       __syncthreads();
     }
   }
@@ -912,8 +914,10 @@ size_t gamg_layout(const pfm_gamg_s *g, int max_iter, GamgWork *w, char *base) {
     D.map = H.ldu->d_map;
     D.gs_fptr = H.ldu->d_gs_fwd_ptr;
     D.gs_fcells = H.ldu->d_gs_fwd_cells;
+    D.gs_fent = H.ldu->d_gs_fwd_ent;
     D.gs_bptr = H.ldu->d_gs_bwd_ptr;
     D.gs_bcells = H.ldu->d_gs_bwd_cells;
+    D.gs_bent = H.ldu->d_gs_bwd_ent;
     D.gs_nfl = (int32_t)H.ldu->gs_fwd_ptr.size() - 1;
     D.gs_nbl = (int32_t)H.ldu->gs_bwd_ptr.size() - 1;
     t.packed[k] = (double *)take(sizeof(double) * (size_t)std::max<int64_t>(H.ldu->nslots, 1));
@@ -1010,8 +1014,10 @@ size_t gamg_mixed_layout(const pfm_gamg_s *g, int max_outer, int fp32_levels, Ga
     D.map = D64.map;
     D.gs_fptr = D64.gs_fptr;
     D.gs_fcells = D64.gs_fcells;
+    D.gs_fent = D64.gs_fent;
     D.gs_bptr = D64.gs_bptr;
     D.gs_bcells = D64.gs_bcells;
+    D.gs_bent = D64.gs_bent;
     D.gs_nfl = D64.gs_nfl;
     D.gs_nbl = D64.gs_nbl;
     D.f2c = D64.f2c;
@@ -1137,10 +1143,10 @@ void enqueue_cycle(const pfm_gamg_s *g, const CycleLevels<T> &C, const KState *s
       const pfm_ldu h = g->lv[k].ldu;
       for (int pass = 0; pass < 2; ++pass) {
         const std::vector<int32_t> &ptr = pass ? h->gs_bwd_ptr : h->gs_fwd_ptr;
-        const int32_t *cells = pass ? h->d_gs_bwd_cells : h->d_gs_fwd_cells;
+        const int4 *ents = pass ? h->d_gs_bwd_ent : h->d_gs_fwd_ent;
         for (size_t l = 0; l + 1 < ptr.size(); ++l) {
           const int cnt = ptr[l + 1] - ptr[l];
-          k_gs_level<V><<<grid1d(cnt), kBlock, 0, s>>>(D, cur, cells + ptr[l], cnt, st);
+          k_gs_level<V><<<grid1d(cnt), kBlock, 0, s>>>(D, cur, ents + ptr[l], cnt, st);
         }
       }
       return;

@@ -578,6 +578,8 @@ int pfm_ldu_destroy(pfm_ldu h) {
   (void)hipFree(h->d_gs_fwd_cells);
   (void)hipFree(h->d_gs_bwd_ptr);
   (void)hipFree(h->d_gs_bwd_cells);
+  (void)hipFree(h->d_gs_fwd_ent);
+  (void)hipFree(h->d_gs_bwd_ent);
   poll_pool_free(h->poll);
   (void)hipSetDevice(dev_prev);
   delete h;
@@ -622,6 +624,24 @@ int pfm_ldu_create(int32_t n, int32_t nf, const int32_t *own, const int32_t *nei
       (st = upload(&h->d_gs_bwd_cells, L.gs_bwd_cells))) {
     pfm_ldu_destroy(h);
     return st;
+  }
+  {
+    // Gauss-Seidel schedule entries: each scheduled cell with its row's
+    // lower-slot base and group widths, so a sweep reads them in one load.
+    auto entries = [&](const std::vector<int32_t> &cells) {
+      std::vector<int4> e(cells.size());
+      for (size_t k = 0; k < cells.size(); ++k) {
+        const int32_t c = cells[k];
+        const SliceInfo si = L.slices[c / kSlice];
+        e[k] = make_int4(c, si.ptr + c % kSlice, (int)(uint16_t)si.wl | ((int)si.wu << 16), 0);
+      }
+      return e;
+    };
+    if ((st = upload(&h->d_gs_fwd_ent, entries(L.gs_fwd_cells))) ||
+        (st = upload(&h->d_gs_bwd_ent, entries(L.gs_bwd_cells)))) {
+      pfm_ldu_destroy(h);
+      return st;
+    }
   }
   {
     // Column offsets from the row in int16 when every one fits: the PCG's

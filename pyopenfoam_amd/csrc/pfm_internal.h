@@ -148,6 +148,7 @@ struct pfm_ldu_s {
   std::vector<int32_t> gs_fwd_ptr, gs_bwd_ptr;  // host copies of GS level offsets
   int32_t *d_gs_fwd_ptr = nullptr, *d_gs_fwd_cells = nullptr;
   int32_t *d_gs_bwd_ptr = nullptr, *d_gs_bwd_cells = nullptr;
+  int4 *d_gs_fwd_ent = nullptr, *d_gs_bwd_ent = nullptr;  // per scheduled cell: gs_entry()
   pfm::PollPool poll;                   // pinned stop-flag slots of the solves
   bool oriented_sorted = false;         // every owner < neighbour, owners non-decreasing
   pfm_progress_fn progress = nullptr;   // optional host progress report

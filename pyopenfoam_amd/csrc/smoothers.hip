@@ -12,8 +12,7 @@ namespace {
 inline int grid1d(int64_t n) { return (int)((n + kBlock - 1) / kBlock); }
 
 // One sweep level: its cells are mutually independent.
-__global__ __launch_bounds__(kBlock) void k_gs_level(const int32_t *__restrict__ cells, int count,
-                                                      const SliceInfo *__restrict__ sl,
+__global__ __launch_bounds__(kBlock) void k_gs_level(const int4 *__restrict__ ents, int count,
                                                       const int32_t *__restrict__ col,
                                                       const int32_t *__restrict__ map,
                                                       const double *__restrict__ val,
@@ -21,7 +20,7 @@ __global__ __launch_bounds__(kBlock) void k_gs_level(const int32_t *__restrict__
                                                       const double *__restrict__ b, double *x,
                                                       double omega) {
   const int t = blockIdx.x * kBlock + threadIdx.x;
-  if (t < count) gs_cell(cells[t], sl, col, map, val, diag, b, x, omega);
+  if (t < count) gs_entry(ents[t], col, map, val, diag, b, x, omega, true);
 }
 
 constexpr int kGsThreads = 1024;
@@ -29,19 +28,19 @@ constexpr int kGsThreads = 1024;
 // Small meshes: one workgroup walks every level of n_sweeps symmetric sweeps,
 // with a barrier between levels.
 __global__ __launch_bounds__(kGsThreads) void k_gs_sweeps_wg(
-    int n_sweeps, const int32_t *__restrict__ fptr, int nfl, const int32_t *__restrict__ fcells,
-    const int32_t *__restrict__ bptr, int nbl, const int32_t *__restrict__ bcells,
-    const SliceInfo *__restrict__ sl, const int32_t *__restrict__ col,
+    int n_sweeps, const int32_t *__restrict__ fptr, int nfl, const int4 *__restrict__ fents,
+    const int32_t *__restrict__ bptr, int nbl, const int4 *__restrict__ bents,
+    const int32_t *__restrict__ col,
     const int32_t *__restrict__ map, const double *__restrict__ val,
     const double *__restrict__ diag, const double *__restrict__ b, double *x, double omega) {
   for (int s = 0; s < n_sweeps; ++s) {
     for (int pass = 0; pass < 2; ++pass) {
       const int32_t *ptr = pass ? bptr : fptr;
-      const int32_t *cells = pass ? bcells : fcells;
+      const int4 *ents = pass ? bents : fents;
       const int nl = pass ? nbl : nfl;
       for (int l = 0; l < nl; ++l) {
         for (int k = ptr[l] + (int)threadIdx.x; k < ptr[l + 1]; k += kGsThreads)
-          gs_cell(cells[k], sl, col, map, val, diag, b, x, omega);
+          gs_entry(ents[k], col, map, val, diag, b, x, omega, true);
         __syncthreads();
       }
     }
@@ -195,22 +194,20 @@ __global__ __launch_bounds__(kBlock) void k_sum_sqrt(const double *__restrict__ 
 // SmoothSolver (smooth_solver.py:96-232).
 
 // One level of a forward Gauss-Seidel sweep with direct assignment.
-__global__ __launch_bounds__(kBlock) void k_sm_gs_level(const int32_t *__restrict__ cells, int count,
-                                                         const SliceInfo *__restrict__ sl,
+__global__ __launch_bounds__(kBlock) void k_sm_gs_level(const int4 *__restrict__ ents, int count,
                                                          const int32_t *__restrict__ col,
                                                          const int32_t *__restrict__ map,
                                                          const double *__restrict__ val,
                                                          const double *__restrict__ diag,
                                                          const double *__restrict__ b, double *x,
                                                          const KState *st) {
-  if (st->done) return;
+  const bool live = !st->done;  // checked before the store, off the load chain
   const int t = blockIdx.x * kBlock + threadIdx.x;
-  if (t < count) gs_cell<double, true>(cells[t], sl, col, map, val, diag, b, x, 1.0);
+  if (t < count) gs_entry<double, true>(ents[t], col, map, val, diag, b, x, 1.0, live);
 }
 
 __global__ __launch_bounds__(kGsThreads) void k_sm_gs_wg(int n_sweeps, const int32_t *__restrict__ fptr,
-                                                          int nfl, const int32_t *__restrict__ fcells,
-                                                          const SliceInfo *__restrict__ sl,
+                                                          int nfl, const int4 *__restrict__ fents,
                                                           const int32_t *__restrict__ col,
                                                           const int32_t *__restrict__ map,
                                                           const double *__restrict__ val,
@@ -221,7 +218,7 @@ __global__ __launch_bounds__(kGsThreads) void k_sm_gs_wg(int n_sweeps, const int
   for (int sw = 0; sw < n_sweeps; ++sw)
     for (int l = 0; l < nfl; ++l) {
       for (int k = fptr[l] + (int)threadIdx.x; k < fptr[l + 1]; k += kGsThreads)
-        gs_cell<double, true>(fcells[k], sl, col, map, val, diag, b, x, 1.0);
+        gs_entry<double, true>(fents[k], col, map, val, diag, b, x, 1.0, true);
       __syncthreads();
     }
 }
@@ -323,18 +320,18 @@ int pfm_gs_smooth(pfm_ldu h, const double *diag, const double *packed, const dou
   const int nfl = (int)h->gs_fwd_ptr.size() - 1, nbl = (int)h->gs_bwd_ptr.size() - 1;
   const int levels = std::max(nfl, nbl);
   if (h->n < 2048 * levels) {
-    k_gs_sweeps_wg<<<1, kGsThreads, 0, s>>>(n_sweeps, h->d_gs_fwd_ptr, nfl, h->d_gs_fwd_cells,
-                                            h->d_gs_bwd_ptr, nbl, h->d_gs_bwd_cells, h->d_slices,
-                                            h->d_col, h->d_map, packed, diag, b, x, omega);
+    k_gs_sweeps_wg<<<1, kGsThreads, 0, s>>>(n_sweeps, h->d_gs_fwd_ptr, nfl, h->d_gs_fwd_ent,
+                                            h->d_gs_bwd_ptr, nbl, h->d_gs_bwd_ent, h->d_col,
+                                            h->d_map, packed, diag, b, x, omega);
   } else {
     for (int sw = 0; sw < n_sweeps; ++sw) {
       for (int pass = 0; pass < 2; ++pass) {
         const std::vector<int32_t> &ptr = pass ? h->gs_bwd_ptr : h->gs_fwd_ptr;
-        const int32_t *cells = pass ? h->d_gs_bwd_cells : h->d_gs_fwd_cells;
+        const int4 *ents = pass ? h->d_gs_bwd_ent : h->d_gs_fwd_ent;
         for (size_t l = 0; l + 1 < ptr.size(); ++l) {
           const int cnt = ptr[l + 1] - ptr[l];
-          k_gs_level<<<grid1d(cnt), kBlock, 0, s>>>(cells + ptr[l], cnt, h->d_slices, h->d_col,
-                                                    h->d_map, packed, diag, b, x, omega);
+          k_gs_level<<<grid1d(cnt), kBlock, 0, s>>>(ents + ptr[l], cnt, h->d_col, h->d_map, packed,
+                                                    diag, b, x, omega);
         }
       }
     }
@@ -480,16 +477,14 @@ int pfm_smooth_solve(pfm_ldu h, const pfm_solve_params *prm, int smoother, int32
         const int it = enq + k + 1;
         if (smoother == PFM_SMOOTH_GAUSS_SEIDEL) {
           if (wg) {
-            k_sm_gs_wg<<<1, kGsThreads, 0, s>>>(n_sweeps, h->d_gs_fwd_ptr, nfl, h->d_gs_fwd_cells,
-                                                h->d_slices, h->d_col, h->d_map, packed, diag, b,
-                                                x, w.st);
+            k_sm_gs_wg<<<1, kGsThreads, 0, s>>>(n_sweeps, h->d_gs_fwd_ptr, nfl, h->d_gs_fwd_ent,
+                                                h->d_col, h->d_map, packed, diag, b, x, w.st);
           } else {
             for (int sw = 0; sw < n_sweeps; ++sw)
               for (int l = 0; l < nfl; ++l) {
                 const int bb = h->gs_fwd_ptr[l], cnt = h->gs_fwd_ptr[l + 1] - bb;
-                k_sm_gs_level<<<grid1d(cnt), kBlock, 0, s>>>(h->d_gs_fwd_cells + bb, cnt,
-                                                             h->d_slices, h->d_col, h->d_map,
-                                                             packed, diag, b, x, w.st);
+                k_sm_gs_level<<<grid1d(cnt), kBlock, 0, s>>>(h->d_gs_fwd_ent + bb, cnt, h->d_col,
+                                                             h->d_map, packed, diag, b, x, w.st);
               }
           }
         } else {
