@@ -167,13 +167,17 @@ def roofline_probe(args) -> int:
 
 
 def spmv_kernel_name(case) -> str:
-    """The kernel pfm_spmv streams with: the paired int16 column offsets when
-    every coupling's column is within 32767 of its row (the handle then keeps
+    """The kernel pfm_spmv streams with: the paired 16-bit column offsets when
+    every coupling's column is within 32767 of its row, or within 65535 on a
+    mesh whose owners all precede their neighbours (the handle then keeps
     them), else the int32 columns."""
     import numpy as np
 
-    band = int(np.abs(case.owner.astype(np.int64) - case.neighbour).max()) if len(case.owner) else 0
-    if band <= 32767 and os.environ.get("PFM_SPMV_COLS") != "int32":
+    nf = case.mesh.n_internal_faces
+    d = case.neighbour.astype(np.int64) - case.owner[:nf]
+    band = int(np.abs(d).max()) if nf else 0
+    oriented = bool(nf == 0 or d.min() > 0)
+    if (band <= 32767 or (oriented and band <= 65535)) and os.environ.get("PFM_SPMV_COLS") != "int32":
         return "k_spmv_p2<double, true>"
     return "k_spmv<double, true>"
 

@@ -201,15 +201,17 @@ inline int krylov_col_form(const pfm_ldu_s *h) {
     if (std::strcmp(e, "int16") == 0) return (int)kCols16;
     return -1;
   }();
-  if (forced == kCols32 || !h->d_col16) return kCols32;
-  if (forced == kCols16 || !h->d_colp2) return kCols16;
+  if (forced == kCols32) return kCols32;
+  if (forced == kCols16 || !h->d_colp2) return h->d_col16 ? kCols16 : kCols32;
   return kColsPair;
 }
 
-// Paired int16 column offsets.  Word j of lane l in slice s, at
+// Paired 16-bit column offsets.  Word j of lane l in slice s, at
 // pair_base(s) + 64 j + l, holds the offsets from the row of the slice's slots
 // 2j (low half) and 2j+1 (high half), slots counted over the lower group then
-// the upper group; a missing or padding slot holds kColPad16.  A 4-byte load
+// the upper group.  The word before the array names the encoding: signed
+// (col = row + int16, kColPad16 = padding) or oriented (col = row + u in the
+// lower group, row - u in the upper one, u = 0 = padding).  A 4-byte load
 // per lane then fetches two slots' columns: half the index bytes of int32
 // columns with as many bytes per load instruction (a 2-byte load per slot
 // moves half as much per instruction and was slower than int32 columns).
@@ -217,8 +219,14 @@ inline int krylov_col_form(const pfm_ldu_s *h) {
 // for 64 ceil((wl + wu) / 2); the words past them are never read.
 __device__ __forceinline__ int pair_base(const SliceInfo si, int s) { return si.ptr / 2 + 32 * s; }
 
-__device__ __forceinline__ int pair_col(uint32_t w, int hi, int row) {
-  const int v = hi ? (int)(int16_t)(w >> 16) : (int)(int16_t)(w & 0xffffu);
+__device__ __forceinline__ bool pair_oriented(const uint32_t *__restrict__ cp) {
+  return cp[-1] == kPairOriented;
+}
+
+__device__ __forceinline__ int pair_col(uint32_t w, int hi, int row, bool lower, bool ori) {
+  const uint32_t u = hi ? (w >> 16) : (w & 0xffffu);
+  if (ori) return u == 0 ? -1 : (lower ? row + (int)u : row - (int)u);
+  const int v = (int)(int16_t)u;
   return v == kColPad16 ? -1 : row + v;
 }
 
@@ -228,7 +236,7 @@ __device__ __forceinline__ int pair_col(uint32_t w, int hi, int row) {
 // two sums in face order.
 template <int WL, typename T, typename Gather, bool kUpperFirst, bool kNT>
 __device__ __forceinline__ T row_pair_fixed(int wu, int ptr, int pb, int lane, int row, T dx,
-                                            const uint32_t *__restrict__ cp,
+                                            const uint32_t *__restrict__ cp, bool ori,
                                             const T *__restrict__ val, Gather gx) {
   constexpr int U = 4;
   constexpr int NS = WL + U;
@@ -243,7 +251,7 @@ __device__ __forceinline__ T row_pair_fixed(int wu, int ptr, int pb, int lane, i
   for (int j = 0; j < NS; ++j) v[j] = j < nsl ? ld<kNT>(val + ptr + j * kSlice + lane) : T(0);
   int c[NS];
 #pragma unroll
-  for (int j = 0; j < NS; ++j) c[j] = j < nsl ? pair_col(w[j >> 1], j & 1, row) : -1;
+  for (int j = 0; j < NS; ++j) c[j] = j < nsl ? pair_col(w[j >> 1], j & 1, row, j < WL, ori) : -1;
   T xv[NS];
 #pragma unroll
   for (int j = 0; j < NS; ++j) xv[j] = c[j] >= 0 ? gx(c[j]) : T(0);
@@ -264,13 +272,14 @@ __device__ __forceinline__ T row_spmv_pair(const SliceInfo si, int s, int lane, 
                                            const T *__restrict__ val, Gather gx) {
   const int wl = si.wl, wu = si.wu;
   const int pb = pair_base(si, s);
+  const bool ori = pair_oriented(cp);  // one scalar load per wave
   if (wu <= 4) {
     switch (wl) {  // wave-uniform
-      case 0: return row_pair_fixed<0, T, Gather, kUpperFirst, kNT>(wu, si.ptr, pb, lane, row, dx, cp, val, gx);
-      case 1: return row_pair_fixed<1, T, Gather, kUpperFirst, kNT>(wu, si.ptr, pb, lane, row, dx, cp, val, gx);
-      case 2: return row_pair_fixed<2, T, Gather, kUpperFirst, kNT>(wu, si.ptr, pb, lane, row, dx, cp, val, gx);
-      case 3: return row_pair_fixed<3, T, Gather, kUpperFirst, kNT>(wu, si.ptr, pb, lane, row, dx, cp, val, gx);
-      case 4: return row_pair_fixed<4, T, Gather, kUpperFirst, kNT>(wu, si.ptr, pb, lane, row, dx, cp, val, gx);
+      case 0: return row_pair_fixed<0, T, Gather, kUpperFirst, kNT>(wu, si.ptr, pb, lane, row, dx, cp, ori, val, gx);
+      case 1: return row_pair_fixed<1, T, Gather, kUpperFirst, kNT>(wu, si.ptr, pb, lane, row, dx, cp, ori, val, gx);
+      case 2: return row_pair_fixed<2, T, Gather, kUpperFirst, kNT>(wu, si.ptr, pb, lane, row, dx, cp, ori, val, gx);
+      case 3: return row_pair_fixed<3, T, Gather, kUpperFirst, kNT>(wu, si.ptr, pb, lane, row, dx, cp, ori, val, gx);
+      case 4: return row_pair_fixed<4, T, Gather, kUpperFirst, kNT>(wu, si.ptr, pb, lane, row, dx, cp, ori, val, gx);
       default: break;
     }
   }
@@ -278,7 +287,7 @@ __device__ __forceinline__ T row_spmv_pair(const SliceInfo si, int s, int lane, 
   T accL = T(0), accU = T(0);
   for (int j = 0; j < wl + wu; ++j) {
     const uint32_t wd = ld<kNT>(cp + pb + (j >> 1) * kSlice + lane);
-    const int c = pair_col(wd, j & 1, row);
+    const int c = pair_col(wd, j & 1, row, j < wl, ori);
     if (c < 0) continue;
     const T p = ld<kNT>(val + si.ptr + j * kSlice + lane) * gx(c);
     if (j < wl)

@@ -565,7 +565,7 @@ int pfm_ldu_destroy(pfm_ldu h) {
   (void)hipFree(h->d_slices);
   (void)hipFree(h->d_col);
   (void)hipFree(h->d_col16);
-  (void)hipFree(h->d_colp2);
+  (void)hipFree(h->d_colp2_alloc);
   (void)hipFree(h->d_map);
   (void)hipFree(h->d_own);
   (void)hipFree(h->d_nei);
@@ -670,25 +670,39 @@ int pfm_ldu_create(int32_t n, int32_t nf, const int32_t *own, const int32_t *nei
       pfm_ldu_destroy(h);
       return st;
     }
-    if (fits) {
-      // The same offsets two slots per 4-byte word (pair_base layout): slice s
-      // owns 32 (wl + wu + 1) words from ptr/2 + 32 s.
-      std::vector<uint32_t> p2((size_t)(L.nslots / 2 + (int64_t)32 * L.nslices), 0x80008000u);
+    // The offsets two slots per 4-byte word (pair_base layout: slice s owns
+    // 32 (wl + wu + 1) words from ptr/2 + 32 s), in one of two encodings,
+    // recorded in the word before the array (kPairHeader words precede it):
+    // on an oriented topology (every owner < its neighbour) whose columns lie
+    // within 65535 of their rows, unsigned magnitudes (a lower-group column is
+    // row + u, an upper-group one row - u, 0 = padding: twice the signed
+    // reach); otherwise the signed int16 offsets above when they fit.
+    bool ori = !(env && env[0] == '0');
+    for (int32_t f = 0; f < nf && ori; ++f) ori = own[f] < nei[f] && nei[f] - own[f] <= 65535;
+    if (ori || fits) {
+      const uint32_t pad = ori ? 0u : 0x8000u;
+      const int64_t nwords = L.nslots / 2 + (int64_t)32 * L.nslices;
+      std::vector<uint32_t> p2((size_t)(kPairHeader + nwords), pad | (pad << 16));
+      p2[kPairHeader - 1] = ori ? kPairOriented : kPairSigned;
       for (size_t s = 0; s < L.slices.size(); ++s) {
         const SliceInfo si = L.slices[s];
-        const int64_t pb = si.ptr / 2 + (int64_t)32 * s;
+        const int64_t pb = kPairHeader + si.ptr / 2 + (int64_t)32 * s;
         const int width = si.wl + si.wu;
         for (int j = 0; j < width; ++j)
           for (int lane = 0; lane < kSlice; ++lane) {
-            const uint32_t half = (uint16_t)c16[(int64_t)si.ptr + (int64_t)j * kSlice + lane];
+            const int32_t c = L.col[(int64_t)si.ptr + (int64_t)j * kSlice + lane];
+            const int64_t d = (int64_t)c - ((int64_t)s * kSlice + lane);
+            uint32_t half = pad;
+            if (c >= 0) half = ori ? (uint32_t)(j < si.wl ? d : -d) : (uint32_t)(uint16_t)(int16_t)d;
             uint32_t &w = p2[pb + (int64_t)(j >> 1) * kSlice + lane];
             w = (j & 1) ? ((w & 0xffffu) | (half << 16)) : ((w & 0xffff0000u) | half);
           }
       }
-      if ((st = upload(&h->d_colp2, p2))) {
+      if ((st = upload(&h->d_colp2_alloc, p2))) {
         pfm_ldu_destroy(h);
         return st;
       }
+      h->d_colp2 = h->d_colp2_alloc + kPairHeader;
     }
   }
   if ((st = poll_pool_init(h->poll)) != PFM_OK) {

@@ -53,6 +53,12 @@ constexpr int kBlock = 256;    // threads per workgroup
 constexpr int kWavesPerBlock = kBlock / kSlice;
 constexpr int kXcds = 8;       // MI355X: 8 XCDs, workgroups dealt round-robin
 
+// Paired column offsets: the allocation holds kPairHeader words before the
+// array (256-byte aligned), the last of which names the encoding.
+constexpr int kPairHeader = 64;
+constexpr uint32_t kPairSigned = 1;    // int16 col - row, -32768 = padding
+constexpr uint32_t kPairOriented = 2;  // uint16 |col - row|, lower +, upper -, 0 = padding
+
 // Per-slice layout: slot offset, lower and upper slot counts.
 struct SliceInfo {
   int32_t ptr;
@@ -136,7 +142,8 @@ struct pfm_ldu_s {
   pfm::SliceInfo *d_slices = nullptr;
   int32_t *d_col = nullptr;
   int16_t *d_col16 = nullptr;           // column - row per slot, when all fit (else null)
-  uint32_t *d_colp2 = nullptr;          // the same offsets paired per lane (pair_base), or null
+  uint32_t *d_colp2 = nullptr;          // column offsets paired per lane (pair_base), or null
+  uint32_t *d_colp2_alloc = nullptr;    // its allocation: kPairHeader words, then d_colp2
   int32_t *d_map = nullptr;
   int32_t *d_own = nullptr;
   int32_t *d_nei = nullptr;
