@@ -122,16 +122,21 @@ def test_factor_unordered_faces(gpu_device):
     assert it == io and bits_equal(xs.cpu().numpy(), xo)
 
 
-def test_spmv_and_pcg_wide_bandwidth(gpu_device):
-    """A randomly permuted 36^3 mesh without the RCM renumbering: column
-    offsets beyond the int16 range, so the SpMV and PCG kernels read the int32
-    columns; both stay bit-exact (the renumbered cases above read the int16
-    row offsets)."""
+@pytest.mark.parametrize("n,band_lo,band_hi", [(36, 32767, 65535), (42, 65535, None)])
+def test_spmv_and_pcg_wide_bandwidth(gpu_device, n, band_lo, band_hi):
+    """Randomly permuted meshes without the RCM renumbering (every owner still
+    precedes its neighbour).  At 36^3 the column offsets exceed the int16
+    range but stay within 65535: the SpMV and PCG kernels read the paired
+    offsets as unsigned magnitudes.  At 42^3 they exceed 65535 too and the
+    kernels read the int32 columns.  Both stay bit-exact (the renumbered cases
+    above read signed-range offsets)."""
     from oracle import krylov as okry
     from pyopenfoam_amd import PCGSolver
     from tests.helpers import make_case
-    c = make_case(36, jitter=0.0, sigma=1.0, seed=2, renumber=False)
-    assert np.abs(c.owner - c.neighbour).max() > 32767
+    c = make_case(n, jitter=0.0, sigma=1.0, seed=2, renumber=False)
+    band = int(np.abs(c.owner - c.neighbour).max())
+    assert band > band_lo and (band_hi is None or band <= band_hi)
+    assert (c.neighbour > c.owner).all()
     nfi = c.mesh.n_internal_faces
     d, lo, up = oldu.laplacian(c.gamma, c.geo.mag_sf[:nfi], c.geo.delta[:nfi], c.geo.cell_volumes,
                                c.owner, c.neighbour)
