@@ -69,18 +69,51 @@ __global__ __launch_bounds__(kBlock) void k_pe_rows(int n, int nf, int nslices,
   const int row = s * kSlice + lane;
   double d1 = 0.0, d2 = 0.0, t1 = 0.0, t2 = 0.0, t3 = 0.0;
   int base = si.ptr + lane;
-  for (int j = 0; j < si.wl; ++j, base += kSlice) {
-    const int f = map[base];
-    if (f >= 0) {
-      d1 += -lower[f];
-      t1 += -phi[f];
+  if (si.wl <= 4 && si.wu <= 4) {
+    // Every face index of the row loaded at once, then every gather, then the
+    // sums in slot order: two dependent round trips instead of one per slot.
+    constexpr int U = 4;
+    const int bu = base + si.wl * kSlice;
+    int fl[U], fu[U];
+#pragma unroll
+    for (int j = 0; j < U; ++j) {
+      fl[j] = j < si.wl ? map[base + j * kSlice] : -1;
+      fu[j] = j < si.wu ? map[bu + j * kSlice] : -1;
     }
-  }
-  for (int j = 0; j < si.wu; ++j, base += kSlice) {
-    const int f = map[base];
-    if (f >= 0) {
-      d2 += -upper[f];
-      t2 += phi[f];
+    double lv[U], pl[U], uv[U], pu[U];
+#pragma unroll
+    for (int j = 0; j < U; ++j) {
+      lv[j] = fl[j] >= 0 ? lower[fl[j]] : 0.0;
+      pl[j] = fl[j] >= 0 ? phi[fl[j]] : 0.0;
+      uv[j] = fu[j] >= 0 ? upper[fu[j]] : 0.0;
+      pu[j] = fu[j] >= 0 ? phi[fu[j]] : 0.0;
+    }
+#pragma unroll
+    for (int j = 0; j < U; ++j)
+      if (fl[j] >= 0) {
+        d1 += -lv[j];
+        t1 += -pl[j];
+      }
+#pragma unroll
+    for (int j = 0; j < U; ++j)
+      if (fu[j] >= 0) {
+        d2 += -uv[j];
+        t2 += pu[j];
+      }
+  } else {
+    for (int j = 0; j < si.wl; ++j, base += kSlice) {
+      const int f = map[base];
+      if (f >= 0) {
+        d1 += -lower[f];
+        t1 += -phi[f];
+      }
+    }
+    for (int j = 0; j < si.wu; ++j, base += kSlice) {
+      const int f = map[base];
+      if (f >= 0) {
+        d2 += -upper[f];
+        t2 += phi[f];
+      }
     }
   }
   if (row < n) {
@@ -140,22 +173,55 @@ __global__ __launch_bounds__(kBlock) void k_cvel(int n, int nf, int nslices,
   double gx = 0.0, gy = 0.0, gz = 0.0;
   int base = si.ptr + lane;
   const double prow = row < n ? p[row] : 0.0;
-  for (int j = 0; j < si.wl + si.wu; ++j, base += kSlice) {
-    const int f = map[base];
-    if (f < 0) continue;
-    const double wf = w[f];
-    const double pc = p[col[base]];
-    const double p_own = j < si.wl ? prow : pc, p_nei = j < si.wl ? pc : prow;
+  // One slot's contribution, in slot order (owner side: + p_f S_f; neighbour
+  // side: + (-p_f S_f)).
+  auto add = [&](bool owner_side, double wf, double pc, double sx, double sy, double sz) {
+    const double p_own = owner_side ? prow : pc, p_nei = owner_side ? pc : prow;
     const double pf = wf * p_own + (1.0 - wf) * p_nei;
-    const double cx = pf * sf[3 * f], cy = pf * sf[3 * f + 1], cz = pf * sf[3 * f + 2];
-    if (j < si.wl) {  // owner side: + p_f S_f
+    const double cx = pf * sx, cy = pf * sy, cz = pf * sz;
+    if (owner_side) {
       gx += cx;
       gy += cy;
       gz += cz;
-    } else {          // neighbour side: + (-p_f S_f)
+    } else {
       gx += -cx;
       gy += -cy;
       gz += -cz;
+    }
+  };
+  if (si.wl <= 4 && si.wu <= 4) {
+    // Each group's face indices and columns loaded at once, then its gathers,
+    // then its slots in order: two dependent round trips per group instead
+    // of one per slot.
+    constexpr int U = 4;
+    for (int g = 0; g < 2; ++g) {
+      const int width = g ? si.wu : si.wl;
+      const int gb = g ? base + si.wl * kSlice : base;
+      int f[U], c[U];
+#pragma unroll
+      for (int j = 0; j < U; ++j) {
+        f[j] = j < width ? map[gb + j * kSlice] : -1;
+        c[j] = j < width ? col[gb + j * kSlice] : -1;
+      }
+      double wf[U], pc[U], sx[U], sy[U], sz[U];
+#pragma unroll
+      for (int j = 0; j < U; ++j) {
+        const bool ok = f[j] >= 0;
+        wf[j] = ok ? w[f[j]] : 0.0;
+        pc[j] = ok ? p[c[j]] : 0.0;
+        sx[j] = ok ? sf[3 * f[j]] : 0.0;
+        sy[j] = ok ? sf[3 * f[j] + 1] : 0.0;
+        sz[j] = ok ? sf[3 * f[j] + 2] : 0.0;
+      }
+#pragma unroll
+      for (int j = 0; j < U; ++j)
+        if (f[j] >= 0) add(g == 0, wf[j], pc[j], sx[j], sy[j], sz[j]);
+    }
+  } else {
+    for (int j = 0; j < si.wl + si.wu; ++j, base += kSlice) {
+      const int f = map[base];
+      if (f < 0) continue;
+      add(j < si.wl, w[f], p[col[base]], sf[3 * f], sf[3 * f + 1], sf[3 * f + 2]);
     }
   }
   if (row >= n) return;
