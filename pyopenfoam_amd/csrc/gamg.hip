@@ -28,6 +28,7 @@
 // reference's sequential evaluation; only ||r|| depends on the reduction tree.
 
 #include <cmath>
+#include <cstdlib>
 #include <cstring>
 #include <type_traits>
 
@@ -177,6 +178,54 @@ __global__ __launch_bounds__(kBlock) void k_gs_level(LevelDevT<T> L, int cur,
   const int t = blockIdx.x * kBlock + threadIdx.x;
   if (t < count)
     gs_entry<T>(ents[t], L.col, L.map, L.packed, L.diag, L.b, L.x[cur], T(1), live);
+}
+
+// One symmetric Gauss-Seidel sweep of a whole level by one workgroup, which
+// walks the level's dependency levels with a barrier each (a level with few
+// cells per dependency level: a launch per dependency level costs more than
+// its work).  Each thread loads its first schedule entry of the next
+// dependency level while the current one runs.  Same cell updates, in place.
+constexpr int kGsWgThreads = 1024;
+template <typename T>
+__global__ __launch_bounds__(kGsWgThreads) void k_gs_wg(LevelDevT<T> L, int cur,
+                                                         const KState *st) {
+  if (st->done) return;
+  const int t = (int)threadIdx.x;
+  T *x = L.x[cur];
+  for (int pass = 0; pass < 2; ++pass) {
+    const int32_t *ptr = pass ? L.gs_bptr : L.gs_fptr;
+    const int4 *ents = pass ? L.gs_bent : L.gs_fent;
+    const int nl = pass ? L.gs_nbl : L.gs_nfl;
+    if (nl <= 0) continue;
+    int b = ptr[0], e = ptr[1];
+    bool has = b + t < e;
+    int4 f = has ? ents[b + t] : make_int4(0, 0, 0, 0);
+    for (int l = 0; l < nl; ++l) {
+      const int en = l + 1 < nl ? ptr[l + 2] : e;  // dependency level l + 1: [e, en)
+      const bool hn = e + t < en;
+      const int4 fn = hn ? ents[e + t] : make_int4(0, 0, 0, 0);
+      if (has) gs_entry<T>(f, L.col, L.map, L.packed, L.diag, L.b, x, T(1), true);
+      for (int k = b + t + kGsWgThreads; k < e; k += kGsWgThreads)
+        gs_entry<T>(ents[k], L.col, L.map, L.packed, L.diag, L.b, x, T(1), true);
+      __syncthreads();
+      b = e;
+      e = en;
+      has = hn;
+      f = fn;
+    }
+  }
+}
+
+// Levels with at most this many cells per Gauss-Seidel dependency level (one
+// per thread of k_gs_wg) run their sweeps in k_gs_wg: a dependency level
+// there still costs ~5 us (the schedule entry, the couplings and the gathered
+// x are dependent round trips, then the barrier), so only levels whose cells
+// fit one pass of the workgroup gain over a launch each (128^3 pair GAMG:
+// 90.1 -> 86.4 ms per cycle; 2048 cells: 105 ms, 4096: 138 ms).
+// PFM_GS_WG_CELLS overrides, read at each enqueue; 0 = never.
+int gs_wg_cells() {
+  const char *e = std::getenv("PFM_GS_WG_CELLS");
+  return e ? std::atoi(e) : kGsWgThreads;
 }
 
 template <typename T>
@@ -1141,6 +1190,12 @@ void enqueue_cycle(const pfm_gamg_s *g, const CycleLevels<T> &C, const KState *s
     using V = typename std::remove_const<typename std::remove_pointer<decltype(D.packed)>::type>::type;
     if (gs) {  // one symmetric sweep, level by level, in place
       const pfm_ldu h = g->lv[k].ldu;
+      const int64_t ndep = std::max<int64_t>(
+          std::max<int64_t>((int64_t)h->gs_fwd_ptr.size() - 1, (int64_t)h->gs_bwd_ptr.size() - 1), 1);
+      if ((int64_t)h->n <= (int64_t)gs_wg_cells() * ndep) {  // few cells per dependency level
+        k_gs_wg<V><<<1, kGsWgThreads, 0, s>>>(D, cur, st);
+        return;
+      }
       for (int pass = 0; pass < 2; ++pass) {
         const std::vector<int32_t> &ptr = pass ? h->gs_bwd_ptr : h->gs_fwd_ptr;
         const int4 *ents = pass ? h->d_gs_bwd_ent : h->d_gs_fwd_ent;

@@ -86,15 +86,19 @@ def test_gamg_pair_galerkin_converges(gpu_device):
     assert np.linalg.norm(c.b - oldu.ax(d, lo, up, c.owner, c.neighbour, xr)) / np.linalg.norm(c.b) < 2e-8
 
 
+@pytest.mark.parametrize("wg_cells", ["1024", "0"])
 @pytest.mark.parametrize("n,jit,sig", [(8, 0.2, 1.0), (24, 0.2, 1.0)])
-def test_gamg_gauss_seidel_bit_exact(gpu_device, n, jit, sig):
+def test_gamg_gauss_seidel_bit_exact(gpu_device, n, jit, sig, wg_cells, monkeypatch):
     """Extension level_smoother="GaussSeidel": the reference V-cycle with every
     smoothing step a symmetric Gauss-Seidel sweep; bit-exact with the oracle's
-    V-cycle over GaussSeidelSmoother after 6 cycles (24^3: the fine level is
-    swept by per-level launches, the small levels inside the one-workgroup
-    tail; coarse levels carry faces of both orientations)."""
+    V-cycle over GaussSeidelSmoother after 6 cycles.  The launched levels are
+    swept by one workgroup walking their dependency levels (few cells per
+    dependency level, the default) or by one launch per dependency level
+    (PFM_GS_WG_CELLS=0); the small levels run inside the one-workgroup tail;
+    coarse levels carry faces of both orientations."""
+    monkeypatch.setenv("PFM_GS_WG_CELLS", wg_cells)
     c, d, lo, up = system(n, jit, sig)
-    print(f"gamg gs n={n}", flush=True)
+    print(f"gamg gs n={n} wg_cells={wg_cells}", flush=True)
     m = _matrix(c.n_cells, c.owner, c.neighbour, d, lo, up, gpu_device)
     ncyc = 6
     solver = GAMGSolver(tolerance=1e-300, rel_tol=1e-300, max_iter=ncyc,
