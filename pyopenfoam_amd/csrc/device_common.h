@@ -327,6 +327,21 @@ __device__ __forceinline__ SliceInfo load_slice_lane(const SliceInfo *sl, int s)
 // face index visits the faces in the reference's face order.
 // kAssign: x[c] = new value (SmoothSolver's sweep, smooth_solver.py:170-180)
 // instead of x[c] + omega (new - x[c]) (GaussSeidelSmoother).
+// x[c] from the row sum (diag x_c + couplings in face order), as
+// GaussSeidelSmoother (kAssign: SmoothSolver) forms it.
+template <typename T, bool kAssign>
+__device__ __forceinline__ void gs_store(int c, T d, T xc, T row, const T *__restrict__ b, T *x,
+                                         T omega, bool live) {
+  const T off = row - d * xc;
+  if (live && fabs(d) > T(1e-30)) {
+    const T nv = (b[c] - off) / d;
+    if (kAssign)
+      x[c] = nv;
+    else
+      x[c] = xc + omega * (nv - xc);
+  }
+}
+
 // The update of cell c whose lower slots start at bl (slot ptr + lane) in a
 // slice of group widths wl, wu; `live` false computes without storing (a
 // sweep enqueued after its solve stopped).
@@ -413,14 +428,7 @@ __device__ __forceinline__ void gs_core(int c, int bl, int wl, int wu,
     }
   }
   }
-  const T off = row - d * xc;
-  if (live && fabs(d) > T(1e-30)) {
-    const T nv = (b[c] - off) / d;
-    if (kAssign)
-      x[c] = nv;
-    else
-      x[c] = xc + omega * (nv - xc);
-  }
+  gs_store<T, kAssign>(c, d, xc, row, b, x, omega, live);
 }
 
 template <typename T, bool kAssign = false>
@@ -435,15 +443,42 @@ __device__ __forceinline__ void gs_cell(int c, const SliceInfo *__restrict__ sl,
                       true);
 }
 
-// gs_core from a schedule entry (cell, lower-slot base, wl | wu << 16).
+// The update of a schedule entry's cell (GsEnt): the columns already in face
+// order, so the coefficients, the gathered x values, diag, b and x_c are all
+// loaded at once after the entry; rows of more than 8 couplings go through
+// gs_core's merge by the slot map.
 template <typename T, bool kAssign = false>
-__device__ __forceinline__ void gs_entry(const int4 e, const int32_t *__restrict__ col,
+__device__ __forceinline__ void gs_entry(const GsEnt &e, const int32_t *__restrict__ col,
                                          const int32_t *__restrict__ map,
                                          const T *__restrict__ val,
                                          const T *__restrict__ diag,
                                          const T *__restrict__ b, T *x, T omega, bool live) {
-  gs_core<T, kAssign>(e.x, e.y, e.z & 0xffff, (int)((unsigned)e.z >> 16), col, map, val, diag, b,
-                      x, omega, live);
+  const int c = e.a.x, bl = e.a.y, wl = e.a.z & 0xffff, wu = (int)((unsigned)e.a.z >> 16);
+  const int mask = e.a.w & 0xff, n = (e.a.w >> 8) & 0xff;
+  if (n > 8) {
+    gs_core<T, kAssign>(c, bl, wl, wu, col, map, val, diag, b, x, omega, live);
+    return;
+  }
+  const int cols[8] = {e.b.x, e.b.y, e.b.z, e.b.w, e.c.x, e.c.y, e.c.z, e.c.w};
+  const int bu = bl + wl * kSlice;
+  const T d = diag[c];
+  const T xc = x[c];
+  T v[8], xv[8];
+  int il = 0, iu = 0;
+#pragma unroll
+  for (int k = 0; k < 8; ++k) {
+    const bool lo = (mask >> k) & 1;
+    const int slot = lo ? bl + il * kSlice : bu + iu * kSlice;
+    il += lo ? 1 : 0;
+    iu += lo ? 0 : 1;
+    v[k] = k < n ? val[slot] : T(0);
+    xv[k] = k < n ? x[cols[k]] : T(0);
+  }
+  T row = d * xc;
+#pragma unroll
+  for (int k = 0; k < 8; ++k)
+    if (k < n) row += v[k] * xv[k];
+  gs_store<T, kAssign>(c, d, xc, row, b, x, omega, live);
 }
 
 }  // namespace pfm

@@ -172,7 +172,7 @@ __global__ __launch_bounds__(kBlock) void k_jacobi(LevelDevT<T> L, int cur, cons
 // store, so its load does not head the chain of dependent loads.
 template <typename T>
 __global__ __launch_bounds__(kBlock) void k_gs_level(LevelDevT<T> L, int cur,
-                                                      const int4 *__restrict__ ents, int count,
+                                                      const GsEnt *__restrict__ ents, int count,
                                                       const KState *st) {
   const bool live = !st->done;
   const int t = blockIdx.x * kBlock + threadIdx.x;
@@ -194,16 +194,16 @@ __global__ __launch_bounds__(kGsWgThreads) void k_gs_wg(LevelDevT<T> L, int cur,
   T *x = L.x[cur];
   for (int pass = 0; pass < 2; ++pass) {
     const int32_t *ptr = pass ? L.gs_bptr : L.gs_fptr;
-    const int4 *ents = pass ? L.gs_bent : L.gs_fent;
+    const GsEnt *ents = pass ? L.gs_bent : L.gs_fent;
     const int nl = pass ? L.gs_nbl : L.gs_nfl;
     if (nl <= 0) continue;
     int b = ptr[0], e = ptr[1];
     bool has = b + t < e;
-    int4 f = has ? ents[b + t] : make_int4(0, 0, 0, 0);
+    GsEnt f = has ? ents[b + t] : GsEnt{};
     for (int l = 0; l < nl; ++l) {
       const int en = l + 1 < nl ? ptr[l + 2] : e;  // dependency level l + 1: [e, en)
       const bool hn = e + t < en;
-      const int4 fn = hn ? ents[e + t] : make_int4(0, 0, 0, 0);
+      const GsEnt fn = hn ? ents[e + t] : GsEnt{};
       if (has) gs_entry<T>(f, L.col, L.map, L.packed, L.diag, L.b, x, T(1), true);
       for (int k = b + t + kGsWgThreads; k < e; k += kGsWgThreads)
         gs_entry<T>(ents[k], L.col, L.map, L.packed, L.diag, L.b, x, T(1), true);
@@ -325,7 +325,7 @@ __device__ void tail_jacobi(const LevelDev &L, int cur) {
 __device__ void tail_gs(const LevelDev &L, int cur) {
   for (int pass = 0; pass < 2; ++pass) {
     const int32_t *ptr = pass ? L.gs_bptr : L.gs_fptr;
-    const int4 *ents = pass ? L.gs_bent : L.gs_fent;
+    const GsEnt *ents = pass ? L.gs_bent : L.gs_fent;
     const int nl = pass ? L.gs_nbl : L.gs_nfl;
     for (int l = 0; l < nl; ++l) {
       for (int k = ptr[l] + (int)threadIdx.x; k < ptr[l + 1]; k += kTailThreads)
@@ -1198,7 +1198,7 @@ void enqueue_cycle(const pfm_gamg_s *g, const CycleLevels<T> &C, const KState *s
       }
       for (int pass = 0; pass < 2; ++pass) {
         const std::vector<int32_t> &ptr = pass ? h->gs_bwd_ptr : h->gs_fwd_ptr;
-        const int4 *ents = pass ? h->d_gs_bwd_ent : h->d_gs_fwd_ent;
+        const GsEnt *ents = pass ? h->d_gs_bwd_ent : h->d_gs_fwd_ent;
         for (size_t l = 0; l + 1 < ptr.size(); ++l) {
           const int cnt = ptr[l + 1] - ptr[l];
           k_gs_level<V><<<grid1d(cnt), kBlock, 0, s>>>(D, cur, ents + ptr[l], cnt, st);

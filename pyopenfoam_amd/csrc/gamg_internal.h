@@ -37,7 +37,7 @@ struct LevelDevT {
   // Gauss-Seidel smoothing: slot -> face map and the sweep level schedules
   const int32_t *map;
   const int32_t *gs_fptr, *gs_fcells, *gs_bptr, *gs_bcells;
-  const int4 *gs_fent, *gs_bent;  // the schedules as gs_entry() entries
+  const GsEnt *gs_fent, *gs_bent;  // the schedules as gs_entry() entries
   int32_t gs_nfl, gs_nbl;
 };
 using LevelDev = LevelDevT<double>;

@@ -626,14 +626,42 @@ int pfm_ldu_create(int32_t n, int32_t nf, const int32_t *own, const int32_t *nei
     return st;
   }
   {
-    // Gauss-Seidel schedule entries: each scheduled cell with its row's
-    // lower-slot base and group widths, so a sweep reads them in one load.
+    // Gauss-Seidel schedule entries (GsEnt): each scheduled cell with its
+    // lower-slot base, group widths and its columns merged into face order,
+    // so a sweep reads the entry, then the coefficients and x values at once.
     auto entries = [&](const std::vector<int32_t> &cells) {
-      std::vector<int4> e(cells.size());
+      std::vector<GsEnt> e(cells.size());
       for (size_t k = 0; k < cells.size(); ++k) {
         const int32_t c = cells[k];
         const SliceInfo si = L.slices[c / kSlice];
-        e[k] = make_int4(c, si.ptr + c % kSlice, (int)(uint16_t)si.wl | ((int)si.wu << 16), 0);
+        const int64_t bl = si.ptr + c % kSlice, bu = bl + (int64_t)si.wl * kSlice;
+        int32_t cols[8] = {0, 0, 0, 0, 0, 0, 0, 0};
+        int n = 0, mask = 0, il = 0, iu = 0;
+        bool wide = false;
+        while (true) {  // merge the valid prefixes of both groups by face index
+          const bool hl = il < si.wl && L.col[bl + (int64_t)il * kSlice] >= 0;
+          const bool hu = iu < si.wu && L.col[bu + (int64_t)iu * kSlice] >= 0;
+          if (!hl && !hu) break;
+          const bool take_l =
+              hl && (!hu || L.map[bl + (int64_t)il * kSlice] < L.map[bu + (int64_t)iu * kSlice]);
+          if (n == 8) {
+            wide = true;
+            break;
+          }
+          if (take_l) {
+            cols[n] = L.col[bl + (int64_t)il * kSlice];
+            mask |= 1 << n;
+            ++il;
+          } else {
+            cols[n] = L.col[bu + (int64_t)iu * kSlice];
+            ++iu;
+          }
+          ++n;
+        }
+        e[k].a = make_int4(c, (int)bl, (int)(uint16_t)si.wl | ((int)si.wu << 16),
+                           wide ? (255 << 8) : (mask | (n << 8)));
+        e[k].b = make_int4(cols[0], cols[1], cols[2], cols[3]);
+        e[k].c = make_int4(cols[4], cols[5], cols[6], cols[7]);
       }
       return e;
     };

@@ -59,6 +59,15 @@ constexpr int kPairHeader = 64;
 constexpr uint32_t kPairSigned = 1;    // int16 col - row, -32768 = padding
 constexpr uint32_t kPairOriented = 2;  // uint16 |col - row|, lower +, upper -, 0 = padding
 
+// A Gauss-Seidel schedule entry (48 bytes): a = {cell, slot of its first
+// lower coupling (ptr + lane), wl | wu << 16, group mask | n << 8}, then the
+// row's n <= 8 columns in face order (b, c); bit k of the mask says whether
+// the k-th coupling in face order is a lower one.  n = 255: the row has more
+// than 8 couplings and is merged through the slot map instead.
+struct GsEnt {
+  int4 a, b, c;
+};
+
 // Per-slice layout: slot offset, lower and upper slot counts.
 struct SliceInfo {
   int32_t ptr;
@@ -155,7 +164,7 @@ struct pfm_ldu_s {
   std::vector<int32_t> gs_fwd_ptr, gs_bwd_ptr;  // host copies of GS level offsets
   int32_t *d_gs_fwd_ptr = nullptr, *d_gs_fwd_cells = nullptr;
   int32_t *d_gs_bwd_ptr = nullptr, *d_gs_bwd_cells = nullptr;
-  int4 *d_gs_fwd_ent = nullptr, *d_gs_bwd_ent = nullptr;  // per scheduled cell: gs_entry()
+  pfm::GsEnt *d_gs_fwd_ent = nullptr, *d_gs_bwd_ent = nullptr;  // per scheduled cell
   pfm::PollPool poll;                   // pinned stop-flag slots of the solves
   bool oriented_sorted = false;         // every owner < neighbour, owners non-decreasing
   pfm_progress_fn progress = nullptr;   // optional host progress report

@@ -12,7 +12,7 @@ namespace {
 inline int grid1d(int64_t n) { return (int)((n + kBlock - 1) / kBlock); }
 
 // One sweep level: its cells are mutually independent.
-__global__ __launch_bounds__(kBlock) void k_gs_level(const int4 *__restrict__ ents, int count,
+__global__ __launch_bounds__(kBlock) void k_gs_level(const GsEnt *__restrict__ ents, int count,
                                                       const int32_t *__restrict__ col,
                                                       const int32_t *__restrict__ map,
                                                       const double *__restrict__ val,
@@ -28,15 +28,15 @@ constexpr int kGsThreads = 1024;
 // Small meshes: one workgroup walks every level of n_sweeps symmetric sweeps,
 // with a barrier between levels.
 __global__ __launch_bounds__(kGsThreads) void k_gs_sweeps_wg(
-    int n_sweeps, const int32_t *__restrict__ fptr, int nfl, const int4 *__restrict__ fents,
-    const int32_t *__restrict__ bptr, int nbl, const int4 *__restrict__ bents,
+    int n_sweeps, const int32_t *__restrict__ fptr, int nfl, const GsEnt *__restrict__ fents,
+    const int32_t *__restrict__ bptr, int nbl, const GsEnt *__restrict__ bents,
     const int32_t *__restrict__ col,
     const int32_t *__restrict__ map, const double *__restrict__ val,
     const double *__restrict__ diag, const double *__restrict__ b, double *x, double omega) {
   for (int s = 0; s < n_sweeps; ++s) {
     for (int pass = 0; pass < 2; ++pass) {
       const int32_t *ptr = pass ? bptr : fptr;
-      const int4 *ents = pass ? bents : fents;
+      const GsEnt *ents = pass ? bents : fents;
       const int nl = pass ? nbl : nfl;
       for (int l = 0; l < nl; ++l) {
         for (int k = ptr[l] + (int)threadIdx.x; k < ptr[l + 1]; k += kGsThreads)
@@ -194,7 +194,7 @@ __global__ __launch_bounds__(kBlock) void k_sum_sqrt(const double *__restrict__ 
 // SmoothSolver (smooth_solver.py:96-232).
 
 // One level of a forward Gauss-Seidel sweep with direct assignment.
-__global__ __launch_bounds__(kBlock) void k_sm_gs_level(const int4 *__restrict__ ents, int count,
+__global__ __launch_bounds__(kBlock) void k_sm_gs_level(const GsEnt *__restrict__ ents, int count,
                                                          const int32_t *__restrict__ col,
                                                          const int32_t *__restrict__ map,
                                                          const double *__restrict__ val,
@@ -207,7 +207,7 @@ __global__ __launch_bounds__(kBlock) void k_sm_gs_level(const int4 *__restrict__
 }
 
 __global__ __launch_bounds__(kGsThreads) void k_sm_gs_wg(int n_sweeps, const int32_t *__restrict__ fptr,
-                                                          int nfl, const int4 *__restrict__ fents,
+                                                          int nfl, const GsEnt *__restrict__ fents,
                                                           const int32_t *__restrict__ col,
                                                           const int32_t *__restrict__ map,
                                                           const double *__restrict__ val,
@@ -327,7 +327,7 @@ int pfm_gs_smooth(pfm_ldu h, const double *diag, const double *packed, const dou
     for (int sw = 0; sw < n_sweeps; ++sw) {
       for (int pass = 0; pass < 2; ++pass) {
         const std::vector<int32_t> &ptr = pass ? h->gs_bwd_ptr : h->gs_fwd_ptr;
-        const int4 *ents = pass ? h->d_gs_bwd_ent : h->d_gs_fwd_ent;
+        const GsEnt *ents = pass ? h->d_gs_bwd_ent : h->d_gs_fwd_ent;
         for (size_t l = 0; l + 1 < ptr.size(); ++l) {
           const int cnt = ptr[l + 1] - ptr[l];
           k_gs_level<<<grid1d(cnt), kBlock, 0, s>>>(ents + ptr[l], cnt, h->d_col, h->d_map, packed,
