@@ -334,12 +334,12 @@ __device__ __forceinline__ XcdRange xcd_range(int nact) {
   }
 
 // Latency: in the tail of a batch only a few tiles remain active and each
-// wave's slices are the whole of its work, so the row loops below keep two
-// slices' loads in flight (the next slice's operands are loaded before the
-// current slice's stores, or two SpMV rows are formed side by side), and a
-// tile's first operands are loaded before its stop flag and partial sums are
-// read.  Per-row arithmetic and summation order are those of one slice at a
-// time.
+// wave's slices are the whole of its work, so the update kernels' row loops
+// keep two slices' loads in flight (the next slice's operands are loaded
+// before the current slice's stores; the tail-mode SpMV forms two slices side
+// by side), and a tile's first operands are loaded before its stop flag and
+// partial sums are read.  Per-row arithmetic and summation order are those of
+// one slice at a time.
 
 // Ap = A p; tile partial p.Ap.  Tiles of a system that stopped earlier in the
 // chunk compute values nobody reads, so this kernel skips the stop check.
@@ -364,32 +364,22 @@ __global__ __launch_bounds__(kBlock) void k_b_spmv(const int4 *__restrict__ act,
     const int lane = threadIdx.x & 63;
     const int nw = blockDim.x >> 6;
     double pap = 0.0;
-    for (int s = s0 + (threadIdx.x >> 6); s < s1; s += 2 * nw) {
-      // rows of slices s and s + nw (the latter recomputed from s when past
-      // the tile's end, and then not stored), formed side by side
-      const bool two = s + nw < s1;
-      const int sb = two ? s + nw : s;
-      const SliceInfo sa_ = load_slice(sl, s), sb_ = load_slice(sl, sb);
-      const int ra = s * kSlice + lane, rb = sb * kSlice + lane;
-      const double pa = p[ra], pb = p[rb];
-      double va, vb;
-      if constexpr (kPair) {  // paired int16 column offsets
+    // one slice per wave step: each lane adds its rows' products in slice
+    // order s0 + w, s0 + w + nw, ...  (forming two slices side by side, as the
+    // tail-mode kernel does, was 3.30 against 3.24 s for the 512-system batch)
+    for (int s = s0 + (threadIdx.x >> 6); s < s1; s += nw) {
+      const SliceInfo sa_ = load_slice(sl, s);
+      const int ra = s * kSlice + lane;
+      const double pa = p[ra];
+      double va;
+      if constexpr (kPair)  // paired int16 column offsets
         va = row_spmv_pair<double, decltype(gx), false, true>(sa_, s, lane, ra, diag[ra] * pa, cp,
                                                               val, gx);
-        vb = row_spmv_pair<double, decltype(gx), false, true>(sb_, sb, lane, rb, diag[rb] * pb, cp,
-                                                              val, gx);
-      } else {
+      else
         va = row_spmv_g<double, decltype(gx), false, ColAbsT<true>, true>(
             sa_, lane, diag[ra] * pa, ColAbsT<true>{col}, val, gx);
-        vb = row_spmv_g<double, decltype(gx), false, ColAbsT<true>, true>(
-            sb_, lane, diag[rb] * pb, ColAbsT<true>{col}, val, gx);
-      }
       Ap[ra] = va;
       pap += pa * va;
-      if (two) {
-        Ap[rb] = vb;
-        pap += pb * vb;
-      }
     }
     pap = wg_sum(pap, lds);
     if (threadIdx.x == 0) part[t] = pap;
