@@ -321,14 +321,14 @@ __device__ __forceinline__ SliceInfo load_slice_lane(const SliceInfo *sl, int s)
   return si;
 }
 
-// One Gauss-Seidel cell update (solvers/smoothers.py:113-149).  The row's
+// Gauss-Seidel cell updates (solvers/smoothers.py:113-149).  A row's
 // couplings sit in two groups of its slice (owner-faces, column = neighbour,
-// then neighbour-faces, column = owner), each in face order; merging them by
-// face index visits the faces in the reference's face order.
-// kAssign: x[c] = new value (SmoothSolver's sweep, smooth_solver.py:170-180)
-// instead of x[c] + omega (new - x[c]) (GaussSeidelSmoother).
-// x[c] from the row sum (diag x_c + couplings in face order), as
-// GaussSeidelSmoother (kAssign: SmoothSolver) forms it.
+// then neighbour-faces, column = owner), each in face order; merged by face
+// index they follow the reference's face order.  kAssign: x[c] = new value
+// (SmoothSolver's sweep, smooth_solver.py:170-180) instead of
+// x[c] + omega (new - x[c]) (GaussSeidelSmoother).
+//
+// x[c] from the row sum (diag x_c + couplings in face order).
 template <typename T, bool kAssign>
 __device__ __forceinline__ void gs_store(int c, T d, T xc, T row, const T *__restrict__ b, T *x,
                                          T omega, bool live) {
@@ -343,7 +343,9 @@ __device__ __forceinline__ void gs_store(int c, T d, T xc, T row, const T *__res
 }
 
 // The update of cell c whose lower slots start at bl (slot ptr + lane) in a
-// slice of group widths wl, wu; `live` false computes without storing (a
+// slice of group widths wl, wu, for rows of more than 8 couplings: the two
+// groups, each in face order, merged by face index slot by slot (one chain of
+// dependent loads per coupling); `live` false computes without storing (a
 // sweep enqueued after its solve stopped).
 template <typename T, bool kAssign = false>
 __device__ __forceinline__ void gs_core(int c, int bl, int wl, int wu,
@@ -356,61 +358,6 @@ __device__ __forceinline__ void gs_core(int c, int bl, int wl, int wu,
   const T d = diag[c];
   const T xc = x[c];
   T row = d * xc;
-  if (wl <= 4 && wu <= 4) {
-    // All of the row's couplings loaded at once (no chain of dependent loads
-    // through the merge), the products formed, then added in face order: an
-    // entry's position in the merged order is its index in its group plus
-    // the number of the other group's faces before it (a face is in only one
-    // group of a row, so no two faces tie).  Padding slots are a suffix of
-    // each group and are skipped.
-    constexpr int U = 4;
-    int cl[U], cu[U], ml[U], mu[U];
-    T vl[U], vu[U];
-#pragma unroll
-    for (int j = 0; j < U; ++j) {
-      const bool hl = j < wl, hu = j < wu;
-      cl[j] = hl ? col[bl + j * kSlice] : -1;
-      ml[j] = hl ? map[bl + j * kSlice] : 0;
-      vl[j] = hl ? val[bl + j * kSlice] : T(0);
-      cu[j] = hu ? col[bu + j * kSlice] : -1;
-      mu[j] = hu ? map[bu + j * kSlice] : 0;
-      vu[j] = hu ? val[bu + j * kSlice] : T(0);
-    }
-    T pl[U], pu[U];
-#pragma unroll
-    for (int j = 0; j < U; ++j) {
-      pl[j] = cl[j] >= 0 ? vl[j] * x[cl[j]] : T(0);
-      pu[j] = cu[j] >= 0 ? vu[j] * x[cu[j]] : T(0);
-    }
-    int rl[U], ru[U];
-#pragma unroll
-    for (int i = 0; i < U; ++i) {
-      rl[i] = i;
-      ru[i] = i;
-#pragma unroll
-      for (int j = 0; j < U; ++j) {
-        rl[i] += (cu[j] >= 0 && mu[j] < ml[i]) ? 1 : 0;
-        ru[i] += (cl[j] >= 0 && ml[j] < mu[i]) ? 1 : 0;
-      }
-    }
-#pragma unroll
-    for (int k = 0; k < 2 * U; ++k) {
-      bool has = false;
-      T pick = T(0);
-#pragma unroll
-      for (int i = 0; i < U; ++i) {
-        if (cl[i] >= 0 && rl[i] == k) {
-          pick = pl[i];
-          has = true;
-        }
-        if (cu[i] >= 0 && ru[i] == k) {
-          pick = pu[i];
-          has = true;
-        }
-      }
-      if (has) row += pick;
-    }
-  } else {
   int jl = 0, ju = 0;
   int cl = jl < wl ? col[bl] : -1;
   int cu = ju < wu ? col[bu] : -1;
@@ -427,20 +374,7 @@ __device__ __forceinline__ void gs_core(int c, int bl, int wl, int wu,
       cu = ju < wu ? col[bu + ju * kSlice] : -1;
     }
   }
-  }
   gs_store<T, kAssign>(c, d, xc, row, b, x, omega, live);
-}
-
-template <typename T, bool kAssign = false>
-__device__ __forceinline__ void gs_cell(int c, const SliceInfo *__restrict__ sl,
-                                        const int32_t *__restrict__ col,
-                                        const int32_t *__restrict__ map,
-                                        const T *__restrict__ val,
-                                        const T *__restrict__ diag,
-                                        const T *__restrict__ b, T *x, T omega) {
-  const SliceInfo si = load_slice_lane(sl, c / kSlice);  // lanes hold cells of any slice
-  gs_core<T, kAssign>(c, si.ptr + c % kSlice, si.wl, si.wu, col, map, val, diag, b, x, omega,
-                      true);
 }
 
 // The update of a schedule entry's cell (GsEnt): the columns already in face
