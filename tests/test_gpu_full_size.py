@@ -108,19 +108,25 @@ def test_full_size_solves_reach_tolerance(gpu_device):
     assert relg.item() < 1e-8, relg.item()
 
 
-@pytest.mark.parametrize("coarse", ["reference", "galerkin"])
-def test_full_size_gamg_cycles_bit_exact(gpu_device, coarse):
+@pytest.mark.parametrize("coarse,smoother", [("reference", "Jacobi"), ("galerkin", "Jacobi"),
+                                              ("galerkin", "GaussSeidel")])
+def test_full_size_gamg_cycles_bit_exact(gpu_device, coarse, smoother):
     """The pair hierarchy at 2.1M cells (20 levels down to a few cells: grid
     launches, the one-workgroup tail, the one-wave levels): level sizes as the
-    oracle's pair maps, and x after two Jacobi V-cycles bit for bit as the
-    oracle's V-cycle over the oracle's maps and coarse matrices (the
-    reference's, or with the Galerkin coarse diagonal the bench's GAMG uses)."""
+    oracle's pair maps, and x after V-cycles bit for bit as the oracle's V-cycle
+    over the oracle's maps and coarse matrices (the reference's, or with the
+    Galerkin coarse diagonal the bench's GAMG uses): two with Jacobi
+    smoothing, one with symmetric Gauss-Seidel sweeps (configs[1]'s smoother:
+    one launch per dependency level on the large levels, one workgroup walking
+    the dependency levels on the smaller ones)."""
     S = _full(gpu_device)
     c, mat = S["case"], S["mat"]
-    ncyc = 2
-    print(f"full-size gamg cycles coarse={coarse}", flush=True)
+    gs = smoother == "GaussSeidel"
+    ncyc = 1 if gs else 2
+    print(f"full-size gamg cycles coarse={coarse} smoother={smoother}", flush=True)
+    kw = dict(level_smoother="GaussSeidel") if gs else {}
     solver = GAMGSolver(tolerance=1e-300, rel_tol=1e-300, max_iter=ncyc, agglomeration="pair",
-                        coarse=coarse, max_levels=20)
+                        coarse=coarse, max_levels=20, **kw)
     sizes = solver.hierarchy(mat).level_sizes()
     x, it, _ = solver(mat, mat.source,
                       torch.zeros(c.n_cells, dtype=torch.float64, device=gpu_device))
@@ -139,7 +145,8 @@ def test_full_size_gamg_cycles_bit_exact(gpu_device, coarse):
     xo = np.zeros(c.n_cells)
     b = mat.source.cpu().numpy()
     for _ in range(ncyc):
-        xo = ogamg.v_cycle(levels, aggs, 0, xo, b, 2, 2)
+        xo = ogamg.v_cycle(levels, aggs, 0, xo, b, 2, 2,
+                           ogamg.smooth_gs if gs else ogamg.smooth)
     assert it == ncyc
     assert bits_equal(x.cpu().numpy(), xo)
 
