@@ -77,6 +77,20 @@ __device__ __forceinline__ double row_av(const SliceInfo si, int s, int lane, in
                                                                       ColAbsT<kNT>{col}, val, gx);
 }
 
+// One row of A v with the handle's paired column offsets when cp is given,
+// else its int32 columns (kUpperFirst: the association of the reference's
+// Ax_T, for the transposed packed values).
+template <bool kUpperFirst = false>
+__device__ __forceinline__ double row_any(const SliceInfo si, int s, int lane, int row, double dx,
+                                          const int32_t *__restrict__ col,
+                                          const uint32_t *__restrict__ cp,
+                                          const double *__restrict__ val,
+                                          const double *__restrict__ v) {
+  auto gx = [v](int c) { return v[c]; };
+  if (cp) return row_spmv_pair<double, decltype(gx), kUpperFirst>(si, s, lane, row, dx, cp, val, gx);
+  return row_spmv_g<double, decltype(gx), kUpperFirst>(si, lane, dx, ColAbs{col}, val, gx);
+}
+
 // ---------------------------------------------------------------- PCG ------
 // Per iteration i: k_pcg_spmv (Ap, p.Ap partials) -> k_pcg_update (every
 // workgroup reduces p.Ap itself: breakdown test, alpha; r; r.r and r.z
@@ -313,6 +327,7 @@ __global__ __launch_bounds__(kBlock) void k_pcg_pupdate(int n, int nslices, int 
 // r = b - A x; rt = r; p = r; y = M^-1 p; partial r.r (= rt.r).
 __global__ __launch_bounds__(kBlock) void k_bs_init(int n, int nslices, const SliceInfo *__restrict__ sl,
                                                      const int32_t *__restrict__ col,
+                                                     const uint32_t *__restrict__ cp,
                                                      const double *__restrict__ val,
                                                      const double *__restrict__ diag,
                                                      const double *__restrict__ rD,
@@ -326,7 +341,7 @@ __global__ __launch_bounds__(kBlock) void k_bs_init(int n, int nslices, const Sl
   FOR_ROWS({
     const SliceInfo si = load_slice(sl, s);
     const double dx = row < n ? diag[row] * x[row] : 0.0;
-    const double ax = row_spmv<double>(si, lane, dx, col, val, x);
+    const double ax = row_any(si, s, lane, row, dx, col, cp, val, x);
     if (row < n) {
       const double ri = b[row] - ax;
       r[row] = ri;
@@ -356,6 +371,7 @@ __global__ __launch_bounds__(kBlock) void k_bs_init_fin(const double *__restrict
 // v = A y; partial rt.v.
 __global__ __launch_bounds__(kBlock) void k_bs_spmv1(int n, int nslices, const SliceInfo *__restrict__ sl,
                                                       const int32_t *__restrict__ col,
+                                                      const uint32_t *__restrict__ cp,
                                                       const double *__restrict__ val,
                                                       const double *__restrict__ diag,
                                                       const double *__restrict__ y,
@@ -368,7 +384,7 @@ __global__ __launch_bounds__(kBlock) void k_bs_spmv1(int n, int nslices, const S
   FOR_ROWS({
     const SliceInfo si = load_slice(sl, s);
     const double dx = row < n ? diag[row] * y[row] : 0.0;
-    const double av = row_spmv<double>(si, lane, dx, col, val, y);
+    const double av = row_any(si, s, lane, row, dx, col, cp, val, y);
     if (row < n) {
       v[row] = av;
       acc[0] += rt[row] * av;
@@ -414,6 +430,7 @@ __global__ __launch_bounds__(kBlock) void k_bs_s(int n, int nslices, int it,
 __global__ __launch_bounds__(kBlock) void k_bs_spmv2(int n, int nslices, int it,
                                                       const SliceInfo *__restrict__ sl,
                                                       const int32_t *__restrict__ col,
+                                                      const uint32_t *__restrict__ cp,
                                                       const double *__restrict__ val,
                                                       const double *__restrict__ diag,
                                                       const double *__restrict__ z,
@@ -436,7 +453,7 @@ __global__ __launch_bounds__(kBlock) void k_bs_spmv2(int n, int nslices, int it,
     const SliceInfo si = load_slice(sl, s);
     const double zi = row < n ? z[row] : 0.0;
     const double dx = row < n ? diag[row] * zi : 0.0;
-    const double at = row_spmv<double>(si, lane, dx, col, val, z);
+    const double at = row_any(si, s, lane, row, dx, col, cp, val, z);
     if (row < n) {
       t[row] = at;
       acc[0] += at * at;
@@ -559,6 +576,7 @@ __global__ __launch_bounds__(kBlock) void k_bs_pupdate(int n, int nslices, int i
 __global__ __launch_bounds__(kBlock) void k_bicg_init(int n, int nslices,
                                                        const SliceInfo *__restrict__ sl,
                                                        const int32_t *__restrict__ col,
+                                                       const uint32_t *__restrict__ cp,
                                                        const double *__restrict__ val,
                                                        const double *__restrict__ diag,
                                                        const double *__restrict__ rD,
@@ -573,7 +591,7 @@ __global__ __launch_bounds__(kBlock) void k_bicg_init(int n, int nslices,
   FOR_ROWS({
     const SliceInfo si = load_slice(sl, s);
     const double dx = row < n ? diag[row] * x[row] : 0.0;
-    const double ax = row_spmv<double>(si, lane, dx, col, val, x);
+    const double ax = row_any(si, s, lane, row, dx, col, cp, val, x);
     if (row < n) {
       const double ri = b[row] - ax;
       const double zi = rD ? ri / rD[row] : ri;
@@ -607,6 +625,7 @@ __global__ __launch_bounds__(kBlock) void k_bicg_init_fin(const double *__restri
 __global__ __launch_bounds__(kBlock) void k_bicg_spmv(int n, int nslices,
                                                        const SliceInfo *__restrict__ sl,
                                                        const int32_t *__restrict__ col,
+                                                       const uint32_t *__restrict__ cp,
                                                        const double *__restrict__ val,
                                                        const double *__restrict__ val_t,
                                                        const double *__restrict__ diag,
@@ -622,10 +641,9 @@ __global__ __launch_bounds__(kBlock) void k_bicg_spmv(int n, int nslices,
   FOR_ROWS({
     const SliceInfo si = load_slice(sl, s);
     const double dx = row < n ? diag[row] * p[row] : 0.0;
-    const double av = row_spmv<double>(si, lane, dx, col, val, p);
+    const double av = row_any(si, s, lane, row, dx, col, cp, val, p);
     // A^T p in the association of the reference's Ax_T
-    auto ld = [p](int c) { return p[c]; };
-    const double at = row_spmv_g<double, decltype(ld), true>(si, lane, dx, ColAbs{col}, val_t, ld);
+    const double at = row_any<true>(si, s, lane, row, dx, col, cp, val_t, p);
     if (row < n) {
       v[row] = av;
       t[row] = at;
@@ -939,7 +957,8 @@ int pfm_pbicgstab_solve(pfm_ldu h, const pfm_solve_params *prm, const double *di
   const int n = h->n, ns = h->nslices;
   k_state_init<<<1, 1, 0, s>>>(w.st, prm->tolerance, prm->rel_tol, prm->max_iter, prm->min_iter,
                                hist_cap);
-  k_bs_init<<<G, kBlock, 0, s>>>(n, ns, h->d_slices, h->d_col, packed, diag, pd, b, x, r, rt, p,
+  const uint32_t *cp = pcg_col_form(h) == kColsPair ? h->d_colp2 : nullptr;
+  k_bs_init<<<G, kBlock, 0, s>>>(n, ns, h->d_slices, h->d_col, cp, packed, diag, pd, b, x, r, rt, p,
                                  y, w.part);
   k_bs_init_fin<<<1, kBlock, 0, s>>>(w.part, G, w.st, w.hist);
   PFM_CHECK_HIP(hipGetLastError());
@@ -947,10 +966,10 @@ int pfm_pbicgstab_solve(pfm_ldu h, const pfm_solve_params *prm, const double *di
   // monitor(r), rho, beta, p, y.  Partial regions: A = [0, G) rt.v, B = [G, 2G)
   // s.s, C = [2G, 4G) t.t t.z, D = [4G, 6G) r.r rt.r.
   rc = drive(h, w.st, prm->max_iter, s, [&](int it) {
-    k_bs_spmv1<<<G, kBlock, 0, s>>>(n, ns, h->d_slices, h->d_col, packed, diag, y, rt, v, w.part,
+    k_bs_spmv1<<<G, kBlock, 0, s>>>(n, ns, h->d_slices, h->d_col, cp, packed, diag, y, rt, v, w.part,
                                     w.st);
     k_bs_s<<<G, kBlock, 0, s>>>(n, ns, it, pd, r, v, sv, z, w.part, w.st);
-    k_bs_spmv2<<<G, kBlock, 0, s>>>(n, ns, it, h->d_slices, h->d_col, packed, diag, z, t, w.part,
+    k_bs_spmv2<<<G, kBlock, 0, s>>>(n, ns, it, h->d_slices, h->d_col, cp, packed, diag, z, t, w.part,
                                     w.st, w.hist);
     k_bs_update<<<G, kBlock, 0, s>>>(n, ns, y, z, sv, t, rt, x, r, w.part, w.st);
     k_bs_pupdate<<<G, kBlock, 0, s>>>(n, ns, it, pd, r, v, p, y, w.part, w.st, w.hist);
@@ -989,12 +1008,13 @@ int pfm_pbicg_solve(pfm_ldu h, const pfm_solve_params *prm, const double *diag,
   const int n = h->n, ns = h->nslices;
   k_state_init<<<1, 1, 0, s>>>(w.st, prm->tolerance, prm->rel_tol, prm->max_iter, prm->min_iter,
                                hist_cap);
-  k_bicg_init<<<G, kBlock, 0, s>>>(n, ns, h->d_slices, h->d_col, packed, diag, pd, b, x, r, rt, p,
+  const uint32_t *cp = pcg_col_form(h) == kColsPair ? h->d_colp2 : nullptr;
+  k_bicg_init<<<G, kBlock, 0, s>>>(n, ns, h->d_slices, h->d_col, cp, packed, diag, pd, b, x, r, rt, p,
                                    w.part);
   k_bicg_init_fin<<<1, kBlock, 0, s>>>(w.part, G, w.st, w.hist);
   PFM_CHECK_HIP(hipGetLastError());
   rc = drive(h, w.st, prm->max_iter, s, [&](int it) {
-    k_bicg_spmv<<<G, kBlock, 0, s>>>(n, ns, h->d_slices, h->d_col, packed, packed_t, diag, pd, p,
+    k_bicg_spmv<<<G, kBlock, 0, s>>>(n, ns, h->d_slices, h->d_col, cp, packed, packed_t, diag, pd, p,
                                      rt, v, t, w.part, w.st);
     k_bicg_update<<<G, kBlock, 0, s>>>(n, ns, it, pd, p, v, t, x, r, rt, w.part, w.st);
     k_bicg_pupdate<<<G, kBlock, 0, s>>>(n, ns, it, pd, r, p, w.part, w.st, w.hist);
