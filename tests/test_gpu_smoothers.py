@@ -74,6 +74,30 @@ def test_gauss_seidel_matches_oracle(gpu_device, n, renumber, swap, sweeps, omeg
     assert bits_equal(x.cpu().numpy(), xo)
 
 
+@pytest.mark.parametrize("n,nf", [(300, 2000), (200000, 700000)])
+def test_gauss_seidel_wide_rows(gpu_device, n, nf):
+    """Random graphs whose rows often hold more than 8 couplings (merged
+    through the slot map instead of the schedule entry's columns), faces of
+    both orientations in random order: 300 cells (one workgroup walks the
+    levels) and 200,000 cells (one launch per level)."""
+    rng = np.random.default_rng(n)
+    own = rng.integers(0, n, nf)
+    nei = rng.integers(0, n, nf)
+    keep = own != nei
+    own, nei = own[keep], nei[keep]
+    deg = np.bincount(own, minlength=n) + np.bincount(nei, minlength=n)
+    assert deg.max() > 8
+    lo = -rng.uniform(0.1, 1.0, own.shape[0])
+    up = -rng.uniform(0.1, 1.0, own.shape[0])
+    d = np.bincount(own, -lo, n) + np.bincount(nei, -up, n) + 0.5
+    b = rng.uniform(-1.0, 1.0, n)
+    x0 = rng.standard_normal(n)
+    m = _matrix(n, own, nei, d, lo, up, gpu_device)
+    x = GaussSeidelSmoother(m, omega=0.9).smooth(torch.from_numpy(x0), torch.from_numpy(b), 2)
+    xo = ogamg.gauss_seidel(d, lo, up, own, nei, b, x0, 2, omega=0.9)
+    assert bits_equal(x.cpu().numpy(), xo)
+
+
 @pytest.mark.parametrize("n_iter,omega", [(1, 2.0 / 3.0), (2, 2.0 / 3.0), (5, 1.0)])
 def test_jacobi_matches_oracle(gpu_device, n_iter, omega):
     c, d, lo, up = system(16, 0.2, 1.0)
