@@ -1419,6 +1419,9 @@ int pfm_gamg_create(pfm_ldu fine, const pfm_gamg_params *prm, pfm_gamg *out) {
     st = PFM_ERR_ARG;
   }
   if (st == PFM_OK) st = poll_pool_init(g->poll);
+  if (st == PFM_OK && g->prm.smoother == PFM_SMOOTHER_GAUSS_SEIDEL)  // every level sweeps
+    for (size_t k = 0; k < g->lv.size() && st == PFM_OK; ++k)
+      st = ensure_gs_entries(g->lv[k].ldu, nullptr);
   if (st != PFM_OK) {
     pfm_gamg_destroy(g);
     return st;

@@ -334,6 +334,50 @@ __global__ __launch_bounds__(kBlock) void k_dilu_gather(int nf, const int32_t *_
   lu[k] = lower[f] * upper[f];
 }
 
+// Gauss-Seidel schedule entries (GsEnt), one thread per scheduled cell: the
+// cell, its lower-slot base, group widths and its columns merged into face
+// order (the valid prefixes of both groups by face index), so a sweep reads
+// the entry, then the coefficients and x values at once.
+__global__ __launch_bounds__(kBlock) void k_gs_entries(int count, const int32_t *__restrict__ cells,
+                                                        const SliceInfo *__restrict__ sl,
+                                                        const int32_t *__restrict__ col,
+                                                        const int32_t *__restrict__ map,
+                                                        GsEnt *__restrict__ out) {
+  const int t = blockIdx.x * kBlock + threadIdx.x;
+  if (t >= count) return;
+  const int c = cells[t];
+  const SliceInfo si = load_slice_lane(sl, c / kSlice);
+  const int bl = si.ptr + c % kSlice, bu = bl + si.wl * kSlice;
+  int cols[8] = {0, 0, 0, 0, 0, 0, 0, 0};
+  int n = 0, mask = 0, il = 0, iu = 0;
+  bool wide = false;
+  while (true) {
+    const bool hl = il < si.wl && col[bl + il * kSlice] >= 0;
+    const bool hu = iu < si.wu && col[bu + iu * kSlice] >= 0;
+    if (!hl && !hu) break;
+    if (n == 8) {
+      wide = true;
+      break;
+    }
+    const bool take_l = hl && (!hu || map[bl + il * kSlice] < map[bu + iu * kSlice]);
+    if (take_l) {
+      cols[n] = col[bl + il * kSlice];
+      mask |= 1 << n;
+      ++il;
+    } else {
+      cols[n] = col[bu + iu * kSlice];
+      ++iu;
+    }
+    ++n;
+  }
+  GsEnt e;
+  e.a = make_int4(c, bl, (int)(uint16_t)si.wl | ((int)si.wu << 16),
+                  wide ? (255 << 8) : (mask | (n << 8)));
+  e.b = make_int4(cols[0], cols[1], cols[2], cols[3]);
+  e.c = make_int4(cols[4], cols[5], cols[6], cols[7]);
+  out[t] = e;
+}
+
 constexpr int kDiluThreads = 1024;
 
 // Only the d values depend on the previous level: each thread loads its
@@ -555,6 +599,38 @@ int upload(T **dst, const std::vector<T> &src) {
 
 }  // namespace
 
+namespace pfm {
+// The handle's Gauss-Seidel schedule entries, built on first use (48 bytes per
+// cell and direction: only solves that sweep Gauss-Seidel need them).  The
+// build finishes before the entries are published, so a solve on another
+// stream never reads them half-built.
+int ensure_gs_entries(pfm_ldu_s *h, hipStream_t s) {
+  std::lock_guard<std::mutex> lock(h->gs_ent_mu);
+  if (h->d_gs_fwd_ent || h->n == 0) return PFM_OK;
+  const int nf = (int)h->gs_fwd_ptr.back(), nb = (int)h->gs_bwd_ptr.back();
+  GsEnt *fwd = nullptr, *bwd = nullptr;
+  PFM_CHECK_HIP(hipMalloc((void **)&fwd, sizeof(GsEnt) * (size_t)std::max(nf, 1)));
+  if (hipMalloc((void **)&bwd, sizeof(GsEnt) * (size_t)std::max(nb, 1)) != hipSuccess) {
+    (void)hipFree(fwd);
+    set_error("hipMalloc of the Gauss-Seidel schedule entries failed");
+    return PFM_ERR_HIP;
+  }
+  if (nf) k_gs_entries<<<grid1d(nf), kBlock, 0, s>>>(nf, h->d_gs_fwd_cells, h->d_slices, h->d_col,
+                                                     h->d_map, fwd);
+  if (nb) k_gs_entries<<<grid1d(nb), kBlock, 0, s>>>(nb, h->d_gs_bwd_cells, h->d_slices, h->d_col,
+                                                     h->d_map, bwd);
+  if (hipGetLastError() != hipSuccess || hipStreamSynchronize(s) != hipSuccess) {
+    (void)hipFree(fwd);
+    (void)hipFree(bwd);
+    set_error("building the Gauss-Seidel schedule entries failed");
+    return PFM_ERR_HIP;
+  }
+  h->d_gs_bwd_ent = bwd;
+  h->d_gs_fwd_ent = fwd;
+  return PFM_OK;
+}
+}  // namespace pfm
+
 extern "C" {
 
 int pfm_ldu_destroy(pfm_ldu h) {
@@ -624,52 +700,6 @@ int pfm_ldu_create(int32_t n, int32_t nf, const int32_t *own, const int32_t *nei
       (st = upload(&h->d_gs_bwd_cells, L.gs_bwd_cells))) {
     pfm_ldu_destroy(h);
     return st;
-  }
-  {
-    // Gauss-Seidel schedule entries (GsEnt): each scheduled cell with its
-    // lower-slot base, group widths and its columns merged into face order,
-    // so a sweep reads the entry, then the coefficients and x values at once.
-    auto entries = [&](const std::vector<int32_t> &cells) {
-      std::vector<GsEnt> e(cells.size());
-      for (size_t k = 0; k < cells.size(); ++k) {
-        const int32_t c = cells[k];
-        const SliceInfo si = L.slices[c / kSlice];
-        const int64_t bl = si.ptr + c % kSlice, bu = bl + (int64_t)si.wl * kSlice;
-        int32_t cols[8] = {0, 0, 0, 0, 0, 0, 0, 0};
-        int n = 0, mask = 0, il = 0, iu = 0;
-        bool wide = false;
-        while (true) {  // merge the valid prefixes of both groups by face index
-          const bool hl = il < si.wl && L.col[bl + (int64_t)il * kSlice] >= 0;
-          const bool hu = iu < si.wu && L.col[bu + (int64_t)iu * kSlice] >= 0;
-          if (!hl && !hu) break;
-          const bool take_l =
-              hl && (!hu || L.map[bl + (int64_t)il * kSlice] < L.map[bu + (int64_t)iu * kSlice]);
-          if (n == 8) {
-            wide = true;
-            break;
-          }
-          if (take_l) {
-            cols[n] = L.col[bl + (int64_t)il * kSlice];
-            mask |= 1 << n;
-            ++il;
-          } else {
-            cols[n] = L.col[bu + (int64_t)iu * kSlice];
-            ++iu;
-          }
-          ++n;
-        }
-        e[k].a = make_int4(c, (int)bl, (int)(uint16_t)si.wl | ((int)si.wu << 16),
-                           wide ? (255 << 8) : (mask | (n << 8)));
-        e[k].b = make_int4(cols[0], cols[1], cols[2], cols[3]);
-        e[k].c = make_int4(cols[4], cols[5], cols[6], cols[7]);
-      }
-      return e;
-    };
-    if ((st = upload(&h->d_gs_fwd_ent, entries(L.gs_fwd_cells))) ||
-        (st = upload(&h->d_gs_bwd_ent, entries(L.gs_bwd_cells)))) {
-      pfm_ldu_destroy(h);
-      return st;
-    }
   }
   {
     // Column offsets from the row in int16 when every one fits: the PCG's

@@ -4,6 +4,7 @@
 #include <hip/hip_runtime.h>
 
 #include <atomic>
+#include <mutex>
 #include <cstdint>
 #include <vector>
 
@@ -164,7 +165,8 @@ struct pfm_ldu_s {
   std::vector<int32_t> gs_fwd_ptr, gs_bwd_ptr;  // host copies of GS level offsets
   int32_t *d_gs_fwd_ptr = nullptr, *d_gs_fwd_cells = nullptr;
   int32_t *d_gs_bwd_ptr = nullptr, *d_gs_bwd_cells = nullptr;
-  pfm::GsEnt *d_gs_fwd_ent = nullptr, *d_gs_bwd_ent = nullptr;  // per scheduled cell
+  pfm::GsEnt *d_gs_fwd_ent = nullptr, *d_gs_bwd_ent = nullptr;  // ensure_gs_entries()
+  std::mutex gs_ent_mu;
   pfm::PollPool poll;                   // pinned stop-flag slots of the solves
   bool oriented_sorted = false;         // every owner < neighbour, owners non-decreasing
   pfm_progress_fn progress = nullptr;   // optional host progress report
@@ -179,6 +181,10 @@ namespace pfm {
 int pcg_solve(pfm_ldu h, const pfm_solve_params *prm, const double *diag, const double *packed,
               const double *rD, const double *b, double *x, void *work, double *history_h,
               pfm_solve_result *res, void *stream, bool may_allocate);
+
+// Builds the handle's Gauss-Seidel schedule entries if they are not built yet
+// (allocates; synchronises `s` once).
+int ensure_gs_entries(pfm_ldu_s *h, hipStream_t s);
 
 // Calls the handle's progress report when its interval has passed since `last`.
 void report_progress(const pfm_ldu_s *h, int32_t iterations, double &last);

@@ -317,6 +317,8 @@ int pfm_gs_smooth(pfm_ldu h, const double *diag, const double *packed, const dou
   PFM_NONNULL(x);
   PFM_REQUIRE(h->nslots == 0 || packed, "null packed coefficients");
   hipStream_t s = (hipStream_t)stream;
+  int st = ensure_gs_entries(h, s);
+  if (st != PFM_OK) return st;
   const int nfl = (int)h->gs_fwd_ptr.size() - 1, nbl = (int)h->gs_bwd_ptr.size() - 1;
   const int levels = std::max(nfl, nbl);
   if (h->n < 2048 * levels) {
@@ -445,6 +447,10 @@ int pfm_smooth_solve(pfm_ldu h, const pfm_solve_params *prm, int smoother, int32
   PFM_REQUIRE(smoother == PFM_SMOOTH_GAUSS_SEIDEL || rD, "preconditioner diagonal missing");
   PFM_REQUIRE(h->nf == 0 || (lower && upper), "null face coefficients");
   hipStream_t s = (hipStream_t)stream;
+  if (smoother == PFM_SMOOTH_GAUSS_SEIDEL) {
+    const int est = ensure_gs_entries(h, s);
+    if (est != PFM_OK) return est;
+  }
   SmoothWork w;
   smooth_layout(h, prm->max_iter, &w, (char *)work);
   const int hist_cap = prm->max_iter + 1;
