@@ -401,6 +401,18 @@ def bench_batch(args, dev):
     seq_ms = seq_s * 1e3 * ns / k_s
     log(f"one at a time: {seq_s * 1e3:.1f} ms for {k_s} sampled systems -> ~{seq_ms:.0f} ms "
         f"for {ns} ({seq_ms / ms:.1f}x the batched solve)")
+    # algorithmic bytes: each system's PCG iteration moves 24 F_k + 112 N_k (the
+    # single-system figure), counted over the iterations each system ran
+    nk = np.diff(bc.cell_offsets).astype(np.float64)
+    fk = np.diff(bc.face_offsets).astype(np.float64)
+    per_iter = 24.0 * fk + 112.0 * nk
+    batch_bytes = float((it.astype(np.float64) * per_iter).sum())
+    gbs = batch_bytes / (ms * 1e-3) / 1e9
+    full_us = float(B.autotune_us[B.tile_slices])
+    full_gbs = float(per_iter.sum()) / (full_us * 1e-6) / 1e9
+    log(f"batch bandwidth: {batch_bytes / 1e9:.1f} GB algorithmic over the solve, {gbs:.0f} GB/s "
+        f"({gbs / HBM_PEAK_GBS:.3f} of peak); every system active: {full_gbs:.0f} GB/s "
+        f"({full_gbs / HBM_PEAK_GBS:.3f})")
     return {"n_systems": ns, "n_cells": N, "n_internal_faces": F,
             "sizes": f"n ~ U{{{args.batch_nmin}..{args.batch_nmax}}} seed 0",
             "ms": round(ms, 2), "ms_per_system": round(ms / ns, 4), "max_iter": max_iter,
@@ -414,6 +426,12 @@ def bench_batch(args, dev):
             "autotune_us_per_iter": {str(k): round(v, 2) for k, v in B.autotune_us.items()},
             "adjoint": {"forward_ms": round(tf * 1e3, 2), "backward_ms": round(tb * 1e3, 2),
                         "ratio": round(tb / tf, 3)},
+            "algorithmic_bytes": batch_bytes, "gbs_algorithmic": round(gbs, 1),
+            "frac_of_hbm_peak": round(gbs / HBM_PEAK_GBS, 4),
+            "full_load": {"us_per_iteration": round(full_us, 2),
+                          "bytes_per_iteration": float(per_iter.sum()),
+                          "gbs": round(full_gbs, 1),
+                          "frac_of_hbm_peak": round(full_gbs / HBM_PEAK_GBS, 4)},
             "one_at_a_time_ms_est": round(seq_ms, 1), "one_at_a_time_sample": int(k_s),
             "speedup_vs_one_at_a_time": round(seq_ms / ms, 2),
             "setup_s": {"cases_host": round(t_gen, 1), "handle_autotune": round(t_create, 1)}}
