@@ -119,3 +119,86 @@ def test_solve_pressure_equation(gpu_device):
     r = (mat.source - mat.Ax(p)).norm().item()
     assert r <= 1e-8 * mat.source.norm().item()
     assert abs(p[0].item()) < 1e-6 * p.abs().max().item()
+
+
+def _graph(kind, rng):
+    """Internal faces (own, nei) and boundary owners of a random topology:
+    'shuffled' -- 300 cells, random orientation and order, rows wider than
+    four couplings per side; 'shuffled_far' -- 80,000 cells, columns beyond
+    the 16-bit offsets; 'sorted_far' -- owner < neighbour sorted by owner,
+    columns beyond 65,535; 'sorted_near' -- the same within a narrow band."""
+    if kind == "shuffled":
+        n, m = 300, 2000
+        own, nei = rng.integers(0, n, m), rng.integers(0, n, m)
+    elif kind == "shuffled_far":
+        n, m = 80000, 240000
+        own, nei = rng.integers(0, n, m), rng.integers(0, n, m)
+    else:
+        n = 80000 if kind == "sorted_far" else 4000
+        reach = n if kind == "sorted_far" else 40
+        own = np.repeat(np.arange(n), 3)
+        nei = own + rng.integers(1, reach, own.shape[0])
+    keep = (own != nei) & (nei < n)
+    own, nei = own[keep], nei[keep]
+    if kind.startswith("sorted"):
+        key = np.unique(own.astype(np.int64) * n + nei)
+        own, nei = key // n, key % n
+    bown = rng.integers(0, n, n // 3)
+    return n, own.astype(np.int64), nei.astype(np.int64), bown.astype(np.int64)
+
+
+@pytest.mark.parametrize("kind", ["shuffled", "shuffled_far", "sorted_far", "sorted_near"])
+def test_pressure_kernels_on_random_topologies(gpu_device, kind):
+    """Assembly, face-flux and velocity corrections through the C ABI on
+    topologies the mesh tests do not reach: faces in any order and
+    orientation, rows of more than four couplings per side, columns beyond the
+    16-bit offsets, owner-sorted faces with and without them.  Bit-exact
+    against the oracle."""
+    from pyopenfoam_amd import LduMatrix, _lib
+    from pyopenfoam_amd.core.device import current_stream
+    print(f"pressure kernels on {kind}", flush=True)
+    rng = np.random.default_rng(17)
+    n, own, nei, bown = _graph(kind, rng)
+    nfi, nb = own.shape[0], bown.shape[0]
+    nfa = nfi + nb
+    own_all = np.concatenate([own, bown])
+    m = LduMatrix(n, torch.from_numpy(own), torch.from_numpy(nei), device=gpu_device)
+    info = m.addressing.info
+    if kind == "shuffled":
+        assert info.max_lower > 4 and info.max_upper > 4
+    h = m.addressing.handle
+    ap = rng.uniform(0.5, 3.0, n) * np.where(rng.random(n) < 0.3, -1.0, 1.0)
+    w = rng.uniform(0.05, 0.95, nfi)
+    mag = rng.uniform(0.5, 2.0, nfi)
+    dl = rng.uniform(0.5, 2.0, nfi)
+    vol = rng.uniform(0.5, 2.0, n)
+    phi = rng.normal(size=nfa)
+    p = rng.normal(size=n)
+    hb = rng.normal(size=(n, 3))
+    sf = rng.normal(size=(nfa, 3))
+    order = np.argsort(bown, kind="stable").astype(np.int32)
+    bptr = np.zeros(n + 1, np.int64)
+    np.cumsum(np.bincount(bown, minlength=n), out=bptr[1:])
+    T = lambda a: _t(a, gpu_device)
+    d_bptr, d_bface = T(bptr.astype(np.int32)), T(order)
+    d_ap, d_w, d_mag, d_dl, d_vol = T(ap), T(w), T(mag), T(dl), T(vol)
+    d_phi, d_p, d_hb, d_sf = T(phi), T(p), T(hb), T(sf)
+    L, q, s = _lib.lib(), _lib.ptr, current_stream(gpu_device)
+    diag, lower = torch.empty(n, dtype=torch.float64, device=gpu_device), torch.empty(
+        nfi, dtype=torch.float64, device=gpu_device)
+    upper, src = torch.empty_like(lower), torch.empty_like(diag)
+    _lib.check(L.pfm_pressure_assemble(h, nfa, q(d_bptr), q(d_bface), q(d_phi), q(d_ap), q(d_w),
+                                       q(d_mag), q(d_dl), q(d_vol), q(diag), q(lower), q(upper),
+                                       q(src), s), "assemble")
+    flux = torch.empty_like(d_phi)
+    _lib.check(L.pfm_correct_face_flux(h, nfa, q(d_phi), q(d_p), q(d_ap), q(d_w), q(d_mag),
+                                       q(d_dl), q(flux), s), "flux")
+    u = torch.empty_like(d_hb)
+    _lib.check(L.pfm_correct_velocity(h, nfa, q(d_bptr), q(d_bface), q(d_hb), q(d_p), q(d_ap),
+                                      q(d_w), q(d_sf), q(d_vol), 0.7, q(u), s), "velocity")
+    d, lo, up, sr = opr.assemble(phi, ap, w, mag, dl, vol, own_all, nei)
+    assert bits_equal(lower.cpu().numpy(), lo) and bits_equal(upper.cpu().numpy(), up)
+    assert bits_equal(diag.cpu().numpy(), d) and bits_equal(src.cpu().numpy(), sr)
+    assert bits_equal(flux.cpu().numpy(), opr.correct_face_flux(phi, p, ap, w, mag, dl, own_all, nei))
+    assert bits_equal(u.cpu().numpy(), opr.correct_velocity(hb, p, ap, w, sf, vol, own_all, nei,
+                                                            alpha=0.7))
