@@ -22,12 +22,10 @@ inline int grid1d(int64_t n) { return (int)((n + kBlock - 1) / kBlock); }
 __device__ inline double clamp_min(double x, double lo) { return x < lo ? lo : x; }
 __device__ inline double inv_ap(double a) { return 1.0 / clamp_min(fabs(a), 1e-30); }
 
-// Face-interpolated 1/A_p: w*inv_P + (1 - w)*inv_N (pressure_equation.py:102-111, 313).
-__device__ inline double inv_face(const double *__restrict__ ap, const double *__restrict__ w,
-                                  const int32_t *__restrict__ own, const int32_t *__restrict__ nei,
-                                  int f) {
-  const double wf = w[f];
-  return wf * inv_ap(ap[own[f]]) + (1.0 - wf) * inv_ap(ap[nei[f]]);
+// Face-interpolated 1/A_p: w*inv_P + (1 - w)*inv_N (pressure_equation.py:102-111, 313)
+// of a face with weight wf, owner o and neighbour q.
+__device__ inline double inv_face(const double *__restrict__ ap, double wf, int o, int q) {
+  return wf * inv_ap(ap[o]) + (1.0 - wf) * inv_ap(ap[q]);
 }
 
 // lower = -c/V_P, upper = -c/V_N with c = (1/A_p)_f |S_f| delta_f (:117-129).
@@ -42,9 +40,12 @@ __global__ __launch_bounds__(kBlock) void k_pe_faces(int nf, const int32_t *__re
                                                       double *__restrict__ upper) {
   const int f = blockIdx.x * kBlock + threadIdx.x;
   if (f >= nf) return;
-  const double c = (inv_face(ap, w, own, nei, f) * mag[f]) * delta[f];
-  lower[f] = (-c) / vol[own[f]];
-  upper[f] = (-c) / vol[nei[f]];
+  // The face arrays are read once: nontemporal loads keep the gathered A_p
+  // and volumes cached.
+  const int o = ld<true>(own + f), q = ld<true>(nei + f);
+  const double c = (inv_face(ap, ld<true>(w + f), o, q) * ld<true>(mag + f)) * ld<true>(delta + f);
+  lower[f] = (-c) / vol[o];
+  upper[f] = (-c) / vol[q];
 }
 
 // Per row: diag = sum(c/V_P over owner faces) + sum(c/V_N over neighbour
@@ -141,8 +142,10 @@ __global__ __launch_bounds__(kBlock) void k_cff(int nf_all, int nf,
     out[f] = phi[f];
     return;
   }
-  const double dp = p[own[f]] - p[nei[f]];
-  out[f] = phi[f] + (((inv_face(ap, w, own, nei, f) * dp) * mag[f]) * delta[f]);
+  const int o = ld<true>(own + f), q = ld<true>(nei + f);  // face arrays: read once
+  const double dp = p[o] - p[q];
+  out[f] = ld<true>(phi + f) +
+           (((inv_face(ap, ld<true>(w + f), o, q) * dp) * ld<true>(mag + f)) * ld<true>(delta + f));
 }
 
 // Gauss gradient with linearly interpolated face pressure (boundary faces
