@@ -341,6 +341,11 @@ __device__ __forceinline__ XcdRange xcd_range(int nact) {
 // partial sums are read.  Per-row arithmetic and summation order are those of
 // one slice at a time.
 
+// The vector streams of the 256-thread update kernels (r, Ap, rD, p, x: 150 MB
+// each for the 512-system batch) are read nontemporally, which keeps the
+// gathered p and the small per-system data cached (3.58 -> 3.41 s).
+__device__ __forceinline__ double ldnt(const double *a) { return __builtin_nontemporal_load(a); }
+
 // Ap = A p; tile partial p.Ap.  Tiles of a system that stopped earlier in the
 // chunk compute values nobody reads, so this kernel skips the stop check.
 template <bool kPair>
@@ -356,8 +361,9 @@ __global__ __launch_bounds__(kBlock) void k_b_spmv(const int4 *__restrict__ act,
                                                     const BGlobal *__restrict__ g,
                                                     const BSys *sys) {
   __shared__ double lds[kMaxWaves];
-  // The batch's coefficients (1.3 GB for the 512-system batch) are read once
-  // per iteration: nontemporal loads keep the vectors cached (3.73 -> 3.64 s).
+  // The batch's coefficients (1.3 GB for the 512-system batch) and diagonal
+  // are read once per iteration: nontemporal loads keep the vectors cached
+  // (3.73 -> 3.64 s; the diagonal's, 611 -> 598 us per full-load iteration).
   auto gx = [p](int c) { return p[c]; };
   FOR_ACTIVE_TILES(false, {
     (void)k;
@@ -373,11 +379,11 @@ __global__ __launch_bounds__(kBlock) void k_b_spmv(const int4 *__restrict__ act,
       const double pa = p[ra];
       double va;
       if constexpr (kPair)  // paired int16 column offsets
-        va = row_spmv_pair<double, decltype(gx), false, true>(sa_, s, lane, ra, diag[ra] * pa, cp,
-                                                              val, gx);
+        va = row_spmv_pair<double, decltype(gx), false, true>(sa_, s, lane, ra, ldnt(diag + ra) * pa,
+                                                              cp, val, gx);
       else
         va = row_spmv_g<double, decltype(gx), false, ColAbsT<true>, true>(
-            sa_, lane, diag[ra] * pa, ColAbsT<true>{col}, val, gx);
+            sa_, lane, ldnt(diag + ra) * pa, ColAbsT<true>{col}, val, gx);
       Ap[ra] = va;
       pap += pa * va;
     }
@@ -386,10 +392,6 @@ __global__ __launch_bounds__(kBlock) void k_b_spmv(const int4 *__restrict__ act,
   })
 }
 
-// The vector streams of the 256-thread update kernels (r, Ap, rD, p, x: 150 MB
-// each for the 512-system batch) are read nontemporally, which keeps the
-// gathered p and the small per-system data cached (3.58 -> 3.41 s).
-__device__ __forceinline__ double ldnt(const double *a) { return __builtin_nontemporal_load(a); }
 
 // alpha per system (kept for k_b_pupdate); r -= alpha Ap; tile partials r.r,
 // r.z.  A breakdown ends the system before this iteration's x update.
