@@ -417,6 +417,130 @@ __global__ __launch_bounds__(kDiluThreads) void k_dilu_sweep(const int32_t *__re
   }
 }
 
+// The same sweep with the diagonal in LDS.  Each face level's faces touch
+// cells spread over the whole mesh, but a cell is touched only within a
+// window of levels (first to last face holding it): 16,449 cells are live at
+// once at 128^3 (1,041 at 32^3, ~n^2).  The host gives each cell an LDS slot
+// for its window.  Level l is one phase between barriers: its faces (the
+// arithmetic of dilu_face, on LDS), the write-back to d of the cells whose
+// window ended at level l - 1, and the load of the cells whose window starts
+// at level l + 1.  The three touch different slots: a slot written back in
+// phase l is given to another cell from phase l + 1 on.  After the last
+// level, the cells whose window ended there are written back.  Each of the
+// kT threads holds at most one face and one phase entry per level (the host
+// picks kT = 512 or 1024 to fit), fetched kDiluAhead levels before use into a
+// register ring; the level pointers come a chunk of kDiluChunk levels at a
+// time into LDS.  The fetches are unconditional (clamped indices, validity
+// kept beside the raw values) so that the compiler's waits before a ring
+// entry is used cover the loads of earlier levels only.  At 128^3 (33,147
+// levels, at most 254 faces and 130 phase entries per level) the sweep takes
+// 14.2 ms with 512 threads, 16.6 ms with 1024, 15.0 ms with 256; on the
+// global diagonal it took 34.7-36.8 ms.
+constexpr int kDiluSlots = 19000;   // 152,000 B of the 160 KB LDS
+constexpr int kDiluAhead = 8;
+constexpr int kDiluUnroll = 2;
+constexpr int kDiluChunk = 256;
+
+__device__ __forceinline__ void dilu_face_lds(uint32_t w, double a, double *dl) {
+  const int sp = (int)(w & 0xffffu), sq = (int)(w >> 16);
+  const double dp = dl[sp], dq = dl[sq];
+  if (fabs(dq) > 1e-30) dl[sp] = dp - a / dq;
+  if (fabs(dp) > 1e-30) dl[sq] = dq - a / dp;
+}
+
+// Phase lists: iptr[0] .. iptr[1] the cells loaded before level 0, iptr[l + 1]
+// .. iptr[l + 2] those of phase l (0 <= l <= nl; phase nl holds write-backs
+// only).  Entries (cell, slot) load, (cell, -(slot + 1)) write back.  nf,
+// nio >= 1.
+template <int kT>
+__global__ __launch_bounds__(kT) void k_dilu_lds(int nl, int nf, int nio,
+                                                  const int32_t *__restrict__ fptr,
+                                                  const uint32_t *__restrict__ fslot,
+                                                  const double *__restrict__ lu,
+                                                  const int32_t *__restrict__ iptr,
+                                                  const int2 *__restrict__ io,
+                                                  const double *__restrict__ din,
+                                                  double *__restrict__ d) {
+  __shared__ double dl[kDiluSlots];
+  __shared__ int32_t pb[2][2][kDiluChunk + 1];  // chunk buffer, {face, phase} level offsets
+  const int t = (int)threadIdx.x;
+  if (nl <= 0) return;
+  // Level offsets of chunk c (levels c*C .. c*C + C, clamped to nl) into
+  // buffer c & 1.
+  auto load_chunk = [&](int c) {
+    const int l0 = c * kDiluChunk;
+    for (int i = t; i < 2 * (kDiluChunk + 1); i += kT) {
+      const int which = i / (kDiluChunk + 1), j = i - which * (kDiluChunk + 1);
+      const int l = min(l0 + j, nl);
+      pb[c & 1][which][j] = which == 0 ? fptr[l] : iptr[l + 1];
+    }
+  };
+  for (int k = iptr[0] + t; k < iptr[1]; k += kT) dl[io[k].y] = din[k];
+  load_chunk(0);
+  load_chunk(1);
+  __syncthreads();
+  uint32_t rw[kDiluAhead];
+  double ra[kDiluAhead], rv[kDiluAhead];
+  int2 re[kDiluAhead];
+  int rok[kDiluAhead];  // bit 0: a face, bit 1: a phase entry
+  // This thread's face and phase entry of level l (levels past the end read
+  // the empty range of level nl) into ring slot j.
+  auto fetch = [&](int j, int l) {
+    const int lc = min(l, nl);
+    const int c = lc / kDiluChunk, o = lc - c * kDiluChunk;
+    const int32_t *pf = pb[c & 1][0], *pi = pb[c & 1][1];
+    const int fb = pf[o] + t, ib = pi[o] + t;
+    rok[j] = (fb < pf[o + 1] ? 1 : 0) | (ib < pi[o + 1] ? 2 : 0);
+    const int fi = min(fb, nf - 1), ii = min(ib, nio - 1);
+    rw[j] = fslot[fi];
+    ra[j] = lu[fi];
+    re[j] = io[ii];
+    rv[j] = din[ii];
+  };
+#pragma unroll
+  for (int j = 0; j < kDiluAhead; ++j) fetch(j, j);
+  // Two turns of the ring per loop iteration: the compiler's waits are exact
+  // inside the unrolled body and drain the outstanding loads only where the
+  // loop closes.
+  for (int l0 = 0; l0 < nl; l0 += kDiluUnroll * kDiluAhead) {
+#pragma unroll
+    for (int jj = 0; jj < kDiluUnroll * kDiluAhead; ++jj) {
+      const int j = jj % kDiluAhead, l = l0 + jj;
+      if (l >= nl) break;
+      const int c = l / kDiluChunk, o = l - c * kDiluChunk;
+      if (o == 0 && c > 0) load_chunk(c + 1);  // chunk c - 1 is no longer read
+      const int ok = rok[j];
+      const uint32_t w = rw[j];
+      const double a = ra[j], v = rv[j];
+      const int2 e = re[j];
+      fetch(j, l + kDiluAhead);
+      if (ok & 1) dilu_face_lds(w, a, dl);
+      if (ok & 2) {
+        if (e.y >= 0)
+          dl[e.y] = v;
+        else
+          d[e.x] = dl[-e.y - 1];
+      }
+      __syncthreads();
+    }
+  }
+  for (int k = iptr[nl + 1] + t; k < iptr[nl + 2]; k += kT) {  // phase nl
+    const int2 e = io[k];
+    d[e.x] = dl[-e.y - 1];
+  }
+}
+
+// The initial value of every cell the DILU LDS sweep loads (din[k] = d[cell]
+// for a load entry), read before the sweep so that its loads need no index.
+__global__ __launch_bounds__(kBlock) void k_dilu_din(int nio, const int2 *__restrict__ io,
+                                                      const double *__restrict__ d,
+                                                      double *__restrict__ din) {
+  const int k = blockIdx.x * kBlock + threadIdx.x;
+  if (k >= nio) return;
+  const int2 e = io[k];
+  din[k] = e.y >= 0 ? d[e.x] : 0.0;
+}
+
 // Small schedules: one launch per face level (used when the level count is
 // low enough that launches are cheaper than the single-workgroup sweep).
 __global__ __launch_bounds__(kBlock) void k_dilu_level(const int32_t *__restrict__ faces, int count,
@@ -588,6 +712,13 @@ __global__ __launch_bounds__(kBlock) void k_dir_grad(int nd, const int32_t *__re
 
 constexpr int kDiluLaunchLevels = 64;
 
+// PFM_DILU_LDS=0 keeps the DILU sweep on the global-memory diagonal (for
+// comparing the two forms; read at every factorisation).
+bool dilu_lds_enabled() {
+  const char *e = std::getenv("PFM_DILU_LDS");
+  return !(e && std::strcmp(e, "0") == 0);
+}
+
 inline int grid1d(int64_t n) { return (int)((n + kBlock - 1) / kBlock); }
 
 template <typename T>
@@ -631,6 +762,137 @@ int ensure_gs_entries(pfm_ldu_s *h, hipStream_t s) {
   h->d_gs_fwd_ent = fwd;
   return PFM_OK;
 }
+
+int ensure_dilu_slots(pfm_ldu_s *h, hipStream_t s) {
+  std::lock_guard<std::mutex> lock(h->dilu_slots_mu);
+  if (h->dilu_slots_state != 0) return PFM_OK;
+  const int nf = h->nf, n = h->n;
+  const int nl = (int)h->dilu_level_ptr.size() - 1;
+  if (nf == 0 || nl <= 0) {
+    h->dilu_slots_state = -1;
+    return PFM_OK;
+  }
+  std::vector<int32_t> faces(nf), own(nf), nei(nf);
+  PFM_CHECK_HIP(hipMemcpyAsync(faces.data(), h->d_dilu_faces, sizeof(int32_t) * nf,
+                               hipMemcpyDeviceToHost, s));
+  PFM_CHECK_HIP(hipMemcpyAsync(own.data(), h->d_own, sizeof(int32_t) * nf, hipMemcpyDeviceToHost, s));
+  PFM_CHECK_HIP(hipMemcpyAsync(nei.data(), h->d_nei, sizeof(int32_t) * nf, hipMemcpyDeviceToHost, s));
+  PFM_CHECK_HIP(hipStreamSynchronize(s));
+  const std::vector<int32_t> &lp = h->dilu_level_ptr;
+  // each cell's window of levels
+  std::vector<int32_t> first(n, INT32_MAX), last(n, -1);
+  for (int l = 0; l < nl; ++l)
+    for (int k = lp[l]; k < lp[l + 1]; ++k) {
+      const int f = faces[k];
+      for (const int c : {own[f], nei[f]}) {
+        first[c] = std::min(first[c], l);
+        last[c] = std::max(last[c], l);
+      }
+    }
+  // cells by first and by last level (counting sorts)
+  std::vector<int32_t> fptr(nl + 2, 0), lptr(nl + 2, 0);
+  for (int c = 0; c < n; ++c)
+    if (last[c] >= 0) {
+      ++fptr[first[c] + 1];
+      ++lptr[last[c] + 1];
+    }
+  for (int l = 0; l <= nl; ++l) {
+    fptr[l + 1] += fptr[l];
+    lptr[l + 1] += lptr[l];
+  }
+  std::vector<int32_t> byfirst(fptr[nl + 1]), bylast(lptr[nl + 1]);
+  {
+    std::vector<int32_t> fp(fptr.begin(), fptr.end() - 1), lq(lptr.begin(), lptr.end() - 1);
+    for (int c = 0; c < n; ++c)
+      if (last[c] >= 0) {
+        byfirst[fp[first[c]]++] = c;
+        bylast[lq[last[c]]++] = c;
+      }
+  }
+  // slots; phase l: write-backs (window ended at l - 1), loads (window starts
+  // at l + 1); a slot written back in phase l is reused from phase l + 1 on
+  std::vector<int32_t> slot(n, -1), freelist, released;
+  std::vector<int2> io;
+  std::vector<int32_t> iptr(nl + 3, 0);
+  io.reserve(2 * (size_t)fptr[nl + 1]);
+  int nslots = 0;
+  auto take = [&](int c) {
+    int v;
+    if (!freelist.empty()) {
+      v = freelist.back();
+      freelist.pop_back();
+    } else {
+      v = nslots++;
+    }
+    slot[c] = v;
+    io.push_back(make_int2(c, v));
+  };
+  auto write_back = [&](int l) {  // cells whose window ended at level l
+    for (int k = lptr[l]; k < lptr[l + 1]; ++k) {
+      const int c = bylast[k];
+      io.push_back(make_int2(c, -(slot[c] + 1)));
+      released.push_back(slot[c]);
+    }
+  };
+  for (int k = fptr[0]; k < fptr[1]; ++k) take(byfirst[k]);  // before level 0
+  iptr[1] = (int32_t)io.size();
+  for (int l = 0; l < nl; ++l) {
+    if (l > 0) write_back(l - 1);
+    if (l + 1 < nl)
+      for (int k = fptr[l + 1]; k < fptr[l + 2]; ++k) take(byfirst[k]);
+    freelist.insert(freelist.end(), released.begin(), released.end());
+    released.clear();
+    iptr[l + 2] = (int32_t)io.size();
+  }
+  write_back(nl - 1);  // phase nl
+  iptr[nl + 2] = (int32_t)io.size();
+  int widest = 0;  // faces or phase entries of the fullest level
+  for (int l = 0; l < nl; ++l)
+    widest = std::max(widest, std::max(lp[l + 1] - lp[l], iptr[l + 2] - iptr[l + 1]));
+  if (nslots > kDiluSlots || io.empty() || widest > kDiluThreads) {
+    h->dilu_slots_state = -1;
+    return PFM_OK;
+  }
+  h->dilu_lds_threads = widest <= 512 ? 512 : kDiluThreads;
+  std::vector<uint32_t> fslot(nf);
+  for (int k = 0; k < nf; ++k) {
+    const int f = faces[k];
+    fslot[k] = (uint32_t)slot[own[f]] | ((uint32_t)slot[nei[f]] << 16);
+  }
+  uint32_t *d_fslot = nullptr;
+  int2 *d_io = nullptr;
+  int32_t *d_iptr = nullptr;
+  double *d_din = nullptr;
+  const size_t nio = io.size();
+  auto fail = [&] {
+    (void)hipFree(d_fslot);
+    (void)hipFree(d_io);
+    (void)hipFree(d_iptr);
+    (void)hipFree(d_din);
+    set_error("building the DILU slot schedule failed");
+    return PFM_ERR_HIP;
+  };
+  if (hipMalloc((void **)&d_fslot, sizeof(uint32_t) * nf) != hipSuccess ||
+      hipMalloc((void **)&d_io, sizeof(int2) * std::max<size_t>(nio, 1)) != hipSuccess ||
+      hipMalloc((void **)&d_iptr, sizeof(int32_t) * (nl + 3)) != hipSuccess ||
+      hipMalloc((void **)&d_din, sizeof(double) * nio) != hipSuccess)
+    return fail();
+  if (hipMemcpyAsync(d_fslot, fslot.data(), sizeof(uint32_t) * nf, hipMemcpyHostToDevice, s) !=
+          hipSuccess ||
+      (nio && hipMemcpyAsync(d_io, io.data(), sizeof(int2) * nio, hipMemcpyHostToDevice, s) !=
+                  hipSuccess) ||
+      hipMemcpyAsync(d_iptr, iptr.data(), sizeof(int32_t) * (nl + 3), hipMemcpyHostToDevice, s) !=
+          hipSuccess ||
+      hipStreamSynchronize(s) != hipSuccess)
+    return fail();
+  h->d_dilu_fslot = d_fslot;
+  h->d_dilu_io = d_io;
+  h->d_dilu_iptr = d_iptr;
+  h->d_dilu_din = d_din;
+  h->n_dilu_io = (int32_t)nio;
+  h->dilu_slots_state = 1;
+  return PFM_OK;
+}
 }  // namespace pfm
 
 extern "C" {
@@ -652,6 +914,10 @@ int pfm_ldu_destroy(pfm_ldu h) {
   (void)hipFree(h->d_dilu_level_ptr);
   (void)hipFree(h->d_dilu_pq);
   (void)hipFree(h->d_dilu_lu);
+  (void)hipFree(h->d_dilu_fslot);
+  (void)hipFree(h->d_dilu_io);
+  (void)hipFree(h->d_dilu_iptr);
+  (void)hipFree(h->d_dilu_din);
   (void)hipFree(h->d_gs_fwd_ptr);
   (void)hipFree(h->d_gs_fwd_cells);
   (void)hipFree(h->d_gs_bwd_ptr);
@@ -988,10 +1254,28 @@ int pfm_dilu_factor(pfm_ldu h, int transpose, const double *diag, const double *
         PFM_CHECK_HIP(hipMalloc((void **)&h->d_dilu_pq, sizeof(int2) * (size_t)h->nf));
         PFM_CHECK_HIP(hipMalloc((void **)&h->d_dilu_lu, sizeof(double) * (size_t)h->nf));
       }
+      if (dilu_lds_enabled()) {
+        const int rc = ensure_dilu_slots(h, s);
+        if (rc != PFM_OK) return rc;
+      }
       k_dilu_gather<<<grid1d(h->nf), kBlock, 0, s>>>(h->nf, h->d_dilu_faces, h->d_own, h->d_nei,
                                                      lo, up, (int2 *)h->d_dilu_pq, h->d_dilu_lu);
-      k_dilu_sweep<<<1, kDiluThreads, 0, s>>>(h->d_dilu_level_ptr, nl, (const int2 *)h->d_dilu_pq,
-                                               h->d_dilu_lu, rD);
+      if (dilu_lds_enabled() && h->dilu_slots_state == 1) {
+        if (h->n_dilu_io)
+          k_dilu_din<<<grid1d(h->n_dilu_io), kBlock, 0, s>>>(h->n_dilu_io, h->d_dilu_io, rD,
+                                                             h->d_dilu_din);
+        if (h->dilu_lds_threads == 512)
+          k_dilu_lds<512><<<1, 512, 0, s>>>(nl, h->nf, h->n_dilu_io, h->d_dilu_level_ptr,
+                                            h->d_dilu_fslot, h->d_dilu_lu, h->d_dilu_iptr,
+                                            h->d_dilu_io, h->d_dilu_din, rD);
+        else
+          k_dilu_lds<kDiluThreads><<<1, kDiluThreads, 0, s>>>(
+              nl, h->nf, h->n_dilu_io, h->d_dilu_level_ptr, h->d_dilu_fslot, h->d_dilu_lu,
+              h->d_dilu_iptr, h->d_dilu_io, h->d_dilu_din, rD);
+      } else {
+        k_dilu_sweep<<<1, kDiluThreads, 0, s>>>(h->d_dilu_level_ptr, nl,
+                                                 (const int2 *)h->d_dilu_pq, h->d_dilu_lu, rD);
+      }
     }
     PFM_LAUNCH_CHECK();
   }

@@ -162,6 +162,17 @@ struct pfm_ldu_s {
   int32_t *d_dilu_level_ptr = nullptr;
   void *d_dilu_pq = nullptr;            // level-ordered (owner, neighbour), lazily allocated
   double *d_dilu_lu = nullptr;          // level-ordered lower*upper
+  // LDS-resident DILU sweep (ensure_dilu_slots): per face in level order the
+  // LDS slots of its owner and neighbour (low / high 16 bits); per phase the
+  // cells written back and loaded, as (cell, -(slot + 1)) or (cell, slot).
+  int dilu_slots_state = 0;             // 0 not built, 1 built, -1 does not fit in LDS
+  int dilu_lds_threads = 0;             // threads of the sweep (512 or 1024)
+  std::mutex dilu_slots_mu;
+  uint32_t *d_dilu_fslot = nullptr;
+  int2 *d_dilu_io = nullptr;
+  int32_t *d_dilu_iptr = nullptr;       // levels + 3: the loads before level 0, phases 0 .. levels
+  double *d_dilu_din = nullptr;         // initial value of each loaded cell, per factor call
+  int32_t n_dilu_io = 0;
   std::vector<int32_t> gs_fwd_ptr, gs_bwd_ptr;  // host copies of GS level offsets
   int32_t *d_gs_fwd_ptr = nullptr, *d_gs_fwd_cells = nullptr;
   int32_t *d_gs_bwd_ptr = nullptr, *d_gs_bwd_cells = nullptr;
@@ -185,6 +196,11 @@ int pcg_solve(pfm_ldu h, const pfm_solve_params *prm, const double *diag, const 
 // Builds the handle's Gauss-Seidel schedule entries if they are not built yet
 // (allocates; synchronises `s` once).
 int ensure_gs_entries(pfm_ldu_s *h, hipStream_t s);
+
+// Builds the handle's LDS slot schedule of the DILU sweep if it is not built
+// yet (host work over a download of the face levels; synchronises `s`), and
+// records whether the cells live at once fit the workgroup's LDS.
+int ensure_dilu_slots(pfm_ldu_s *h, hipStream_t s);
 
 // Calls the handle's progress report when its interval has passed since `last`.
 void report_progress(const pfm_ldu_s *h, int32_t iterations, double &last);

@@ -86,6 +86,49 @@ def test_dilu_factor_bit_exact(gpu_device, n, jit, sig):
     assert bits_equal(rd, opre.dilu_build(d, lo, up, c.owner, c.neighbour))
 
 
+@pytest.mark.parametrize("lds", ["1", "0"])
+def test_dilu_sweep_forms(gpu_device, monkeypatch, lds):
+    """The DILU sweep with the diagonal held in LDS (one slot per cell for the
+    window of face levels that touch it, slots reused) and on the global
+    diagonal (PFM_DILU_LDS=0): both bit-exact against the oracle's sequential
+    recurrence, on a 24^3 jittered mesh and on random banded topologies, all
+    past the level count below which the factor is launched level by level:
+    faces shuffled in order and orientation (a level of 754 faces: the
+    1024-thread form of the LDS sweep), sorted with either orientation (the
+    512-thread form), and led by 1,400 disjoint faces (a level wider than the
+    LDS sweep takes: the factor falls back to the global diagonal)."""
+    monkeypatch.setenv("PFM_DILU_LDS", lds)
+    print(f"dilu sweep, PFM_DILU_LDS={lds}", flush=True)
+    c, d, lo, up = system(24, 0.2, 1.0)
+    m = _matrix(c, d, lo, up, gpu_device)
+    assert m.addressing.info.n_dilu_levels > 64
+    assert bits_equal(DILUPreconditioner(m).diagonal.cpu().numpy(),
+                      opre.dilu_build(d, lo, up, c.owner, c.neighbour))
+    rng = np.random.default_rng(29)
+    n, m = 3000, 20000
+    a = rng.integers(0, n, m)
+    b = np.clip(a + rng.integers(1, 40, m), 0, n - 1)
+    flip = rng.random(m) < 0.5
+    own, nei = np.where(flip, b, a), np.where(flip, a, b)
+    keep = own != nei
+    own, nei = own[keep], nei[keep]
+    # shuffled, sorted by cell pair with either orientation, led by disjoint faces
+    key = np.minimum(own, nei) * n + np.maximum(own, nei)
+    pairs = np.arange(1400)
+    wide = (np.concatenate([2 * pairs, own]), np.concatenate([2 * pairs + 1, nei]))
+    for o, q in ((own, nei), (own[np.argsort(key, kind="stable")], nei[np.argsort(key, kind="stable")]),
+                 wide):
+        lo2 = -rng.uniform(0.1, 1.0, o.shape[0])
+        up2 = -rng.uniform(0.1, 1.0, o.shape[0])
+        d2 = np.bincount(o, -lo2, n) + np.bincount(q, -up2, n) + 0.5
+        m2 = LduMatrix(n, torch.from_numpy(o), torch.from_numpy(q), device=gpu_device)
+        m2.diag, m2.lower, m2.upper = (torch.from_numpy(d2), torch.from_numpy(lo2),
+                                       torch.from_numpy(up2))
+        assert m2.addressing.info.n_dilu_levels > 64
+        assert bits_equal(DILUPreconditioner(m2).diagonal.cpu().numpy(),
+                          opre.dilu_build_py(d2, lo2, up2, o, q))
+
+
 def test_factor_unordered_faces(gpu_device):
     """Faces not in upper-triangular order (owner > neighbour, shuffled): the
     level schedules must still reproduce the sequential recurrences."""
