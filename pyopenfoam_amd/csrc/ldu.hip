@@ -836,13 +836,17 @@ int ensure_dilu_slots(pfm_ldu_s *h, hipStream_t s) {
   };
   for (int k = fptr[0]; k < fptr[1]; ++k) take(byfirst[k]);  // before level 0
   iptr[1] = (int32_t)io.size();
-  for (int l = 0; l < nl; ++l) {
+  for (int l = 0; l < nl && nslots <= kDiluSlots; ++l) {
     if (l > 0) write_back(l - 1);
     if (l + 1 < nl)
       for (int k = fptr[l + 1]; k < fptr[l + 2]; ++k) take(byfirst[k]);
     freelist.insert(freelist.end(), released.begin(), released.end());
     released.clear();
     iptr[l + 2] = (int32_t)io.size();
+  }
+  if (nslots > kDiluSlots) {  // more cells live at once than LDS holds
+    h->dilu_slots_state = -1;
+    return PFM_OK;
   }
   write_back(nl - 1);  // phase nl
   iptr[nl + 2] = (int32_t)io.size();
