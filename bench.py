@@ -633,7 +633,7 @@ def gs_launches_per_cycle(hier, solver) -> int:
 
 
 def bench_precision(args, dev):
-    """the 256^3 jittered mesh with lognormal(0, 1.5) diffusivity.
+    """The 256^3 jittered mesh with lognormal(0, 1.5) diffusivity.
     FP64 pair/Galerkin GAMG and its mixed-precision form (FP32 V-cycles on the
     levels above 4096 cells inside FP64 refinement on the true residual, FP64
     coarse levels and coarsest solve) to rel 1e-8; the SpMV at this size (1.6 GB
@@ -990,7 +990,7 @@ def main() -> int:
             log(f"gamg {name}: {gms:.1f} ms, {cyc} cycles ({gms / cyc:.2f} ms/cycle), converged "
                 f"{gs.last_info.converged}, true rel residual {rt:.2e}, {hier.n_levels} levels")
 
-    # --- precision: 256^3, lognormal(0, 1.5) --------------------
+    # --- precision: 256^3, lognormal(0, 1.5) -----------------------------------
     precision = None
     if not args.no_precision:
         precision = bench_precision(args, dev)

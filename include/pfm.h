@@ -209,7 +209,7 @@ int pfm_relax(int32_t n, double alpha, const double *diag, const double *source,
               void *stream);
 
 /* Achievable-bandwidth probes (no reference counterpart: measurement of the
- * ceiling the bandwidth-bound kernels are judged against).  mode 0 reads
+ * ceiling the bandwidth-bound kernels are compared with).  mode 0 reads
  * nbytes of src with nontemporal 16-byte loads and writes one partial sum per
  * wave to dst (pfm_bandwidth_probe_out_bytes() bytes); mode 1 copies nbytes
  * from src to dst.  nbytes: a multiple of 16; buffers 16-byte aligned. */

@@ -116,7 +116,7 @@ def test_full_size_gamg_cycles_bit_exact(gpu_device, coarse, smoother):
     oracle's pair maps, and x after V-cycles bit for bit as the oracle's V-cycle
     over the oracle's maps and coarse matrices (the reference's, or with the
     Galerkin coarse diagonal the bench's GAMG uses): two with Jacobi
-    smoothing, one with symmetric Gauss-Seidel sweeps (configs[1]'s smoother:
+    smoothing, one with symmetric Gauss-Seidel sweeps (the pair GAMG's level smoother:
     one launch per dependency level on the large levels, one workgroup walking
     the dependency levels on the smaller ones)."""
     S = _full(gpu_device)
