@@ -434,11 +434,11 @@ __global__ __launch_bounds__(kDiluThreads) void k_dilu_sweep(const int32_t *__re
 // kept beside the raw values) so that the compiler's waits before a ring
 // entry is used cover the loads of earlier levels only.  At 128^3 (33,147
 // levels, at most 254 faces and 130 phase entries per level) the sweep takes
-// 14.2 ms with 512 threads, 16.6 ms with 1024, 15.0 ms with 256; on the
-// global diagonal it took 34.7-36.8 ms.
+// 14.2 ms with 512 threads (16.6 ms with 1024, 15.0 ms with 256, at two turns
+// of the ring per loop iteration); on the global diagonal it took 34.7-36.8 ms.
 constexpr int kDiluSlots = 19000;   // 152,000 B of the 160 KB LDS
 constexpr int kDiluAhead = 8;
-constexpr int kDiluUnroll = 2;
+constexpr int kDiluUnroll = 4;
 constexpr int kDiluChunk = 256;
 
 __device__ __forceinline__ void dilu_face_lds(uint32_t w, double a, double *dl) {
@@ -499,9 +499,9 @@ __global__ __launch_bounds__(kT) void k_dilu_lds(int nl, int nf, int nio,
   };
 #pragma unroll
   for (int j = 0; j < kDiluAhead; ++j) fetch(j, j);
-  // Two turns of the ring per loop iteration: the compiler's waits are exact
-  // inside the unrolled body and drain the outstanding loads only where the
-  // loop closes.
+  // kDiluUnroll turns of the ring per loop iteration: the compiler's waits are
+  // exact inside the unrolled body and drain the outstanding loads only where
+  // the loop closes (14.8 / 14.5 / 14.2 / 14.2 ms for 1 / 2 / 4 / 8 turns).
   for (int l0 = 0; l0 < nl; l0 += kDiluUnroll * kDiluAhead) {
 #pragma unroll
     for (int jj = 0; jj < kDiluUnroll * kDiluAhead; ++jj) {
