@@ -913,6 +913,7 @@ int pfm_ldu_destroy(pfm_ldu h) {
   (void)hipFree(h->d_map);
   (void)hipFree(h->d_own);
   (void)hipFree(h->d_nei);
+  (void)hipFree(h->d_own_start);
   (void)hipFree(h->d_dic_cells);
   (void)hipFree(h->d_dilu_faces);
   (void)hipFree(h->d_dilu_level_ptr);
@@ -972,6 +973,15 @@ int pfm_ldu_create(int32_t n, int32_t nf, const int32_t *own, const int32_t *nei
       (st = upload(&h->d_gs_bwd_cells, L.gs_bwd_cells))) {
     pfm_ldu_destroy(h);
     return st;
+  }
+  if (h->oriented_sorted) {  // first owner-face of each row (faces sorted by owner)
+    std::vector<int32_t> os(n + 1, nf);
+    for (int32_t f = nf - 1; f >= 0; --f) os[own[f]] = f;
+    for (int32_t r = n - 1; r >= 0; --r) os[r] = std::min(os[r], os[r + 1]);
+    if ((st = upload(&h->d_own_start, os))) {
+      pfm_ldu_destroy(h);
+      return st;
+    }
   }
   {
     // Column offsets from the row in int16 when every one fits: the PCG's

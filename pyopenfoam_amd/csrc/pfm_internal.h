@@ -157,6 +157,7 @@ struct pfm_ldu_s {
   int32_t *d_map = nullptr;
   int32_t *d_own = nullptr;
   int32_t *d_nei = nullptr;
+  int32_t *d_own_start = nullptr;       // n + 1: first owner-face of each row (oriented_sorted only)
   int32_t *d_dic_cells = nullptr;
   int32_t *d_dilu_faces = nullptr;
   int32_t *d_dilu_level_ptr = nullptr;

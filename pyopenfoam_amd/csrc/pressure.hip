@@ -160,15 +160,18 @@ __global__ __launch_bounds__(kBlock) void k_cff(int nf_all, int nf,
 // index and column, then every gather, then the sums in slot order (two
 // dependent round trips for rows of at most 4 + 4 couplings).  kPair reads
 // the columns from the paired 16-bit offsets.  kOwnRun (faces oriented and
-// sorted by owner): a row's owner-side faces are consecutive, so one face
-// index (its first lower slot's) gives all of them.  At 128^3: 158 us with the
-// per-group round trips, 130 us in this form.
+// sorted by owner): a row's owner-side faces are consecutive from its entry
+// of own_start, loaded with the row's operands.  HbyA and U, read and written
+// once, go nontemporally.  At 128^3: 158 us with the per-group round trips,
+// 140 us with the owner-side run from the first lower slot's face index and
+// plain HbyA and U, 131 us in this form.
 template <bool kPair, bool kOwnRun>
 __global__ __launch_bounds__(kBlock) void k_cvel(int n, int nf, int nslices,
                                                   const SliceInfo *__restrict__ sl,
                                                   const int32_t *__restrict__ map,
                                                   const int32_t *__restrict__ col,
                                                   const uint32_t *__restrict__ cp,
+                                                  const int32_t *__restrict__ own_start,
                                                   const double *__restrict__ w,
                                                   const double *__restrict__ p,
                                                   const double *__restrict__ sf,
@@ -187,10 +190,11 @@ __global__ __launch_bounds__(kBlock) void k_cvel(int n, int nf, int nslices,
   const double prow = in ? p[row] : 0.0;
   const double v0 = in ? vol[row] : 1.0;
   const double apr = in ? ap[row] : 1.0;
-  const double h0 = in ? hbya[3 * row] : 0.0;
-  const double h1 = in ? hbya[3 * row + 1] : 0.0;
-  const double h2 = in ? hbya[3 * row + 2] : 0.0;
+  const double h0 = in ? ld<true>(hbya + 3 * row) : 0.0;
+  const double h1 = in ? ld<true>(hbya + 3 * row + 1) : 0.0;
+  const double h2 = in ? ld<true>(hbya + 3 * row + 2) : 0.0;
   const int b0 = in ? bptr[row] : 0, b1 = in ? bptr[row + 1] : 0;
+  const int f0 = kOwnRun && in ? own_start[row] : 0;
   double gx = 0.0, gy = 0.0, gz = 0.0;
   // One slot's contribution, in slot order (owner side: + p_f S_f; neighbour
   // side: + (-p_f S_f)).
@@ -226,7 +230,6 @@ __global__ __launch_bounds__(kBlock) void k_cvel(int n, int nf, int nslices,
 #pragma unroll
       for (int j = 0; j < U; ++j) c[j] = j < nsl ? col[b + j * kSlice] : -1;
     }
-    const int f0 = kOwnRun && wl > 0 ? map[b] : -1;
 #pragma unroll
     for (int j = 0; j < U; ++j) {
       if (kOwnRun && j < wl)
@@ -277,9 +280,9 @@ __global__ __launch_bounds__(kBlock) void k_cvel(int n, int nf, int nslices,
     uy = uy * sc;
     uz = uz * sc;
   }
-  u[3 * row] = ux;
-  u[3 * row + 1] = uy;
-  u[3 * row + 2] = uz;
+  __builtin_nontemporal_store(ux, u + 3 * row);
+  __builtin_nontemporal_store(uy, u + 3 * row + 1);
+  __builtin_nontemporal_store(uz, u + 3 * row + 2);
 }
 
 constexpr int kAdjThreads = 1024;
@@ -394,13 +397,15 @@ int pfm_correct_velocity(pfm_ldu h, int32_t n_faces, const int32_t *bnd_ptr,
   hipStream_t s = (hipStream_t)stream;
   const int G = blocks_for_slices(h->nslices);
 #define PFM_CVEL_ARGS                                                                    \
-  h->n, h->nf, h->nslices, h->d_slices, h->d_map, h->d_col, h->d_colp2, weights, p, face_areas, \
+  h->n, h->nf, h->nslices, h->d_slices, h->d_map, h->d_col, h->d_colp2, h->d_own_start, weights, \
+      p, face_areas, \
       bnd_ptr, bnd_face, vol, a_p, hbya, alpha, u_out
-  if (h->d_colp2 && h->oriented_sorted)
+  const bool run = h->oriented_sorted && h->d_own_start;
+  if (h->d_colp2 && run)
     k_cvel<true, true><<<G, kBlock, 0, s>>>(PFM_CVEL_ARGS);
   else if (h->d_colp2)
     k_cvel<true, false><<<G, kBlock, 0, s>>>(PFM_CVEL_ARGS);
-  else if (h->oriented_sorted)
+  else if (run)
     k_cvel<false, true><<<G, kBlock, 0, s>>>(PFM_CVEL_ARGS);
   else
     k_cvel<false, false><<<G, kBlock, 0, s>>>(PFM_CVEL_ARGS);
