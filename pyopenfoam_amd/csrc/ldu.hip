@@ -581,9 +581,12 @@ __global__ __launch_bounds__(kBlock) void k_lap_faces(int nf, const double *__re
 // diag_i = (sum over owner-faces of c/V_i) + (sum over neighbour-faces of c/V_i);
 // c/V_P == -lower_f and c/V_N == -upper_f exactly, so the sums read the
 // freshly written face coefficients through the slot map.
+// own_start (owner-sorted faces, else null): a row's owner faces are
+// own_start[row] .. own_start[row + 1] - 1, so the lower group needs no map.
 __global__ __launch_bounds__(kBlock) void k_lap_diag(int n, int nslices,
                                                       const SliceInfo *__restrict__ sl,
                                                       const int32_t *__restrict__ map,
+                                                      const int32_t *__restrict__ own_start,
                                                       const double *__restrict__ lower,
                                                       const double *__restrict__ upper,
                                                       double *__restrict__ diag) {
@@ -594,9 +597,15 @@ __global__ __launch_bounds__(kBlock) void k_lap_diag(int n, int nslices,
   const int row = s * kSlice + lane;
   double s1 = 0.0, s2 = 0.0;
   int base = si.ptr + lane;
-  for (int j = 0; j < si.wl; ++j, base += kSlice) {
-    const int f = map[base];
-    if (f >= 0) s1 += -lower[f];
+  if (own_start) {
+    if (row < n)
+      for (int f = own_start[row]; f < own_start[row + 1]; ++f) s1 += -lower[f];
+    base += si.wl * kSlice;
+  } else {
+    for (int j = 0; j < si.wl; ++j, base += kSlice) {
+      const int f = map[base];
+      if (f >= 0) s1 += -lower[f];
+    }
   }
   for (int j = 0; j < si.wu; ++j, base += kSlice) {
     const int f = map[base];
@@ -1378,8 +1387,9 @@ int pfm_laplacian_assemble(pfm_ldu h, const double *gamma, const double *mag_sf,
     k_lap_faces<<<grid1d(h->nf), kBlock, 0, s>>>(h->nf, gamma, mag_sf, delta, vol, h->d_own,
                                                  h->d_nei, lower, upper);
   }
-  k_lap_diag<<<blocks_for_slices(h->nslices), kBlock, 0, s>>>(h->n, h->nslices, h->d_slices,
-                                                              h->d_map, lower, upper, diag);
+  k_lap_diag<<<blocks_for_slices(h->nslices), kBlock, 0, s>>>(
+      h->n, h->nslices, h->d_slices, h->d_map, h->oriented_sorted ? h->d_own_start : nullptr,
+      lower, upper, diag);
   if (n_dir > 0) {
     PFM_NONNULL(dir_gamma);
     PFM_NONNULL(dir_mag_sf);
