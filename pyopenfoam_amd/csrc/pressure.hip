@@ -55,6 +55,7 @@ __global__ __launch_bounds__(kBlock) void k_pe_faces(int nf, const int32_t *__re
 __global__ __launch_bounds__(kBlock) void k_pe_rows(int n, int nf, int nslices,
                                                      const SliceInfo *__restrict__ sl,
                                                      const int32_t *__restrict__ map,
+                                                     const int32_t *__restrict__ own_start,
                                                      const double *__restrict__ lower,
                                                      const double *__restrict__ upper,
                                                      const double *__restrict__ phi,
@@ -76,9 +77,15 @@ __global__ __launch_bounds__(kBlock) void k_pe_rows(int n, int nf, int nslices,
     constexpr int U = 4;
     const int bu = base + si.wl * kSlice;
     int fl[U], fu[U];
+    // owner-sorted faces: the row's owner faces follow from own_start, no map
+    const int o0 = own_start && row < n ? own_start[row] : 0;
+    const int o1 = own_start && row < n ? own_start[row + 1] : 0;
 #pragma unroll
     for (int j = 0; j < U; ++j) {
-      fl[j] = j < si.wl ? map[base + j * kSlice] : -1;
+      if (own_start)
+        fl[j] = o0 + j < o1 ? o0 + j : -1;
+      else
+        fl[j] = j < si.wl ? map[base + j * kSlice] : -1;
       fu[j] = j < si.wu ? map[bu + j * kSlice] : -1;
     }
     double lv[U], pl[U], uv[U], pu[U];
@@ -350,7 +357,8 @@ int pfm_pressure_assemble(pfm_ldu h, int32_t n_faces, const int32_t *bnd_ptr,
     k_pe_faces<<<grid1d(h->nf), kBlock, 0, s>>>(h->nf, h->d_own, h->d_nei, a_p, weights, mag_sf,
                                                  delta, vol, lower, upper);
   k_pe_rows<<<blocks_for_slices(h->nslices), kBlock, 0, s>>>(
-      h->n, h->nf, h->nslices, h->d_slices, h->d_map, lower, upper, phi_hbya, bnd_ptr, bnd_face,
+      h->n, h->nf, h->nslices, h->d_slices, h->d_map,
+      h->oriented_sorted ? h->d_own_start : nullptr, lower, upper, phi_hbya, bnd_ptr, bnd_face,
       vol, diag, source);
   PFM_CHECK_HIP(hipGetLastError());
   return PFM_OK;
